@@ -1,0 +1,418 @@
+// extern "C" entry points of include/gis.h.
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/gis.h"
+#include "gis_internal.hpp"
+
+namespace gis {
+
+int fail(gis_context* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+int hip_fail(gis_context* h, hipError_t e, const char* what) {
+    if (h) h->err = std::string(what) + ": " + hipGetErrorString(e);
+    return kErrHip;
+}
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+int upload(gis_context* h, DevMem& dst, const void* src, size_t bytes) {
+    GIS_HIP(h, dst.ensure(bytes));
+    if (bytes) GIS_HIP(h, hipMemcpyAsync(dst.p, src, bytes, hipMemcpyHostToDevice, h->stream));
+    return kOK;
+}
+
+int copy_d2d(gis_context* h, DevMem& dst, const void* src, size_t bytes) {
+    GIS_HIP(h, dst.ensure(bytes));
+    if (bytes) GIS_HIP(h, hipMemcpyAsync(dst.p, src, bytes, hipMemcpyDeviceToDevice, h->stream));
+    return kOK;
+}
+
+int set_values_common(gis_context* h) {
+    GIS_TRY(gather_values(h));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    return kOK;
+}
+
+SolveArgs to_args(int method, const gis_params* p, int hist_cap) {
+    gis_params d;
+    gis_default_params(&d);
+    if (!p) p = &d;
+    SolveArgs a;
+    a.method = method;
+    a.error = p->error;
+    a.max_steps = p->max_steps;
+    a.omega = p->omega;
+    a.power_steps = p->power_steps;
+    a.int_bound = p->int_bound;
+    a.hist_cap = hist_cap;
+    return a;
+}
+
+// x and b are staged into the handle's io buffer: [x | b]
+int solve_staged(gis_context* h, const SolveArgs& a, double* x_host_or_dev,
+                 const double* b_host_or_dev, bool on_device, double* hist, int hist_cap,
+                 int* hist_len, double* norm0, double* normf) {
+    const size_t n = (size_t)h->N;
+    GIS_HIP(h, h->io.ensure(sizeof(double) * 2 * n));
+    double* dx = h->io.as<double>();
+    double* db = dx + n;
+    const hipMemcpyKind kin = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind kout = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    GIS_HIP(h, hipMemcpyAsync(dx, x_host_or_dev, sizeof(double) * n, kin, h->stream));
+    GIS_HIP(h, hipMemcpyAsync(db, b_host_or_dev, sizeof(double) * n, kin, h->stream));
+    int len = 0;
+    const int st = device_solve(h, a, dx, db, &len, norm0, normf);
+    if (st != kOK && st < kBreakdown) return st;
+    GIS_HIP(h, hipMemcpyAsync(x_host_or_dev, dx, sizeof(double) * n, kout, h->stream));
+    if (hist && hist_cap > 0) {
+        const int m = std::min(len, std::min(hist_cap, a.hist_cap));
+        GIS_HIP(h, hipMemcpyAsync(hist, h->hist.p, sizeof(double) * (size_t)m,
+                                  hipMemcpyDeviceToHost, h->stream));
+    }
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    if (hist_len) *hist_len = len;
+    return st;
+}
+
+}  // namespace
+}  // namespace gis
+
+using namespace gis;
+
+extern "C" {
+
+GIS_API void gis_default_params(gis_params* p) {
+    if (!p) return;
+    p->error = 1e-4;
+    p->max_steps = 0;
+    p->omega = 0.0;
+    p->power_steps = 30;
+    p->int_bound = 1e-10;
+}
+
+GIS_API const char* gis_status_string(int s) {
+    switch (s) {
+        case GIS_OK: return "ok";
+        case GIS_ERR_ARG: return "invalid argument";
+        case GIS_ERR_HIP: return "HIP runtime error";
+        case GIS_ERR_SHAPE: return "size mismatch or limit exceeded";
+        case GIS_ERR_STATE: return "matrix pattern or values not set";
+        case GIS_ERR_NOMEM: return "out of memory";
+        case GIS_BREAKDOWN: return "BiCG breakdown";
+        case GIS_NOT_CONVERGED: return "iteration cap reached";
+        case GIS_NAN: return "residual is NaN";
+        default: return "unknown status";
+    }
+}
+
+GIS_API int gis_create(gis_handle* out, int device) {
+    if (!out) return GIS_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return GIS_ERR_HIP;
+    if (device < 0 || device >= ndev) return GIS_ERR_ARG;
+    gis_context* h = new (std::nothrow) gis_context();
+    if (!h) return GIS_ERR_NOMEM;
+    h->device = device;
+    DeviceGuard g(device);
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return GIS_ERR_HIP;
+    }
+    h->own_stream = true;
+    if (ensure_state(h) != kOK) {
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return GIS_ERR_HIP;
+    }
+    *out = h;
+    return GIS_OK;
+}
+
+GIS_API int gis_destroy(gis_handle h) {
+    if (!h) return GIS_ERR_ARG;
+    {
+        DeviceGuard g(h->device);
+        (void)hipStreamSynchronize(h->stream);
+        if (h->st_host) (void)hipHostFree(h->st_host);
+        h->st_host = nullptr;
+        if (h->own_stream) (void)hipStreamDestroy(h->stream);
+        delete h;   // DevMem destructors free device memory with the device current
+    }
+    return GIS_OK;
+}
+
+GIS_API int gis_set_stream(gis_handle h, void* stream) {
+    if (!h) return GIS_ERR_ARG;
+    DeviceGuard g(h->device);
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    if (h->own_stream) GIS_HIP(h, hipStreamDestroy(h->stream));
+    h->own_stream = false;
+    h->stream = static_cast<hipStream_t>(stream);
+    return GIS_OK;
+}
+
+GIS_API int gis_reset_stream(gis_handle h) {
+    if (!h) return GIS_ERR_ARG;
+    if (h->own_stream) return GIS_OK;
+    DeviceGuard g(h->device);
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    GIS_HIP(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->own_stream = true;
+    return GIS_OK;
+}
+
+GIS_API int gis_synchronize(gis_handle h) {
+    if (!h) return GIS_ERR_ARG;
+    DeviceGuard g(h->device);
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    return GIS_OK;
+}
+
+GIS_API const char* gis_last_error(gis_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+static int set_pattern_impl(gis_handle h, int n_cells, int n_faces, const int* la,
+                            const int* ua, bool dev) {
+    if (!h) return GIS_ERR_ARG;
+    if (n_cells < 1 || n_faces < 0 || (n_faces > 0 && (!la || !ua)))
+        return fail(h, GIS_ERR_ARG, "set_pattern: need n_cells >= 1, n_faces >= 0, addressing");
+    DeviceGuard g(h->device);
+    const size_t b = sizeof(int) * (size_t)n_faces;
+    if (dev) {
+        GIS_TRY(copy_d2d(h, h->lower_addr, la, b));
+        GIS_TRY(copy_d2d(h, h->upper_addr, ua, b));
+    } else {
+        GIS_TRY(upload(h, h->lower_addr, la, b));
+        GIS_TRY(upload(h, h->upper_addr, ua, b));
+    }
+    GIS_TRY(build_pattern(h, n_cells, n_faces, h->lower_addr.as<int>(), h->upper_addr.as<int>()));
+    GIS_TRY(ensure_reduction(h, n_cells));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    return GIS_OK;
+}
+
+GIS_API int gis_set_pattern(gis_handle h, int n_cells, int n_faces, const int* lower_addr,
+                            const int* upper_addr) {
+    return set_pattern_impl(h, n_cells, n_faces, lower_addr, upper_addr, false);
+}
+
+GIS_API int gis_set_pattern_device(gis_handle h, int n_cells, int n_faces,
+                                   const int* d_lower_addr, const int* d_upper_addr) {
+    return set_pattern_impl(h, n_cells, n_faces, d_lower_addr, d_upper_addr, true);
+}
+
+static int set_values_impl(gis_handle h, const double* diag, const double* lower,
+                           const double* upper, bool dev) {
+    if (!h) return GIS_ERR_ARG;
+    if (!h->has_pattern) return fail(h, GIS_ERR_STATE, "set_values before set_pattern");
+    if (!diag || (h->M > 0 && (!lower || !upper))) return fail(h, GIS_ERR_ARG, "null coefficients");
+    DeviceGuard g(h->device);
+    const size_t bn = sizeof(double) * (size_t)h->N;
+    const size_t bm = sizeof(double) * (size_t)h->M;
+    if (dev) {
+        GIS_TRY(copy_d2d(h, h->diag, diag, bn));
+        GIS_TRY(copy_d2d(h, h->lower, lower, bm));
+        GIS_TRY(copy_d2d(h, h->upper, upper, bm));
+    } else {
+        GIS_TRY(upload(h, h->diag, diag, bn));
+        GIS_TRY(upload(h, h->lower, lower, bm));
+        GIS_TRY(upload(h, h->upper, upper, bm));
+    }
+    return set_values_common(h);
+}
+
+GIS_API int gis_set_values(gis_handle h, const double* diag, const double* lower,
+                           const double* upper) {
+    return set_values_impl(h, diag, lower, upper, false);
+}
+
+GIS_API int gis_set_values_device(gis_handle h, const double* d_diag, const double* d_lower,
+                                  const double* d_upper) {
+    return set_values_impl(h, d_diag, d_lower, d_upper, true);
+}
+
+GIS_API int gis_get_size(gis_handle h, int* n_rows, long long* nnz) {
+    if (!h) return GIS_ERR_ARG;
+    if (!h->has_pattern) return fail(h, GIS_ERR_STATE, "pattern not set");
+    if (n_rows) *n_rows = h->N;
+    if (nnz) *nnz = h->nnz;
+    return GIS_OK;
+}
+
+static int get_csr_impl(gis_handle h, int* rp, int* ci, double* v, bool transpose) {
+    if (!h) return GIS_ERR_ARG;
+    if (!h->has_pattern) return fail(h, GIS_ERR_STATE, "pattern not set");
+    DeviceGuard g(h->device);
+    if (rp)
+        GIS_HIP(h, hipMemcpyAsync(rp, h->rowptr.p, sizeof(int) * ((size_t)h->N + 1),
+                                  hipMemcpyDeviceToHost, h->stream));
+    if (ci)
+        GIS_HIP(h, hipMemcpyAsync(ci, h->col.p, sizeof(int) * (size_t)h->nnz,
+                                  hipMemcpyDeviceToHost, h->stream));
+    if (v) {
+        if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
+        if (transpose) GIS_TRY(ensure_valT(h));
+        GIS_HIP(h, hipMemcpyAsync(v, transpose ? h->valT.p : h->val.p,
+                                  sizeof(double) * (size_t)h->nnz, hipMemcpyDeviceToHost,
+                                  h->stream));
+    }
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    return GIS_OK;
+}
+
+GIS_API int gis_get_csr(gis_handle h, int* row_offsets, int* column_indices, double* values) {
+    return get_csr_impl(h, row_offsets, column_indices, values, false);
+}
+
+GIS_API int gis_get_csr_transpose(gis_handle h, int* row_offsets, int* column_indices,
+                                  double* values) {
+    return get_csr_impl(h, row_offsets, column_indices, values, true);
+}
+
+GIS_API int gis_spmv_device(gis_handle h, const double* d_x, double* d_y) {
+    if (!h || !d_x || !d_y) return GIS_ERR_ARG;
+    DeviceGuard g(h->device);
+    GIS_TRY(launch_spmv(h, d_x, d_y));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    return GIS_OK;
+}
+
+GIS_API int gis_spmv_device_async(gis_handle h, const double* d_x, double* d_y) {
+    if (!h || !d_x || !d_y) return GIS_ERR_ARG;
+    DeviceGuard g(h->device);
+    return launch_spmv(h, d_x, d_y);
+}
+
+GIS_API int gis_spmv32_device_async(gis_handle h, const float* d_x, float* d_y) {
+    if (!h || !d_x || !d_y) return GIS_ERR_ARG;
+    DeviceGuard g(h->device);
+    return launch_spmv32(h, d_x, d_y);
+}
+
+GIS_API int gis_spmv_transpose_device(gis_handle h, const double* d_x, double* d_y) {
+    if (!h || !d_x || !d_y) return GIS_ERR_ARG;
+    DeviceGuard g(h->device);
+    GIS_TRY(launch_spmv_t(h, d_x, d_y));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    return GIS_OK;
+}
+
+GIS_API int gis_dot_device(gis_handle h, const double* d_a, const double* d_b, double* out) {
+    if (!h || !d_a || !d_b || !out) return GIS_ERR_ARG;
+    if (!h->has_pattern) return fail(h, GIS_ERR_STATE, "pattern not set (vector length)");
+    DeviceGuard g(h->device);
+    return device_dot(h, d_a, d_b, out);
+}
+
+GIS_API int gis_set_schedule(gis_handle h, int spmv64, int spmv32, int block_iters) {
+    if (!h) return GIS_ERR_ARG;
+    if (spmv64 > 2 || spmv32 > 2) return fail(h, GIS_ERR_ARG, "schedule id out of range");
+    if (spmv64 >= 0) h->sched.spmv64 = static_cast<SpmvKind>(spmv64);
+    if (spmv32 >= 0) h->sched.spmv32 = static_cast<SpmvKind>(spmv32);
+    if (block_iters > 0) h->sched.block_iters = block_iters;
+    return GIS_OK;
+}
+
+GIS_API int gis_get_schedule(gis_handle h, int* spmv64, int* spmv32, int* block_iters) {
+    if (!h) return GIS_ERR_ARG;
+    if (spmv64) *spmv64 = static_cast<int>(h->sched.spmv64);
+    if (spmv32) *spmv32 = static_cast<int>(h->sched.spmv32);
+    if (block_iters) *block_iters = h->sched.block_iters;
+    return GIS_OK;
+}
+
+static int norm_factor_impl(gis_handle h, const double* b, const double* x, double* nf,
+                            bool dev) {
+    if (!h || !b || !x || !nf) return GIS_ERR_ARG;
+    if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
+    DeviceGuard g(h->device);
+    const size_t n = (size_t)h->N;
+    GIS_HIP(h, h->io.ensure(sizeof(double) * 2 * n));
+    double* dx = h->io.as<double>();
+    double* db = dx + n;
+    const hipMemcpyKind k = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    GIS_HIP(h, hipMemcpyAsync(dx, x, sizeof(double) * n, k, h->stream));
+    GIS_HIP(h, hipMemcpyAsync(db, b, sizeof(double) * n, k, h->stream));
+    return device_norm_factor(h, db, dx, nf);
+}
+
+GIS_API int gis_norm_factor(gis_handle h, const double* b, const double* x, double* nf) {
+    return norm_factor_impl(h, b, x, nf, false);
+}
+
+GIS_API int gis_norm_factor_device(gis_handle h, const double* d_b, const double* d_x,
+                                   double* nf) {
+    return norm_factor_impl(h, d_b, d_x, nf, true);
+}
+
+static int solve_impl(gis_handle h, int method, double* x, const double* b,
+                      const gis_params* params, double* hist, int hist_cap, int* hist_len,
+                      bool dev) {
+    if (!h || !x || !b) return GIS_ERR_ARG;
+    if (method < GIS_CG || method > GIS_RICHARDSON) return fail(h, GIS_ERR_ARG, "unknown method");
+    if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
+    DeviceGuard g(h->device);
+    SolveArgs a = to_args(method, params, std::max(1, hist_cap));
+    return solve_staged(h, a, x, b, dev, hist, hist_cap, hist_len, nullptr, nullptr);
+}
+
+GIS_API int gis_solve(gis_handle h, int method, double* x, const double* b,
+                      const gis_params* params, double* hist, int hist_cap, int* hist_len) {
+    return solve_impl(h, method, x, b, params, hist, hist_cap, hist_len, false);
+}
+
+GIS_API int gis_solve_device(gis_handle h, int method, double* d_x, const double* d_b,
+                             const gis_params* params, double* hist, int hist_cap,
+                             int* hist_len) {
+    return solve_impl(h, method, d_x, d_b, params, hist, hist_cap, hist_len, true);
+}
+
+GIS_API int gis_foam_solve(gis_handle h, int method, double* psi, const double* source,
+                           double tolerance, long long max_iter, gis_performance* perf) {
+    if (!h || !psi || !source) return GIS_ERR_ARG;
+    if (method < GIS_CG || method > GIS_RICHARDSON) return fail(h, GIS_ERR_ARG, "unknown method");
+    if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
+    DeviceGuard g(h->device);
+    double nf = 0.0;
+    GIS_TRY(norm_factor_impl(h, source, psi, &nf, false));
+    gis_params p;
+    gis_default_params(&p);
+    p.error = tolerance * nf;   // cuFoamCG.C:68-74: error * norm_factor
+    p.max_steps = max_iter;
+    p.omega = 0.0;              // cuFoamRichardson.C:66
+    p.power_steps = 30;
+    p.int_bound = 1e-10;        // cuFoamBiCG.C:68
+    SolveArgs a = to_args(method, &p, 1);
+    int len = 0;
+    double n0 = 0.0, nfin = 0.0;
+    const int st = solve_staged(h, a, psi, source, false, nullptr, 0, &len, &n0, &nfin);
+    if (st != GIS_OK && st < GIS_BREAKDOWN) return st;
+    if (perf) {
+        perf->initial_residual = n0;
+        perf->final_residual = nfin;
+        perf->n_iterations = len - 1;
+        perf->status = st;
+        perf->norm_factor = nf;
+    }
+    return st;
+}
+
+}  // extern "C"

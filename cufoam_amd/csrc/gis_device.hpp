@@ -1,0 +1,321 @@
+// Device-side building blocks of the gis kernels for gfx950 (wave64).
+//
+// Arithmetic contract (identical in oracle/gis_oracle.c, so results can be
+// compared bit for bit):
+//   * compiled with -ffp-contract=off; fused multiply-adds only via fma()/fmaf();
+//   * CSR row sums are sequential in storage order, products rounded first;
+//   * every reduction has ONE order fixed by the vector length: leaves of
+//     kLeaf = 4 consecutive elements accumulated from +0.0, then a pairwise
+//     binary tree over the leaves.  A workgroup of kWG threads owns an aligned
+//     tile of kWG*kLeaf rows (one subtree); partials of tiles are folded in
+//     aligned groups of kGroup by the last-arriving workgroup of each group,
+//     so the result never depends on the launch grid or the device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gis {
+
+constexpr int kWG = 256;              // threads per workgroup (4 wave64)
+constexpr int kLeaf = 4;              // consecutive rows per reduction leaf
+constexpr int kTile = kWG * kLeaf;    // rows per workgroup tile (one partial)
+constexpr int kGroup = 64;            // partials folded per group (one wave)
+constexpr int kMaxLevels = 6;         // kGroup^5 * kTile rows >> 2^31
+constexpr int kMaxNV = 4;             // reductions carried by one kernel
+
+// Cross-workgroup reduction workspace (device pointers, passed by value).
+struct RedWs {
+    double* part[kMaxLevels];         // level-L partials, count[L] * kMaxNV doubles
+    unsigned* ticket[kMaxLevels];     // one arrival counter per group of level L
+    int count[kMaxLevels];            // number of partials at level L
+    int nlevels;                      // levels until count == 1
+};
+
+// Solver scalars that live on the device between launches.
+struct KState {
+    double rho0, rho1, alpha, beta;   // Krylov scalars
+    double norm, norm0;               // last and initial residual norm
+    double error;                     // absolute stopping threshold
+    double omega;                     // Richardson relaxation factor
+    double int_bound;                 // BiCG breakdown bound
+    double aux[8];                    // per-solver extras (norm factor, x_bar, ...)
+    double* hist;                     // residual history (device), hist_cap entries
+    long long max_steps;              // iteration cap (already made positive)
+    int k;                            // completed iterations
+    int done;                         // 1: stop, later launches are no-ops
+    int status;                       // solve outcome code
+    int hist_cap;
+    int flag;                         // per-solver control word
+    int pad;
+};
+
+// ---------------------------------------------------------------------------
+// cache-bypassing agent-scope accesses for the cross-workgroup partials.
+// Agent-scope fences on gfx950 write back and invalidate the L2; the partials
+// travel through atomics that bypass the non-coherent caches instead, and the
+// fences below only order the memory operations of this wave.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void st_agent(double* p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
+                       static_cast<unsigned long long>(__double_as_longlong(v)),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double ld_agent(const double* p) {
+    return __longlong_as_double(static_cast<long long>(
+        __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT)));
+}
+
+__device__ __forceinline__ void order_release() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));   // vmcnt(0)
+}
+
+__device__ __forceinline__ void order_acquire() {
+    __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ unsigned ticket_add(unsigned* t) {
+    return __hip_atomic_fetch_add(t, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void ticket_reset(unsigned* t) {
+    __hip_atomic_store(t, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// pairwise trees
+// ---------------------------------------------------------------------------
+// Butterfly over the 64 lanes: afterwards every lane holds the pairwise-tree
+// sum of the wave's 64 values (addition is commutative, so partner lanes agree).
+template <int NV>
+__device__ __forceinline__ void wave_tree(double (&v)[NV]) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] = v[i] + __shfl_xor(v[i], m, 64);
+    }
+}
+
+// Pairwise tree over the kWG thread values; the result lands in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_tree(double (&v)[NV], double (*sh)[kMaxNV]) {
+    wave_tree<NV>(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) sh[w][i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) v[i] = (sh[0][i] + sh[1][i]) + (sh[2][i] + sh[3][i]);
+    }
+}
+
+// LDS a kernel hands to finish_tile().
+struct RedShared {
+    double w[4][kMaxNV];
+    int last;
+};
+
+// Fold this workgroup's leaf values into the global reduction.  The tile's
+// subtree is formed here; the last workgroup to arrive in each aligned group
+// of kGroup tiles folds the group, level by level, and the very last one calls
+// fin(totals) on lane 0 of wave 0.  Must be reached by every thread of the
+// workgroup exactly once per launch.
+template <int NV, class Fin>
+__device__ __forceinline__ void finish_tile(const RedWs& red, int tile, double (&v)[NV],
+                                            RedShared& sh, Fin&& fin) {
+    block_tree<NV>(v, sh.w);
+    if (red.count[0] == 1) {
+        if (threadIdx.x == 0) fin(v);
+        return;
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) st_agent(&red.part[0][(size_t)tile * kMaxNV + i], v[i]);
+        order_release();
+        const int g = tile / kGroup;
+        const int members = min(kGroup, red.count[0] - g * kGroup);
+        const unsigned prev = ticket_add(&red.ticket[0][g]);
+        const int last = (prev == (unsigned)(members - 1));
+        if (last) ticket_reset(&red.ticket[0][g]);
+        sh.last = last;
+    }
+    __syncthreads();
+    if (!sh.last || threadIdx.x >= 64) return;
+
+    // wave 0 of a group-final workgroup
+    const int lane = threadIdx.x;
+    int idx = tile / kGroup;   // this group's index = partial index one level up
+    for (int L = 0; L + 1 < kMaxLevels; ++L) {
+        order_acquire();
+        double w[NV];
+        const int src = idx * kGroup + lane;
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+            w[i] = (src < red.count[L]) ? ld_agent(&red.part[L][(size_t)src * kMaxNV + i]) : 0.0;
+        wave_tree<NV>(w);
+        if (red.count[L + 1] == 1) {
+            if (lane == 0) fin(w);
+            return;
+        }
+        int last = 0;
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i)
+                st_agent(&red.part[L + 1][(size_t)idx * kMaxNV + i], w[i]);
+            order_release();
+            const int g = idx / kGroup;
+            const int members = min(kGroup, red.count[L + 1] - g * kGroup);
+            const unsigned prev = ticket_add(&red.ticket[L + 1][g]);
+            last = (prev == (unsigned)(members - 1));
+            if (last) ticket_reset(&red.ticket[L + 1][g]);
+        }
+        last = __shfl(last, 0, 64);
+        if (!last) return;
+        idx = idx / kGroup;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CSR tile SpMV: y[row0 .. row0+nrows) = sum_j a_j * X(col_j), written to
+// sh_y.  X(c) is evaluated on the fly (it may combine several vectors).
+// Each row is summed sequentially from 0 in storage order.  Products are
+// staged through LDS in chunks of at most CAP nonzeros so that val/col are
+// read with unit stride across the workgroup.
+// ---------------------------------------------------------------------------
+template <typename T, int CAP>
+struct SpmvShared {
+    int rp[kTile + 1];
+    T prod[CAP];
+    int end;
+};
+
+template <typename T, int CAP, class XOp>
+__device__ __forceinline__ void tile_spmv(const int* __restrict__ rowptr,
+                                          const int* __restrict__ col,
+                                          const T* __restrict__ val, int row0, int nrows,
+                                          XOp&& xop, SpmvShared<T, CAP>& sh,
+                                          T* __restrict__ sh_y) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i <= nrows; i += kWG) sh.rp[i] = rowptr[row0 + i];
+    __syncthreads();
+    int r = 0;
+    while (r < nrows) {
+        if (tid == 0) {
+            const int base = sh.rp[r];
+            int e;
+            if (sh.rp[r + 1] - base > CAP) {
+                e = r;   // a single row longer than the staging buffer
+            } else {
+                int lo = r + 1, hi = nrows;   // invariant: rp[lo]-base <= CAP
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (sh.rp[mid] - base <= CAP) lo = mid; else hi = mid - 1;
+                }
+                e = lo;
+            }
+            sh.end = e;
+        }
+        __syncthreads();
+        const int e = sh.end;
+        if (e == r) {
+            if (tid == 0) {
+                T s = T(0);
+                for (int j = sh.rp[r]; j < sh.rp[r + 1]; ++j) s = s + val[j] * xop(col[j]);
+                sh_y[r] = s;
+            }
+            __syncthreads();
+            r = r + 1;
+            continue;
+        }
+        const int nz0 = sh.rp[r];
+        const int cnt = sh.rp[e] - nz0;
+#pragma unroll 4
+        for (int j = tid; j < cnt; j += kWG) sh.prod[j] = val[nz0 + j] * xop(col[nz0 + j]);
+        __syncthreads();
+        for (int i = r + tid; i < e; i += kWG) {
+            const int a = sh.rp[i] - nz0;
+            const int b = sh.rp[i + 1] - nz0;
+            T s = T(0);
+            for (int j = a; j < b; ++j) s = s + sh.prod[j];
+            sh_y[i] = s;
+        }
+        __syncthreads();
+        r = e;
+    }
+}
+
+// Unstaged variant: each thread sums its own kLeaf consecutive rows straight
+// from global memory (same order, same result; fewer LDS passes, strided loads).
+template <typename T, class XOp>
+__device__ __forceinline__ void tile_spmv_direct(const int* __restrict__ rowptr,
+                                                 const int* __restrict__ col,
+                                                 const T* __restrict__ val, int row0,
+                                                 int nrows, XOp&& xop, T* __restrict__ sh_y) {
+    const int t0 = threadIdx.x * kLeaf;
+#pragma unroll
+    for (int k = 0; k < kLeaf; ++k) {
+        const int i = t0 + k;
+        if (i < nrows) {
+            const int a = rowptr[row0 + i];
+            const int b = rowptr[row0 + i + 1];
+            T s = T(0);
+            for (int j = a; j < b; ++j) s = s + val[j] * xop(col[j]);
+            sh_y[i] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// kLeaf consecutive elements owned by one thread
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load4(const double* __restrict__ p, int i0, int n, double (&o)[4]) {
+    if (i0 + 3 < n) {
+        const double2 a = *reinterpret_cast<const double2*>(p + i0);
+        const double2 b = *reinterpret_cast<const double2*>(p + i0 + 2);
+        o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k] = (i0 + k < n) ? p[i0 + k] : 0.0;
+    }
+}
+
+__device__ __forceinline__ void store4(double* __restrict__ p, int i0, int n, const double (&o)[4]) {
+    if (i0 + 3 < n) {
+        *reinterpret_cast<double2*>(p + i0) = make_double2(o[0], o[1]);
+        *reinterpret_cast<double2*>(p + i0 + 2) = make_double2(o[2], o[3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i0 + k < n) p[i0 + k] = o[k];
+    }
+}
+
+__device__ __forceinline__ void load4(const float* __restrict__ p, int i0, int n, float (&o)[4]) {
+    if (i0 + 3 < n) {
+        const float4 a = *reinterpret_cast<const float4*>(p + i0);
+        o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k] = (i0 + k < n) ? p[i0 + k] : 0.0f;
+    }
+}
+
+__device__ __forceinline__ void store4(float* __restrict__ p, int i0, int n, const float (&o)[4]) {
+    if (i0 + 3 < n) {
+        *reinterpret_cast<float4*>(p + i0) = make_float4(o[0], o[1], o[2], o[3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i0 + k < n) p[i0 + k] = o[k];
+    }
+}
+
+}  // namespace gis

@@ -1,0 +1,164 @@
+// Host-side context of the gis C-ABI: one handle = one device, one HIP stream,
+// one cached LDU pattern (device CSR + value permutation) and its workspaces.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "gis_device.hpp"
+
+namespace gis {
+
+// Owning device allocation that only grows.
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            bytes = 0;
+            if (e != hipSuccess) return e;
+        }
+        if (n == 0) n = 256;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+    ~DevMem() { release(); }
+};
+
+// SpMV schedule: how a tile's products are formed.  Every schedule yields
+// bit-identical results (same per-row order), so the choice is pure tuning.
+enum class SpmvKind : int { Staged2K = 0, Staged4K = 1, Direct = 2 };
+
+struct Schedule {
+    SpmvKind spmv64 = SpmvKind::Staged4K;   // FP64 kernels
+    SpmvKind spmv32 = SpmvKind::Staged4K;   // FP32 kernels
+    int block_iters = 64;                   // max iterations launched between host checks
+};
+
+}  // namespace gis
+
+struct gis_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+
+    // pattern (cached per mesh)
+    int N = 0;
+    int M = 0;
+    long long nnz = 0;
+    bool has_pattern = false;
+    bool has_values = false;
+    bool has_val32 = false;
+    bool has_valT = false;
+    gis::DevMem rowptr, col, perm, nbr_cnt;     // CSR pattern, value permutation
+    gis::DevMem lower_addr, upper_addr;         // LDU addressing copies
+    gis::DevMem diag, lower, upper;             // LDU coefficients (device copies)
+    gis::DevMem val, valT, val32;               // CSR values: A, A^T, float(A)
+    gis::DevMem dinv;                           // per-row helper (1/a_ii or sqrt(1/a_ii))
+
+    // workspaces
+    gis::DevMem vec;                            // solver vectors
+    gis::DevMem io;                             // staging for host <-> device vectors
+    gis::DevMem red_mem;
+    gis::RedWs red{};
+    int red_rows = -1;
+    gis::DevMem state_mem;
+    gis::KState* st = nullptr;                  // device state
+    gis::KState* st_host = nullptr;             // pinned mirror
+    gis::DevMem hist;
+    gis::DevMem scratch;                        // pattern-build temporaries
+
+    gis::Schedule sched;
+    std::string err;
+};
+
+namespace gis {
+
+// status codes (mirrored in include/gis.h)
+enum : int {
+    kOK = 0,
+    kErrArg = 1,
+    kErrHip = 2,
+    kErrShape = 3,
+    kErrState = 4,
+    kErrNoMem = 5,
+    kBreakdown = 10,
+    kNotConverged = 11,
+    kNaN = 12,
+};
+
+int fail(gis_context* h, int code, const std::string& msg);
+int hip_fail(gis_context* h, hipError_t e, const char* what);
+
+#define GIS_HIP(h, expr)                                        \
+    do {                                                        \
+        hipError_t _e = (expr);                                 \
+        if (_e != hipSuccess) return ::gis::hip_fail((h), _e, #expr); \
+    } while (0)
+
+#define GIS_TRY(expr)             \
+    do {                          \
+        int _s = (expr);          \
+        if (_s != ::gis::kOK) return _s; \
+    } while (0)
+
+inline int tiles_of(long long n) { return (int)((n + kTile - 1) / kTile); }
+
+// reductions / state
+int ensure_reduction(gis_context* h, int rows);
+int ensure_state(gis_context* h);
+int read_state(gis_context* h);
+
+// pattern + values (gis_pattern.hip)
+int build_pattern(gis_context* h, int N, int M, const int* d_lower_addr, const int* d_upper_addr);
+int gather_values(gis_context* h);          // val (and valT / val32 when present) from diag/lower/upper
+int ensure_val32(gis_context* h);
+int ensure_valT(gis_context* h);
+
+// kernels (gis_solvers.hip)
+int launch_spmv(gis_context* h, const double* x, double* y);
+int launch_spmv_t(gis_context* h, const double* x, double* y);
+int launch_spmv32(gis_context* h, const float* x, float* y);
+int device_dot(gis_context* h, const double* a, const double* b, double* out);
+int device_norm_factor(gis_context* h, const double* d_b, const double* d_x, double* nf);
+
+struct SolveArgs {
+    int method;
+    double error;
+    long long max_steps;    // <= 0: unbounded
+    double omega;
+    int power_steps;
+    double int_bound;
+    int hist_cap;
+};
+
+// x (in/out) and b are device vectors owned by the handle's io workspace.
+int device_solve(gis_context* h, const SolveArgs& a, double* d_x, const double* d_b,
+                 int* hist_len, double* norm0, double* norm_final);
+
+struct MixedArgs {
+    double rel_tol;
+    double inner_rel_tol;
+    long long max_iter;
+    int mode;               // 0 reliable updates, 1 restarted refinement
+    int hist_cap;
+};
+
+int device_solve_mixed(gis_context* h, const MixedArgs& a, double* d_x, const double* d_b,
+                       int* hist_len, int* iters, double* rel_res);
+
+}  // namespace gis

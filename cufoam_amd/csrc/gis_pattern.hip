@@ -1,0 +1,282 @@
+// Device LDU -> CSR conversion (replaces the host loop of gis/ldu2csr.h:7-42).
+//
+// Row c of the CSR holds, in this order (gis/ldu2csr.h:19-39):
+//   lower faces with neighbour c, ascending face index  (col = owner,     val = lower[f])
+//   the diagonal                                        (col = c,         val = diag[c])
+//   upper faces owned by c, ascending face index        (col = neighbour, val = upper[f])
+// so row_offsets[c] = c + losortStart[c] + ownerStart[c].
+//
+// The pattern is built once per mesh (integer work only, bit-exact and
+// deterministic: atomics only count or claim slots that a per-row sort then
+// orders); a per-nonzero code `perm` remembers where each CSR value comes
+// from, so every later coefficient update is one gather:
+//   perm[p] in [0, M)      -> lower[f]
+//   perm[p] in [M, M+N)    -> diag[c]
+//   perm[p] in [M+N, 2M+N) -> upper[f]
+// A^T of an LDU matrix has the same pattern with lower and upper swapped.
+#include "gis_internal.hpp"
+
+namespace gis {
+
+namespace {
+
+constexpr int kBlk = 256;
+
+__global__ __launch_bounds__(kBlk) void k_count(int N, int M, const int* __restrict__ la,
+                                                const int* __restrict__ ua, int* own_cnt,
+                                                int* nbr_cnt, int* first_own, int* err) {
+    const int f = blockIdx.x * kBlk + threadIdx.x;
+    if (f >= M) return;
+    const int l = la[f];
+    const int u = ua[f];
+    bool bad = l < 0 || l >= N || u < 0 || u >= N;
+    const int lp = f > 0 ? la[f - 1] : -1;
+    if (f > 0 && lp > l) bad = true;   // owners must be sorted (upper-triangular order)
+    if (bad) {
+        atomicOr(err, 1);
+        return;
+    }
+    atomicAdd(&own_cnt[l], 1);
+    atomicAdd(&nbr_cnt[u], 1);
+    if (lp != l) first_own[l] = f;
+}
+
+__global__ __launch_bounds__(kBlk) void k_rowlen(int N, const int* __restrict__ own_cnt,
+                                                 const int* __restrict__ nbr_cnt, int* rowlen) {
+    const int c = blockIdx.x * kBlk + threadIdx.x;
+    if (c < N) rowlen[c] = 1 + own_cnt[c] + nbr_cnt[c];
+    else if (c == N) rowlen[c] = 0;
+}
+
+// exclusive scan of 1024-element blocks (256 threads x 4)
+__global__ __launch_bounds__(kBlk) void k_scan_block(const int* __restrict__ in, int* out,
+                                                     int* block_sums, int n) {
+    __shared__ int sh[kBlk / 64 + 1];
+    const int base = blockIdx.x * (kBlk * 4) + threadIdx.x * 4;
+    int v[4];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        s += v[k];
+    }
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    int x = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int i = 0; i < kBlk / 64; ++i) {
+            const int t = sh[i];
+            sh[i] = acc;
+            acc += t;
+        }
+        sh[kBlk / 64] = acc;
+    }
+    __syncthreads();
+    int run = x - s + sh[w];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0 && block_sums) block_sums[blockIdx.x] = sh[kBlk / 64];
+}
+
+__global__ __launch_bounds__(kBlk) void k_scan_add(int* out, const int* __restrict__ offs, int n) {
+    const int i = blockIdx.x * (kBlk * 4) + threadIdx.x * 4;
+    const int o = offs[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (i + k < n) out[i + k] += o;
+}
+
+__global__ __launch_bounds__(kBlk) void k_fill_diag(int N, int M, const int* __restrict__ rowptr,
+                                                    const int* __restrict__ nbr_cnt, int* col,
+                                                    int* perm) {
+    const int c = blockIdx.x * kBlk + threadIdx.x;
+    if (c >= N) return;
+    const int pos = rowptr[c] + nbr_cnt[c];
+    col[pos] = c;
+    perm[pos] = M + c;
+}
+
+__global__ __launch_bounds__(kBlk) void k_fill_faces(int N, int M, const int* __restrict__ la,
+                                                     const int* __restrict__ ua,
+                                                     const int* __restrict__ rowptr,
+                                                     const int* __restrict__ nbr_cnt,
+                                                     const int* __restrict__ first_own, int* fill,
+                                                     int* col, int* perm) {
+    const int f = blockIdx.x * kBlk + threadIdx.x;
+    if (f >= M) return;
+    const int o = la[f];
+    const int u = ua[f];
+    const int up = rowptr[o] + nbr_cnt[o] + 1 + (f - first_own[o]);
+    col[up] = u;
+    perm[up] = M + N + f;
+    const int slot = atomicAdd(&fill[u], 1);
+    perm[rowptr[u] + slot] = f;   // lower part, ordered by k_sort_lower
+}
+
+__global__ __launch_bounds__(kBlk) void k_sort_lower(int N, const int* __restrict__ la,
+                                                     const int* __restrict__ rowptr,
+                                                     const int* __restrict__ nbr_cnt, int* col,
+                                                     int* perm) {
+    const int c = blockIdx.x * kBlk + threadIdx.x;
+    if (c >= N) return;
+    const int a = rowptr[c];
+    const int n = nbr_cnt[c];
+    for (int i = 1; i < n; ++i) {   // insertion sort by face index (rows are short)
+        const int v = perm[a + i];
+        int j = i - 1;
+        while (j >= 0 && perm[a + j] > v) {
+            perm[a + j + 1] = perm[a + j];
+            --j;
+        }
+        perm[a + j + 1] = v;
+    }
+    for (int i = 0; i < n; ++i) col[a + i] = la[perm[a + i]];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlk) void k_gather(long long nnz, int N, int M,
+                                                 const int* __restrict__ perm,
+                                                 const double* __restrict__ diag,
+                                                 const double* __restrict__ lower,
+                                                 const double* __restrict__ upper, T* val) {
+    const long long p = (long long)blockIdx.x * kBlk + threadIdx.x;
+    if (p >= nnz) return;
+    const int code = perm[p];
+    double v;
+    if (code < M) v = lower[code];
+    else if (code < M + N) v = diag[code - M];
+    else v = upper[code - M - N];
+    val[p] = static_cast<T>(v);
+}
+
+int scan_exclusive(gis_context* h, const int* in, int* out, int n, int* tmp, size_t tmp_ints) {
+    const int nb = (n + kBlk * 4 - 1) / (kBlk * 4);
+    if (nb <= 1) {
+        hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(kBlk), 0, h->stream, in, out, nullptr, n);
+        GIS_HIP(h, hipGetLastError());
+        return kOK;
+    }
+    if ((size_t)2 * nb > tmp_ints) return fail(h, kErrNoMem, "scan scratch too small");
+    int* sums = tmp;
+    int* sums_scanned = tmp + nb;
+    hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(kBlk), 0, h->stream, in, out, sums, n);
+    GIS_HIP(h, hipGetLastError());
+    GIS_TRY(scan_exclusive(h, sums, sums_scanned, nb, tmp + 2 * nb, tmp_ints - 2 * nb));
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kBlk), 0, h->stream, out, sums_scanned, n);
+    GIS_HIP(h, hipGetLastError());
+    return kOK;
+}
+
+inline dim3 grid_for(long long n) { return dim3((unsigned)((n + kBlk - 1) / kBlk)); }
+
+}  // namespace
+
+int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua) {
+    const long long nnz = (long long)N + 2LL * M;
+    if (N < 1 || M < 0 || nnz >= (1LL << 31))
+        return fail(h, kErrShape, "n_cells must be >= 1 and n_cells + 2*n_faces < 2^31");
+    h->has_pattern = false;
+    h->has_values = h->has_val32 = h->has_valT = false;
+    GIS_HIP(h, h->rowptr.ensure(sizeof(int) * ((size_t)N + 1)));
+    GIS_HIP(h, h->col.ensure(sizeof(int) * (size_t)nnz));
+    GIS_HIP(h, h->perm.ensure(sizeof(int) * (size_t)nnz));
+    GIS_HIP(h, h->nbr_cnt.ensure(sizeof(int) * (size_t)N));
+    // scratch: own_cnt[N] first_own[N] fill[N] rowlen[N+1] err[1] + scan temporaries
+    const size_t scan_tmp = 4 * ((size_t)N / (kBlk * 4) + 64);
+    const size_t scratch_ints = 4 * (size_t)N + 2 + scan_tmp;
+    GIS_HIP(h, h->scratch.ensure(sizeof(int) * scratch_ints));
+    int* own_cnt = h->scratch.as<int>();
+    int* first_own = own_cnt + N;
+    int* fill = first_own + N;
+    int* rowlen = fill + N;
+    int* err = rowlen + N + 1;
+    int* tmp = err + 1;
+    int* nbr_cnt = h->nbr_cnt.as<int>();
+    GIS_HIP(h, hipMemsetAsync(own_cnt, 0, sizeof(int) * (3 * (size_t)N + 2), h->stream));
+    GIS_HIP(h, hipMemsetAsync(err, 0, sizeof(int), h->stream));
+    GIS_HIP(h, hipMemsetAsync(nbr_cnt, 0, sizeof(int) * (size_t)N, h->stream));
+    if (M > 0) {
+        hipLaunchKernelGGL(k_count, grid_for(M), dim3(kBlk), 0, h->stream, N, M, d_la, d_ua,
+                           own_cnt, nbr_cnt, first_own, err);
+        GIS_HIP(h, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_rowlen, grid_for((long long)N + 1), dim3(kBlk), 0, h->stream, N, own_cnt,
+                       nbr_cnt, rowlen);
+    GIS_HIP(h, hipGetLastError());
+    GIS_TRY(scan_exclusive(h, rowlen, h->rowptr.as<int>(), N + 1, tmp, scan_tmp));
+    int herr = 0;
+    GIS_HIP(h, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    if (herr)
+        return fail(h, kErrArg,
+                    "face addressing invalid: need 0 <= lowerAddr, upperAddr < n_cells and "
+                    "lowerAddr sorted ascending");
+    hipLaunchKernelGGL(k_fill_diag, grid_for(N), dim3(kBlk), 0, h->stream, N, M,
+                       h->rowptr.as<int>(), nbr_cnt, h->col.as<int>(), h->perm.as<int>());
+    GIS_HIP(h, hipGetLastError());
+    if (M > 0) {
+        hipLaunchKernelGGL(k_fill_faces, grid_for(M), dim3(kBlk), 0, h->stream, N, M, d_la, d_ua,
+                           h->rowptr.as<int>(), nbr_cnt, first_own, fill, h->col.as<int>(),
+                           h->perm.as<int>());
+        GIS_HIP(h, hipGetLastError());
+        hipLaunchKernelGGL(k_sort_lower, grid_for(N), dim3(kBlk), 0, h->stream, N, d_la,
+                           h->rowptr.as<int>(), nbr_cnt, h->col.as<int>(), h->perm.as<int>());
+        GIS_HIP(h, hipGetLastError());
+    }
+    h->N = N;
+    h->M = M;
+    h->nnz = nnz;
+    h->has_pattern = true;
+    return kOK;
+}
+
+int gather_values(gis_context* h) {
+    if (!h->has_pattern) return fail(h, kErrState, "pattern not set");
+    GIS_HIP(h, h->val.ensure(sizeof(double) * (size_t)h->nnz));
+    hipLaunchKernelGGL(k_gather<double>, grid_for(h->nnz), dim3(kBlk), 0, h->stream, h->nnz, h->N,
+                       h->M, h->perm.as<int>(), h->diag.as<double>(), h->lower.as<double>(),
+                       h->upper.as<double>(), h->val.as<double>());
+    GIS_HIP(h, hipGetLastError());
+    h->has_values = true;
+    h->has_val32 = false;
+    h->has_valT = false;
+    return kOK;
+}
+
+int ensure_val32(gis_context* h) {
+    if (!h->has_values) return fail(h, kErrState, "values not set");
+    if (h->has_val32) return kOK;
+    GIS_HIP(h, h->val32.ensure(sizeof(float) * (size_t)h->nnz));
+    hipLaunchKernelGGL(k_gather<float>, grid_for(h->nnz), dim3(kBlk), 0, h->stream, h->nnz, h->N,
+                       h->M, h->perm.as<int>(), h->diag.as<double>(), h->lower.as<double>(),
+                       h->upper.as<double>(), h->val32.as<float>());
+    GIS_HIP(h, hipGetLastError());
+    h->has_val32 = true;
+    return kOK;
+}
+
+int ensure_valT(gis_context* h) {
+    if (!h->has_values) return fail(h, kErrState, "values not set");
+    if (h->has_valT) return kOK;
+    GIS_HIP(h, h->valT.ensure(sizeof(double) * (size_t)h->nnz));
+    // A^T: same pattern, lower <-> upper
+    hipLaunchKernelGGL(k_gather<double>, grid_for(h->nnz), dim3(kBlk), 0, h->stream, h->nnz, h->N,
+                       h->M, h->perm.as<int>(), h->diag.as<double>(), h->upper.as<double>(),
+                       h->lower.as<double>(), h->valT.as<double>());
+    GIS_HIP(h, hipGetLastError());
+    h->has_valT = true;
+    return kOK;
+}
+
+}  // namespace gis

@@ -1,0 +1,138 @@
+"""Host-side mirror of cuFoam's OpenFOAM plugin surface (lduMatrix::solver).
+
+cuFoam registers eight ``lduMatrix::solver`` subclasses that OpenFOAM selects
+by name from ``fvSolution`` (``solver cuFoamCG; tolerance ...; maxIter ...;``).
+This module keeps the same names, constructor arguments and ``solve``
+contract over the gis C ABI, so a caller (or a test) drives the library
+exactly as OpenFOAM drives the wrappers:
+
+    solver = lduMatrix.solver.New("cuFoamCG", "p", matrix, controls)
+    perf = solver.solve(psi, source)        # psi updated in place
+
+Semantics reproduced from the reference (cuFoamCG/cuFoamCG.C:38-89 and the
+seven siblings):
+  * ``tolerance`` is absolute on the L2 residual, scaled by NormFactor
+    (gis/gis.cu:106-123); ``maxIter`` is the iteration cap (<= 0 unbounded);
+  * ``relTol``, ``minIter`` and ``cmpt`` are accepted and ignored, as there;
+  * the returned residuals are the unnormalised L2 norms history.front() and
+    history.back(), nIterations = history.size() - 1;
+  * cuFoamCG and cuFoamRichardson are registered for symmetric matrices only
+    (cuFoamCG.C:14-15, cuFoamRichardson.C:11-12); the others for both.
+
+The coupling arguments (interfaces, coupleBouCoeffs) are accepted and, as in
+the reference, not used: processor and cyclic patches are not coupled.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .gis import Gis, Performance
+
+
+@dataclass
+class lduMatrix:
+    """Minimal lduMatrix: coefficients plus lduAddressing (owner/neighbour)."""
+    diag: np.ndarray
+    lower: np.ndarray
+    upper: np.ndarray
+    lowerAddr: np.ndarray
+    upperAddr: np.ndarray
+
+    @classmethod
+    def from_system(cls, sys) -> "lduMatrix":
+        return cls(sys.diag, sys.lower, sys.upper, sys.lower_addr, sys.upper_addr)
+
+    def symmetric(self) -> bool:
+        return bool(np.array_equal(self.lower, self.upper))
+
+    def asymmetric(self) -> bool:
+        return not self.symmetric()
+
+    def size(self) -> int:
+        return int(self.diag.shape[0])
+
+
+# name -> (gis method, registered for symmetric, registered for asymmetric)
+_TABLE = {
+    "cuFoamCG": ("CG", True, False),
+    "cuFoamBiCG": ("BiCG", True, True),
+    "cuFoamJacobi": ("Jacobi", True, True),
+    "cuFoamRichardson": ("Richardson", True, False),
+    "cuFoamMR": ("MR", True, True),
+    "cuFoamMRJacobi": ("MRJacobi", True, True),
+    "cuFoamJMR": ("JMR", True, True),
+    "cuFoamMRS": ("MRS", True, True),
+}
+
+
+class FatalError(RuntimeError):
+    pass
+
+
+class cuFoamSolver:
+    """Base of the eight wrappers (one per TypeName)."""
+
+    typeName = ""
+
+    def __init__(self, fieldName: str, matrix: lduMatrix, coupleBouCoeffs=None,
+                 coupleIntCoeffs=None, interfaces=None, solverControls: dict | None = None,
+                 gis: Gis | None = None):
+        controls = dict(solverControls or {})
+        self.fieldName = fieldName
+        self.matrix = matrix
+        self.tolerance = float(controls.get("tolerance", 1e-6))
+        self.relTol = float(controls.get("relTol", 0.0))        # ignored, as in the reference
+        self.maxIter = int(controls.get("maxIter", 1000))
+        self.minIter = int(controls.get("minIter", 0))          # ignored, as in the reference
+        self._gis = gis if gis is not None else Gis()
+        self._pattern_key = None
+
+    def _upload(self):
+        key = (id(self.matrix.lowerAddr), self.matrix.size())
+        if key != self._pattern_key:
+            self._gis.set_pattern(self.matrix.size(), self.matrix.lowerAddr,
+                                  self.matrix.upperAddr)
+            self._pattern_key = key
+        self._gis.set_values(self.matrix.diag, self.matrix.lower, self.matrix.upper)
+
+    def solve(self, psi: np.ndarray, source: np.ndarray, cmpt: int = 0) -> Performance:
+        if psi.dtype != np.float64 or not psi.flags.c_contiguous:
+            raise FatalError("psi must be a contiguous float64 field")
+        self._upload()
+        src = np.ascontiguousarray(source, dtype=np.float64)
+        perf = self._gis.foam_solve(_TABLE[self.typeName][0], psi, src, self.tolerance,
+                                    self.maxIter)
+        perf.solver = self.typeName
+        return perf
+
+
+def _make(name: str):
+    return type(name, (cuFoamSolver,), {"typeName": name})
+
+
+cuFoamCG = _make("cuFoamCG")
+cuFoamBiCG = _make("cuFoamBiCG")
+cuFoamJacobi = _make("cuFoamJacobi")
+cuFoamRichardson = _make("cuFoamRichardson")
+cuFoamMR = _make("cuFoamMR")
+cuFoamMRJacobi = _make("cuFoamMRJacobi")
+cuFoamJMR = _make("cuFoamJMR")
+cuFoamMRS = _make("cuFoamMRS")
+
+_CLASSES = {c.typeName: c for c in (cuFoamCG, cuFoamBiCG, cuFoamJacobi, cuFoamRichardson,
+                                      cuFoamMR, cuFoamMRJacobi, cuFoamJMR, cuFoamMRS)}
+
+
+def New(name: str, fieldName: str, matrix: lduMatrix, solverControls: dict,
+        gis: Gis | None = None) -> cuFoamSolver:
+    """lduMatrix::solver::New: runtime selection by name and matrix symmetry."""
+    if name not in _TABLE:
+        raise FatalError(f"Unknown {'symmetric' if matrix.symmetric() else 'asymmetric'} "
+                         f"matrix solver {name}; valid: {sorted(_TABLE)}")
+    _, sym_ok, asym_ok = _TABLE[name]
+    if matrix.symmetric() and not sym_ok or matrix.asymmetric() and not asym_ok:
+        kind = "symmetric" if matrix.symmetric() else "asymmetric"
+        raise FatalError(f"Unknown {kind} matrix solver {name}")
+    return _CLASSES[name](fieldName, matrix, None, None, None, solverControls, gis=gis)

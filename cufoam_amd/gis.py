@@ -1,0 +1,303 @@
+"""ctypes binding of the gis C ABI (include/gis.h, libgis_amd.so).
+
+Host arrays are numpy arrays; device arrays are torch tensors on a ROCm
+device (``tensor.is_cuda``) and take the ``_device`` entry points, so data
+stays resident in HBM.  Every call raises ``GisError`` on an error status;
+solve outcomes (breakdown, iteration cap, NaN) are returned, not raised.
+There is no CPU fallback: without the library or a GPU these calls fail.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import build as _build
+
+OK, ERR_ARG, ERR_HIP, ERR_SHAPE, ERR_STATE, ERR_NOMEM = 0, 1, 2, 3, 4, 5
+BREAKDOWN, NOT_CONVERGED, NAN = 10, 11, 12
+
+METHODS = {"CG": 0, "BiCG": 1, "Jacobi": 2, "MR": 3, "MRJacobi": 4, "JMR": 5, "MRS": 6,
+           "Richardson": 7}
+
+
+class GisError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"gis status {status}: {msg}")
+        self.status = status
+
+
+class _Params(C.Structure):
+    _fields_ = [("error", C.c_double), ("max_steps", C.c_longlong), ("omega", C.c_double),
+                ("power_steps", C.c_int), ("int_bound", C.c_double)]
+
+
+class _Perf(C.Structure):
+    _fields_ = [("initial_residual", C.c_double), ("final_residual", C.c_double),
+                ("n_iterations", C.c_int), ("status", C.c_int), ("norm_factor", C.c_double)]
+
+
+_lib = None
+
+# (name, restype, argtypes) of every extern "C" symbol in include/gis.h
+_vp, _i, _ll, _d = C.c_void_p, C.c_int, C.c_longlong, C.c_double
+_ip, _llp, _dp = C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_double)
+SIGNATURES = {
+    "gis_default_params": (None, [C.POINTER(_Params)]),
+    "gis_create": (_i, [C.POINTER(_vp), _i]),
+    "gis_destroy": (_i, [_vp]),
+    "gis_set_stream": (_i, [_vp, _vp]),
+    "gis_reset_stream": (_i, [_vp]),
+    "gis_synchronize": (_i, [_vp]),
+    "gis_status_string": (C.c_char_p, [_i]),
+    "gis_last_error": (C.c_char_p, [_vp]),
+    "gis_set_pattern": (_i, [_vp, _i, _i, _vp, _vp]),
+    "gis_set_pattern_device": (_i, [_vp, _i, _i, _vp, _vp]),
+    "gis_set_values": (_i, [_vp, _vp, _vp, _vp]),
+    "gis_set_values_device": (_i, [_vp, _vp, _vp, _vp]),
+    "gis_get_size": (_i, [_vp, _ip, _llp]),
+    "gis_get_csr": (_i, [_vp, _vp, _vp, _vp]),
+    "gis_get_csr_transpose": (_i, [_vp, _vp, _vp, _vp]),
+    "gis_spmv_device": (_i, [_vp, _vp, _vp]),
+    "gis_spmv_transpose_device": (_i, [_vp, _vp, _vp]),
+    "gis_dot_device": (_i, [_vp, _vp, _vp, _dp]),
+    "gis_spmv_device_async": (_i, [_vp, _vp, _vp]),
+    "gis_spmv32_device_async": (_i, [_vp, _vp, _vp]),
+    "gis_set_schedule": (_i, [_vp, _i, _i, _i]),
+    "gis_get_schedule": (_i, [_vp, _ip, _ip, _ip]),
+    "gis_norm_factor": (_i, [_vp, _vp, _vp, _dp]),
+    "gis_norm_factor_device": (_i, [_vp, _vp, _vp, _dp]),
+    "gis_solve": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
+    "gis_solve_device": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
+    "gis_foam_solve": (_i, [_vp, _i, _vp, _vp, _d, _ll, C.POINTER(_Perf)]),
+}
+
+
+def library_path() -> str:
+    return _build.LIB
+
+
+def load(build_if_missing: bool = True):
+    """Load libgis_amd.so (building it first only if it does not exist)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        if not build_if_missing:
+            raise GisError(ERR_STATE, f"{path} is not built")
+        _build.build()
+    L = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        if hasattr(L, name):
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+    _lib = L
+    return L
+
+
+def _is_dev(a) -> bool:
+    try:
+        import torch
+    except ImportError:
+        return False
+    return isinstance(a, torch.Tensor) and a.is_cuda
+
+
+def _host(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _ptr(a):
+    if _is_dev(a):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+@dataclass
+class Performance:
+    """solverPerformance as cuFoam's wrappers fill it (cuFoamCG.C:80-83)."""
+    solver: str
+    initial_residual: float
+    final_residual: float
+    n_iterations: int
+    status: int
+    norm_factor: float
+
+
+class Gis:
+    """One handle: a device, a stream, a cached LDU pattern and its values."""
+
+    def __init__(self, device: int = 0):
+        self._L = load()
+        self._h = C.c_void_p()
+        st = self._L.gis_create(C.byref(self._h), device)
+        if st != OK:
+            raise GisError(st, "gis_create failed (is a GPU visible?)")
+        self.device = device
+        self.n = 0
+        self.nnz = 0
+        self.use_torch_stream()
+
+    def use_torch_stream(self):
+        """Run on torch's current stream of this device, so device tensors that torch
+        produced are complete before gis reads them (the handle's own stream is
+        non-blocking and would not wait for them)."""
+        try:
+            import torch
+        except ImportError:
+            return
+        if torch.cuda.is_available():
+            self.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+
+    # -- lifetime ------------------------------------------------------------
+    def close(self):
+        if self._h:
+            self._L.gis_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, st: int, what: str) -> int:
+        if st != OK and st < BREAKDOWN:
+            msg = self._L.gis_last_error(self._h)
+            raise GisError(st, f"{what}: {msg.decode() if msg else ''}")
+        return st
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream_ptr: int | None):
+        """Run on the given hipStream_t (an int; 0 is the default stream); None goes
+        back to the handle's own stream."""
+        if stream_ptr is None:
+            self._check(self._L.gis_reset_stream(self._h), "reset_stream")
+        else:
+            self._check(self._L.gis_set_stream(self._h, C.c_void_p(stream_ptr)), "set_stream")
+
+    # -- matrix --------------------------------------------------------------
+    def set_pattern(self, n_cells: int, lower_addr, upper_addr):
+        if _is_dev(lower_addr):
+            st = self._L.gis_set_pattern_device(self._h, n_cells, int(lower_addr.numel()),
+                                                _ptr(lower_addr), _ptr(upper_addr))
+        else:
+            la = _host(lower_addr, np.int32)
+            ua = _host(upper_addr, np.int32)
+            st = self._L.gis_set_pattern(self._h, n_cells, la.shape[0], _ptr(la), _ptr(ua))
+        self._check(st, "set_pattern")
+        n = C.c_int(0)
+        nnz = C.c_longlong(0)
+        self._check(self._L.gis_get_size(self._h, C.byref(n), C.byref(nnz)), "get_size")
+        self.n, self.nnz = n.value, nnz.value
+
+    def set_values(self, diag, lower, upper):
+        if _is_dev(diag):
+            st = self._L.gis_set_values_device(self._h, _ptr(diag), _ptr(lower), _ptr(upper))
+        else:
+            d, lo, up = _host(diag, np.float64), _host(lower, np.float64), _host(upper, np.float64)
+            st = self._L.gis_set_values(self._h, _ptr(d), _ptr(lo), _ptr(up))
+        self._check(st, "set_values")
+
+    def set_system(self, sys):
+        self.set_pattern(sys.n_cells, sys.lower_addr, sys.upper_addr)
+        self.set_values(sys.diag, sys.lower, sys.upper)
+
+    def get_csr(self, transpose: bool = False):
+        rp = np.zeros(self.n + 1, np.int32)
+        col = np.zeros(max(self.nnz, 1), np.int32)
+        val = np.zeros(max(self.nnz, 1), np.float64)
+        f = self._L.gis_get_csr_transpose if transpose else self._L.gis_get_csr
+        self._check(f(self._h, _ptr(rp), _ptr(col), _ptr(val)), "get_csr")
+        return rp, col[:self.nnz], val[:self.nnz]
+
+    # -- kernels on device tensors ---------------------------------------------
+    def spmv(self, x, y=None, transpose: bool = False):
+        import torch
+        if y is None:
+            y = torch.empty_like(x)
+        f = self._L.gis_spmv_transpose_device if transpose else self._L.gis_spmv_device
+        self._check(f(self._h, _ptr(x), _ptr(y)), "spmv")
+        return y
+
+    def spmv_async(self, x, y):
+        f = self._L.gis_spmv32_device_async if x.dtype.itemsize == 4 else self._L.gis_spmv_device_async
+        self._check(f(self._h, _ptr(x), _ptr(y)), "spmv_async")
+
+    def dot(self, a, b) -> float:
+        out = C.c_double(0.0)
+        self._check(self._L.gis_dot_device(self._h, _ptr(a), _ptr(b), C.byref(out)), "dot")
+        return out.value
+
+    def norm_factor(self, b, x) -> float:
+        out = C.c_double(0.0)
+        if _is_dev(b):
+            st = self._L.gis_norm_factor_device(self._h, _ptr(b), _ptr(x), C.byref(out))
+        else:
+            b, x = _host(b, np.float64), _host(x, np.float64)
+            st = self._L.gis_norm_factor(self._h, _ptr(b), _ptr(x), C.byref(out))
+        self._check(st, "norm_factor")
+        return out.value
+
+    def set_schedule(self, spmv64: int = -1, spmv32: int = -1, block_iters: int = -1):
+        self._check(self._L.gis_set_schedule(self._h, spmv64, spmv32, block_iters),
+                    "set_schedule")
+
+    def get_schedule(self) -> tuple[int, int, int]:
+        a, b, c = C.c_int(0), C.c_int(0), C.c_int(0)
+        self._check(self._L.gis_get_schedule(self._h, C.byref(a), C.byref(b), C.byref(c)),
+                    "get_schedule")
+        return a.value, b.value, c.value
+
+    def synchronize(self):
+        self._check(self._L.gis_synchronize(self._h), "synchronize")
+
+    # -- solvers ---------------------------------------------------------------
+    def solve(self, method: str, b, x0=None, error: float = 1e-4, max_steps: int = 0,
+              omega: float = 0.0, power_steps: int = 30, int_bound: float = 1e-10,
+              hist_cap: int | None = None):
+        """SingleGPU::<method>(x, A, b, error, max_steps, ...) -> (x, hist, status).
+
+        With numpy inputs x is a new numpy array; with device tensors x0 is
+        updated in place and returned.
+        """
+        p = _Params(error, max_steps, omega, power_steps, int_bound)
+        if hist_cap is None:
+            hist_cap = (max_steps if max_steps > 0 else 10000) + 1
+        hist = np.zeros(max(hist_cap, 1))
+        hl = C.c_int(0)
+        m = METHODS[method]
+        if _is_dev(b):
+            import torch
+            x = x0 if x0 is not None else torch.zeros_like(b)
+            st = self._L.gis_solve_device(self._h, m, _ptr(x), _ptr(b), C.byref(p), _ptr(hist),
+                                          hist_cap, C.byref(hl))
+        else:
+            b = _host(b, np.float64)
+            x = np.array(x0 if x0 is not None else np.zeros_like(b), dtype=np.float64, copy=True)
+            st = self._L.gis_solve(self._h, m, _ptr(x), _ptr(b), C.byref(p), _ptr(hist),
+                                   hist_cap, C.byref(hl))
+        self._check(st, f"solve({method})")
+        return x, hist[:min(hl.value, hist_cap)].copy(), st
+
+    def foam_solve(self, method: str, psi, source, tolerance: float, max_iter: int):
+        """cuFoam<method>::solve: psi (numpy) is updated in place; returns Performance."""
+        perf = _Perf()
+        st = self._L.gis_foam_solve(self._h, METHODS[method], _ptr(psi), _ptr(source),
+                                    tolerance, max_iter, C.byref(perf))
+        self._check(st, f"foam_solve({method})")
+        return Performance("cuFoam" + method, perf.initial_residual, perf.final_residual,
+                           perf.n_iterations, perf.status, perf.norm_factor)

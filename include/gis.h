@@ -1,0 +1,148 @@
+/*
+ * gis.h -- C ABI of the MI355X (gfx950) gis sparse iterative solver library.
+ *
+ * Replaces cuFoam's C++ template ABI gis/gis.h:1-95 (free functions taking
+ * SingleGPU::SparseCSR / SingleGPU::Vector by reference) with a handle-based
+ * extern "C" interface over plain pointers and sizes.  The library owns all
+ * device memory behind the handle; the OpenFOAM lduMatrix::solver wrappers
+ * (cuFoamCG.C:38-89 and siblings) hand over host arrays.  Every `_device`
+ * variant takes device pointers instead (e.g. from torch tensors) and keeps
+ * the data resident.
+ *
+ * Conventions
+ *   - n_cells = N (matrix order), n_faces = M (one upper and one lower
+ *     coefficient per face), int32 indices, FP64 values (gis/gis.h:7-8).
+ *   - lower_addr[f] = owner, upper_addr[f] = neighbour, faces sorted by owner
+ *     (OpenFOAM upper-triangular order).  A(owner, nbr) = upper[f],
+ *     A(nbr, owner) = lower[f].
+ *   - All calls are synchronous with respect to the host unless named
+ *     `_async`; work runs on the handle's HIP stream.
+ *   - Return value: GIS_OK, an error code (< 10), or a solve outcome (>= 10)
+ *     which is not an error: the outputs are valid.
+ *   - Handles are independent; one handle must not be used from two threads
+ *     at once.
+ */
+#ifndef GIS_AMD_GIS_H
+#define GIS_AMD_GIS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GIS_API __attribute__((visibility("default")))
+
+/* ---- status codes (replace exit(1) in check.h:21-29 and BiCG's bool) ---- */
+#define GIS_OK 0
+#define GIS_ERR_ARG 1          /* invalid argument or addressing                  */
+#define GIS_ERR_HIP 2          /* HIP runtime error (message: gis_last_error)     */
+#define GIS_ERR_SHAPE 3        /* size mismatch / limits (SolverSingleGPU.h:177)   */
+#define GIS_ERR_STATE 4        /* pattern or values not set                      */
+#define GIS_ERR_NOMEM 5
+#define GIS_BREAKDOWN 10       /* BiCG |xi| < int_bound (SolverSingleGPU.h:629)   */
+#define GIS_NOT_CONVERGED 11   /* iteration cap reached above the threshold       */
+#define GIS_NAN 12             /* residual norm became NaN                         */
+
+/* ---- solver methods (gis/gis.h:13-87; wrappers cuFoam<Name>) ---- */
+#define GIS_CG 0
+#define GIS_BICG 1
+#define GIS_JACOBI 2
+#define GIS_MR 3
+#define GIS_MRJACOBI 4
+#define GIS_JMR 5              /* PreconditionedMRJacobi (gis/gis.cu:84-92)        */
+#define GIS_MRS 6
+#define GIS_RICHARDSON 7
+
+typedef struct gis_context *gis_handle;
+
+/* Solve controls.  Defaults (gis_default_params) reproduce gis/gis.h:13-87. */
+typedef struct {
+    double error;        /* absolute ||r||_2 threshold (CG :412 uses norm > error) */
+    long long max_steps; /* <= 0: unbounded (SolverSingleGPU.h:394-395)            */
+    double omega;        /* Richardson factor; |omega| < 1e-15 -> OptimizePower     */
+    int power_steps;     /* Richardson power iterations (30)                        */
+    double int_bound;    /* BiCG breakdown bound (1e-10)                            */
+} gis_params;
+
+/* solverPerformance fields the wrappers fill (cuFoamCG.C:80-83) */
+typedef struct {
+    double initial_residual; /* history.front(): unnormalised ||r_0||_2              */
+    double final_residual;   /* history.back()                                      */
+    int n_iterations;        /* history.size() - 1                                   */
+    int status;              /* GIS_OK / GIS_BREAKDOWN / GIS_NOT_CONVERGED / GIS_NAN  */
+    double norm_factor;      /* NormFactor(A, b, x) (gis/gis.cu:106-123)              */
+} gis_performance;
+
+GIS_API void gis_default_params(gis_params *p);
+
+/* ---- lifetime ---- */
+GIS_API int gis_create(gis_handle *h, int device);
+GIS_API int gis_destroy(gis_handle h);
+/* Run on a caller stream (hipStream_t; NULL is the default stream) instead of
+ * the handle's own non-blocking stream; gis_reset_stream goes back to it. */
+GIS_API int gis_set_stream(gis_handle h, void *stream);
+GIS_API int gis_reset_stream(gis_handle h);
+GIS_API int gis_synchronize(gis_handle h);
+GIS_API const char *gis_status_string(int status);
+GIS_API const char *gis_last_error(gis_handle h);
+
+/* ---- matrix: LDU pattern (cached per mesh) and coefficients ----
+ * gis_set_pattern replaces the addressing half of getSparseCSRMat
+ * (gis/ldu2csr.h:7-42): the CSR pattern and the value permutation are built
+ * on the device once and reused by every gis_set_values. */
+GIS_API int gis_set_pattern(gis_handle h, int n_cells, int n_faces, const int *lower_addr,
+                            const int *upper_addr);
+GIS_API int gis_set_pattern_device(gis_handle h, int n_cells, int n_faces,
+                                   const int *d_lower_addr, const int *d_upper_addr);
+/* The value half of getSparseCSRMat (gis/ldu2csr.h:23-36): one device gather. */
+GIS_API int gis_set_values(gis_handle h, const double *diag, const double *lower,
+                           const double *upper);
+GIS_API int gis_set_values_device(gis_handle h, const double *d_diag, const double *d_lower,
+                                  const double *d_upper);
+GIS_API int gis_get_size(gis_handle h, int *n_rows, long long *nnz);
+/* Copy the device CSR (row_offsets[N+1], column_indices[nnz], values[nnz]) to host. */
+GIS_API int gis_get_csr(gis_handle h, int *row_offsets, int *column_indices, double *values);
+/* CSR of A^T (Transpose, gis/gis.cu:4-7): same pattern for LDU, lower <-> upper. */
+GIS_API int gis_get_csr_transpose(gis_handle h, int *row_offsets, int *column_indices,
+                                  double *values);
+
+/* ---- BLAS-1 / SpMV on device vectors of length N (SolverSingleGPU.h:103-211) ---- */
+GIS_API int gis_spmv_device(gis_handle h, const double *d_x, double *d_y);
+GIS_API int gis_spmv_transpose_device(gis_handle h, const double *d_x, double *d_y);
+GIS_API int gis_dot_device(gis_handle h, const double *d_a, const double *d_b, double *out);
+/* Enqueue one SpMV on the handle stream without synchronising (timing loops). */
+GIS_API int gis_spmv_device_async(gis_handle h, const double *d_x, double *d_y);
+GIS_API int gis_spmv32_device_async(gis_handle h, const float *d_x, float *d_y);
+
+/* ---- launch schedule (tuning only: every schedule gives bit-identical results).
+ * spmv64 / spmv32: 0 = LDS-staged, 2048 nonzeros per chunk; 1 = LDS-staged,
+ * 4096; 2 = unstaged.  block_iters: iterations enqueued between host checks
+ * of the convergence flag.  Negative values leave a field unchanged. ---- */
+GIS_API int gis_set_schedule(gis_handle h, int spmv64, int spmv32, int block_iters);
+GIS_API int gis_get_schedule(gis_handle h, int *spmv64, int *spmv32, int *block_iters);
+
+/* ---- NormFactor (gis/gis.cu:106-123) ---- */
+GIS_API int gis_norm_factor(gis_handle h, const double *b, const double *x, double *nf);
+GIS_API int gis_norm_factor_device(gis_handle h, const double *d_b, const double *d_x,
+                                   double *nf);
+
+/* ---- solve A x = b with one method; x holds x0 on entry (host pointers).
+ * hist (optional, hist_cap entries) receives the residual norms
+ * ||r_0||, ||r_1||, ...; *hist_len gets the full count (iterations + 1). ---- */
+GIS_API int gis_solve(gis_handle h, int method, double *x, const double *b,
+                      const gis_params *params, double *hist, int hist_cap, int *hist_len);
+GIS_API int gis_solve_device(gis_handle h, int method, double *d_x, const double *d_b,
+                             const gis_params *params, double *hist, int hist_cap,
+                             int *hist_len);
+
+/* ---- the cuFoam<Name>::solve sequence (cuFoamCG.C:45-83 and siblings):
+ * nf = NormFactor(A, source, psi); solve with error = tolerance * nf and
+ * max_steps = max_iter (BiCG int_bound 1e-10, Richardson omega 0 / 30 power
+ * steps, MRS invM = D^-1); psi is updated in place. ---- */
+GIS_API int gis_foam_solve(gis_handle h, int method, double *psi, const double *source,
+                           double tolerance, long long max_iter, gis_performance *perf);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GIS_AMD_GIS_H */

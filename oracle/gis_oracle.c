@@ -1,0 +1,679 @@
+/*
+ * gis_oracle.c -- CPU restatement of cuFoam's gis sparse iterative solver path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The
+ * product library (cufoam_amd/csrc) never links or calls it.
+ *
+ * What it restates (reference paths relative to the cuFoam tree):
+ *   or_ldu2csr_pattern / or_ldu2csr_values   gis/ldu2csr.h:19-39
+ *   or_transpose                            gis/SparseSystem/SparseCPU.h:369-399
+ *   or_spmv / or_residual                   gis/SparseSystem/kernels.h:206-237, :409-441
+ *   or_dot / or_sum / or_dist               gis/SparseSystem/SolverSingleGPU.h:74-128, :217-242
+ *   or_norm_factor                          gis/gis.cu:106-123
+ *   or_cg                                   gis/SparseSystem/SolverSingleGPU.h:366-445
+ *   or_bicg                                 gis/SparseSystem/SolverSingleGPU.h:568-675
+ *   or_richardson (+ optimize_power,
+ *     find_diag_abs_min)                    gis/SparseSystem/SolverSingleGPU.h:686-819
+ *   or_mr                                   gis/SparseSystem/SolverSingleGPU.h:825-890
+ *   or_jacobi                               gis/SparseSystem/SolverSingleGPU.h:924-994
+ *   or_mrjacobi                             gis/SparseSystem/SolverSingleGPU.h:1000-1073
+ *   or_jmr (PreconditionedMRJacobi)         gis/SparseSystem/SolverSingleGPU.h:1079-1155
+ *   or_mrs (+ JacobiPrecondition)           gis/SparseSystem/SolverSingleGPU.h:1236-1303,
+ *                                           gis/SparseSystem/SparseCPU.h:428-446
+ *   or_foam_solve                           cuFoamCG/cuFoamCG.C:45-83 and the seven
+ *                                           sibling wrappers (constants: cuFoamBiCG.C:68,
+ *                                           cuFoamRichardson.C:66, cuFoamMRS.C:50)
+ *
+ * Arithmetic contract shared with the HIP kernels (so GPU results can be
+ * compared bit for bit):
+ *   - built with -ffp-contract=off; fused multiply-adds appear only as fma().
+ *   - vector updates v = a + s*w are fma(s, w, a), as nvcc contracts the
+ *     reference's AddKernel1 (kernels.h:332-341).
+ *   - CSR row sums are sequential in storage order, each product rounded:
+ *     y = 0; y = y + a_j * x_cj  (no fma).
+ *   - reductions (dot, squared norm, distance, sum) use ONE fixed order that
+ *     depends on the vector length only: leaves of 4 consecutive elements
+ *     accumulated sequentially from +0.0 (dot terms by fma), then a pairwise
+ *     binary tree over the leaves (zero padded to a power of two).  The
+ *     reference's own order depends on the launch grid (launch.h:126-138) and
+ *     on a host-side serial sum of per-block partials, so no bit pattern of
+ *     it is reproducible; this order replaces it.
+ *   - division and sqrt are IEEE correctly rounded.
+ *
+ * Definitions used where OpenFOAM code is absent from the container
+ * (parity of these against OpenFOAM itself is unpinned; they are pinned by
+ * the definitions, the closed form rowptr[c] = c + losortStart[c] +
+ * ownerStart[c], and an independent counting-sort restatement in tests):
+ *   ownerStart[c]  = first face whose owner >= c (faces sorted by owner),
+ *                    ownerStart[N] = M;
+ *   losort         = face indices grouped by neighbour, ascending neighbour,
+ *                    ascending face index within a group;
+ *   losortStart[c] = first losort position whose neighbour >= c,
+ *                    losortStart[N] = M.
+ */
+#include <float.h>
+#include <limits.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OR_API __attribute__((visibility("default")))
+#define OR_HOT __attribute__((target_clones("fma", "default")))
+
+enum { OR_CG = 0, OR_BICG = 1, OR_JACOBI = 2, OR_MR = 3, OR_MRJACOBI = 4,
+       OR_JMR = 5, OR_MRS = 6, OR_RICHARDSON = 7 };
+
+enum { OR_OK = 0, OR_ERR_ARG = 1, OR_ERR_NOMEM = 5, OR_BREAKDOWN = 10 };
+
+/* ------------------------------------------------------------------------ */
+/* canonical reduction: 4-element leaves + pairwise tree                     */
+/* ------------------------------------------------------------------------ */
+#define OR_LEAF 4
+
+typedef struct {
+    double stack[64];
+    int64_t count;
+} tree_acc;
+
+static inline void tree_push(tree_acc *t, double v) {
+    int64_t c = t->count;
+    int lvl = 0;
+    while (c & 1) {
+        v = t->stack[lvl] + v;
+        c >>= 1;
+        ++lvl;
+    }
+    t->stack[lvl] = v;
+    t->count++;
+}
+
+static inline double tree_final(const tree_acc *t) {
+    double acc = 0.0;
+    int have = 0;
+    for (int lvl = 0; lvl < 63; ++lvl) {
+        if ((t->count >> lvl) & 1) {
+            acc = have ? t->stack[lvl] + acc : t->stack[lvl];
+            have = 1;
+        }
+    }
+    return have ? acc : 0.0;
+}
+
+OR_API double or_tree_sum(int64_t n, const double *leaf) {
+    tree_acc t = {.count = 0};
+    for (int64_t i = 0; i < n; ++i) tree_push(&t, leaf[i]);
+    return tree_final(&t);
+}
+
+OR_HOT OR_API double or_dot(int64_t n, const double *a, const double *b) {
+    tree_acc t = {.count = 0};
+    for (int64_t i = 0; i < n; i += OR_LEAF) {
+        int64_t e = i + OR_LEAF < n ? i + OR_LEAF : n;
+        double s = 0.0;
+        for (int64_t k = i; k < e; ++k) s = fma(a[k], b[k], s);
+        tree_push(&t, s);
+    }
+    return tree_final(&t);
+}
+
+OR_HOT OR_API double or_dot_f32(int64_t n, const float *a, const float *b) {
+    tree_acc t = {.count = 0};
+    for (int64_t i = 0; i < n; i += OR_LEAF) {
+        int64_t e = i + OR_LEAF < n ? i + OR_LEAF : n;
+        double s = 0.0;
+        for (int64_t k = i; k < e; ++k) s = fma((double)a[k], (double)b[k], s);
+        tree_push(&t, s);
+    }
+    return tree_final(&t);
+}
+
+OR_API double or_sum(int64_t n, const double *a) {
+    tree_acc t = {.count = 0};
+    for (int64_t i = 0; i < n; i += OR_LEAF) {
+        int64_t e = i + OR_LEAF < n ? i + OR_LEAF : n;
+        double s = 0.0;
+        for (int64_t k = i; k < e; ++k) s = s + a[k];
+        tree_push(&t, s);
+    }
+    return tree_final(&t);
+}
+
+/* sqrt(sum (a-b)^2)  -- DistanceKernel (kernels.h:266-312) + SolverSingleGPU.h:241 */
+OR_HOT OR_API double or_dist(int64_t n, const double *a, const double *b) {
+    tree_acc t = {.count = 0};
+    for (int64_t i = 0; i < n; i += OR_LEAF) {
+        int64_t e = i + OR_LEAF < n ? i + OR_LEAF : n;
+        double s = 0.0;
+        for (int64_t k = i; k < e; ++k) {
+            double d = a[k] - b[k];
+            s = fma(d, d, s);
+        }
+        tree_push(&t, s);
+    }
+    return sqrt(tree_final(&t));
+}
+
+static inline double norm2(int64_t n, const double *a) { return sqrt(or_dot(n, a, a)); }
+
+/* ------------------------------------------------------------------------ */
+/* LDU -> CSR  (gis/ldu2csr.h:19-39)                                         */
+/* ------------------------------------------------------------------------ */
+/* perm codes: [0, M) lower face f; [M, M+N) diagonal of cell c; [M+N, 2M+N) upper face f */
+OR_API int or_ldu2csr_pattern(int N, int M, const int *lower_addr, const int *upper_addr,
+                              int *rowptr, int *col, int *perm) {
+    if (N < 0 || M < 0) return OR_ERR_ARG;
+    for (int f = 0; f < M; ++f) {
+        if (lower_addr[f] < 0 || lower_addr[f] >= N || upper_addr[f] < 0 || upper_addr[f] >= N)
+            return OR_ERR_ARG;
+        if (f > 0 && lower_addr[f] < lower_addr[f - 1]) return OR_ERR_ARG;
+    }
+    int *owner_start = (int *)malloc(sizeof(int) * ((size_t)N + 1));
+    int *losort_start = (int *)calloc((size_t)N + 2, sizeof(int));
+    int *losort = (int *)malloc(sizeof(int) * ((size_t)M + 1));
+    if (!owner_start || !losort_start || !losort) {
+        free(owner_start); free(losort_start); free(losort);
+        return OR_ERR_NOMEM;
+    }
+    /* ownerStart: first face with owner >= c */
+    {
+        int f = 0;
+        for (int c = 0; c <= N; ++c) {
+            while (f < M && lower_addr[f] < c) ++f;
+            owner_start[c] = f;
+        }
+    }
+    /* losort: stable grouping of faces by neighbour */
+    for (int f = 0; f < M; ++f) losort_start[upper_addr[f] + 1]++;
+    for (int c = 0; c < N; ++c) losort_start[c + 1] += losort_start[c];
+    {
+        int *fill = (int *)malloc(sizeof(int) * ((size_t)N + 1));
+        if (!fill) { free(owner_start); free(losort_start); free(losort); return OR_ERR_NOMEM; }
+        memcpy(fill, losort_start, sizeof(int) * ((size_t)N + 1));
+        for (int f = 0; f < M; ++f) losort[fill[upper_addr[f]]++] = f;
+        free(fill);
+    }
+    int n = 0;
+    for (int cell = 0; cell < N; ++cell) {
+        rowptr[cell] = n;
+        for (int idx = losort_start[cell]; idx < losort_start[cell + 1]; ++idx) {
+            int face = losort[idx];
+            col[n] = lower_addr[face];
+            perm[n] = face;
+            ++n;
+        }
+        col[n] = cell;
+        perm[n] = M + cell;
+        ++n;
+        for (int face = owner_start[cell]; face < owner_start[cell + 1]; ++face) {
+            col[n] = upper_addr[face];
+            perm[n] = M + N + face;
+            ++n;
+        }
+    }
+    rowptr[N] = n;
+    free(owner_start); free(losort_start); free(losort);
+    return OR_OK;
+}
+
+OR_API void or_ldu2csr_values(int N, int M, int64_t nnz, const int *perm, const double *diag,
+                              const double *lower, const double *upper, double *val) {
+    for (int64_t p = 0; p < nnz; ++p) {
+        int code = perm[p];
+        if (code < M) val[p] = lower[code];
+        else if (code < M + N) val[p] = diag[code - M];
+        else val[p] = upper[code - M - N];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* general CSR transpose  (SparseCPU.h:369-399, counting sort)               */
+/* ------------------------------------------------------------------------ */
+OR_API void or_transpose(int n_rows, int n_cols, const int *rowptr, const int *col,
+                         const double *val, int *t_rowptr, int *t_col, double *t_val) {
+    int64_t nnz = rowptr[n_rows];
+    for (int i = 0; i <= n_cols; ++i) t_rowptr[i] = 0;
+    for (int64_t j = 0; j < nnz; ++j) t_rowptr[col[j]]++;
+    int sum = 0;
+    for (int i = 0; i < n_cols; ++i) {
+        int tmp = t_rowptr[i];
+        t_rowptr[i] = sum;
+        sum += tmp;
+    }
+    t_rowptr[n_cols] = (int)nnz;
+    int *curr = (int *)malloc(sizeof(int) * ((size_t)n_cols + 1));
+    memcpy(curr, t_rowptr, sizeof(int) * ((size_t)n_cols + 1));
+    for (int i = 0; i < n_rows; ++i) {
+        for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) {
+            int c = col[j];
+            t_val[curr[c]] = val[j];
+            t_col[curr[c]] = i;
+            curr[c]++;
+        }
+    }
+    free(curr);
+}
+
+/* ------------------------------------------------------------------------ */
+/* SpMV and residual                                                          */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int n;
+    const int *rowptr;
+    const int *col;
+    const double *val;
+} csr_t;
+
+OR_API void or_spmv(int N, const int *rowptr, const int *col, const double *val,
+                    const double *x, double *y) {
+    for (int i = 0; i < N; ++i) {
+        double s = 0.0;
+        for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) s = s + val[j] * x[col[j]];
+        y[i] = s;
+    }
+}
+
+OR_API void or_spmv_f32(int N, const int *rowptr, const int *col, const float *val,
+                        const float *x, float *y) {
+    for (int i = 0; i < N; ++i) {
+        float s = 0.0f;
+        for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) s = s + val[j] * x[col[j]];
+        y[i] = s;
+    }
+}
+
+OR_API void or_residual(int N, const int *rowptr, const int *col, const double *val,
+                        const double *b, const double *x, double *r) {
+    for (int i = 0; i < N; ++i) {
+        double s = 0.0;
+        for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) s = s + val[j] * x[col[j]];
+        r[i] = b[i] - s;
+    }
+}
+
+static void spmv(const csr_t *A, const double *x, double *y) {
+    or_spmv(A->n, A->rowptr, A->col, A->val, x, y);
+}
+
+static void residual(const csr_t *A, const double *b, const double *x, double *r) {
+    or_residual(A->n, A->rowptr, A->col, A->val, b, x, r);
+}
+
+/* v = a + s * w   (AddKernel1 under contraction) */
+OR_HOT static void axpy(int64_t n, double s, const double *w, const double *a, double *v) {
+    for (int64_t i = 0; i < n; ++i) v[i] = fma(s, w[i], a[i]);
+}
+
+/* diagonal of A: FindDiag scans the row for the column equal to the row (kernels.h:447-464) */
+static void find_diag(const csr_t *A, double *d) {
+    for (int i = 0; i < A->n; ++i) {
+        for (int j = A->rowptr[i]; j < A->rowptr[i + 1] && A->col[j] <= i; ++j)
+            if (A->col[j] == i) d[i] = A->val[j];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* NormFactor  (gis/gis.cu:106-123)                                          */
+/* ------------------------------------------------------------------------ */
+OR_API double or_norm_factor(int N, const int *rowptr, const int *col, const double *val,
+                             const double *b, const double *x) {
+    csr_t A = {N, rowptr, col, val};
+    double x_bar = or_sum(N, x) / N;
+    double *X = (double *)malloc(sizeof(double) * (size_t)N * 3);
+    double *Y = X + N, *Yb = X + 2 * (size_t)N;
+    for (int i = 0; i < N; ++i) X[i] = x_bar;
+    spmv(&A, x, Y);
+    spmv(&A, X, Yb);
+    double nf = or_dist(N, Y, Yb) + or_dist(N, b, Yb) + 1e-20;
+    free(X);
+    return nf;
+}
+
+/* ------------------------------------------------------------------------ */
+/* history recording (DetailType::Error, support.h:7-9)                      */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double *hist;
+    int cap;
+    int len;
+} hist_t;
+
+static inline void hpush(hist_t *h, double v) {
+    if (h->hist && h->len < h->cap) h->hist[h->len] = v;
+    h->len++;
+}
+
+static inline int64_t unbounded(int max_steps) { return max_steps <= 0 ? INT_MAX : max_steps; }
+
+/* ------------------------------------------------------------------------ */
+/* CG  (SolverSingleGPU.h:366-445)                                           */
+/* ------------------------------------------------------------------------ */
+static int cg(const csr_t *A, const double *b, double *x, double error, int max_steps, hist_t *h) {
+    const int N = A->n;
+    double *r = (double *)malloc(sizeof(double) * (size_t)N * 3);
+    if (!r) return OR_ERR_NOMEM;
+    double *p = r + N, *q = r + 2 * (size_t)N;
+    residual(A, b, x, r);
+    memcpy(p, r, sizeof(double) * N);
+    double rho0 = or_dot(N, r, r);
+    double rho1 = 0;
+    double norm_r = sqrt(rho0);
+    int64_t ms = unbounded(max_steps);
+    hpush(h, norm_r);
+    for (int64_t k = 1; k <= ms && norm_r > error; ++k) {
+        if (k > 1) axpy(N, rho0 / rho1, p, r, p);
+        spmv(A, p, q);
+        double xi = rho0 / or_dot(N, p, q);
+        axpy(N, xi, p, x, x);
+        axpy(N, -xi, q, r, r);
+        rho1 = rho0;
+        rho0 = or_dot(N, r, r);
+        norm_r = sqrt(rho0);
+        hpush(h, norm_r);
+    }
+    free(r);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* BiCG with an explicit transpose  (SolverSingleGPU.h:568-675)              */
+/* ------------------------------------------------------------------------ */
+static int bicg(const csr_t *A, const csr_t *At, const double *b, double *x, double error,
+                int max_steps, double int_bound, hist_t *h) {
+    const int N = A->n;
+    double *r = (double *)malloc(sizeof(double) * (size_t)N * 6);
+    if (!r) return OR_ERR_NOMEM;
+    double *rs = r + N, *p = r + 2 * (size_t)N, *ps = r + 3 * (size_t)N;
+    double *q = r + 4 * (size_t)N, *qs = r + 5 * (size_t)N;
+    residual(A, b, x, r);
+    memcpy(rs, r, sizeof(double) * N);
+    memcpy(p, r, sizeof(double) * N);
+    memcpy(ps, rs, sizeof(double) * N);
+    double rho0 = or_dot(N, r, rs);
+    double rho1 = 0;
+    double norm_r = norm2(N, r);
+    int64_t ms = unbounded(max_steps);
+    hpush(h, norm_r);
+    int status = OR_OK;
+    for (int64_t k = 1; k <= ms; ++k) {
+        spmv(A, p, q);
+        double xi = rho0 / or_dot(N, ps, q);
+        if (xi > -int_bound && xi < int_bound) {
+            status = OR_BREAKDOWN;
+            break;
+        }
+        axpy(N, xi, p, x, x);
+        axpy(N, -xi, q, r, r);
+        norm_r = norm2(N, r);
+        hpush(h, norm_r);
+        if (norm_r < error) break;
+        spmv(At, ps, qs);
+        axpy(N, -xi, qs, rs, rs);
+        rho1 = rho0;
+        rho0 = or_dot(N, r, rs);
+        double theta = rho0 / rho1;
+        axpy(N, theta, p, r, p);
+        axpy(N, theta, ps, rs, ps);
+    }
+    free(r);
+    return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Richardson + OptimizePower + FindDiagAbsMin  (SolverSingleGPU.h:686-819) */
+/* ------------------------------------------------------------------------ */
+OR_API double or_find_diag_abs_min(int N, const int *rowptr, const int *col, const double *val) {
+    csr_t A = {N, rowptr, col, val};
+    double *d = (double *)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
+    for (int i = 0; i < N; ++i) d[i] = 0.0;
+    find_diag(&A, d);
+    double m = INFINITY;
+    for (int i = 0; i < N; ++i) m = fmin(m, fabs(d[i]));
+    free(d);
+    double result = DBL_MAX;
+    if (result > m) result = m;
+    return result;
+}
+
+OR_API double or_optimize_power(int N, const int *rowptr, const int *col, const double *val,
+                                int max_steps) {
+    csr_t A = {N, rowptr, col, val};
+    double *x1 = (double *)malloc(sizeof(double) * (size_t)N * 3);
+    double *x2 = x1 + N, *Av = x1 + 2 * (size_t)N;
+    for (int i = 0; i < N; ++i) {
+        x1[i] = 0.0;
+        x2[i] = 1.0 / N;
+    }
+    for (int s = 0; s < max_steps; ++s) {
+        memcpy(x1, x2, sizeof(double) * N);
+        spmv(&A, x1, x2);
+        double norm_x2 = norm2(N, x2);
+        double alpha = 1 / norm_x2;
+        for (int i = 0; i < N; ++i) x2[i] = alpha * x2[i];
+    }
+    spmv(&A, x2, Av);
+    double max_eigenvalue = or_dot(N, x2, Av);
+    double min_eigenvalue = or_find_diag_abs_min(N, rowptr, col, val);
+    free(x1);
+    return 2.0 / (max_eigenvalue + min_eigenvalue);
+}
+
+static int richardson(const csr_t *A, const double *b, double *x, double error, double omega,
+                      int power_steps, int max_steps, hist_t *h, double *omega_used) {
+    const int N = A->n;
+    if (-1e-15 < omega && omega < 1e-15)
+        omega = or_optimize_power(N, A->rowptr, A->col, A->val, power_steps);
+    if (omega_used) *omega_used = omega;
+    double *r = (double *)malloc(sizeof(double) * (size_t)N);
+    if (!r) return OR_ERR_NOMEM;
+    residual(A, b, x, r);
+    double norm_r = norm2(N, r);
+    int64_t ms = unbounded(max_steps);
+    hpush(h, norm_r);
+    int64_t k = 1;
+    while (k++ <= ms && norm_r > error) {
+        axpy(N, omega, r, x, x);
+        residual(A, b, x, r);
+        norm_r = norm2(N, r);
+        hpush(h, norm_r);
+    }
+    free(r);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* MR  (SolverSingleGPU.h:825-890)                                           */
+/* ------------------------------------------------------------------------ */
+static int mr(const csr_t *A, const double *b, double *x, double error, int max_steps, hist_t *h) {
+    const int N = A->n;
+    double *r = (double *)malloc(sizeof(double) * (size_t)N * 2);
+    if (!r) return OR_ERR_NOMEM;
+    double *Ar = r + N;
+    residual(A, b, x, r);
+    double norm_r = norm2(N, r);
+    int64_t ms = unbounded(max_steps);
+    hpush(h, norm_r);
+    int64_t k = 1;
+    while (k++ <= ms && norm_r > error) {
+        spmv(A, r, Ar);
+        double omega = or_dot(N, r, Ar) / or_dot(N, Ar, Ar);
+        axpy(N, omega, r, x, x);
+        axpy(N, -omega, Ar, r, r);
+        norm_r = norm2(N, r);
+        hpush(h, norm_r);
+    }
+    free(r);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Jacobi (residual form)  (SolverSingleGPU.h:924-994)                       */
+/* ------------------------------------------------------------------------ */
+static int jacobi(const csr_t *A, const double *b, double *x, double error, int max_steps,
+                  hist_t *h) {
+    const int N = A->n;
+    double *r = (double *)malloc(sizeof(double) * (size_t)N * 4);
+    if (!r) return OR_ERR_NOMEM;
+    double *D_1 = r + N, *z = r + 2 * (size_t)N, *w = r + 3 * (size_t)N;
+    residual(A, b, x, r);
+    double norm_r = norm2(N, r);
+    find_diag(A, D_1);
+    for (int i = 0; i < N; ++i) D_1[i] = 1.0 / D_1[i];
+    int64_t ms = unbounded(max_steps);
+    hpush(h, norm_r);
+    int64_t k = 1;
+    while (k++ <= ms && norm_r > error) {
+        for (int i = 0; i < N; ++i) z[i] = D_1[i] * r[i];
+        spmv(A, z, w);
+        axpy(N, 1.0, z, x, x);
+        axpy(N, -1.0, w, r, r);
+        norm_r = norm2(N, r);
+        hpush(h, norm_r);
+    }
+    free(r);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* MRJacobi  (SolverSingleGPU.h:1000-1073)                                   */
+/* ------------------------------------------------------------------------ */
+static int mrjacobi(const csr_t *A, const double *b, double *x, double error, int max_steps,
+                    hist_t *h) {
+    const int N = A->n;
+    double *r = (double *)malloc(sizeof(double) * (size_t)N * 4);
+    if (!r) return OR_ERR_NOMEM;
+    double *D_1 = r + N, *z = r + 2 * (size_t)N, *w = r + 3 * (size_t)N;
+    residual(A, b, x, r);
+    double norm_r = norm2(N, r);
+    find_diag(A, D_1);
+    for (int i = 0; i < N; ++i) D_1[i] = 1.0 / D_1[i];
+    int64_t ms = unbounded(max_steps);
+    hpush(h, norm_r);
+    int64_t k = 1;
+    while (k++ <= ms && norm_r > error) {
+        for (int i = 0; i < N; ++i) z[i] = D_1[i] * r[i];
+        spmv(A, z, w);
+        double alpha = or_dot(N, r, w) / or_dot(N, w, w);
+        axpy(N, alpha, z, x, x);
+        axpy(N, -alpha, w, r, r);
+        norm_r = norm2(N, r);
+        hpush(h, norm_r);
+    }
+    free(r);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* JMR = PreconditionedMRJacobi  (SolverSingleGPU.h:1079-1155)               */
+/* ------------------------------------------------------------------------ */
+static int jmr(const csr_t *A, const double *b, double *x, double error, int max_steps,
+               hist_t *h) {
+    const int N = A->n;
+    double *r = (double *)malloc(sizeof(double) * (size_t)N * 4);
+    if (!r) return OR_ERR_NOMEM;
+    double *S = r + N, *z = r + 2 * (size_t)N, *w = r + 3 * (size_t)N;
+    find_diag(A, S);
+    for (int i = 0; i < N; ++i) S[i] = sqrt(1.0 / S[i]);
+    residual(A, b, x, r);
+    for (int i = 0; i < N; ++i) r[i] = S[i] * r[i];
+    double norm_r = norm2(N, r);
+    int64_t ms = unbounded(max_steps);
+    hpush(h, norm_r);
+    int64_t k = 1;
+    while (k++ <= ms && norm_r > error) {
+        for (int i = 0; i < N; ++i) z[i] = S[i] * r[i];
+        spmv(A, z, w);
+        for (int i = 0; i < N; ++i) w[i] = S[i] * w[i];
+        double alpha = or_dot(N, r, w) / or_dot(N, w, w);
+        axpy(N, alpha, z, x, x);
+        axpy(N, -alpha, w, r, r);
+        norm_r = norm2(N, r);
+        hpush(h, norm_r);
+    }
+    free(r);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* MRS with invM = D^-1 as a diagonal CSR  (SolverSingleGPU.h:1236-1303)     */
+/* ------------------------------------------------------------------------ */
+static int mrs(const csr_t *A, const double *b, double *x, double error, int max_steps,
+               hist_t *h) {
+    const int N = A->n;
+    /* JacobiPrecondition (SparseCPU.h:428-446): invD(i,i) = 1 / a_ii */
+    int *inv_rp = (int *)malloc(sizeof(int) * ((size_t)N + 1) + sizeof(int) * (size_t)N);
+    double *inv_v = (double *)malloc(sizeof(double) * (size_t)N);
+    double *r = (double *)malloc(sizeof(double) * (size_t)N * 3);
+    if (!inv_rp || !inv_v || !r) { free(inv_rp); free(inv_v); free(r); return OR_ERR_NOMEM; }
+    int *inv_c = inv_rp + N + 1;
+    inv_rp[0] = 0;
+    for (int i = 0; i < N; ++i) {
+        int j = A->rowptr[i];
+        while (j < A->rowptr[i + 1] && A->col[j] < i) ++j;
+        if (j == A->rowptr[i + 1] || A->col[j] != i) {
+            free(inv_rp); free(inv_v); free(r);
+            return OR_ERR_ARG;
+        }
+        inv_v[i] = 1 / A->val[j];
+        inv_c[i] = i;
+        inv_rp[i + 1] = i + 1;
+    }
+    csr_t invM = {N, inv_rp, inv_c, inv_v};
+    double *z = r + N, *w = r + 2 * (size_t)N;
+    residual(A, b, x, r);
+    double norm_r = norm2(N, r);
+    int64_t ms = unbounded(max_steps);
+    hpush(h, norm_r);
+    int64_t k = 1;
+    while (k++ <= ms && norm_r > error) {
+        spmv(&invM, r, z);
+        spmv(A, z, w);
+        double alpha = or_dot(N, r, w) / or_dot(N, w, w);
+        axpy(N, alpha, z, x, x);
+        axpy(N, -alpha, w, r, r);
+        norm_r = norm2(N, r);
+        hpush(h, norm_r);
+    }
+    free(inv_rp); free(inv_v); free(r);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* dispatch                                                                   */
+/* ------------------------------------------------------------------------ */
+/*
+ * Solve A x = b with one of the eight methods.  x holds x0 on entry.
+ * At (CSR of A^T) is only read by BiCG.  hist receives up to hist_cap
+ * residual norms; *hist_len gets the full count (iterations = len - 1).
+ * omega/power_steps: Richardson; int_bound: BiCG breakdown bound.
+ * *omega_used receives Richardson's relaxation factor (may be NULL).
+ */
+OR_API int or_solve(int method, int N, const int *rowptr, const int *col, const double *val,
+                    const int *t_rowptr, const int *t_col, const double *t_val,
+                    const double *b, double *x, double error, int max_steps, double omega,
+                    int power_steps, double int_bound, double *hist, int hist_cap,
+                    int *hist_len, double *omega_used) {
+    csr_t A = {N, rowptr, col, val};
+    csr_t At = {N, t_rowptr, t_col, t_val};
+    hist_t h = {hist, hist_cap, 0};
+    int st;
+    switch (method) {
+        case OR_CG: st = cg(&A, b, x, error, max_steps, &h); break;
+        case OR_BICG:
+            if (!t_rowptr || !t_col || !t_val) return OR_ERR_ARG;
+            st = bicg(&A, &At, b, x, error, max_steps, int_bound, &h);
+            break;
+        case OR_JACOBI: st = jacobi(&A, b, x, error, max_steps, &h); break;
+        case OR_MR: st = mr(&A, b, x, error, max_steps, &h); break;
+        case OR_MRJACOBI: st = mrjacobi(&A, b, x, error, max_steps, &h); break;
+        case OR_JMR: st = jmr(&A, b, x, error, max_steps, &h); break;
+        case OR_MRS: st = mrs(&A, b, x, error, max_steps, &h); break;
+        case OR_RICHARDSON:
+            st = richardson(&A, b, x, error, omega, power_steps, max_steps, &h, omega_used);
+            break;
+        default: return OR_ERR_ARG;
+    }
+    if (hist_len) *hist_len = h.len;
+    return st;
+}

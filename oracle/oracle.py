@@ -1,0 +1,346 @@
+"""ctypes front end of the CPU oracle (oracle/gis_oracle*.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` -- never by the product package.
+
+Every function returns fresh numpy arrays; inputs are not modified.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libgis_oracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref_sparsecpu.so")
+
+METHODS = {"CG": 0, "BiCG": 1, "Jacobi": 2, "MR": 3, "MRJacobi": 4, "JMR": 5, "MRS": 6,
+           "Richardson": 7}
+BREAKDOWN = 10
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_lib = None
+
+
+def build(force: bool = False) -> None:
+    """Compile the oracle with its Makefile (gcc only; no GPU toolchain needed)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+def build_ref() -> bool:
+    """Compile the reference's SparseCPU.h driver when the cuFoam sources are present."""
+    if not os.path.isdir("/root/reference/gis/SparseSystem"):
+        return os.path.exists(REF_PATH)
+    subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+    return True
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(LIB_PATH)
+        L.or_dot.restype = C.c_double
+        L.or_dot.argtypes = [C.c_int64, _f64p, _f64p]
+        L.or_dot_f32.restype = C.c_double
+        L.or_dot_f32.argtypes = [C.c_int64, _f32p, _f32p]
+        L.or_sum.restype = C.c_double
+        L.or_sum.argtypes = [C.c_int64, _f64p]
+        L.or_dist.restype = C.c_double
+        L.or_dist.argtypes = [C.c_int64, _f64p, _f64p]
+        L.or_tree_sum.restype = C.c_double
+        L.or_tree_sum.argtypes = [C.c_int64, _f64p]
+        L.or_ldu2csr_pattern.restype = C.c_int
+        L.or_ldu2csr_pattern.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, _i32p, _i32p]
+        L.or_ldu2csr_values.restype = None
+        L.or_ldu2csr_values.argtypes = [C.c_int, C.c_int, C.c_int64, _i32p, _f64p, _f64p, _f64p,
+                                        _f64p]
+        L.or_transpose.restype = None
+        L.or_transpose.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p]
+        L.or_spmv.restype = None
+        L.or_spmv.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.or_spmv_f32.restype = None
+        L.or_spmv_f32.argtypes = [C.c_int, _i32p, _i32p, _f32p, _f32p, _f32p]
+        L.or_residual.restype = None
+        L.or_residual.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p]
+        L.or_norm_factor.restype = C.c_double
+        L.or_norm_factor.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.or_find_diag_abs_min.restype = C.c_double
+        L.or_find_diag_abs_min.argtypes = [C.c_int, _i32p, _i32p, _f64p]
+        L.or_optimize_power.restype = C.c_double
+        L.or_optimize_power.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_int]
+        L.or_solve.restype = C.c_int
+        L.or_solve.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, C.c_void_p, C.c_void_p,
+                               C.c_void_p, _f64p, _f64p, C.c_double, C.c_int, C.c_double,
+                               C.c_int, C.c_double, _f64p, C.c_int, C.POINTER(C.c_int),
+                               C.POINTER(C.c_double)]
+        if hasattr(L, "or_mixed_cg"):
+            L.or_mixed_cg.restype = C.c_int
+            L.or_mixed_cg.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double,
+                                      C.c_double, C.c_int, C.c_int, _f64p, C.c_int,
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        _lib = L
+    return _lib
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ---------------------------------------------------------------------------
+# formats
+# ---------------------------------------------------------------------------
+@dataclass
+class Csr:
+    n: int
+    rowptr: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rowptr[-1])
+
+    def scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.val, self.col, self.rowptr), shape=(self.n, self.n))
+
+
+def ldu2csr_pattern(n_cells, lower_addr, upper_addr):
+    """(rowptr, col, perm) of gis/ldu2csr.h:19-39; perm codes lower f / M+c diag / M+N+f upper."""
+    la = _c(lower_addr, np.int32)
+    ua = _c(upper_addr, np.int32)
+    M = la.shape[0]
+    nnz = n_cells + 2 * M
+    rowptr = np.zeros(n_cells + 1, np.int32)
+    col = np.zeros(max(nnz, 1), np.int32)
+    perm = np.zeros(max(nnz, 1), np.int32)
+    st = lib().or_ldu2csr_pattern(n_cells, M, la, ua, rowptr, col, perm)
+    if st != 0:
+        raise ValueError(f"or_ldu2csr_pattern failed with status {st}")
+    return rowptr, col[:nnz], perm[:nnz]
+
+
+def ldu2csr(sys) -> Csr:
+    """CPU CSR of an LduSystem, as getSparseCSRMat builds it (gis/ldu2csr.h:7-42)."""
+    rowptr, col, perm = ldu2csr_pattern(sys.n_cells, sys.lower_addr, sys.upper_addr)
+    nnz = perm.shape[0]
+    val = np.zeros(max(nnz, 1))
+    lib().or_ldu2csr_values(sys.n_cells, sys.n_faces, nnz, _c(perm if nnz else np.zeros(1), np.int32),
+                            _c(sys.diag, np.float64), _c(sys.lower if sys.n_faces else np.zeros(1), np.float64),
+                            _c(sys.upper if sys.n_faces else np.zeros(1), np.float64), val)
+    return Csr(sys.n_cells, rowptr, col, val[:nnz])
+
+
+def transpose(A: Csr) -> Csr:
+    """A^T by the counting sort of SparseCPU.h:369-399."""
+    t_rowptr = np.zeros(A.n + 1, np.int32)
+    t_col = np.zeros(max(A.nnz, 1), np.int32)
+    t_val = np.zeros(max(A.nnz, 1))
+    lib().or_transpose(A.n, A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                       _c(A.val, np.float64), t_rowptr, t_col, t_val)
+    return Csr(A.n, t_rowptr, t_col[:A.nnz], t_val[:A.nnz])
+
+
+# ---------------------------------------------------------------------------
+# kernels
+# ---------------------------------------------------------------------------
+def dot(a, b) -> float:
+    a = _c(a, np.float64)
+    return lib().or_dot(a.shape[0], a, _c(b, np.float64))
+
+
+def dot_f32(a, b) -> float:
+    a = _c(a, np.float32)
+    return lib().or_dot_f32(a.shape[0], a, _c(b, np.float32))
+
+
+def vsum(a) -> float:
+    a = _c(a, np.float64)
+    return lib().or_sum(a.shape[0], a)
+
+
+def dist(a, b) -> float:
+    a = _c(a, np.float64)
+    return lib().or_dist(a.shape[0], a, _c(b, np.float64))
+
+
+def tree_sum(leaves) -> float:
+    leaves = _c(leaves, np.float64)
+    return lib().or_tree_sum(leaves.shape[0], leaves)
+
+
+def spmv(A: Csr, x) -> np.ndarray:
+    y = np.zeros(A.n)
+    lib().or_spmv(A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32), _c(A.val, np.float64),
+                  _c(x, np.float64), y)
+    return y
+
+
+def spmv_f32(A: Csr, x) -> np.ndarray:
+    y = np.zeros(A.n, np.float32)
+    lib().or_spmv_f32(A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                      _c(A.val, np.float32), _c(x, np.float32), y)
+    return y
+
+
+def residual(A: Csr, b, x) -> np.ndarray:
+    r = np.zeros(A.n)
+    lib().or_residual(A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32), _c(A.val, np.float64),
+                      _c(b, np.float64), _c(x, np.float64), r)
+    return r
+
+
+def norm_factor(A: Csr, b, x) -> float:
+    return lib().or_norm_factor(A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                                _c(A.val, np.float64), _c(b, np.float64), _c(x, np.float64))
+
+
+def find_diag_abs_min(A: Csr) -> float:
+    return lib().or_find_diag_abs_min(A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                                      _c(A.val, np.float64))
+
+
+def optimize_power(A: Csr, steps: int = 30) -> float:
+    return lib().or_optimize_power(A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                                   _c(A.val, np.float64), steps)
+
+
+@dataclass
+class SolveResult:
+    x: np.ndarray
+    hist: np.ndarray
+    hist_len: int
+    status: int
+    omega: float = 0.0
+
+    @property
+    def iterations(self) -> int:
+        return self.hist_len - 1
+
+
+def solve(method: str, A: Csr, b, x0, error: float, max_steps: int, At: Csr | None = None,
+          omega: float = 0.0, power_steps: int = 30, int_bound: float = 1e-10,
+          hist_cap: int | None = None) -> SolveResult:
+    """One gis solver (SingleGPU::<method> semantics) on the CPU; x0 is copied."""
+    m = METHODS[method]
+    x = _c(np.array(x0, dtype=np.float64, copy=True), np.float64)
+    if hist_cap is None:
+        hist_cap = (max_steps if max_steps > 0 else 100000) + 1
+    hist = np.zeros(max(hist_cap, 1))
+    hl = C.c_int(0)
+    om = C.c_double(0.0)
+    keep = []
+    if m == METHODS["BiCG"]:
+        if At is None:
+            At = transpose(A)
+        keep = [_c(At.rowptr, np.int32), _c(At.col, np.int32), _c(At.val, np.float64)]
+        tptrs = [k.ctypes.data for k in keep]
+    else:
+        tptrs = [None, None, None]
+    st = lib().or_solve(m, A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                        _c(A.val, np.float64), tptrs[0], tptrs[1], tptrs[2], _c(b, np.float64),
+                        x, error, max_steps, omega, power_steps, int_bound, hist, hist_cap,
+                        C.byref(hl), C.byref(om))
+    del keep
+    if st not in (0, BREAKDOWN):
+        raise RuntimeError(f"or_solve({method}) failed with status {st}")
+    n = min(hl.value, hist_cap)
+    return SolveResult(x, hist[:n].copy(), hl.value, st, om.value)
+
+
+@dataclass
+class Performance:
+    """The three numbers cuFoam's wrappers put into solverPerformance."""
+    solver: str
+    initial_residual: float
+    final_residual: float
+    n_iterations: int
+    status: int
+
+
+def foam_solve(method: str, sys, psi, source, tolerance: float, max_iter: int,
+               A: Csr | None = None) -> tuple[np.ndarray, Performance]:
+    """cuFoam<method>::solve on the CPU (cuFoamCG/cuFoamCG.C:45-83 and siblings).
+
+    error = tolerance * NormFactor(A, b, x); the solver runs with max_steps = maxIter;
+    initialResidual = hist.front(), finalResidual = hist.back(), nIterations = size-1.
+    """
+    if A is None:
+        A = ldu2csr(sys)
+    nf = norm_factor(A, source, psi)
+    res = solve(method, A, source, psi, tolerance * nf, max_iter,
+                At=transpose(A) if method == "BiCG" else None)
+    perf = Performance("cuFoam" + method, float(res.hist[0]), float(res.hist[-1]),
+                       res.hist_len - 1, res.status)
+    return res.x, perf
+
+
+def mixed_cg(A: Csr, b, x0, rel_tol: float = 1e-10, inner_rel_tol: float = 1e-4,
+             max_iter: int = 100000, mode: int = 0, hist_cap: int = 256):
+    """Mixed-precision CG (FP32 inner Krylov, FP64 residual/correction/norms).
+
+    mode 0: reliable updates (the Krylov direction survives each FP64 update);
+    mode 1: restarted iterative refinement (inner CG restarted from zero).
+    Returns (x, true-residual history, inner iterations, status).
+    """
+    x = _c(np.array(x0, dtype=np.float64, copy=True), np.float64)
+    hist = np.zeros(hist_cap)
+    hl = C.c_int(0)
+    iters = C.c_int(0)
+    st = lib().or_mixed_cg(A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                           _c(A.val, np.float64), _c(b, np.float64), x, rel_tol, inner_rel_tol,
+                           max_iter, mode, hist, hist_cap, C.byref(hl), C.byref(iters))
+    return x, hist[:min(hl.value, hist_cap)].copy(), iters.value, st
+
+
+# ---------------------------------------------------------------------------
+# the reference's own SparseCPU.h, compiled where the sources exist
+# ---------------------------------------------------------------------------
+_ref = None
+
+
+def ref_lib():
+    """ctypes handle of oracle/_ref/libref_sparsecpu.so, or None if unbuildable here."""
+    global _ref
+    if _ref is None:
+        try:
+            build_ref()
+        except Exception:
+            return None
+        if not os.path.exists(REF_PATH):
+            return None
+        R = C.CDLL(REF_PATH)
+        R.ref_transpose.restype = None
+        R.ref_transpose.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p]
+        R.ref_jacobi_precondition.restype = C.c_int
+        R.ref_jacobi_precondition.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p]
+        _ref = R
+    return _ref
+
+
+def ref_transpose(A: Csr) -> Csr:
+    R = ref_lib()
+    t_rowptr = np.zeros(A.n + 1, np.int32)
+    t_col = np.zeros(max(A.nnz, 1), np.int32)
+    t_val = np.zeros(max(A.nnz, 1))
+    R.ref_transpose(A.n, A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                    _c(A.val, np.float64), t_rowptr, t_col, t_val)
+    return Csr(A.n, t_rowptr, t_col[:A.nnz], t_val[:A.nnz])
+
+
+def ref_jacobi_precondition(A: Csr) -> np.ndarray:
+    R = ref_lib()
+    out = np.zeros(A.n)
+    st = R.ref_jacobi_precondition(A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                                   _c(A.val, np.float64), out)
+    if st != 0:
+        raise ValueError("D is not invertible")
+    return out
