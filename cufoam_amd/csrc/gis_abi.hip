@@ -324,7 +324,7 @@ GIS_API int gis_dot_device(gis_handle h, const double* d_a, const double* d_b, d
 
 GIS_API int gis_set_schedule(gis_handle h, int spmv64, int spmv32, int block_iters) {
     if (!h) return GIS_ERR_ARG;
-    if (spmv64 > 2 || spmv32 > 2) return fail(h, GIS_ERR_ARG, "schedule id out of range");
+    if (spmv64 > 3 || spmv32 > 3) return fail(h, GIS_ERR_ARG, "schedule id out of range");
     if (spmv64 >= 0) h->sched.spmv64 = static_cast<SpmvKind>(spmv64);
     if (spmv32 >= 0) h->sched.spmv32 = static_cast<SpmvKind>(spmv32);
     if (block_iters > 0) h->sched.block_iters = block_iters;

@@ -251,6 +251,75 @@ __device__ __forceinline__ void tile_spmv(const int* __restrict__ rowptr,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Sliced layout (SELL-64): rows are bundled 64 to a slice (one row per lane of
+// a wave64); slice s stores its slots slot-major, slot j of lane i at
+// soff[s] + 64*j + i, padding marked by column -1.  Slot j of a row is its
+// j-th CSR entry, so the sequential row sum below is the CSR row sum bit for
+// bit.  Every load of values and columns is unit-stride across the wave and
+// the x gathers of neighbouring rows hit neighbouring addresses.
+// A workgroup tile of kTile rows = 16 slices; wave w owns slices 4w..4w+3 and
+// writes y to sh_y in row order (each wave touches only its own 256 rows).
+// ---------------------------------------------------------------------------
+constexpr int kSlice = 64;
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T, class XOp>
+__device__ __forceinline__ T sell_row(const int* __restrict__ scol, const T* __restrict__ sval,
+                                      int a, int len, int lane, XOp&& xop) {
+    T acc = T(0);
+    for (int j = 0; j < len; j += 8) {
+        int cc[8];
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = a + (j + u) * kSlice + lane;
+            if (j + u < len) {
+                cc[u] = __builtin_nontemporal_load(scol + idx);
+                v[u] = __builtin_nontemporal_load(sval + idx);
+            } else {
+                cc[u] = -1;
+                v[u] = T(0);
+            }
+        }
+        T xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = cc[u] >= 0 ? xop(cc[u]) : T(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (cc[u] >= 0) acc = acc + v[u] * xv[u];
+    }
+    return acc;
+}
+
+template <typename T, class XOp>
+__device__ __forceinline__ void tile_sell(const int* __restrict__ soff,
+                                          const int* __restrict__ scol,
+                                          const T* __restrict__ sval, int N, int row0,
+                                          XOp&& xop, T* __restrict__ sh_y) {
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int nslices = (N + kSlice - 1) / kSlice;
+    const int s0 = row0 / kSlice + w * 4;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int s = s0 + c;
+        T y = T(0);
+        if (s < nslices) {
+            const int a = soff[s];
+            const int len = (soff[s + 1] - a) / kSlice;
+            y = sell_row<T>(scol, sval, a, len, lane, xop);
+        }
+        sh_y[(w * 4 + c) * kSlice + lane] = y;
+    }
+    wave_sync_lds();
+}
+
 // Unstaged variant: each thread sums its own kLeaf consecutive rows straight
 // from global memory (same order, same result; fewer LDS passes, strided loads).
 template <typename T, class XOp>

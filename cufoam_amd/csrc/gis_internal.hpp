@@ -41,11 +41,11 @@ struct DevMem {
 
 // SpMV schedule: how a tile's products are formed.  Every schedule yields
 // bit-identical results (same per-row order), so the choice is pure tuning.
-enum class SpmvKind : int { Staged2K = 0, Staged4K = 1, Direct = 2 };
+enum class SpmvKind : int { Sell = 0, Staged2K = 1, Staged4K = 2, Direct = 3 };
 
 struct Schedule {
-    SpmvKind spmv64 = SpmvKind::Staged4K;   // FP64 kernels
-    SpmvKind spmv32 = SpmvKind::Staged4K;   // FP32 kernels
+    SpmvKind spmv64 = SpmvKind::Sell;       // FP64 kernels
+    SpmvKind spmv32 = SpmvKind::Sell;       // FP32 kernels
     int block_iters = 64;                   // max iterations launched between host checks
 };
 
@@ -68,6 +68,9 @@ struct gis_context {
     gis::DevMem lower_addr, upper_addr;         // LDU addressing copies
     gis::DevMem diag, lower, upper;             // LDU coefficients (device copies)
     gis::DevMem val, valT, val32;               // CSR values: A, A^T, float(A)
+    long long n_slots = 0;                      // sliced layout (64-row slices)
+    gis::DevMem soff, scol, sperm;              // slice offsets, slot columns / codes
+    gis::DevMem sval, svalT, sval32;            // slot values: A, A^T, float(A)
     gis::DevMem dinv;                           // per-row helper (1/a_ii or sqrt(1/a_ii))
 
     // workspaces

@@ -1,0 +1,175 @@
+// Kernel plumbing shared by the solver translation units: the matrix view,
+// the SpMV schedules (all bit-identical), tile bookkeeping and the solver
+// set-up kernel.
+#pragma once
+
+#include "gis_internal.hpp"
+
+namespace gis {
+
+// Device view of one matrix in both layouts (CSR + sliced); a schedule reads
+// whichever it needs.
+template <typename T>
+struct Mat {
+    const int* rowptr;
+    const int* col;
+    const T* val;
+    const int* soff;
+    const int* scol;
+    const T* sval;
+};
+
+// ---------------------------------------------------------------------------
+// SpMV schedules: y[row0 .. row0+nrows) -> sh_y (tile row order)
+// ---------------------------------------------------------------------------
+struct Sell {
+    template <typename T>
+    struct Shared {};
+    template <typename T, class XOp>
+    __device__ static void run(const Mat<T>& m, int N, int row0, int, XOp&& xop, Shared<T>&,
+                               T* y) {
+        tile_sell<T>(m.soff, m.scol, m.sval, N, row0, xop, y);
+    }
+};
+
+template <int CAP>
+struct Staged {
+    template <typename T>
+    using Shared = SpmvShared<T, CAP>;
+    template <typename T, class XOp>
+    __device__ static void run(const Mat<T>& m, int, int row0, int nrows, XOp&& xop,
+                               Shared<T>& sh, T* y) {
+        tile_spmv<T, CAP>(m.rowptr, m.col, m.val, row0, nrows, xop, sh, y);
+    }
+};
+
+struct Direct {
+    template <typename T>
+    struct Shared {};
+    template <typename T, class XOp>
+    __device__ static void run(const Mat<T>& m, int, int row0, int nrows, XOp&& xop,
+                               Shared<T>&, T* y) {
+        tile_spmv_direct<T>(m.rowptr, m.col, m.val, row0, nrows, xop, y);
+    }
+};
+
+#define GIS_SPMV_DISPATCH(kind, POLICY, ...)          \
+    switch (kind) {                                   \
+        case SpmvKind::Staged2K: {                    \
+            using POLICY = Staged<2048>;              \
+            __VA_ARGS__;                              \
+        } break;                                      \
+        case SpmvKind::Staged4K: {                    \
+            using POLICY = Staged<4096>;              \
+            __VA_ARGS__;                              \
+        } break;                                      \
+        case SpmvKind::Direct: {                      \
+            using POLICY = Direct;                    \
+            __VA_ARGS__;                              \
+        } break;                                      \
+        default: {                                    \
+            using POLICY = Sell;                      \
+            __VA_ARGS__;                              \
+        } break;                                      \
+    }
+
+// row range of this workgroup's tile
+struct TileRows {
+    int tile, row0, nrows, i0;   // i0: first of this thread's kLeaf rows
+    __device__ TileRows(int N) {
+        tile = blockIdx.x;
+        row0 = tile * kTile;
+        nrows = min(kTile, N - row0);
+        i0 = row0 + threadIdx.x * kLeaf;
+    }
+};
+
+__device__ __forceinline__ void record_norm(KState* st, double norm) {
+    const int k = st->k + 1;
+    st->k = k;
+    st->norm = norm;
+    if (k < st->hist_cap) st->hist[k] = norm;
+}
+
+// ---------------------------------------------------------------------------
+// solver set-up: r = b - A x (ResidualKernel, kernels.h:409-441), optionally
+// r = s o r (JMR, SolverSingleGPU.h:1098-1100), a copy of r, and
+// rho0 = (r, r), norm0 = sqrt(rho0), hist[0].
+// done_rule 0: done when !(norm > error)   (loop guards of CG, Jacobi, MR, ...)
+// done_rule 1: never done before the first iteration (BiCG, :621)
+// ---------------------------------------------------------------------------
+template <class P>
+__global__ __launch_bounds__(kWG) void k_setup(Mat<double> m, int N,
+                                               const double* __restrict__ b,
+                                               const double* __restrict__ x,
+                                               const double* __restrict__ scale,
+                                               double* __restrict__ r, double* __restrict__ r2,
+                                               KState* st, RedWs red, int done_rule) {
+    __shared__ typename P::template Shared<double> sh;
+    __shared__ double sh_y[kTile];
+    __shared__ RedShared shr;
+    TileRows t(N);
+    P::run(m, N, t.row0, t.nrows, [&](int c) { return x[c]; }, sh, sh_y);
+    double bv[4], rv[4], sv[4];
+    load4(b, t.i0, N, bv);
+    if (scale) load4(scale, t.i0, N, sv);
+    double leaf[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        rv[k] = bv[k] - sh_y[threadIdx.x * kLeaf + k];
+        if (scale) rv[k] = sv[k] * rv[k];
+        if (t.i0 + k < N) leaf[0] = fma(rv[k], rv[k], leaf[0]);
+    }
+    store4(r, t.i0, N, rv);
+    if (r2) store4(r2, t.i0, N, rv);
+    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+        const double norm = sqrt(v[0]);
+        st->rho0 = v[0];
+        st->rho1 = 0.0;
+        st->norm = norm;
+        st->norm0 = norm;
+        st->k = 0;
+        st->status = kOK;
+        if (st->hist_cap > 0) st->hist[0] = norm;
+        st->done = (done_rule == 0) ? !(norm > st->error) : 0;
+    });
+}
+
+// host helpers shared by the solver files
+inline dim3 tiles_grid(int N) { return dim3((unsigned)tiles_of(N)); }
+
+inline Mat<double> mat64(const gis_context* h) {
+    return Mat<double>{h->rowptr.as<int>(), h->col.as<int>(), h->val.as<double>(),
+                       h->soff.as<int>(), h->scol.as<int>(), h->sval.as<double>()};
+}
+
+inline Mat<double> mat64T(const gis_context* h) {
+    return Mat<double>{h->rowptr.as<int>(), h->col.as<int>(), h->valT.as<double>(),
+                       h->soff.as<int>(), h->scol.as<int>(), h->svalT.as<double>()};
+}
+
+inline Mat<float> mat32(const gis_context* h) {
+    return Mat<float>{h->rowptr.as<int>(), h->col.as<int>(), h->val32.as<float>(),
+                      h->soff.as<int>(), h->scol.as<int>(), h->sval32.as<float>()};
+}
+
+// Enqueue iterations in blocks until the device reports done.
+template <class Iter>
+int run_blocks(gis_context* h, long long max_steps, Iter&& launch_iteration) {
+    long long launched = 0;
+    int block = 4;
+    for (;;) {
+        for (int i = 0; i < block && launched < max_steps; ++i, ++launched)
+            GIS_TRY(launch_iteration());
+        GIS_HIP(h, hipGetLastError());
+        GIS_TRY(read_state(h));
+        if (h->st_host->done || launched >= max_steps) break;
+        block = block * 2 < h->sched.block_iters ? block * 2 : h->sched.block_iters;
+        if (block < 1) block = 1;
+    }
+    return kOK;
+}
+
+int prepare_solve(gis_context* h, const SolveArgs& a, int nvec);
+
+}  // namespace gis

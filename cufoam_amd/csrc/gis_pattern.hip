@@ -144,6 +144,7 @@ __global__ __launch_bounds__(kBlk) void k_sort_lower(int N, const int* __restric
     for (int i = 0; i < n; ++i) col[a + i] = la[perm[a + i]];
 }
 
+// value of one CSR / slot position from its code (-1: padding slot)
 template <typename T>
 __global__ __launch_bounds__(kBlk) void k_gather(long long nnz, int N, int M,
                                                  const int* __restrict__ perm,
@@ -154,10 +155,44 @@ __global__ __launch_bounds__(kBlk) void k_gather(long long nnz, int N, int M,
     if (p >= nnz) return;
     const int code = perm[p];
     double v;
-    if (code < M) v = lower[code];
+    if (code < 0) v = 0.0;
+    else if (code < M) v = lower[code];
     else if (code < M + N) v = diag[code - M];
     else v = upper[code - M - N];
     val[p] = static_cast<T>(v);
+}
+
+// sliced layout: slot count of each 64-row slice = 64 * (longest row)
+__global__ __launch_bounds__(kBlk) void k_slice_len(int N, int S, const int* __restrict__ rowptr,
+                                                    int* slots) {
+    const int s = blockIdx.x * kBlk + threadIdx.x;
+    if (s > S) return;
+    if (s == S) {
+        slots[s] = 0;
+        return;
+    }
+    int len = 0;
+    const int r1 = min(N, (s + 1) * kSlice);
+    for (int r = s * kSlice; r < r1; ++r) len = max(len, rowptr[r + 1] - rowptr[r]);
+    slots[s] = len * kSlice;
+}
+
+__global__ __launch_bounds__(kBlk) void k_sell_fill(int N, const int* __restrict__ rowptr,
+                                                    const int* __restrict__ col,
+                                                    const int* __restrict__ perm,
+                                                    const int* __restrict__ soff, int* scol,
+                                                    int* sperm) {
+    const int r = blockIdx.x * kBlk + threadIdx.x;
+    if (r >= N) return;
+    const int s = r / kSlice;
+    const int lane = r % kSlice;
+    const int a = rowptr[r];
+    const int len = rowptr[r + 1] - a;
+    const int base = soff[s] + lane;
+    for (int j = 0; j < len; ++j) {
+        scol[base + j * kSlice] = col[a + j];
+        sperm[base + j * kSlice] = perm[a + j];
+    }
 }
 
 int scan_exclusive(gis_context* h, const int* in, int* out, int n, int* tmp, size_t tmp_ints) {
@@ -234,6 +269,27 @@ int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua
                            h->rowptr.as<int>(), nbr_cnt, h->col.as<int>(), h->perm.as<int>());
         GIS_HIP(h, hipGetLastError());
     }
+    // sliced copy of the pattern (64-row slices, slot-major)
+    const int S = (N + kSlice - 1) / kSlice;
+    GIS_HIP(h, h->soff.ensure(sizeof(int) * ((size_t)S + 1)));
+    int* slots = rowlen;   // reuse: S + 1 <= N + 1
+    hipLaunchKernelGGL(k_slice_len, grid_for((long long)S + 1), dim3(kBlk), 0, h->stream, N, S,
+                       h->rowptr.as<int>(), slots);
+    GIS_HIP(h, hipGetLastError());
+    GIS_TRY(scan_exclusive(h, slots, h->soff.as<int>(), S + 1, tmp, scan_tmp));
+    int n_slots = 0;
+    GIS_HIP(h, hipMemcpyAsync(&n_slots, h->soff.as<int>() + S, sizeof(int),
+                              hipMemcpyDeviceToHost, h->stream));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    GIS_HIP(h, h->scol.ensure(sizeof(int) * (size_t)n_slots));
+    GIS_HIP(h, h->sperm.ensure(sizeof(int) * (size_t)n_slots));
+    GIS_HIP(h, hipMemsetAsync(h->scol.p, 0xff, sizeof(int) * (size_t)n_slots, h->stream));
+    GIS_HIP(h, hipMemsetAsync(h->sperm.p, 0xff, sizeof(int) * (size_t)n_slots, h->stream));
+    hipLaunchKernelGGL(k_sell_fill, grid_for(N), dim3(kBlk), 0, h->stream, N,
+                       h->rowptr.as<int>(), h->col.as<int>(), h->perm.as<int>(),
+                       h->soff.as<int>(), h->scol.as<int>(), h->sperm.as<int>());
+    GIS_HIP(h, hipGetLastError());
+    h->n_slots = n_slots;
     h->N = N;
     h->M = M;
     h->nnz = nnz;
@@ -241,13 +297,26 @@ int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua
     return kOK;
 }
 
+// CSR values and slot values of one matrix: A (lower, upper) or A^T (swapped)
+template <typename T>
+static int gather_both(gis_context* h, const double* lower, const double* upper, DevMem& csr,
+                       DevMem& sell) {
+    GIS_HIP(h, csr.ensure(sizeof(T) * (size_t)h->nnz));
+    GIS_HIP(h, sell.ensure(sizeof(T) * (size_t)h->n_slots));
+    hipLaunchKernelGGL(k_gather<T>, grid_for(h->nnz), dim3(kBlk), 0, h->stream, h->nnz, h->N,
+                       h->M, h->perm.as<int>(), h->diag.as<double>(), lower, upper, csr.as<T>());
+    GIS_HIP(h, hipGetLastError());
+    hipLaunchKernelGGL(k_gather<T>, grid_for(h->n_slots), dim3(kBlk), 0, h->stream, h->n_slots,
+                       h->N, h->M, h->sperm.as<int>(), h->diag.as<double>(), lower, upper,
+                       sell.as<T>());
+    GIS_HIP(h, hipGetLastError());
+    return kOK;
+}
+
 int gather_values(gis_context* h) {
     if (!h->has_pattern) return fail(h, kErrState, "pattern not set");
-    GIS_HIP(h, h->val.ensure(sizeof(double) * (size_t)h->nnz));
-    hipLaunchKernelGGL(k_gather<double>, grid_for(h->nnz), dim3(kBlk), 0, h->stream, h->nnz, h->N,
-                       h->M, h->perm.as<int>(), h->diag.as<double>(), h->lower.as<double>(),
-                       h->upper.as<double>(), h->val.as<double>());
-    GIS_HIP(h, hipGetLastError());
+    GIS_TRY(gather_both<double>(h, h->lower.as<double>(), h->upper.as<double>(), h->val,
+                                h->sval));
     h->has_values = true;
     h->has_val32 = false;
     h->has_valT = false;
@@ -257,24 +326,18 @@ int gather_values(gis_context* h) {
 int ensure_val32(gis_context* h) {
     if (!h->has_values) return fail(h, kErrState, "values not set");
     if (h->has_val32) return kOK;
-    GIS_HIP(h, h->val32.ensure(sizeof(float) * (size_t)h->nnz));
-    hipLaunchKernelGGL(k_gather<float>, grid_for(h->nnz), dim3(kBlk), 0, h->stream, h->nnz, h->N,
-                       h->M, h->perm.as<int>(), h->diag.as<double>(), h->lower.as<double>(),
-                       h->upper.as<double>(), h->val32.as<float>());
-    GIS_HIP(h, hipGetLastError());
+    GIS_TRY(gather_both<float>(h, h->lower.as<double>(), h->upper.as<double>(), h->val32,
+                               h->sval32));
     h->has_val32 = true;
     return kOK;
 }
 
+// A^T of an LDU matrix: same pattern, lower <-> upper
 int ensure_valT(gis_context* h) {
     if (!h->has_values) return fail(h, kErrState, "values not set");
     if (h->has_valT) return kOK;
-    GIS_HIP(h, h->valT.ensure(sizeof(double) * (size_t)h->nnz));
-    // A^T: same pattern, lower <-> upper
-    hipLaunchKernelGGL(k_gather<double>, grid_for(h->nnz), dim3(kBlk), 0, h->stream, h->nnz, h->N,
-                       h->M, h->perm.as<int>(), h->diag.as<double>(), h->upper.as<double>(),
-                       h->lower.as<double>(), h->valT.as<double>());
-    GIS_HIP(h, hipGetLastError());
+    GIS_TRY(gather_both<double>(h, h->upper.as<double>(), h->lower.as<double>(), h->valT,
+                                h->svalT));
     h->has_valT = true;
     return kOK;
 }

@@ -1,4 +1,5 @@
-// Krylov / stationary solvers of gis on gfx950, FP64.
+// SpMV, reductions, NormFactor and CG on gfx950, FP64; host-side solver
+// infrastructure (reduction workspace, device state, iteration blocks).
 //
 // Each method keeps its scalars (rho, alpha, ...) in a device KState and
 // advances with a fixed sequence of launches per iteration; reductions finish
@@ -7,82 +8,24 @@
 // reads the `done` flag between blocks; launches after convergence return at
 // once, which preserves the reference's per-iteration stopping rule and
 // therefore its iteration count.
-//
-// Methods and the reference loops they follow (gis/SparseSystem/SolverSingleGPU.h):
-//   CG :366-445   BiCG :568-675   Richardson :762-819 (+OptimizePower :723-752)
-//   MR :825-890   Jacobi :924-994  MRJacobi :1000-1073  JMR :1079-1155  MRS :1236-1303
 #include <algorithm>
 #include <climits>
 #include <cmath>
-#include <vector>
 
-#include "gis_internal.hpp"
+#include "gis_kernels.hpp"
 
 namespace gis {
-
-// ---------------------------------------------------------------------------
-// SpMV schedules
-// ---------------------------------------------------------------------------
-template <int CAP>
-struct Staged {
-    template <typename T>
-    using Shared = SpmvShared<T, CAP>;
-    template <typename T, class XOp>
-    __device__ static void run(const int* rowptr, const int* col, const T* val, int row0,
-                               int nrows, XOp&& xop, Shared<T>& sh, T* y) {
-        tile_spmv<T, CAP>(rowptr, col, val, row0, nrows, xop, sh, y);
-    }
-};
-
-struct Direct {
-    template <typename T>
-    struct Shared {};
-    template <typename T, class XOp>
-    __device__ static void run(const int* rowptr, const int* col, const T* val, int row0,
-                               int nrows, XOp&& xop, Shared<T>&, T* y) {
-        tile_spmv_direct<T>(rowptr, col, val, row0, nrows, xop, y);
-    }
-};
-
-#define GIS_SPMV_DISPATCH(kind, POLICY, ...)          \
-    switch (kind) {                                   \
-        case SpmvKind::Staged2K: {                    \
-            using POLICY = Staged<2048>;              \
-            __VA_ARGS__;                              \
-        } break;                                      \
-        case SpmvKind::Staged4K: {                    \
-            using POLICY = Staged<4096>;              \
-            __VA_ARGS__;                              \
-        } break;                                      \
-        default: {                                    \
-            using POLICY = Direct;                    \
-            __VA_ARGS__;                              \
-        } break;                                      \
-    }
-
-// row range of this workgroup's tile
-struct TileRows {
-    int tile, row0, nrows, i0;   // i0: first of this thread's kLeaf rows
-    __device__ TileRows(int N) {
-        tile = blockIdx.x;
-        row0 = tile * kTile;
-        nrows = min(kTile, N - row0);
-        i0 = row0 + threadIdx.x * kLeaf;
-    }
-};
 
 // ---------------------------------------------------------------------------
 // plain SpMV  y = A x  (SpMvKernel, kernels.h:206-237)
 // ---------------------------------------------------------------------------
 template <class P, typename T>
-__global__ __launch_bounds__(kWG) void k_spmv(const int* __restrict__ rowptr,
-                                              const int* __restrict__ col,
-                                              const T* __restrict__ val, int N,
-                                              const T* __restrict__ x, T* __restrict__ y) {
+__global__ __launch_bounds__(kWG) void k_spmv(Mat<T> m, int N, const T* __restrict__ x,
+                                              T* __restrict__ y) {
     __shared__ typename P::template Shared<T> sh;
     __shared__ T sh_y[kTile];
     TileRows t(N);
-    P::run(rowptr, col, val, t.row0, t.nrows, [&](int c) { return x[c]; }, sh, sh_y);
+    P::run(m, N, t.row0, t.nrows, [&](int c) { return x[c]; }, sh, sh_y);
     T o[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) o[k] = sh_y[threadIdx.x * kLeaf + k];
@@ -125,9 +68,7 @@ __global__ __launch_bounds__(kWG) void k_mean(int N, const double* __restrict__ 
 // NormFactor (gis/gis.cu:106-123):
 //   aux[1] = ||A x - A x_bar||_2 + ||b - A x_bar||_2 + 1e-20, with x_bar = aux[0]
 template <class P>
-__global__ __launch_bounds__(kWG) void k_norm_factor(const int* __restrict__ rowptr,
-                                                     const int* __restrict__ col,
-                                                     const double* __restrict__ val, int N,
+__global__ __launch_bounds__(kWG) void k_norm_factor(Mat<double> m, int N,
                                                      const double* __restrict__ b,
                                                      const double* __restrict__ x, KState* st,
                                                      RedWs red) {
@@ -137,8 +78,8 @@ __global__ __launch_bounds__(kWG) void k_norm_factor(const int* __restrict__ row
     __shared__ RedShared shr;
     TileRows t(N);
     const double x_bar = st->aux[0];
-    P::run(rowptr, col, val, t.row0, t.nrows, [&](int c) { return x[c]; }, sh, sh_y);
-    P::run(rowptr, col, val, t.row0, t.nrows, [&](int) { return x_bar; }, sh, sh_yb);
+    P::run(m, N, t.row0, t.nrows, [&](int c) { return x[c]; }, sh, sh_y);
+    P::run(m, N, t.row0, t.nrows, [&](int) { return x_bar; }, sh, sh_yb);
     double bv[4];
     load4(b, t.i0, N, bv);
     double leaf[2] = {0.0, 0.0};
@@ -158,59 +99,6 @@ __global__ __launch_bounds__(kWG) void k_norm_factor(const int* __restrict__ row
 }
 
 // ---------------------------------------------------------------------------
-// solver set-up: r = b - A x (ResidualKernel, kernels.h:409-441), optionally
-// r = s o r (JMR, SolverSingleGPU.h:1098-1100), copies of r, and
-// rho0 = (r, r), norm0 = sqrt(rho0), hist[0].
-// done_rule 0: done when !(norm > error)   (CG, Jacobi, MR, ... loop guards)
-// done_rule 1: never done before the first iteration (BiCG, :621)
-// ---------------------------------------------------------------------------
-template <class P>
-__global__ __launch_bounds__(kWG) void k_setup(const int* __restrict__ rowptr,
-                                               const int* __restrict__ col,
-                                               const double* __restrict__ val, int N,
-                                               const double* __restrict__ b,
-                                               const double* __restrict__ x,
-                                               const double* __restrict__ scale,
-                                               double* __restrict__ r, double* __restrict__ r2,
-                                               KState* st, RedWs red, int done_rule) {
-    __shared__ typename P::template Shared<double> sh;
-    __shared__ double sh_y[kTile];
-    __shared__ RedShared shr;
-    TileRows t(N);
-    P::run(rowptr, col, val, t.row0, t.nrows, [&](int c) { return x[c]; }, sh, sh_y);
-    double bv[4], rv[4], sv[4];
-    load4(b, t.i0, N, bv);
-    if (scale) load4(scale, t.i0, N, sv);
-    double leaf[1] = {0.0};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        rv[k] = bv[k] - sh_y[threadIdx.x * kLeaf + k];
-        if (scale) rv[k] = sv[k] * rv[k];
-        if (t.i0 + k < N) leaf[0] = fma(rv[k], rv[k], leaf[0]);
-    }
-    store4(r, t.i0, N, rv);
-    if (r2) store4(r2, t.i0, N, rv);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
-        const double norm = sqrt(v[0]);
-        st->rho0 = v[0];
-        st->rho1 = 0.0;
-        st->norm = norm;
-        st->norm0 = norm;
-        st->k = 0;
-        st->status = kOK;
-        if (st->hist_cap > 0) st->hist[0] = norm;
-        st->done = (done_rule == 0) ? !(norm > st->error) : 0;
-    });
-}
-
-__device__ __forceinline__ void record_norm(KState* st, double norm) {
-    const int k = st->k + 1;
-    st->k = k;
-    st->norm = norm;
-    if (k < st->hist_cap) st->hist[k] = norm;
-}
-
-// ---------------------------------------------------------------------------
 // CG  (SolverSingleGPU.h:366-445), two launches per iteration:
 //   cg1: p_k = r + beta p_{k-1} (k > 1; formed on the fly, also at the gathered
 //        columns), q = A p_k, x += alpha_{k-1} p_{k-1} (deferred), (p_k, q)
@@ -219,10 +107,7 @@ __device__ __forceinline__ void record_norm(KState* st, double norm) {
 // p alternates between two buffers (p_k in buffer k&1).
 // ---------------------------------------------------------------------------
 template <class P>
-__global__ __launch_bounds__(kWG) void k_cg1(const int* __restrict__ rowptr,
-                                             const int* __restrict__ col,
-                                             const double* __restrict__ val, int N,
-                                             const double* __restrict__ r,
+__global__ __launch_bounds__(kWG) void k_cg1(Mat<double> m, int N, const double* __restrict__ r,
                                              double* __restrict__ pb0, double* __restrict__ pb1,
                                              double* __restrict__ x, double* __restrict__ q,
                                              KState* st, RedWs red) {
@@ -238,10 +123,10 @@ __global__ __launch_bounds__(kWG) void k_cg1(const int* __restrict__ rowptr,
     const double* p_old = (k & 1) ? pb0 : pb1;
     double* p_new = (k & 1) ? pb1 : pb0;
     if (first) {
-        P::run(rowptr, col, val, t.row0, t.nrows, [&](int c) { return r[c]; }, sh, sh_y);
+        P::run(m, N, t.row0, t.nrows, [&](int c) { return r[c]; }, sh, sh_y);
     } else {
-        P::run(rowptr, col, val, t.row0, t.nrows,
-               [&](int c) { return fma(beta, p_old[c], r[c]); }, sh, sh_y);
+        P::run(m, N, t.row0, t.nrows, [&](int c) { return fma(beta, p_old[c], r[c]); }, sh,
+               sh_y);
     }
     double rv[4], pv[4], xv[4], pn[4], qv[4];
     load4(r, t.i0, N, rv);
@@ -374,40 +259,32 @@ int read_state(gis_context* h) {
     return kOK;
 }
 
-static int push_state(gis_context* h) {
-    GIS_HIP(h, hipMemcpyAsync(h->st, h->st_host, sizeof(KState), hipMemcpyHostToDevice,
-                              h->stream));
-    return kOK;
-}
-
-static inline dim3 tiles_grid(int N) { return dim3((unsigned)tiles_of(N)); }
-
 int launch_spmv(gis_context* h, const double* x, double* y) {
     if (!h->has_values) return fail(h, kErrState, "values not set");
+    const Mat<double> m = mat64(h);
     GIS_SPMV_DISPATCH(h->sched.spmv64, P,
                       hipLaunchKernelGGL((k_spmv<P, double>), tiles_grid(h->N), dim3(kWG), 0,
-                                         h->stream, h->rowptr.as<int>(), h->col.as<int>(),
-                                         h->val.as<double>(), h->N, x, y));
+                                         h->stream, m, h->N, x, y));
     GIS_HIP(h, hipGetLastError());
     return kOK;
 }
 
 int launch_spmv_t(gis_context* h, const double* x, double* y) {
     GIS_TRY(ensure_valT(h));
+    const Mat<double> m = mat64T(h);
     GIS_SPMV_DISPATCH(h->sched.spmv64, P,
                       hipLaunchKernelGGL((k_spmv<P, double>), tiles_grid(h->N), dim3(kWG), 0,
-                                         h->stream, h->rowptr.as<int>(), h->col.as<int>(),
-                                         h->valT.as<double>(), h->N, x, y));
+                                         h->stream, m, h->N, x, y));
     GIS_HIP(h, hipGetLastError());
     return kOK;
 }
 
 int launch_spmv32(gis_context* h, const float* x, float* y) {
     GIS_TRY(ensure_val32(h));
+    const Mat<float> m = mat32(h);
     GIS_SPMV_DISPATCH(h->sched.spmv32, P,
                       hipLaunchKernelGGL((k_spmv<P, float>), tiles_grid(h->N), dim3(kWG), 0,
-                                         h->stream, h->rowptr.as<int>(), h->col.as<int>(),
-                                         h->val32.as<float>(), h->N, x, y));
+                                         h->stream, m, h->N, x, y));
     GIS_HIP(h, hipGetLastError());
     return kOK;
 }
@@ -430,33 +307,17 @@ int device_norm_factor(gis_context* h, const double* d_b, const double* d_x, dou
     hipLaunchKernelGGL(k_mean, tiles_grid(h->N), dim3(kWG), 0, h->stream, h->N, d_x, h->st,
                        h->red);
     GIS_HIP(h, hipGetLastError());
+    const Mat<double> m = mat64(h);
     GIS_SPMV_DISPATCH(h->sched.spmv64, P,
                       hipLaunchKernelGGL((k_norm_factor<P>), tiles_grid(h->N), dim3(kWG), 0,
-                                         h->stream, h->rowptr.as<int>(), h->col.as<int>(),
-                                         h->val.as<double>(), h->N, d_b, d_x, h->st, h->red));
+                                         h->stream, m, h->N, d_b, d_x, h->st, h->red));
     GIS_HIP(h, hipGetLastError());
     GIS_TRY(read_state(h));
     *nf = h->st_host->aux[1];
     return kOK;
 }
 
-// Enqueue iterations in blocks until the device reports done.
-template <class Iter>
-static int run_blocks(gis_context* h, long long max_steps, Iter&& launch_iteration) {
-    long long launched = 0;
-    int block = 4;
-    for (;;) {
-        for (int i = 0; i < block && launched < max_steps; ++i, ++launched)
-            GIS_TRY(launch_iteration());
-        GIS_HIP(h, hipGetLastError());
-        GIS_TRY(read_state(h));
-        if (h->st_host->done || launched >= max_steps) break;
-        block = std::min(block * 2, std::max(4, h->sched.block_iters));
-    }
-    return kOK;
-}
-
-static int prepare(gis_context* h, const SolveArgs& a, int nvec) {
+int prepare_solve(gis_context* h, const SolveArgs& a, int nvec) {
     if (!h->has_values) return fail(h, kErrState, "values not set");
     GIS_TRY(ensure_reduction(h, h->N));
     GIS_TRY(ensure_state(h));
@@ -470,27 +331,28 @@ static int prepare(gis_context* h, const SolveArgs& a, int nvec) {
     s.int_bound = a.int_bound;
     s.hist = h->hist.as<double>();
     s.hist_cap = cap;
+    s.aux[4] = INFINITY;   // running min for FindDiagAbsMin
     *h->st_host = s;
-    return push_state(h);
+    GIS_HIP(h, hipMemcpyAsync(h->st, h->st_host, sizeof(KState), hipMemcpyHostToDevice,
+                              h->stream));
+    return kOK;
 }
 
-static int solve_cg(gis_context* h, const SolveArgs& a, double* x, const double* b) {
-    GIS_TRY(prepare(h, a, 4));
+int solve_cg(gis_context* h, const SolveArgs& a, double* x, const double* b) {
+    GIS_TRY(prepare_solve(h, a, 4));
     const int N = h->N;
     double* r = h->vec.as<double>();
     double* pb0 = r + N;
     double* pb1 = r + 2 * (size_t)N;
     double* q = r + 3 * (size_t)N;
     const dim3 g = tiles_grid(N);
+    const Mat<double> m = mat64(h);
     GIS_SPMV_DISPATCH(h->sched.spmv64, P, {
-        hipLaunchKernelGGL((k_setup<P>), g, dim3(kWG), 0, h->stream, h->rowptr.as<int>(),
-                           h->col.as<int>(), h->val.as<double>(), N, b, x, nullptr, r, nullptr,
-                           h->st, h->red, 0);
+        hipLaunchKernelGGL((k_setup<P>), g, dim3(kWG), 0, h->stream, m, N, b, x, nullptr, r,
+                           nullptr, h->st, h->red, 0);
         GIS_HIP(h, hipGetLastError());
-        const long long ms = a.max_steps <= 0 ? (long long)INT_MAX : a.max_steps;
-        GIS_TRY(run_blocks(h, ms, [&]() -> int {
-            hipLaunchKernelGGL((k_cg1<P>), g, dim3(kWG), 0, h->stream, h->rowptr.as<int>(),
-                               h->col.as<int>(), h->val.as<double>(), N, r, pb0, pb1, x, q,
+        GIS_TRY(run_blocks(h, h->st_host->max_steps, [&]() -> int {
+            hipLaunchKernelGGL((k_cg1<P>), g, dim3(kWG), 0, h->stream, m, N, r, pb0, pb1, x, q,
                                h->st, h->red);
             hipLaunchKernelGGL(k_cg2, g, dim3(kWG), 0, h->stream, N, r, q, h->st, h->red);
             return kOK;
@@ -498,24 +360,6 @@ static int solve_cg(gis_context* h, const SolveArgs& a, double* x, const double*
     });
     hipLaunchKernelGGL(k_cg_flush, g, dim3(kWG), 0, h->stream, N, x, pb0, pb1, h->st);
     GIS_HIP(h, hipGetLastError());
-    return kOK;
-}
-
-int device_solve(gis_context* h, const SolveArgs& a, double* d_x, const double* d_b,
-                 int* hist_len, double* norm0, double* norm_final) {
-    int st;
-    switch (a.method) {
-        case 0: st = solve_cg(h, a, d_x, d_b); break;
-        default: return fail(h, kErrArg, "unknown solver method");
-    }
-    if (st != kOK) return st;
-    GIS_TRY(read_state(h));
-    const KState& s = *h->st_host;
-    if (hist_len) *hist_len = s.k + 1;
-    if (norm0) *norm0 = s.norm0;
-    if (norm_final) *norm_final = s.norm;
-    if (s.status != kOK) return s.status;
-    if (!(s.norm == s.norm)) return kNaN;
     return kOK;
 }
 

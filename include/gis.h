@@ -114,8 +114,9 @@ GIS_API int gis_spmv_device_async(gis_handle h, const double *d_x, double *d_y);
 GIS_API int gis_spmv32_device_async(gis_handle h, const float *d_x, float *d_y);
 
 /* ---- launch schedule (tuning only: every schedule gives bit-identical results).
- * spmv64 / spmv32: 0 = LDS-staged, 2048 nonzeros per chunk; 1 = LDS-staged,
- * 4096; 2 = unstaged.  block_iters: iterations enqueued between host checks
+ * spmv64 / spmv32: 0 = sliced rows (64-row slices, one row per lane, default);
+ * 1 = CSR with products staged in LDS, 2048 nonzeros per chunk; 2 = the same
+ * with 4096; 3 = CSR unstaged.  block_iters: iterations enqueued between host checks
  * of the convergence flag.  Negative values leave a field unchanged. ---- */
 GIS_API int gis_set_schedule(gis_handle h, int spmv64, int spmv32, int block_iters);
 GIS_API int gis_get_schedule(gis_handle h, int *spmv64, int *spmv32, int *block_iters);

@@ -21,6 +21,7 @@ REF_PATH = os.path.join(HERE, "_ref", "libref_sparsecpu.so")
 METHODS = {"CG": 0, "BiCG": 1, "Jacobi": 2, "MR": 3, "MRJacobi": 4, "JMR": 5, "MRS": 6,
            "Richardson": 7}
 BREAKDOWN = 10
+NAN = 12
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -253,6 +254,8 @@ def solve(method: str, A: Csr, b, x0, error: float, max_steps: int, At: Csr | No
     if st not in (0, BREAKDOWN):
         raise RuntimeError(f"or_solve({method}) failed with status {st}")
     n = min(hl.value, hist_cap)
+    if st == 0 and n == hl.value and n > 0 and np.isnan(hist[n - 1]):
+        st = NAN   # same outcome code as the C ABI (GIS_NAN)
     return SolveResult(x, hist[:n].copy(), hl.value, st, om.value)
 
 

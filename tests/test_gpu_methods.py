@@ -1,0 +1,125 @@
+"""GPU parity of the eight gis methods (BiCG, Jacobi, MR, MRJacobi, JMR, MRS,
+Richardson, CG) against the CPU oracle: residual histories and solutions are
+compared bit for bit, under every SpMV schedule, plus the edge cases the
+reference's loops define (breakdown, NaN propagation, zero iterations)."""
+import numpy as np
+import pytest
+
+from cufoam_amd import mesh
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+METHODS = ["CG", "BiCG", "Jacobi", "MR", "MRJacobi", "JMR", "MRS", "Richardson"]
+SIZES = [(1, 0), (4, 1), (8, 3), (16, 2), (32, 0)]
+
+
+def _case(n, seed, symmetric, shift):
+    print(f"[case] n={n} seed={seed} symmetric={symmetric} shift={shift}", flush=True)
+    return mesh.laplacian(n, seed=seed, shift=shift, symmetric=symmetric)
+
+
+def _check(gpu, method, s, x0, error, max_steps, A=None):
+    A = A if A is not None else O.ldu2csr(s)
+    ref = O.solve(method, A, s.b, x0, error, max_steps)
+    x, hist, st = gpu.solve(method, s.b, x0, error=error, max_steps=max_steps,
+                            hist_cap=ref.hist_len + 4)
+    assert st == ref.status, (method, st, ref.status)
+    assert len(hist) == ref.hist_len, (method, len(hist), ref.hist_len)
+    np.testing.assert_array_equal(hist, ref.hist)
+    np.testing.assert_array_equal(x, ref.x)
+    return ref
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("n,seed", SIZES)
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_method_bit_exact_fixed_steps(gpu, method, n, seed, symmetric):
+    s = _case(n, seed, symmetric, 0.1 if not symmetric else 1e-2)
+    gpu.set_system(s)
+    x0 = np.random.default_rng(seed).uniform(-1, 1, s.n_cells) if seed % 2 else s.x0
+    _check(gpu, method, s, x0, 0.0, 25)
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_method_bit_exact_to_convergence(gpu, method):
+    s = _case(16, 4, method in ("CG", "Richardson"), 0.5)
+    gpu.set_system(s)
+    err = 1e-6 * float(np.linalg.norm(s.b))
+    ref = _check(gpu, method, s, s.x0, err, 5000)
+    assert ref.hist[-1] < err or ref.hist[-1] <= err
+
+
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3])
+@pytest.mark.parametrize("method", ["CG", "BiCG", "MRJacobi", "Richardson"])
+def test_schedules_are_bit_identical(gpu, schedule, method):
+    s = _case(32, 6, method in ("CG", "Richardson"), 0.05)
+    gpu.set_system(s)
+    gpu.set_schedule(schedule, schedule, -1)
+    try:
+        _check(gpu, method, s, s.x0, 0.0, 40)
+    finally:
+        gpu.set_schedule(0, 0, -1)
+
+
+def test_bicg_breakdown_forced_from_host(gpu):
+    # A = 1e11 I: xi = 1e-11 < int_bound 1e-10 at the first step (SolverSingleGPU.h:629-630)
+    N = 64
+    s = mesh.LduSystem(N, np.zeros(0, np.int32), np.zeros(0, np.int32), np.full(N, 1e11),
+                       np.zeros(0), np.zeros(0), np.random.default_rng(0).uniform(-1, 1, N),
+                       np.zeros(N))
+    gpu.set_system(s)
+    ref = _check(gpu, "BiCG", s, s.x0, 1e-30, 10)
+    assert ref.status == O.BREAKDOWN and ref.hist_len == 1
+
+
+def test_bicg_zero_rhs_propagates_nan_like_reference(gpu):
+    # r0 = 0 -> rho0 = 0 -> xi = 0/0 = NaN, not caught by the breakdown test
+    s = _case(8, 0, False, 0.1)
+    s.b = np.zeros(s.n_cells)
+    gpu.set_system(s)
+    A = O.ldu2csr(s)
+    ref = O.solve("BiCG", A, s.b, s.x0, 1e-12, 3)
+    x, hist, st = gpu.solve("BiCG", s.b, s.x0, error=1e-12, max_steps=3, hist_cap=8)
+    assert len(hist) == ref.hist_len == 4
+    np.testing.assert_array_equal(hist, ref.hist)
+    assert np.isnan(x).all() and np.isnan(ref.x).all()
+
+
+@pytest.mark.parametrize("method", ["CG", "Jacobi", "MR", "Richardson"])
+def test_zero_iterations_when_initial_residual_small(gpu, method):
+    s = _case(8, 1, True, 1e-2)
+    gpu.set_system(s)
+    _check(gpu, method, s, s.x0, 1e30, 100)
+
+
+@pytest.mark.parametrize("name", ["cuFoamCG", "cuFoamBiCG", "cuFoamJacobi", "cuFoamMR",
+                                  "cuFoamMRJacobi", "cuFoamJMR", "cuFoamMRS",
+                                  "cuFoamRichardson"])
+def test_foam_wrappers_bit_exact(gpu, name):
+    from cufoam_amd import foam
+    sym = name in ("cuFoamCG", "cuFoamRichardson")
+    s = _case(16, 3, sym, 0.2)
+    m = foam.lduMatrix.from_system(s)
+    psi = np.random.default_rng(3).uniform(-1, 1, s.n_cells)
+    psi_ref = psi.copy()
+    solver = foam.New(name, "U", m, {"tolerance": 1e-9, "relTol": 0.1, "maxIter": 300},
+                      gis=gpu)
+    perf = solver.solve(psi, s.b)
+    ref_x, ref = O.foam_solve(name[len("cuFoam"):], s, psi_ref, s.b, 1e-9, 300)
+    assert perf.n_iterations == ref.n_iterations
+    assert perf.initial_residual == ref.initial_residual
+    assert perf.final_residual == ref.final_residual
+    np.testing.assert_array_equal(psi, ref_x)
+
+
+def test_foam_selection_rules():
+    from cufoam_amd import foam
+    s = _case(4, 0, False, 0.1)
+    m = foam.lduMatrix.from_system(s)
+    with pytest.raises(foam.FatalError):
+        foam.New("cuFoamCG", "p", m, {})          # symmetric-only (cuFoamCG.C:14-15)
+    with pytest.raises(foam.FatalError):
+        foam.New("cuFoamRichardson", "p", m, {})  # symmetric-only (cuFoamRichardson.C:11-12)
+    with pytest.raises(foam.FatalError):
+        foam.New("cuFoamGAMG", "p", m, {})
