@@ -1,0 +1,284 @@
+#!/usr/bin/env python3
+"""Benchmark: CG time to 1e-10 relative residual on the 256^3 Poisson system,
+FP64 vs mixed precision, and CSR SpMV HBM bandwidth, on one MI355X.
+
+A step is one complete solve from x0 = 0 of the 256^3 7-point FV Laplacian
+(shift 1e-4, seed 0, b ~ U[-1,1]) until ||b - A x||_2 / ||b||_2 <= 1e-10, with
+matrix and vectors already resident in HBM.  `value` is the mixed-precision
+solve time (FP32 Krylov loop, FP64 residual/correction/norms); the plain FP64
+CG time of the same solve is reported beside it.  The roofline entry is for
+the dominant kernel of the mixed solve (the fused FP32 SpMV + direction +
+correction kernel), timed live with HIP events on its launch stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CG time-to-1e-10 rel resid, 256^3 Poisson: FP64 vs mixed-prec; SpMV HBM GB/s"
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(N, nnz, sv, sx):
+    """Algorithmic bytes of one CSR SpMV: values + int32 columns + row offsets + x read + y write."""
+    return nnz * (sv + 4) + 4 * (N + 1) + 2 * N * sx
+
+
+def mx1_bytes(N, nnz):
+    """Algorithmic bytes of the fused FP32 iteration kernel: FP32 matrix (values + columns
+    + row offsets) and six FP32 vector passes (read r, p_{k-1}, xc; write p_k, q, xc)."""
+    return nnz * (4 + 4) + 4 * (N + 1) + 6 * N * 4
+
+
+def cpu_baseline(sys_, iters_fp64: int, sample_iters: int):
+    """The oracle's FP64 CG (the same algorithm, bit-identical iterates) on the host:
+    a bounded sample of `sample_iters` iterations, extrapolated to the iteration count
+    the solve needs."""
+    import numpy as np
+    from oracle import oracle as O
+    A = O.ldu2csr(sys_)
+    t0 = time.perf_counter()
+    O.solve("CG", A, sys_.b, sys_.x0, 0.0, sample_iters, hist_cap=sample_iters + 1)
+    dt = time.perf_counter() - t0
+    per_iter = dt / sample_iters
+    return {
+        "value": per_iter * iters_fp64,
+        "unit": "s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{sample_iters} FP64 CG iterations of the same 256^3 system with the C "
+                  f"oracle on 1 host thread ({dt:.1f} s incl. set-up), extrapolated to the "
+                  f"{iters_fp64} iterations the FP64 solve needs",
+        "per_iteration_s": per_iter,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256, help="mesh edge (256 = the named workload)")
+    ap.add_argument("--mode", type=int, default=0, help="mixed mode: 0 reliable updates, 1 restart")
+    ap.add_argument("--inner", type=float, default=1e-2,
+                    help="FP64 update trigger: FP32 residual below inner * last true residual")
+    ap.add_argument("--max-iter", type=int, default=100000,
+                    help="iteration cap per solve (profiling runs use a small cap)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 256 MiB device copy (counter calibration in PMC runs)")
+    ap.add_argument("--no-literal", dest="literal", action="store_false",
+                    help="skip the extra restarted-refinement solve with inner relTol 1e-4")
+    ap.add_argument("--cpu-sample", type=int, default=30, help="oracle CG iterations timed")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the solver runs "
+              "on a single MI355X; see the sharding notes). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from cufoam_amd import mesh
+    from cufoam_amd.gis import Gis
+
+    n = args.n
+    t0 = time.perf_counter()
+    s = mesh.laplacian(n, seed=0, shift=1e-4, symmetric=True)
+    log(f"[bench] generated {n}^3: N={s.n_cells} nnz={s.nnz} in {time.perf_counter()-t0:.1f}s")
+    N, nnz = s.n_cells, s.nnz
+
+    g = Gis(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.set_pattern(N, s.lower_addr, s.upper_addr)
+    t_pattern = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g.set_values(s.diag, s.lower, s.upper)
+    t_values = time.perf_counter() - t0
+    b = torch.from_numpy(s.b).cuda()
+    bnorm = float(np.linalg.norm(s.b))
+    max_iter = args.max_iter
+    if args.calib_copy:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    K, W = args.steps, args.warmup
+    xs = [torch.zeros_like(b) for _ in range(K)]
+
+    # ---- mixed precision (timed region; the dominant kernel is timed live) ---
+    for _ in range(W):
+        g.solve_mixed(b, torch.zeros_like(b), 1e-10, args.inner, max_iter, args.mode)
+    torch.cuda.synchronize()
+    g.set_timing("iter_spmv")
+    t0 = time.perf_counter()
+    res_m = [g.solve_mixed(b, xs[i], 1e-10, args.inner, max_iter, args.mode) for i in range(K)]
+    torch.cuda.synchronize()
+    t_mixed = (time.perf_counter() - t0) / K
+    mx1_ms, mx1_n = g.get_timing("iter_spmv")
+    g.set_timing()
+    _, hist_m, it_m, rel_m, st_m = res_m[-1]
+    log(f"[bench] mixed: {t_mixed:.3f}s/solve inner_iters={it_m} updates={len(hist_m)-1} "
+        f"rel={rel_m:.3e} status={st_m}")
+
+    # ---- plain FP64 CG (timed region) ------------------------------------------
+    for _ in range(W):
+        g.solve("CG", b, torch.zeros_like(b), error=1e-10 * bnorm, max_steps=max_iter,
+                hist_cap=max_iter + 1)
+    torch.cuda.synchronize()
+    g.set_timing("iter_spmv")
+    xd = [torch.zeros_like(b) for _ in range(K)]
+    t0 = time.perf_counter()
+    res_d = [g.solve("CG", b, xd[i], error=1e-10 * bnorm, max_steps=max_iter,
+                     hist_cap=max_iter + 1) for i in range(K)]
+    torch.cuda.synchronize()
+    t_fp64 = (time.perf_counter() - t0) / K
+    cg1_ms, cg1_n = g.get_timing("iter_spmv")
+    g.set_timing()
+    _, hist_d, st_d = res_d[-1]
+    it_d = len(hist_d) - 1
+    log(f"[bench] fp64: {t_fp64:.3f}s/solve iters={it_d} status={st_d}")
+
+    # ---- per-kernel breakdown: one extra solve of each with every slot timed ----
+    slots = ("iter_spmv", "iter_update", "fp64_update")
+    g.set_timing(*slots)
+    g.solve_mixed(b, torch.zeros_like(b), 1e-10, args.inner, max_iter, args.mode)
+    brk_m = {s_: g.get_timing(s_) for s_ in slots}
+    g.set_timing(*slots)
+    g.solve("CG", b, torch.zeros_like(b), error=1e-10 * bnorm, max_steps=max_iter,
+            hist_cap=max_iter + 1)
+    brk_d = {s_: g.get_timing(s_) for s_ in slots[:2]}
+    g.set_timing()
+
+    def avg_us(t):
+        return round(t[0] / max(t[1], 1) * 1e3, 2)
+
+    # true residuals and mixed-vs-FP64 error, recomputed in FP64 on the device
+    r = torch.empty_like(b)
+    g.spmv(xs[-1], r)
+    rel_true_m = float(torch.linalg.vector_norm(b - r)) / bnorm
+    g.spmv(xd[-1], r)
+    rel_true_d = float(torch.linalg.vector_norm(b - r)) / bnorm
+    err_m = float(torch.linalg.vector_norm(xs[-1] - xd[-1]) / torch.linalg.vector_norm(xd[-1]))
+
+    # ---- standalone SpMV bandwidth (sliced schedule, FP64 and FP32) --------
+    x64 = torch.randn(N, dtype=torch.float64, device="cuda")
+    y64 = torch.empty_like(x64)
+    x32, y32 = x64.float(), torch.empty(N, dtype=torch.float32, device="cuda")
+    spmv = {}
+    for name, xx, yy, sv in (("fp64", x64, y64, 8), ("fp32", x32, y32, 4)):
+        for _ in range(3):
+            g.spmv_async(xx, yy)
+        g.synchronize()
+        g.set_timing("spmv")
+        for _ in range(20):
+            g.spmv_async(xx, yy)
+        ms, cnt = g.get_timing("spmv")
+        g.set_timing()
+        us = ms / cnt * 1e3
+        gbs = spmv_bytes(N, nnz, sv, sv) / (us * 1e-6) / 1e9
+        spmv[name] = {"us": round(us, 2), "GB/s": round(gbs, 1),
+                      "frac": round(gbs / PEAK_HBM_GBS, 4),
+                      "GFLOP/s": round(2 * nnz / (us * 1e-6) / 1e9, 1)}
+        log(f"[bench] spmv {name}: {us:.1f} us {gbs:.0f} GB/s")
+
+    # ---- roofline of the dominant kernel (fused FP32 iteration kernel) -----
+    mx1_avg_s = mx1_ms / max(mx1_n, 1) / 1e3
+    ach = mx1_bytes(N, nnz) / mx1_avg_s / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_poisson256.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get("k_mx1_bytes_per_launch")
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic,
+                "kernel": "k_mx1 (FP32 SpMV + direction + deferred correction)",
+                "bytes_per_launch": mx1_bytes(N, nnz),
+                "avg_launch_us": round(mx1_avg_s * 1e6, 2), "launches": mx1_n}
+
+    literal = None
+    if args.literal:
+        x = torch.zeros_like(b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, hl, itl, rell, stl = g.solve_mixed(b, x, 1e-10, 1e-4, max_iter, 1)
+        torch.cuda.synchronize()
+        literal = {"mode": "restarted refinement", "inner_rel_tol": 1e-4,
+                   "s": round(time.perf_counter() - t0, 5), "iterations": itl,
+                   "fp64_updates": len(hl) - 1, "rel_residual": rell, "status": stl}
+        log(f"[bench] literal restarted refinement: {literal}")
+
+    cpu = None
+    if not args.no_cpu and n == 256:
+        try:
+            cpu = cpu_baseline(s, it_d, args.cpu_sample)
+            log(f"[bench] cpu baseline: {cpu['value']:.1f}s extrapolated")
+        except Exception as e:  # the oracle is optional at run time
+            log(f"[bench] cpu baseline skipped: {e}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(t_mixed, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(t_mixed * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 outer",
+        "data": "synthetic",
+        "config": {"workload": f"poisson{n}: {n}^3 7-point FV Laplacian in OpenFOAM LDU "
+                               f"layout, off-diagonals -(1+0.5u), shift 1e-4, seed 0, "
+                               f"b~U[-1,1], x0=0, CG to ||b-Ax||/||b|| <= 1e-10",
+                   "n_cells": N, "nnz": nnz,
+                   "mixed_mode": "reliable updates" if args.mode == 0 else "restarted refinement",
+                   "inner_rel_tol": args.inner},
+        "fp64_cg_s": round(t_fp64, 5),
+        "speedup_mixed_vs_fp64": round(t_fp64 / t_mixed, 4),
+        "iterations_fp64": it_d,
+        "iterations_mixed": it_m,
+        "fp64_updates_mixed": len(hist_m) - 1,
+        "rel_residual_mixed": rel_true_m,
+        "rel_residual_fp64": rel_true_d,
+        "rel_error_mixed_vs_fp64": err_m,
+        "fp64_iteration_us": round(t_fp64 / max(it_d, 1) * 1e6, 2),
+        "mixed_iteration_us": round(t_mixed / max(it_m, 1) * 1e6, 2),
+        "kernel_avg_us": {"k_mx1": avg_us((mx1_ms, mx1_n)), "k_cg1": avg_us((cg1_ms, cg1_n)),
+                          "instrumented_solve": {
+                              "k_mx1": avg_us(brk_m["iter_spmv"]),
+                              "k_mx2": avg_us(brk_m["iter_update"]),
+                              "k_mxu": avg_us(brk_m["fp64_update"]),
+                              "k_cg1": avg_us(brk_d["iter_spmv"]),
+                              "k_cg2": avg_us(brk_d["iter_update"])}},
+        "spmv": spmv,
+        "setup_s": {"pattern": round(t_pattern, 4), "values": round(t_values, 4)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if literal is not None:
+        out["mixed_restarted_inner_1e-4"] = literal
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -154,6 +154,7 @@ GIS_API int gis_destroy(gis_handle h) {
         (void)hipStreamSynchronize(h->stream);
         if (h->st_host) (void)hipHostFree(h->st_host);
         h->st_host = nullptr;
+        timer_release(h);
         if (h->own_stream) (void)hipStreamDestroy(h->stream);
         delete h;   // DevMem destructors free device memory with the device current
     }
@@ -298,13 +299,40 @@ GIS_API int gis_spmv_device(gis_handle h, const double* d_x, double* d_y) {
 GIS_API int gis_spmv_device_async(gis_handle h, const double* d_x, double* d_y) {
     if (!h || !d_x || !d_y) return GIS_ERR_ARG;
     DeviceGuard g(h->device);
-    return launch_spmv(h, d_x, d_y);
+    GIS_TRY(timer_begin(h, 3));
+    GIS_TRY(launch_spmv(h, d_x, d_y));
+    return timer_end(h);
 }
 
 GIS_API int gis_spmv32_device_async(gis_handle h, const float* d_x, float* d_y) {
     if (!h || !d_x || !d_y) return GIS_ERR_ARG;
     DeviceGuard g(h->device);
-    return launch_spmv32(h, d_x, d_y);
+    GIS_TRY(timer_begin(h, 3));
+    GIS_TRY(launch_spmv32(h, d_x, d_y));
+    return timer_end(h);
+}
+
+GIS_API int gis_set_timing(gis_handle h, int slot_mask) {
+    if (!h) return GIS_ERR_ARG;
+    DeviceGuard g(h->device);
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    GIS_TRY(timer_collect(h));
+    h->timer.mask = (unsigned)slot_mask & ((1u << kTimerSlots) - 1);
+    for (int i = 0; i < kTimerSlots; ++i) {
+        h->timer.total_ms[i] = 0.0;
+        h->timer.count[i] = 0;
+    }
+    return GIS_OK;
+}
+
+GIS_API int gis_get_timing(gis_handle h, int slot, double* total_ms, long long* count) {
+    if (!h || slot < 0 || slot >= kTimerSlots) return GIS_ERR_ARG;
+    DeviceGuard g(h->device);
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    GIS_TRY(timer_collect(h));
+    if (total_ms) *total_ms = h->timer.total_ms[slot];
+    if (count) *count = h->timer.count[slot];
+    return GIS_OK;
 }
 
 GIS_API int gis_spmv_transpose_device(gis_handle h, const double* d_x, double* d_y) {
@@ -383,6 +411,57 @@ GIS_API int gis_solve_device(gis_handle h, int method, double* d_x, const double
                              const gis_params* params, double* hist, int hist_cap,
                              int* hist_len) {
     return solve_impl(h, method, d_x, d_b, params, hist, hist_cap, hist_len, true);
+}
+
+static int solve_mixed_impl(gis_handle h, double* x, const double* b, double rel_tol,
+                            double inner_rel_tol, long long max_iter, int mode, double* hist,
+                            int hist_cap, int* hist_len, int* iterations, double* rel_residual,
+                            bool dev) {
+    if (!h || !x || !b) return GIS_ERR_ARG;
+    if (mode != 0 && mode != 1) return fail(h, GIS_ERR_ARG, "mixed mode must be 0 or 1");
+    if (!(rel_tol >= 0.0) || !(inner_rel_tol >= 0.0))
+        return fail(h, GIS_ERR_ARG, "tolerances must be >= 0");
+    if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
+    DeviceGuard g(h->device);
+    const size_t n = (size_t)h->N;
+    GIS_HIP(h, h->io.ensure(sizeof(double) * 2 * n));
+    double* dx = h->io.as<double>();
+    double* db = dx + n;
+    const hipMemcpyKind kin = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind kout = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    GIS_HIP(h, hipMemcpyAsync(dx, x, sizeof(double) * n, kin, h->stream));
+    GIS_HIP(h, hipMemcpyAsync(db, b, sizeof(double) * n, kin, h->stream));
+    MixedArgs a{rel_tol, inner_rel_tol, max_iter, mode, std::max(1, hist_cap)};
+    int len = 0, it = 0;
+    double rr = 0.0;
+    const int st = device_solve_mixed(h, a, dx, db, &len, &it, &rr);
+    if (st != GIS_OK && st < GIS_BREAKDOWN) return st;
+    GIS_HIP(h, hipMemcpyAsync(x, dx, sizeof(double) * n, kout, h->stream));
+    if (hist && hist_cap > 0) {
+        const int m = std::min(len, hist_cap);
+        GIS_HIP(h, hipMemcpyAsync(hist, h->hist.p, sizeof(double) * (size_t)m,
+                                  hipMemcpyDeviceToHost, h->stream));
+    }
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    if (hist_len) *hist_len = len;
+    if (iterations) *iterations = it;
+    if (rel_residual) *rel_residual = rr;
+    return st;
+}
+
+GIS_API int gis_solve_mixed(gis_handle h, double* x, const double* b, double rel_tol,
+                            double inner_rel_tol, long long max_iter, int mode, double* hist,
+                            int hist_cap, int* hist_len, int* iterations, double* rel_residual) {
+    return solve_mixed_impl(h, x, b, rel_tol, inner_rel_tol, max_iter, mode, hist, hist_cap,
+                            hist_len, iterations, rel_residual, false);
+}
+
+GIS_API int gis_solve_mixed_device(gis_handle h, double* d_x, const double* d_b,
+                                   double rel_tol, double inner_rel_tol, long long max_iter,
+                                   int mode, double* hist, int hist_cap, int* hist_len,
+                                   int* iterations, double* rel_residual) {
+    return solve_mixed_impl(h, d_x, d_b, rel_tol, inner_rel_tol, max_iter, mode, hist,
+                            hist_cap, hist_len, iterations, rel_residual, true);
 }
 
 GIS_API int gis_foam_solve(gis_handle h, int method, double* psi, const double* source,

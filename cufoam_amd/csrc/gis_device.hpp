@@ -47,7 +47,7 @@ struct KState {
     int status;                       // solve outcome code
     int hist_cap;
     int flag;                         // per-solver control word
-    int pad;
+    int nhist;                        // entries pushed by solvers with sparse histories
 };
 
 // ---------------------------------------------------------------------------

@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "gis_device.hpp"
 
@@ -49,6 +50,21 @@ struct Schedule {
     int block_iters = 64;                   // max iterations launched between host checks
 };
 
+// Opt-in kernel timing with HIP events on the launch stream.  Slot 0: the
+// SpMV-carrying kernel of each iteration; 1: the vector-update kernel of each
+// iteration; 2: FP64 update kernel of the mixed solver; 3: standalone SpMV.
+constexpr int kTimerSlots = 4;
+
+struct Timer {
+    unsigned mask = 0;                      // bit s: time slot s
+    bool open = false;                      // a start event awaits its end
+    std::vector<hipEvent_t> ev;             // pairs (start, end)
+    std::vector<int> tag;                   // slot of each pair
+    int used = 0;                           // pairs recorded since the last collect
+    double total_ms[kTimerSlots] = {};
+    long long count[kTimerSlots] = {};
+};
+
 }  // namespace gis
 
 struct gis_context {
@@ -86,6 +102,7 @@ struct gis_context {
     gis::DevMem scratch;                        // pattern-build temporaries
 
     gis::Schedule sched;
+    gis::Timer timer;
     std::string err;
 };
 
@@ -119,7 +136,13 @@ int hip_fail(gis_context* h, hipError_t e, const char* what);
         if (_s != ::gis::kOK) return _s; \
     } while (0)
 
-inline int tiles_of(long long n) { return (int)((n + kTile - 1) / kTile); }
+__host__ __device__ inline int tiles_of(long long n) { return (int)((n + kTile - 1) / kTile); }
+
+// kernel timing (gis_solvers.hip)
+int timer_begin(gis_context* h, int slot);    // records the start event of a pair
+int timer_end(gis_context* h);                // records the end event of the open pair
+int timer_collect(gis_context* h);            // after a host sync: accumulate the pairs
+void timer_release(gis_context* h);
 
 // reductions / state
 int ensure_reduction(gis_context* h, int rows);

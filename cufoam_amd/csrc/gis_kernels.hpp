@@ -73,11 +73,12 @@ struct Direct {
         } break;                                      \
     }
 
-// row range of this workgroup's tile
+// row range of a workgroup tile (by default the tile of this workgroup)
 struct TileRows {
     int tile, row0, nrows, i0;   // i0: first of this thread's kLeaf rows
-    __device__ TileRows(int N) {
-        tile = blockIdx.x;
+    __device__ TileRows(int N) : TileRows(N, (int)blockIdx.x) {}
+    __device__ TileRows(int N, int tile_) {
+        tile = tile_;
         row0 = tile * kTile;
         nrows = min(kTile, N - row0);
         i0 = row0 + threadIdx.x * kLeaf;

@@ -256,7 +256,57 @@ int read_state(gis_context* h) {
     GIS_HIP(h, hipMemcpyAsync(h->st_host, h->st, sizeof(KState), hipMemcpyDeviceToHost,
                               h->stream));
     GIS_HIP(h, hipStreamSynchronize(h->stream));
+    return timer_collect(h);
+}
+
+// ---------------------------------------------------------------------------
+// kernel timing
+// ---------------------------------------------------------------------------
+int timer_begin(gis_context* h, int slot) {
+    Timer& t = h->timer;
+    t.open = false;
+    if (!(t.mask & (1u << slot))) return kOK;
+    if ((size_t)(2 * t.used + 2) > t.ev.size()) {
+        for (int i = 0; i < 2; ++i) {
+            hipEvent_t e;
+            GIS_HIP(h, hipEventCreateWithFlags(&e, hipEventDefault));
+            t.ev.push_back(e);
+        }
+        t.tag.push_back(0);
+    }
+    t.tag[t.used] = slot;
+    GIS_HIP(h, hipEventRecord(t.ev[2 * t.used], h->stream));
+    t.open = true;
     return kOK;
+}
+
+int timer_end(gis_context* h) {
+    Timer& t = h->timer;
+    if (!t.open) return kOK;
+    GIS_HIP(h, hipEventRecord(t.ev[2 * t.used + 1], h->stream));
+    t.used++;
+    t.open = false;
+    return kOK;
+}
+
+int timer_collect(gis_context* h) {
+    Timer& t = h->timer;
+    for (int i = 0; i < t.used; ++i) {
+        GIS_HIP(h, hipEventSynchronize(t.ev[2 * i + 1]));
+        float ms = 0.0f;
+        GIS_HIP(h, hipEventElapsedTime(&ms, t.ev[2 * i], t.ev[2 * i + 1]));
+        t.total_ms[t.tag[i]] += ms;
+        t.count[t.tag[i]] += 1;
+    }
+    t.used = 0;
+    return kOK;
+}
+
+void timer_release(gis_context* h) {
+    for (hipEvent_t e : h->timer.ev) (void)hipEventDestroy(e);
+    h->timer.ev.clear();
+    h->timer.tag.clear();
+    h->timer.used = 0;
 }
 
 int launch_spmv(gis_context* h, const double* x, double* y) {
@@ -352,9 +402,13 @@ int solve_cg(gis_context* h, const SolveArgs& a, double* x, const double* b) {
                            nullptr, h->st, h->red, 0);
         GIS_HIP(h, hipGetLastError());
         GIS_TRY(run_blocks(h, h->st_host->max_steps, [&]() -> int {
+            GIS_TRY(timer_begin(h, 0));
             hipLaunchKernelGGL((k_cg1<P>), g, dim3(kWG), 0, h->stream, m, N, r, pb0, pb1, x, q,
                                h->st, h->red);
+            GIS_TRY(timer_end(h));
+            GIS_TRY(timer_begin(h, 1));
             hipLaunchKernelGGL(k_cg2, g, dim3(kWG), 0, h->stream, N, r, q, h->st, h->red);
+            GIS_TRY(timer_end(h));
             return kOK;
         }));
     });

@@ -67,11 +67,15 @@ SIGNATURES = {
     "gis_spmv32_device_async": (_i, [_vp, _vp, _vp]),
     "gis_set_schedule": (_i, [_vp, _i, _i, _i]),
     "gis_get_schedule": (_i, [_vp, _ip, _ip, _ip]),
+    "gis_set_timing": (_i, [_vp, _i]),
+    "gis_get_timing": (_i, [_vp, _i, _dp, _llp]),
     "gis_norm_factor": (_i, [_vp, _vp, _vp, _dp]),
     "gis_norm_factor_device": (_i, [_vp, _vp, _vp, _dp]),
     "gis_solve": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
     "gis_solve_device": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
     "gis_foam_solve": (_i, [_vp, _i, _vp, _vp, _d, _ll, C.POINTER(_Perf)]),
+    "gis_solve_mixed": (_i, [_vp, _vp, _vp, _d, _d, _ll, _i, _vp, _i, _ip, _ip, _dp]),
+    "gis_solve_mixed_device": (_i, [_vp, _vp, _vp, _d, _d, _ll, _i, _vp, _i, _ip, _ip, _dp]),
 }
 
 
@@ -262,6 +266,23 @@ class Gis:
                     "get_schedule")
         return a.value, b.value, c.value
 
+    TIMER_SLOTS = {"iter_spmv": 0, "iter_update": 1, "fp64_update": 2, "spmv": 3}
+
+    def set_timing(self, *slots):
+        """Record HIP-event durations of the named slots (none: stop timing)."""
+        mask = 0
+        for s in slots:
+            mask |= 1 << (self.TIMER_SLOTS[s] if isinstance(s, str) else int(s))
+        self._check(self._L.gis_set_timing(self._h, mask), "set_timing")
+
+    def get_timing(self, slot) -> tuple[float, int]:
+        """(total milliseconds, launches) recorded for a timer slot since set_timing."""
+        s = self.TIMER_SLOTS[slot] if isinstance(slot, str) else int(slot)
+        ms, cnt = C.c_double(0.0), C.c_longlong(0)
+        self._check(self._L.gis_get_timing(self._h, s, C.byref(ms), C.byref(cnt)),
+                    "get_timing")
+        return ms.value, cnt.value
+
     def synchronize(self):
         self._check(self._L.gis_synchronize(self._h), "synchronize")
 
@@ -292,6 +313,27 @@ class Gis:
                                    hist_cap, C.byref(hl))
         self._check(st, f"solve({method})")
         return x, hist[:min(hl.value, hist_cap)].copy(), st
+
+    def solve_mixed(self, b, x0=None, rel_tol: float = 1e-10, inner_rel_tol: float = 1e-4,
+                    max_iter: int = 0, mode: int = 0, hist_cap: int = 256):
+        """Mixed-precision CG -> (x, true-residual history, inner iterations,
+        relative residual, status).  Device tensors are updated in place."""
+        hist = np.zeros(max(hist_cap, 1))
+        hl, it, rr = C.c_int(0), C.c_int(0), C.c_double(0.0)
+        if _is_dev(b):
+            import torch
+            x = x0 if x0 is not None else torch.zeros_like(b)
+            st = self._L.gis_solve_mixed_device(self._h, _ptr(x), _ptr(b), rel_tol,
+                                                inner_rel_tol, max_iter, mode, _ptr(hist),
+                                                hist_cap, C.byref(hl), C.byref(it), C.byref(rr))
+        else:
+            b = _host(b, np.float64)
+            x = np.array(x0 if x0 is not None else np.zeros_like(b), dtype=np.float64, copy=True)
+            st = self._L.gis_solve_mixed(self._h, _ptr(x), _ptr(b), rel_tol, inner_rel_tol,
+                                         max_iter, mode, _ptr(hist), hist_cap, C.byref(hl),
+                                         C.byref(it), C.byref(rr))
+        self._check(st, "solve_mixed")
+        return x, hist[:min(hl.value, hist_cap)].copy(), it.value, rr.value, st
 
     def foam_solve(self, method: str, psi, source, tolerance: float, max_iter: int):
         """cuFoam<method>::solve: psi (numpy) is updated in place; returns Performance."""

@@ -121,6 +121,16 @@ GIS_API int gis_spmv32_device_async(gis_handle h, const float *d_x, float *d_y);
 GIS_API int gis_set_schedule(gis_handle h, int spmv64, int spmv32, int block_iters);
 GIS_API int gis_get_schedule(gis_handle h, int *spmv64, int *spmv32, int *block_iters);
 
+/* ---- kernel timing with HIP events on the handle stream (opt-in; the
+ * DetailType::Time analogue of support.h:7-9 at kernel granularity).
+ * gis_set_timing(h, mask) records the slots whose bit is set in mask (0 stops)
+ * and zeroes the counters.  Slots: 0 = the SpMV-carrying kernel of each solver
+ * iteration, 1 = the vector-update kernel of each iteration, 2 = the FP64
+ * update kernel of gis_solve_mixed, 3 = gis_spmv*_device_async.  Each timed
+ * launch adds two event records to the stream. ---- */
+GIS_API int gis_set_timing(gis_handle h, int slot_mask);
+GIS_API int gis_get_timing(gis_handle h, int slot, double *total_ms, long long *count);
+
 /* ---- NormFactor (gis/gis.cu:106-123) ---- */
 GIS_API int gis_norm_factor(gis_handle h, const double *b, const double *x, double *nf);
 GIS_API int gis_norm_factor_device(gis_handle h, const double *d_b, const double *d_x,
@@ -141,6 +151,25 @@ GIS_API int gis_solve_device(gis_handle h, int method, double *d_x, const double
  * steps, MRS invM = D^-1); psi is updated in place. ---- */
 GIS_API int gis_foam_solve(gis_handle h, int method, double *psi, const double *source,
                            double tolerance, long long max_iter, gis_performance *perf);
+
+/* ---- mixed precision (no counterpart in the reference; for callers that want
+ * an FP64-accurate CG solve at FP32 bandwidth): the Krylov inner loop runs in
+ * FP32, FP64 is kept for the residual recompute r = b - A x, the correction
+ * accumulation x += d and the norms, until ||b - A x||_2 / ||b||_2 <= rel_tol.
+ * mode 0: reliable updates (search direction kept across FP64 updates);
+ * mode 1: restarted iterative refinement (inner CG to inner_rel_tol, restart).
+ * An FP64 update is triggered when the FP32 residual falls below
+ * inner_rel_tol times the last true residual.  max_iter caps the inner
+ * iterations (<= 0: unbounded).  hist gets the FP64 true residual norm at x0
+ * and after every update.  Returns GIS_OK when converged, GIS_NOT_CONVERGED
+ * or GIS_NAN otherwise (x is valid in every case). ---- */
+GIS_API int gis_solve_mixed(gis_handle h, double *x, const double *b, double rel_tol,
+                            double inner_rel_tol, long long max_iter, int mode, double *hist,
+                            int hist_cap, int *hist_len, int *iterations, double *rel_residual);
+GIS_API int gis_solve_mixed_device(gis_handle h, double *d_x, const double *d_b,
+                                   double rel_tol, double inner_rel_tol, long long max_iter,
+                                   int mode, double *hist, int hist_cap, int *hist_len,
+                                   int *iterations, double *rel_residual);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,64 @@
+"""GPU parity of the mixed-precision CG (FP32 Krylov loop, FP64 residual,
+correction and norms) against the oracle's or_mixed_cg, and its convergence
+contract at the workload sizes (checked by an independent FP64 residual)."""
+import numpy as np
+import pytest
+
+from cufoam_amd import mesh
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(n, seed, shift):
+    print(f"[case] n={n} seed={seed} shift={shift}", flush=True)
+    return mesh.laplacian(n, seed=seed, shift=shift, symmetric=True)
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (4, 1), (8, 2), (16, 0), (32, 3)])
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("inner", [1e-4, 1e-2])
+def test_mixed_bit_exact(gpu, n, seed, mode, inner):
+    s = _case(n, seed, 1e-4)
+    gpu.set_system(s)
+    A = O.ldu2csr(s)
+    x0 = np.random.default_rng(seed).uniform(-1, 1, s.n_cells) if seed % 2 else s.x0
+    rx, rh, rit, rst = O.mixed_cg(A, s.b, x0, 1e-10, inner, 5000, mode)
+    x, hist, it, rel, st = gpu.solve_mixed(s.b, x0, 1e-10, inner, 5000, mode)
+    assert st == rst and it == rit
+    np.testing.assert_array_equal(hist, rh)
+    np.testing.assert_array_equal(x, rx)
+
+
+@pytest.mark.parametrize("max_iter", [1, 5, 37])
+def test_mixed_iteration_cap_bit_exact(gpu, max_iter):
+    s = _case(16, 5, 1e-4)
+    gpu.set_system(s)
+    A = O.ldu2csr(s)
+    rx, rh, rit, rst = O.mixed_cg(A, s.b, s.x0, 1e-10, 1e-2, max_iter, 0)
+    x, hist, it, rel, st = gpu.solve_mixed(s.b, s.x0, 1e-10, 1e-2, max_iter, 0)
+    assert (st, it) == (rst, rit) and st == 11
+    np.testing.assert_array_equal(hist, rh)
+    np.testing.assert_array_equal(x, rx)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mixed_128_reaches_fp64_target(gpu, mode):
+    import torch
+    s = _case(128, 0, 1e-4)
+    gpu.set_system(s)
+    b = torch.from_numpy(s.b).cuda()
+    x = torch.zeros_like(b)
+    x, hist, it, rel, st = gpu.solve_mixed(b, x, 1e-10, 1e-4, 20000, mode)
+    assert st == 0
+    A = O.ldu2csr(s)
+    xh = x.cpu().numpy()
+    true_rel = np.linalg.norm(O.residual(A, s.b, xh)) / np.linalg.norm(s.b)
+    assert true_rel <= 1e-10
+    assert abs(true_rel - rel) <= 1e-3 * rel   # device FP64 norm vs host recompute
+    # error against an FP64 CG solve of the same system
+    xd, h64, _ = gpu.solve("CG", s.b, s.x0, error=1e-12 * np.linalg.norm(s.b), max_steps=20000)
+    err = np.linalg.norm(xh - xd) / np.linalg.norm(xd)
+    print(f"[mixed128] mode={mode} inner_iters={it} updates={len(hist) - 1} "
+          f"rel_res={true_rel:.3e} err_vs_fp64={err:.3e}", flush=True)
+    assert err < 1e-4
