@@ -127,6 +127,34 @@ def config_system(name: str, seed: int = 0) -> LduSystem:
     return laplacian(seed=seed, **table[name])
 
 
+def random_ldu(n_cells: int, n_faces: int, seed: int = 0, symmetric: bool = False,
+               shift: float = 0.5) -> LduSystem:
+    """Unstructured LDU system: n_faces distinct random cell pairs (owner < neighbour)
+    in upper-triangular order (sorted by owner, then neighbour), random coefficients,
+    diagonally dominant diagonal.  Rows have irregular lengths, so neighbour groups
+    (losort) interleave owners in a way structured meshes never show."""
+    rng = np.random.default_rng(seed)
+    max_pairs = n_cells * (n_cells - 1) // 2
+    n_faces = min(n_faces, max_pairs)
+    pairs = set()
+    while len(pairs) < n_faces:
+        a = rng.integers(0, n_cells, size=2 * (n_faces - len(pairs)) + 8)
+        c = rng.integers(0, n_cells, size=a.shape[0])
+        for u, v in zip(a.tolist(), c.tolist()):
+            if u != v and len(pairs) < n_faces:
+                pairs.add((min(u, v), max(u, v)))
+    arr = np.array(sorted(pairs), dtype=np.int64).reshape(-1, 2)
+    owner = arr[:, 0].astype(np.int32)
+    neigh = arr[:, 1].astype(np.int32)
+    M = owner.shape[0]
+    upper = -(0.5 + rng.random(M))
+    lower = upper.copy() if symmetric else -(0.5 + rng.random(M))
+    diag = row_abs_sum(n_cells, owner, neigh, lower, upper) + shift
+    b = rng.uniform(-1.0, 1.0, n_cells)
+    return LduSystem(n_cells, owner, neigh, diag, lower, upper, b, np.zeros(n_cells),
+                     meta={"random": True, "seed": seed, "symmetric": symmetric})
+
+
 def batch_edges(count: int = 64, lo: int = 16, hi: int = 64, seed: int = 0) -> np.ndarray:
     """Mesh edges of an uneven batch, n_s ~ U{lo..hi}."""
     rng = np.random.default_rng(seed)
