@@ -19,8 +19,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libgis_amd.so")
 
-SOURCES = ["gis_pattern.hip", "gis_solvers.hip", "gis_abi.hip"]
-EXTRA_SOURCES = ["gis_methods.hip", "gis_mixed.hip", "gis_batch.hip", "gis_adjoint.hip"]
+# every translation unit under csrc/ is part of the library
 
 ARCH = os.environ.get("GIS_OFFLOAD_ARCH", "gfx950")
 
@@ -41,9 +40,7 @@ def flags() -> list[str]:
 
 
 def sources() -> list[str]:
-    out = [os.path.join(CSRC, s) for s in SOURCES]
-    out += [os.path.join(CSRC, s) for s in EXTRA_SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    return out
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
 def _compile(src: str) -> str:
@@ -62,7 +59,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-z,defs", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")

@@ -154,6 +154,8 @@ GIS_API int gis_destroy(gis_handle h) {
         (void)hipStreamSynchronize(h->stream);
         if (h->st_host) (void)hipHostFree(h->st_host);
         h->st_host = nullptr;
+        if (h->mstate_host) (void)hipHostFree(h->mstate_host);
+        h->mstate_host = nullptr;
         timer_release(h);
         if (h->own_stream) (void)hipStreamDestroy(h->stream);
         delete h;   // DevMem destructors free device memory with the device current
@@ -462,6 +464,100 @@ GIS_API int gis_solve_mixed_device(gis_handle h, double* d_x, const double* d_b,
                                    int* iterations, double* rel_residual) {
     return solve_mixed_impl(h, d_x, d_b, rel_tol, inner_rel_tol, max_iter, mode, hist,
                             hist_cap, hist_len, iterations, rel_residual, true);
+}
+
+static int solve_multi_impl(gis_handle h, int method, int nrhs, double* X, const double* B,
+                            const gis_params* params, int transpose, int* iterations,
+                            double* final_residual, int* status, bool dev) {
+    if (!h || !X || !B) return GIS_ERR_ARG;
+    if (nrhs < 1 || nrhs > 8) return fail(h, GIS_ERR_ARG, "nrhs must be in 1..8");
+    if (method != GIS_CG && method != GIS_BICG)
+        return fail(h, GIS_ERR_ARG, "multi-RHS solves support GIS_CG and GIS_BICG");
+    if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
+    DeviceGuard g(h->device);
+    const size_t n = (size_t)h->N * nrhs;
+    GIS_HIP(h, h->io2.ensure(sizeof(double) * 2 * n));
+    double* dx = h->io2.as<double>();
+    double* db = dx + n;
+    const hipMemcpyKind kin = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind kout = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    GIS_HIP(h, hipMemcpyAsync(dx, X, sizeof(double) * n, kin, h->stream));
+    GIS_HIP(h, hipMemcpyAsync(db, B, sizeof(double) * n, kin, h->stream));
+    SolveArgs a = to_args(method, params, 1);
+    const int st = device_solve_multi(h, method, nrhs, transpose != 0, a, dx, db, iterations,
+                                      nullptr, final_residual, status);
+    if (st != GIS_OK && st < GIS_BREAKDOWN) return st;
+    GIS_HIP(h, hipMemcpyAsync(X, dx, sizeof(double) * n, kout, h->stream));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    return st;
+}
+
+GIS_API int gis_solve_multi(gis_handle h, int method, int nrhs, double* X, const double* B,
+                            const gis_params* params, int transpose, int* iterations,
+                            double* final_residual, int* status) {
+    return solve_multi_impl(h, method, nrhs, X, B, params, transpose, iterations,
+                            final_residual, status, false);
+}
+
+GIS_API int gis_solve_multi_device(gis_handle h, int method, int nrhs, double* d_X,
+                                   const double* d_B, const gis_params* params, int transpose,
+                                   int* iterations, double* final_residual, int* status) {
+    return solve_multi_impl(h, method, nrhs, d_X, d_B, params, transpose, iterations,
+                            final_residual, status, true);
+}
+
+static int adjoint_impl(gis_handle h, int method, int nrhs, const double* W, const double* x,
+                        double* lambda, double* d_diag, double* d_lower, double* d_upper,
+                        const gis_params* params, int* iterations, int* status, bool dev) {
+    if (!h || !W || !x || !lambda) return GIS_ERR_ARG;
+    if (nrhs < 1 || nrhs > 8) return fail(h, GIS_ERR_ARG, "nrhs must be in 1..8");
+    if (method != GIS_CG && method != GIS_BICG)
+        return fail(h, GIS_ERR_ARG, "the adjoint solve supports GIS_CG and GIS_BICG");
+    if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
+    DeviceGuard g(h->device);
+    const size_t N = (size_t)h->N, M = (size_t)h->M, k = (size_t)nrhs;
+    // staging: lambda | W | x | d_diag | d_lower | d_upper
+    const size_t tot = 2 * k * N + N + k * N + 2 * k * M;
+    GIS_HIP(h, h->io2.ensure(sizeof(double) * tot));
+    double* dl = h->io2.as<double>();
+    double* dw = dl + k * N;
+    double* dxf = dw + k * N;
+    double* gd = dxf + N;
+    double* glo = gd + k * N;
+    double* gup = glo + k * M;
+    const hipMemcpyKind kin = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind kout = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    GIS_HIP(h, hipMemcpyAsync(dw, W, sizeof(double) * k * N, kin, h->stream));
+    GIS_HIP(h, hipMemcpyAsync(dxf, x, sizeof(double) * N, kin, h->stream));
+    GIS_HIP(h, hipMemsetAsync(dl, 0, sizeof(double) * k * N, h->stream));
+    SolveArgs a = to_args(method, params, 1);
+    const int st = device_solve_multi(h, method, nrhs, true, a, dl, dw, iterations, nullptr,
+                                      nullptr, status);
+    if (st != GIS_OK && st < GIS_BREAKDOWN) return st;
+    GIS_TRY(device_sddmm_fold(h, nrhs, dl, dxf, gd, glo, gup));
+    GIS_HIP(h, hipMemcpyAsync(lambda, dl, sizeof(double) * k * N, kout, h->stream));
+    if (d_diag) GIS_HIP(h, hipMemcpyAsync(d_diag, gd, sizeof(double) * k * N, kout, h->stream));
+    if (d_lower && M)
+        GIS_HIP(h, hipMemcpyAsync(d_lower, glo, sizeof(double) * k * M, kout, h->stream));
+    if (d_upper && M)
+        GIS_HIP(h, hipMemcpyAsync(d_upper, gup, sizeof(double) * k * M, kout, h->stream));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    return st;
+}
+
+GIS_API int gis_adjoint(gis_handle h, int method, int nrhs, const double* W, const double* x,
+                        double* lambda, double* d_diag, double* d_lower, double* d_upper,
+                        const gis_params* params, int* iterations, int* status) {
+    return adjoint_impl(h, method, nrhs, W, x, lambda, d_diag, d_lower, d_upper, params,
+                        iterations, status, false);
+}
+
+GIS_API int gis_adjoint_device(gis_handle h, int method, int nrhs, const double* d_W,
+                               const double* d_x, double* d_lambda, double* d_diag,
+                               double* d_lower, double* d_upper, const gis_params* params,
+                               int* iterations, int* status) {
+    return adjoint_impl(h, method, nrhs, d_W, d_x, d_lambda, d_diag, d_lower, d_upper, params,
+                        iterations, status, true);
 }
 
 GIS_API int gis_foam_solve(gis_handle h, int method, double* psi, const double* source,

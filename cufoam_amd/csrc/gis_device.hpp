@@ -22,7 +22,7 @@ constexpr int kLeaf = 4;              // consecutive rows per reduction leaf
 constexpr int kTile = kWG * kLeaf;    // rows per workgroup tile (one partial)
 constexpr int kGroup = 64;            // partials folded per group (one wave)
 constexpr int kMaxLevels = 6;         // kGroup^5 * kTile rows >> 2^31
-constexpr int kMaxNV = 4;             // reductions carried by one kernel
+constexpr int kMaxNV = 16;            // reductions carried by one kernel (8 RHS x 2)
 
 // Cross-workgroup reduction workspace (device pointers, passed by value).
 struct RedWs {

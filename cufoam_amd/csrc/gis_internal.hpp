@@ -100,6 +100,9 @@ struct gis_context {
     gis::KState* st_host = nullptr;             // pinned mirror
     gis::DevMem hist;
     gis::DevMem scratch;                        // pattern-build temporaries
+    gis::DevMem mstate;                         // per-column states of multi-RHS solves
+    gis::KState* mstate_host = nullptr;         // pinned mirror
+    gis::DevMem io2;                            // staging for multi-RHS / adjoint data
 
     gis::Schedule sched;
     gis::Timer timer;
@@ -175,6 +178,14 @@ struct SolveArgs {
 // x (in/out) and b are device vectors owned by the handle's io workspace.
 int device_solve(gis_context* h, const SolveArgs& a, double* d_x, const double* d_b,
                  int* hist_len, double* norm0, double* norm_final);
+
+// nrhs columns of length N, each contiguous (X[k*N + i]); transpose: solve A^T X = B.
+int device_solve_multi(gis_context* h, int method, int nrhs, bool transpose, const SolveArgs& a,
+                       double* X, const double* B, int* iters, double* norm0, double* normf,
+                       int* status);
+// dL_k/dA = -lambda_k x^T on the pattern, written into LDU-shaped gradients.
+int device_sddmm_fold(gis_context* h, int nrhs, const double* Lam, const double* x,
+                      double* d_diag, double* d_lower, double* d_upper);
 
 struct MixedArgs {
     double rel_tol;

@@ -73,7 +73,24 @@ struct Direct {
         } break;                                      \
     }
 
-// row range of a workgroup tile (by default the tile of this workgroup)
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X (each
+// with its own L2).  Map workgroup i to a tile so that XCD x sweeps one
+// contiguous range of tiles: neighbouring rows -- and the x gathers of a
+// tile -- then stay in one L2.  The map is a bijection on [0, G); results do
+// not depend on it (partials are indexed by tile).
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_tile(int i, int G) {
+    const int q = G / kXcds;
+    const int r = G % kXcds;
+    const int x = i % kXcds;
+    const int j = i / kXcds;
+    return x * q + min(x, r) + j;
+}
+
+// Row range of a workgroup tile.  TileRows(N): tile = workgroup index (best for
+// the SpMV-carrying kernels, measured); StreamRows(N): XCD-contiguous tiles
+// (best for the pure streaming update kernels, measured).
 struct TileRows {
     int tile, row0, nrows, i0;   // i0: first of this thread's kLeaf rows
     __device__ TileRows(int N) : TileRows(N, (int)blockIdx.x) {}
@@ -83,6 +100,10 @@ struct TileRows {
         nrows = min(kTile, N - row0);
         i0 = row0 + threadIdx.x * kLeaf;
     }
+};
+
+struct StreamRows : TileRows {
+    __device__ StreamRows(int N) : TileRows(N, xcd_tile((int)blockIdx.x, (int)gridDim.x)) {}
 };
 
 __device__ __forceinline__ void record_norm(KState* st, double norm) {

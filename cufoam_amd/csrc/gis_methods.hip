@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kWG) void k_mr2(int N, double* __restrict__ r,
                                              KState* st, RedWs red) {
     if (st->done) return;
     __shared__ RedShared shr;
-    TileRows t(N);
+    StreamRows t(N);
     const double alpha = st->alpha;
     const bool za = zero_add != 0;
     double rv[4], wv[4], xv[4];

@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kWG) void k_mx2(int N, float* __restrict__ r,
                                              RedWs red) {
     if (st->done || (st->flag & F_UPD)) return;
     __shared__ RedShared shr;
-    TileRows t(N);
+    StreamRows t(N);
     const float af = (float)st->alpha;
     float rv[4], qv[4];
     load4(r, t.i0, N, rv);
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(kWG) void k_mx_final(int N, double* __restrict__ xb
     const float af = (float)st->alpha;
     const float* p = (k & 1) ? pb1 : pb0;
     const double* xcur = (flag & F_XALT) ? xb1 : xb0;
-    TileRows t(N);
+    StreamRows t(N);
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) {
         const int i = t.i0 + kk;

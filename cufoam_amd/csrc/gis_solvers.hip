@@ -40,7 +40,7 @@ __global__ __launch_bounds__(kWG) void k_dot(int N, const double* __restrict__ a
                                              const double* __restrict__ b, KState* st,
                                              RedWs red, int slot) {
     __shared__ RedShared shr;
-    TileRows t(N);
+    StreamRows t(N);
     double av[4], bv[4];
     load4(a, t.i0, N, av);
     load4(b, t.i0, N, bv);
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kWG) void k_dot(int N, const double* __restrict__ a
 __global__ __launch_bounds__(kWG) void k_mean(int N, const double* __restrict__ x, KState* st,
                                               RedWs red) {
     __shared__ RedShared shr;
-    TileRows t(N);
+    StreamRows t(N);
     double xv[4];
     load4(x, t.i0, N, xv);
     double leaf[1] = {0.0};
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(kWG) void k_cg2(int N, double* __restrict__ r,
                                              RedWs red) {
     if (st->done) return;
     __shared__ RedShared shr;
-    TileRows t(N);
+    StreamRows t(N);
     const double alpha = st->alpha;
     double rv[4], qv[4];
     load4(r, t.i0, N, rv);
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kWG) void k_cg_flush(int N, double* __restrict__ x,
     if (k < 1) return;
     const double alpha = st->alpha;
     const double* p = (k & 1) ? pb1 : pb0;
-    TileRows t(N);
+    StreamRows t(N);
     double xv[4], pv[4];
     load4(x, t.i0, N, xv);
     load4(p, t.i0, N, pv);

@@ -74,6 +74,13 @@ SIGNATURES = {
     "gis_solve": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
     "gis_solve_device": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
     "gis_foam_solve": (_i, [_vp, _i, _vp, _vp, _d, _ll, C.POINTER(_Perf)]),
+    "gis_solve_multi": (_i, [_vp, _i, _i, _vp, _vp, C.POINTER(_Params), _i, _vp, _vp, _vp]),
+    "gis_solve_multi_device": (_i, [_vp, _i, _i, _vp, _vp, C.POINTER(_Params), _i, _vp, _vp,
+                                    _vp]),
+    "gis_adjoint": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_Params), _vp,
+                         _vp]),
+    "gis_adjoint_device": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
+                                C.POINTER(_Params), _vp, _vp]),
     "gis_solve_mixed": (_i, [_vp, _vp, _vp, _d, _d, _ll, _i, _vp, _i, _ip, _ip, _dp]),
     "gis_solve_mixed_device": (_i, [_vp, _vp, _vp, _d, _d, _ll, _i, _vp, _i, _ip, _ip, _dp]),
 }
@@ -334,6 +341,64 @@ class Gis:
                                          C.byref(it), C.byref(rr))
         self._check(st, "solve_mixed")
         return x, hist[:min(hl.value, hist_cap)].copy(), it.value, rr.value, st
+
+    def solve_multi(self, method: str, B, X0=None, error: float = 1e-4, max_steps: int = 0,
+                    int_bound: float = 1e-10, transpose: bool = False):
+        """Several right-hand sides (B: nrhs x N) -> (X, iterations, final norms, status)."""
+        p = _Params(error, max_steps, 0.0, 30, int_bound)
+        if _is_dev(B):
+            import torch
+            nrhs = int(B.shape[0])
+            X = X0 if X0 is not None else torch.zeros_like(B)
+            f = self._L.gis_solve_multi_device
+        else:
+            B = _host(B, np.float64)
+            nrhs = B.shape[0]
+            X = np.array(X0 if X0 is not None else np.zeros_like(B), dtype=np.float64, copy=True)
+            f = self._L.gis_solve_multi
+        it = np.zeros(nrhs, np.int32)
+        nf = np.zeros(nrhs)
+        sts = np.zeros(nrhs, np.int32)
+        st = f(self._h, METHODS[method], nrhs, _ptr(X), _ptr(B), C.byref(p), int(transpose),
+               _ptr(it), _ptr(nf), _ptr(sts))
+        self._check(st, f"solve_multi({method})")
+        return X, it, nf, sts
+
+    def adjoint(self, W, x, method: str = "CG", error: float = 1e-12, max_steps: int = 0):
+        """Gradients of L_k = W[k] . x through x = A^-1 b -> (lambda = dL/db, d_diag,
+        d_lower, d_upper, iterations).  numpy in -> numpy out; device tensors in ->
+        device tensors out."""
+        m = (self.nnz - self.n) // 2
+        p = _Params(error, max_steps, 0.0, 30, 1e-10)
+        if _is_dev(W):
+            import torch
+            if W.dim() == 1:
+                W = W.unsqueeze(0)
+            W = W.contiguous()
+            k, n = W.shape
+            lam = torch.zeros_like(W)
+            dd = torch.zeros_like(W)
+            dlo = torch.zeros((k, max(m, 1)), dtype=W.dtype, device=W.device)
+            dup = torch.zeros_like(dlo)
+            f = self._L.gis_adjoint_device
+            x = x.contiguous()
+        else:
+            W = _host(W, np.float64)
+            if W.ndim == 1:
+                W = W[None, :]
+            x = _host(x, np.float64)
+            k, n = W.shape
+            lam = np.zeros((k, n))
+            dd = np.zeros((k, n))
+            dlo = np.zeros((k, max(m, 1)))
+            dup = np.zeros((k, max(m, 1)))
+            f = self._L.gis_adjoint
+        it = np.zeros(k, np.int32)
+        sts = np.zeros(k, np.int32)
+        st = f(self._h, METHODS[method], k, _ptr(W), _ptr(x), _ptr(lam), _ptr(dd), _ptr(dlo),
+               _ptr(dup), C.byref(p), _ptr(it), _ptr(sts))
+        self._check(st, "adjoint")
+        return lam, dd, dlo[:, :m], dup[:, :m], it
 
     def foam_solve(self, method: str, psi, source, tolerance: float, max_iter: int):
         """cuFoam<method>::solve: psi (numpy) is updated in place; returns Performance."""

@@ -152,6 +152,36 @@ GIS_API int gis_solve_device(gis_handle h, int method, double *d_x, const double
 GIS_API int gis_foam_solve(gis_handle h, int method, double *psi, const double *source,
                            double tolerance, long long max_iter, gis_performance *perf);
 
+/* ---- several right-hand sides sharing the matrix (no counterpart in the
+ * reference, which solves one field component per call; for callers with the
+ * three velocity components of one momentum matrix, or batched adjoint loads):
+ * one SpMM per iteration streams the matrix once for all columns.  X and B are
+ * nrhs x N, each column contiguous (X[k*N + i]); X holds x0 on entry.  Column
+ * k follows exactly the single-RHS method (same scalars, same stopping rule, the
+ * same bits).  method: GIS_CG or GIS_BICG; transpose != 0 solves A^T X = B.
+ * iterations / final_residual / status: nrhs entries each (optional).  1 <= nrhs <= 8. ---- */
+GIS_API int gis_solve_multi(gis_handle h, int method, int nrhs, double *X, const double *B,
+                            const gis_params *params, int transpose, int *iterations,
+                            double *final_residual, int *status);
+GIS_API int gis_solve_multi_device(gis_handle h, int method, int nrhs, double *d_X,
+                                   const double *d_B, const gis_params *params, int transpose,
+                                   int *iterations, double *final_residual, int *status);
+
+/* ---- adjoint of x = A^-1 b (no counterpart in the reference; for callers that
+ * differentiate through the solve): for nrhs losses with dL_k/dx = W[k],
+ * solves A^T lambda_k = W[k] with the same kernels on the transposed matrix
+ * and writes dL_k/db = lambda_k (lambda, nrhs x N) and dL_k/dA_ij =
+ * -lambda_k[i] x[j] on the pattern, folded into LDU form: d_diag (nrhs x N),
+ * d_lower, d_upper (nrhs x n_faces).  x is the forward solution.  Gradient
+ * outputs may be NULL. ---- */
+GIS_API int gis_adjoint(gis_handle h, int method, int nrhs, const double *W, const double *x,
+                        double *lambda, double *d_diag, double *d_lower, double *d_upper,
+                        const gis_params *params, int *iterations, int *status);
+GIS_API int gis_adjoint_device(gis_handle h, int method, int nrhs, const double *d_W,
+                               const double *d_x, double *d_lambda, double *d_diag,
+                               double *d_lower, double *d_upper, const gis_params *params,
+                               int *iterations, int *status);
+
 /* ---- mixed precision (no counterpart in the reference; for callers that want
  * an FP64-accurate CG solve at FP32 bandwidth): the Krylov inner loop runs in
  * FP32, FP64 is kept for the residual recompute r = b - A x, the correction
