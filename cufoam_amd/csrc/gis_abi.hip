@@ -157,6 +157,7 @@ GIS_API int gis_destroy(gis_handle h) {
         if (h->mstate_host) (void)hipHostFree(h->mstate_host);
         h->mstate_host = nullptr;
         timer_release(h);
+        batch_release(h);
         if (h->own_stream) (void)hipStreamDestroy(h->stream);
         delete h;   // DevMem destructors free device memory with the device current
     }
@@ -558,6 +559,46 @@ GIS_API int gis_adjoint_device(gis_handle h, int method, int nrhs, const double*
                                int* iterations, int* status) {
     return adjoint_impl(h, method, nrhs, d_W, d_x, d_lambda, d_diag, d_lower, d_upper, params,
                         iterations, status, true);
+}
+
+GIS_API int gis_batch_set_pattern(gis_handle h, int n_systems, const int* n_cells,
+                                  const int* n_faces, const int* lower_addr,
+                                  const int* upper_addr) {
+    if (!h || !n_cells || !n_faces) return GIS_ERR_ARG;
+    DeviceGuard g(h->device);
+    return batch_set_pattern(h, n_systems, n_cells, n_faces, lower_addr, upper_addr);
+}
+
+GIS_API int gis_batch_set_values(gis_handle h, const double* diag, const double* lower,
+                                 const double* upper) {
+    if (!h || !diag) return GIS_ERR_ARG;
+    DeviceGuard g(h->device);
+    return batch_set_values(h, diag, lower, upper, false);
+}
+
+GIS_API int gis_batch_set_values_device(gis_handle h, const double* d_diag,
+                                        const double* d_lower, const double* d_upper) {
+    if (!h || !d_diag) return GIS_ERR_ARG;
+    DeviceGuard g(h->device);
+    return batch_set_values(h, d_diag, d_lower, d_upper, true);
+}
+
+GIS_API int gis_batch_solve(gis_handle h, double* x, const double* b, const double* error,
+                            long long max_steps, int use_graph, int* iterations,
+                            double* final_residual, int* status) {
+    if (!h || !x || !b || !error) return GIS_ERR_ARG;
+    DeviceGuard g(h->device);
+    return batch_solve(h, x, b, false, error, max_steps, use_graph, iterations, final_residual,
+                       status);
+}
+
+GIS_API int gis_batch_solve_device(gis_handle h, double* d_x, const double* d_b,
+                                   const double* error, long long max_steps, int use_graph,
+                                   int* iterations, double* final_residual, int* status) {
+    if (!h || !d_x || !d_b || !error) return GIS_ERR_ARG;
+    DeviceGuard g(h->device);
+    return batch_solve(h, d_x, d_b, true, error, max_steps, use_graph, iterations,
+                       final_residual, status);
 }
 
 GIS_API int gis_foam_solve(gis_handle h, int method, double* psi, const double* source,

@@ -17,6 +17,9 @@ namespace gis {
 struct DevMem {
     void* p = nullptr;
     size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
     hipError_t ensure(size_t n) {
         if (n <= bytes && p) return hipSuccess;
         if (p) {
@@ -65,6 +68,25 @@ struct Timer {
     long long count[kTimerSlots] = {};
 };
 
+// Batched systems: block-diagonal stack, each system padded to whole tiles.
+struct BatchInfo {
+    bool ready = false;
+    int S = 0;
+    std::vector<int> n, m, pad_off, pack_off, face_off, tile0;
+    DevMem red_mem, meta, st;
+    RedWs* d_red = nullptr;
+    int* d_tile_sys = nullptr;
+    int* d_tile0 = nullptr;
+    int* d_pad_off = nullptr;
+    int* d_pack_off = nullptr;
+    KState* st_host = nullptr;           // pinned
+    hipStream_t cap_stream = nullptr;    // graph capture / replay stream
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_valid = false;
+    int graph_iters = 0;
+    void* graph_key = nullptr;
+};
+
 }  // namespace gis
 
 struct gis_context {
@@ -106,6 +128,7 @@ struct gis_context {
 
     gis::Schedule sched;
     gis::Timer timer;
+    gis::BatchInfo batch;
     std::string err;
 };
 
@@ -186,6 +209,17 @@ int device_solve_multi(gis_context* h, int method, int nrhs, bool transpose, con
 // dL_k/dA = -lambda_k x^T on the pattern, written into LDU-shaped gradients.
 int device_sddmm_fold(gis_context* h, int nrhs, const double* Lam, const double* x,
                       double* d_diag, double* d_lower, double* d_upper);
+
+// batched systems (gis_batch.hip); addressing is per system (local cell ids),
+// faces and coefficients concatenated system by system
+int batch_set_pattern(gis_context* h, int S, const int* n_cells, const int* n_faces,
+                      const int* lower_addr, const int* upper_addr);
+int batch_set_values(gis_context* h, const double* diag, const double* lower,
+                     const double* upper, bool dev);
+int batch_solve(gis_context* h, double* x_packed, const double* b_packed, bool dev,
+                const double* error, long long max_steps, int use_graph, int* iters,
+                double* final_res, int* status);
+void batch_release(gis_context* h);
 
 struct MixedArgs {
     double rel_tol;

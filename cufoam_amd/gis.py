@@ -81,6 +81,11 @@ SIGNATURES = {
                          _vp]),
     "gis_adjoint_device": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
                                 C.POINTER(_Params), _vp, _vp]),
+    "gis_batch_set_pattern": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "gis_batch_set_values": (_i, [_vp, _vp, _vp, _vp]),
+    "gis_batch_set_values_device": (_i, [_vp, _vp, _vp, _vp]),
+    "gis_batch_solve": (_i, [_vp, _vp, _vp, _vp, _ll, _i, _vp, _vp, _vp]),
+    "gis_batch_solve_device": (_i, [_vp, _vp, _vp, _vp, _ll, _i, _vp, _vp, _vp]),
     "gis_solve_mixed": (_i, [_vp, _vp, _vp, _d, _d, _ll, _i, _vp, _i, _ip, _ip, _dp]),
     "gis_solve_mixed_device": (_i, [_vp, _vp, _vp, _d, _d, _ll, _i, _vp, _i, _ip, _ip, _dp]),
 }
@@ -399,6 +404,49 @@ class Gis:
                _ptr(dup), C.byref(p), _ptr(it), _ptr(sts))
         self._check(st, "adjoint")
         return lam, dd, dlo[:, :m], dup[:, :m], it
+
+    # -- batched independent systems -------------------------------------------
+    def batch_set_systems(self, systems):
+        """Stack LduSystems (local addressing each) into one batch on the handle."""
+        n = np.array([s.n_cells for s in systems], np.int32)
+        m = np.array([s.n_faces for s in systems], np.int32)
+        la = np.ascontiguousarray(np.concatenate([s.lower_addr for s in systems]), np.int32)
+        ua = np.ascontiguousarray(np.concatenate([s.upper_addr for s in systems]), np.int32)
+        self._check(self._L.gis_batch_set_pattern(self._h, len(systems), _ptr(n), _ptr(m),
+                                                  _ptr(la), _ptr(ua)), "batch_set_pattern")
+        self.batch_sizes = n
+        self.batch_values(np.concatenate([s.diag for s in systems]),
+                          np.concatenate([s.lower for s in systems]),
+                          np.concatenate([s.upper for s in systems]))
+
+    def batch_values(self, diag, lower, upper):
+        if _is_dev(diag):
+            st = self._L.gis_batch_set_values_device(self._h, _ptr(diag), _ptr(lower), _ptr(upper))
+        else:
+            d, lo, up = _host(diag, np.float64), _host(lower, np.float64), _host(upper, np.float64)
+            st = self._L.gis_batch_set_values(self._h, _ptr(d), _ptr(lo), _ptr(up))
+        self._check(st, "batch_set_values")
+
+    def batch_solve(self, b, x0=None, error=None, max_steps: int = 0, use_graph: bool = True):
+        """CG on every system of the batch -> (x, iterations, final norms, status);
+        b and x are the systems' vectors concatenated."""
+        S = len(self.batch_sizes)
+        err = np.ascontiguousarray(np.broadcast_to(np.asarray(error, np.float64), (S,)))
+        it = np.zeros(S, np.int32)
+        nf = np.zeros(S)
+        sts = np.zeros(S, np.int32)
+        if _is_dev(b):
+            import torch
+            x = x0 if x0 is not None else torch.zeros_like(b)
+            f = self._L.gis_batch_solve_device
+        else:
+            b = _host(b, np.float64)
+            x = np.array(x0 if x0 is not None else np.zeros_like(b), dtype=np.float64, copy=True)
+            f = self._L.gis_batch_solve
+        st = f(self._h, _ptr(x), _ptr(b), _ptr(err), max_steps, int(bool(use_graph)), _ptr(it),
+               _ptr(nf), _ptr(sts))
+        self._check(st, "batch_solve")
+        return x, it, nf, sts
 
     def foam_solve(self, method: str, psi, source, tolerance: float, max_iter: int):
         """cuFoam<method>::solve: psi (numpy) is updated in place; returns Performance."""

@@ -182,6 +182,28 @@ GIS_API int gis_adjoint_device(gis_handle h, int method, int nrhs, const double 
                                double *d_lower, double *d_upper, const gis_params *params,
                                int *iterations, int *status);
 
+/* ---- batched independent systems (no counterpart in the reference, which
+ * solves one matrix per call; for callers with many small or uneven meshes,
+ * e.g. ensembles or per-region solves): CG on every system, one launch per CG
+ * stage for the whole batch, stages replayed from a captured hipGraph when
+ * use_graph != 0.  Each system follows exactly the single-system CG (same
+ * bits, its own iteration count and its own absolute threshold error[s]).
+ * Addressing is per system (local cell ids); faces, coefficients, x and b are
+ * concatenated system by system.  The batch replaces the handle's matrix. ---- */
+GIS_API int gis_batch_set_pattern(gis_handle h, int n_systems, const int *n_cells,
+                                  const int *n_faces, const int *lower_addr,
+                                  const int *upper_addr);
+GIS_API int gis_batch_set_values(gis_handle h, const double *diag, const double *lower,
+                                 const double *upper);
+GIS_API int gis_batch_set_values_device(gis_handle h, const double *d_diag,
+                                        const double *d_lower, const double *d_upper);
+GIS_API int gis_batch_solve(gis_handle h, double *x, const double *b, const double *error,
+                            long long max_steps, int use_graph, int *iterations,
+                            double *final_residual, int *status);
+GIS_API int gis_batch_solve_device(gis_handle h, double *d_x, const double *d_b,
+                                   const double *error, long long max_steps, int use_graph,
+                                   int *iterations, double *final_residual, int *status);
+
 /* ---- mixed precision (no counterpart in the reference; for callers that want
  * an FP64-accurate CG solve at FP32 bandwidth): the Krylov inner loop runs in
  * FP32, FP64 is kept for the residual recompute r = b - A x, the correction
