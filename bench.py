@@ -34,9 +34,13 @@ def spmv_bytes(N, nnz, sv, sx):
     return nnz * (sv + 4) + 4 * (N + 1) + 2 * N * sx
 
 
-def mx1_bytes(N, nnz):
-    """Algorithmic bytes of the fused FP32 iteration kernel: FP32 matrix (values + columns
-    + row offsets) and six FP32 vector passes (read r, p_{k-1}, xc; write p_k, q, xc)."""
+def mx1_bytes(N, nnz, direction):
+    """Algorithmic bytes of the FP32 SpMV-carrying iteration kernel.  Fused form (k_mx1):
+    FP32 matrix (values + columns + row offsets) and six FP32 vector passes (read r,
+    p_{k-1}, xc; write p_k, q, xc).  Split form (k_mx1s): the matrix, p_k read once and
+    q written, i.e. the bytes of one FP32 SpMV."""
+    if direction:
+        return spmv_bytes(N, nnz, 4, 4)
     return nnz * (4 + 4) + 4 * (N + 1) + 6 * N * 4
 
 
@@ -63,11 +67,69 @@ def cpu_baseline(sys_, iters_fp64: int, sample_iters: int):
     }
 
 
+def batch_throughput(reps: int, use_graph: bool = True):
+    """64 SPD systems with edges n_s ~ U{16..64} (seed 0), each solved by CG from x0 = 0
+    to ||r||_2 <= 1e-8 ||b||_2: one batched solve of all 64 against the same 64 solves
+    done one after another through single-system handles (values resident in both)."""
+    import numpy as np
+    import torch
+    from cufoam_amd import mesh
+    from cufoam_amd.gis import Gis
+
+    edges = mesh.batch_edges(64, 16, 64, seed=0)
+    systems = [mesh.laplacian(int(e), seed=100 + i, shift=1e-3) for i, e in enumerate(edges)]
+    err = np.array([1e-8 * np.linalg.norm(s.b) for s in systems])
+    g = Gis(0)
+    g.batch_set_systems(systems)
+    b = torch.from_numpy(np.concatenate([s.b for s in systems])).cuda()
+    singles = []
+    for s in systems:
+        h = Gis(0)
+        h.set_system(s)
+        singles.append((h, torch.from_numpy(s.b).cuda()))
+
+    def run_batch():
+        return g.batch_solve(b, torch.zeros_like(b), err, 100000, use_graph=use_graph)
+
+    def run_seq():
+        its = []
+        for k, (h, bs) in enumerate(singles):
+            _, hist, _ = h.solve("CG", bs, torch.zeros_like(bs), error=err[k],
+                                 max_steps=100000, hist_cap=1)
+            its.append(h.last_iterations)
+        return its
+
+    run_batch()
+    run_seq()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, it_b, _, st_b = run_batch()
+    torch.cuda.synchronize()
+    t_b = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        it_s = run_seq()
+    torch.cuda.synchronize()
+    t_s = (time.perf_counter() - t0) / reps
+    out = {"systems": 64, "cells": int(sum(s.n_cells for s in systems)),
+           "nnz": int(sum(s.nnz for s in systems)), "graph": use_graph,
+           "batched_s": round(t_b, 5), "systems_per_s": round(64 / t_b, 2),
+           "sequential_s": round(t_s, 5), "sequential_systems_per_s": round(64 / t_s, 2),
+           "speedup": round(t_s / t_b, 3), "max_iterations": int(max(it_b)),
+           "iterations_equal": bool(list(map(int, it_b)) == list(map(int, it_s))),
+           "all_converged": bool(np.all(st_b == 0))}
+    for h, _ in singles:
+        h.close()
+    g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="mesh edge (256 = the named workload)")
     ap.add_argument("--mode", type=int, default=0, help="mixed mode: 0 reliable updates, 1 restart")
     ap.add_argument("--inner", type=float, default=1e-2,
@@ -80,6 +142,10 @@ def main():
                     help="skip the extra restarted-refinement solve with inner relTol 1e-4")
     ap.add_argument("--cpu-sample", type=int, default=30, help="oracle CG iterations timed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", dest="tune", action="store_false",
+                    help="keep the default schedule instead of autotuning")
+    ap.add_argument("--no-batch", dest="batch", action="store_false",
+                    help="skip the 64-system batched CG throughput report")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -108,6 +174,13 @@ def main():
     t0 = time.perf_counter()
     g.set_values(s.diag, s.lower, s.upper)
     t_values = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    if args.tune:
+        sched, _, tune_ms = g.autotune(iterations=24)
+    else:
+        sched, tune_ms = g.get_schedule(), (0.0, 0.0)
+    t_tune = time.perf_counter() - t0
+    log(f"[bench] schedule {sched} (autotune {t_tune:.2f}s, best runs {tune_ms} ms)")
     b = torch.from_numpy(s.b).cuda()
     bnorm = float(np.linalg.norm(s.b))
     max_iter = args.max_iter
@@ -156,14 +229,14 @@ def main():
     log(f"[bench] fp64: {t_fp64:.3f}s/solve iters={it_d} status={st_d}")
 
     # ---- per-kernel breakdown: one extra solve of each with every slot timed ----
-    slots = ("iter_spmv", "iter_update", "fp64_update")
+    slots = ("iter_spmv", "iter_update", "fp64_update", "iter_direction")
     g.set_timing(*slots)
     g.solve_mixed(b, torch.zeros_like(b), 1e-10, args.inner, max_iter, args.mode)
     brk_m = {s_: g.get_timing(s_) for s_ in slots}
     g.set_timing(*slots)
     g.solve("CG", b, torch.zeros_like(b), error=1e-10 * bnorm, max_steps=max_iter,
             hist_cap=max_iter + 1)
-    brk_d = {s_: g.get_timing(s_) for s_ in slots[:2]}
+    brk_d = {s_: g.get_timing(s_) for s_ in slots}
     g.set_timing()
 
     def avg_us(t):
@@ -200,18 +273,21 @@ def main():
 
     # ---- roofline of the dominant kernel (fused FP32 iteration kernel) -----
     mx1_avg_s = mx1_ms / max(mx1_n, 1) / 1e3
-    ach = mx1_bytes(N, nnz) / mx1_avg_s / 1e9
+    split32 = sched["fp32_direction"] == 1
+    kname = "k_mx1s" if split32 else "k_mx1"
+    ach = mx1_bytes(N, nnz, split32) / mx1_avg_s / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_poisson256.json")
     if os.path.exists(tf):
         try:
-            traffic = json.load(open(tf)).get("k_mx1_bytes_per_launch")
+            traffic = json.load(open(tf)).get(f"{kname}_bytes_per_launch")
         except Exception:
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic,
-                "kernel": "k_mx1 (FP32 SpMV + direction + deferred correction)",
-                "bytes_per_launch": mx1_bytes(N, nnz),
+                "kernel": ("k_mx1s (FP32 SpMV of the direction + (p, q))" if split32 else
+                           "k_mx1 (FP32 SpMV + direction + deferred correction)"),
+                "bytes_per_launch": mx1_bytes(N, nnz, split32),
                 "avg_launch_us": round(mx1_avg_s * 1e6, 2), "launches": mx1_n}
 
     literal = None
@@ -225,6 +301,11 @@ def main():
                    "s": round(time.perf_counter() - t0, 5), "iterations": itl,
                    "fp64_updates": len(hl) - 1, "rel_residual": rell, "status": stl}
         log(f"[bench] literal restarted refinement: {literal}")
+
+    batched = None
+    if args.batch:
+        batched = batch_throughput(3)
+        log(f"[bench] batched: {batched}")
 
     cpu = None
     if not args.no_cpu and n == 256:
@@ -263,20 +344,27 @@ def main():
         "rel_error_mixed_vs_fp64": err_m,
         "fp64_iteration_us": round(t_fp64 / max(it_d, 1) * 1e6, 2),
         "mixed_iteration_us": round(t_mixed / max(it_m, 1) * 1e6, 2),
-        "kernel_avg_us": {"k_mx1": avg_us((mx1_ms, mx1_n)), "k_cg1": avg_us((cg1_ms, cg1_n)),
+        "kernel_avg_us": {"mixed_spmv_kernel": avg_us((mx1_ms, mx1_n)),
+                          "fp64_spmv_kernel": avg_us((cg1_ms, cg1_n)),
                           "instrumented_solve": {
-                              "k_mx1": avg_us(brk_m["iter_spmv"]),
-                              "k_mx2": avg_us(brk_m["iter_update"]),
-                              "k_mxu": avg_us(brk_m["fp64_update"]),
-                              "k_cg1": avg_us(brk_d["iter_spmv"]),
-                              "k_cg2": avg_us(brk_d["iter_update"])}},
+                              "mixed": {"spmv_kernel": avg_us(brk_m["iter_spmv"]),
+                                        "update_k_mx2": avg_us(brk_m["iter_update"]),
+                                        "fp64_update_k_mxu": avg_us(brk_m["fp64_update"]),
+                                        "direction_k_mx_dir": avg_us(brk_m["iter_direction"])},
+                              "fp64": {"spmv_kernel": avg_us(brk_d["iter_spmv"]),
+                                       "update_k_cg2": avg_us(brk_d["iter_update"]),
+                                       "direction_k_cg_dir": avg_us(brk_d["iter_direction"])}}},
+        "schedule": sched,
         "spmv": spmv,
-        "setup_s": {"pattern": round(t_pattern, 4), "values": round(t_values, 4)},
+        "setup_s": {"pattern": round(t_pattern, 4), "values": round(t_values, 4),
+                    "autotune": round(t_tune, 4)},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     if literal is not None:
         out["mixed_restarted_inner_1e-4"] = literal
+    if batched is not None:
+        out["batched_cg_64_systems"] = batched
     print(json.dumps(out), flush=True)
 
 

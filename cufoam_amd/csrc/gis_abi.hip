@@ -353,20 +353,56 @@ GIS_API int gis_dot_device(gis_handle h, const double* d_a, const double* d_b, d
     return device_dot(h, d_a, d_b, out);
 }
 
-GIS_API int gis_set_schedule(gis_handle h, int spmv64, int spmv32, int block_iters) {
-    if (!h) return GIS_ERR_ARG;
-    if (spmv64 > 3 || spmv32 > 3) return fail(h, GIS_ERR_ARG, "schedule id out of range");
-    if (spmv64 >= 0) h->sched.spmv64 = static_cast<SpmvKind>(spmv64);
-    if (spmv32 >= 0) h->sched.spmv32 = static_cast<SpmvKind>(spmv32);
-    if (block_iters > 0) h->sched.block_iters = block_iters;
+GIS_API int gis_set_schedule(gis_handle h, const gis_schedule* s) {
+    if (!h || !s) return GIS_ERR_ARG;
+    auto coarsen_ok = [](int c) { return c <= 0 || c == 1 || c == 2 || c == 4; };
+    if (s->spmv64 > 3 || s->spmv32 > 3 || s->fp64_direction > 1 || s->fp32_direction > 1 ||
+        !coarsen_ok(s->coarsen64) || !coarsen_ok(s->coarsen32))
+        return fail(h, GIS_ERR_ARG, "schedule field out of range");
+    if (s->spmv64 >= 0) h->sched.spmv64 = static_cast<SpmvKind>(s->spmv64);
+    if (s->spmv32 >= 0) h->sched.spmv32 = static_cast<SpmvKind>(s->spmv32);
+    if (s->block_iters > 0) h->sched.block_iters = s->block_iters;
+    if (s->fp64_direction >= 0) h->sched.fp64_direction = s->fp64_direction;
+    if (s->fp32_direction >= 0) h->sched.fp32_direction = s->fp32_direction;
+    if (s->coarsen64 > 0) h->sched.coarsen64 = s->coarsen64;
+    if (s->coarsen32 > 0) h->sched.coarsen32 = s->coarsen32;
     return GIS_OK;
 }
 
-GIS_API int gis_get_schedule(gis_handle h, int* spmv64, int* spmv32, int* block_iters) {
+GIS_API int gis_get_schedule(gis_handle h, gis_schedule* s) {
+    if (!h || !s) return GIS_ERR_ARG;
+    s->spmv64 = static_cast<int>(h->sched.spmv64);
+    s->spmv32 = static_cast<int>(h->sched.spmv32);
+    s->block_iters = h->sched.block_iters;
+    s->fp64_direction = h->sched.fp64_direction;
+    s->fp32_direction = h->sched.fp32_direction;
+    s->coarsen64 = h->sched.coarsen64;
+    s->coarsen32 = h->sched.coarsen32;
+    return GIS_OK;
+}
+
+GIS_API int gis_autotune(gis_handle h, int targets, int iterations, gis_schedule* chosen,
+                         int* from_cache, double* best_ms) {
     if (!h) return GIS_ERR_ARG;
-    if (spmv64) *spmv64 = static_cast<int>(h->sched.spmv64);
-    if (spmv32) *spmv32 = static_cast<int>(h->sched.spmv32);
-    if (block_iters) *block_iters = h->sched.block_iters;
+    if (targets <= 0 || (targets & ~(GIS_TUNE_FP64 | GIS_TUNE_FP32)))
+        return fail(h, GIS_ERR_ARG, "autotune targets must be GIS_TUNE_FP64 and/or GIS_TUNE_FP32");
+    DeviceGuard g(h->device);
+    int cached_all = 1;
+    for (int what = 1; what <= 2; ++what) {
+        if (!(targets & what)) continue;
+        int spmv = 0, dir = 0, cached = 0;
+        double ms = 0.0;
+        GIS_TRY(device_autotune(h, what, iterations, &spmv, &dir, &cached, &ms));
+        cached_all &= cached;
+        if (best_ms) best_ms[what - 1] = ms;
+    }
+    if (from_cache) *from_cache = cached_all;
+    if (chosen) return gis_get_schedule(h, chosen);
+    return GIS_OK;
+}
+
+GIS_API int gis_autotune_clear(void) {
+    autotune_forget();
     return GIS_OK;
 }
 

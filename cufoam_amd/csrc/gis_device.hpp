@@ -82,6 +82,10 @@ __device__ __forceinline__ unsigned ticket_add(unsigned* t) {
     return __hip_atomic_fetch_add(t, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ unsigned ticket_add_n(unsigned* t, unsigned n) {
+    return __hip_atomic_fetch_add(t, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ void ticket_reset(unsigned* t) {
     __hip_atomic_store(t, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -122,36 +126,13 @@ struct RedShared {
     int last;
 };
 
-// Fold this workgroup's leaf values into the global reduction.  The tile's
-// subtree is formed here; the last workgroup to arrive in each aligned group
-// of kGroup tiles folds the group, level by level, and the very last one calls
-// fin(totals) on lane 0 of wave 0.  Must be reached by every thread of the
-// workgroup exactly once per launch.
+// Upper levels of the global reduction, run by wave 0 of the workgroup that
+// completed group idx of level 0: the last arriver of each aligned group of
+// kGroup partials folds the group, level by level, and the very last one calls
+// fin(totals) on lane 0.
 template <int NV, class Fin>
-__device__ __forceinline__ void finish_tile(const RedWs& red, int tile, double (&v)[NV],
-                                            RedShared& sh, Fin&& fin) {
-    block_tree<NV>(v, sh.w);
-    if (red.count[0] == 1) {
-        if (threadIdx.x == 0) fin(v);
-        return;
-    }
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) st_agent(&red.part[0][(size_t)tile * kMaxNV + i], v[i]);
-        order_release();
-        const int g = tile / kGroup;
-        const int members = min(kGroup, red.count[0] - g * kGroup);
-        const unsigned prev = ticket_add(&red.ticket[0][g]);
-        const int last = (prev == (unsigned)(members - 1));
-        if (last) ticket_reset(&red.ticket[0][g]);
-        sh.last = last;
-    }
-    __syncthreads();
-    if (!sh.last || threadIdx.x >= 64) return;
-
-    // wave 0 of a group-final workgroup
+__device__ __forceinline__ void fold_upper(const RedWs& red, int idx, Fin&& fin) {
     const int lane = threadIdx.x;
-    int idx = tile / kGroup;   // this group's index = partial index one level up
     for (int L = 0; L + 1 < kMaxLevels; ++L) {
         order_acquire();
         double w[NV];
@@ -180,6 +161,63 @@ __device__ __forceinline__ void finish_tile(const RedWs& red, int tile, double (
         if (!last) return;
         idx = idx / kGroup;
     }
+}
+
+// Fold the leaf values of C consecutive tiles (tile0 .. tile0+nt-1, all in one
+// aligned group of kGroup, nt <= C) into the global reduction: each tile's
+// subtree is formed here and stored as its own level-0 partial, and one ticket
+// of weight nt is taken for the workgroup.  Because the partials and the tree
+// above them do not depend on C, every C gives bit-identical totals.  Must be
+// reached by every thread of the workgroup exactly once per launch.
+template <int NV, int C, class Fin>
+__device__ __forceinline__ void finish_tiles(const RedWs& red, int tile0, int nt,
+                                             double (&v)[C][NV], RedShared& sh, Fin&& fin) {
+    static_assert(C * NV <= kMaxNV, "RedShared holds kMaxNV values per wave");
+    double flat[C * NV];
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) flat[c * NV + i] = v[c][i];
+    block_tree<C * NV>(flat, sh.w);
+    if (red.count[0] == 1) {
+        if (threadIdx.x == 0) {
+            double t[NV];
+#pragma unroll
+            for (int i = 0; i < NV; ++i) t[i] = flat[i];
+            fin(t);
+        }
+        return;
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            if (c < nt) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i)
+                    st_agent(&red.part[0][(size_t)(tile0 + c) * kMaxNV + i], flat[c * NV + i]);
+            }
+        }
+        order_release();
+        const int g = tile0 / kGroup;
+        const int members = min(kGroup, red.count[0] - g * kGroup);
+        const unsigned prev = ticket_add_n(&red.ticket[0][g], (unsigned)nt);
+        const int last = (prev + (unsigned)nt == (unsigned)members);
+        if (last) ticket_reset(&red.ticket[0][g]);
+        sh.last = last;
+    }
+    __syncthreads();
+    if (!sh.last || threadIdx.x >= 64) return;
+    fold_upper<NV>(red, tile0 / kGroup, fin);
+}
+
+// One tile per workgroup.
+template <int NV, class Fin>
+__device__ __forceinline__ void finish_tile(const RedWs& red, int tile, double (&v)[NV],
+                                            RedShared& sh, Fin&& fin) {
+    double vv[1][NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) vv[0][i] = v[i];
+    finish_tiles<NV, 1>(red, tile, 1, vv, sh, fin);
 }
 
 // ---------------------------------------------------------------------------

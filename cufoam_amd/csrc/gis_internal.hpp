@@ -51,12 +51,22 @@ struct Schedule {
     SpmvKind spmv64 = SpmvKind::Sell;       // FP64 kernels
     SpmvKind spmv32 = SpmvKind::Sell;       // FP32 kernels
     int block_iters = 64;                   // max iterations launched between host checks
+    // CG direction p_k = r + beta p_{k-1}: 0 formed on the fly inside the SpMV
+    // kernel's gathers; 1 written by a streaming pass first, so that the SpMV
+    // gathers one vector.  Same arithmetic, bit-identical iterates.
+    int fp64_direction = 0;
+    int fp32_direction = 0;
+    // tiles per workgroup of the CG iteration kernels (1, 2 or 4); partials stay
+    // per tile, so the reduction tree and the results do not change
+    int coarsen64 = 1;
+    int coarsen32 = 1;
 };
 
 // Opt-in kernel timing with HIP events on the launch stream.  Slot 0: the
 // SpMV-carrying kernel of each iteration; 1: the vector-update kernel of each
-// iteration; 2: FP64 update kernel of the mixed solver; 3: standalone SpMV.
-constexpr int kTimerSlots = 4;
+// iteration; 2: FP64 update kernel of the mixed solver; 3: standalone SpMV;
+// 4: the separate direction pass.
+constexpr int kTimerSlots = 5;
 
 struct Timer {
     unsigned mask = 0;                      // bit s: time slot s
@@ -228,6 +238,11 @@ struct MixedArgs {
     int mode;               // 0 reliable updates, 1 restarted refinement
     int hist_cap;
 };
+
+// schedule autotuner (gis_tune.hip); what: 1 FP64 kernels, 2 FP32 kernels
+int device_autotune(gis_context* h, int what, int iterations, int* spmv, int* direction,
+                    int* from_cache, double* best_ms);
+void autotune_forget();
 
 int device_solve_mixed(gis_context* h, const MixedArgs& a, double* d_x, const double* d_b,
                        int* hist_len, int* iters, double* rel_res);

@@ -106,6 +106,18 @@ struct StreamRows : TileRows {
     __device__ StreamRows(int N) : TileRows(N, xcd_tile((int)blockIdx.x, (int)gridDim.x)) {}
 };
 
+// Coarsened workgroups: workgroup w covers the C consecutive tiles w*C ..
+// w*C+C-1 (stream = true: w XCD-swizzled first).  nt <= C tiles exist.
+template <int C>
+struct SuperTile {
+    int tile0, nt;
+    __device__ SuperTile(int N, bool stream) {
+        const int s = stream ? xcd_tile((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+        tile0 = s * C;
+        nt = min(C, tiles_of(N) - tile0);
+    }
+};
+
 __device__ __forceinline__ void record_norm(KState* st, double norm) {
     const int k = st->k + 1;
     st->k = k;
@@ -159,6 +171,24 @@ __global__ __launch_bounds__(kWG) void k_setup(Mat<double> m, int N,
 
 // host helpers shared by the solver files
 inline dim3 tiles_grid(int N) { return dim3((unsigned)tiles_of(N)); }
+inline dim3 super_grid(int N, int C) { return dim3((unsigned)((tiles_of(N) + C - 1) / C)); }
+
+// Tiles per workgroup of the iteration kernels (a schedule field: 1, 2 or 4).
+#define GIS_COARSEN_DISPATCH(c, CC, ...)  \
+    switch (c) {                          \
+        case 4: {                         \
+            constexpr int CC = 4;         \
+            __VA_ARGS__;                  \
+        } break;                          \
+        case 2: {                         \
+            constexpr int CC = 2;         \
+            __VA_ARGS__;                  \
+        } break;                          \
+        default: {                        \
+            constexpr int CC = 1;         \
+            __VA_ARGS__;                  \
+        } break;                          \
+    }
 
 inline Mat<double> mat64(const gis_context* h) {
     return Mat<double>{h->rowptr.as<int>(), h->col.as<int>(), h->val.as<double>(),

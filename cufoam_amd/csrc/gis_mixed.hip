@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kWG) void k_mx_setup(Mat<double> m, int N,
     });
 }
 
-template <class P>
+template <class P, int C>
 __global__ __launch_bounds__(kWG) void k_mx1(Mat<float> m, int N, const float* __restrict__ r,
                                              float* __restrict__ pb0, float* __restrict__ pb1,
                                              float* __restrict__ xc, float* __restrict__ q,
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kWG) void k_mx1(Mat<float> m, int N, const float* _
     __shared__ typename P::template Shared<float> sh;
     __shared__ float sh_y[kTile];
     __shared__ RedShared shr;
-    TileRows t(N);
+    const SuperTile<C> sup(N, false);
     const int k = st->k + 1;
     const int flag = st->flag;
     const bool first = (flag & F_RESTART) != 0;
@@ -113,51 +113,133 @@ __global__ __launch_bounds__(kWG) void k_mx1(Mat<float> m, int N, const float* _
     const float af = (float)st->alpha;
     const float* p_old = (k & 1) ? pb0 : pb1;
     float* p_new = (k & 1) ? pb1 : pb0;
-    if (first) {
-        P::run(m, N, t.row0, t.nrows, [&](int c) { return r[c]; }, sh, sh_y);
-    } else {
-        P::run(m, N, t.row0, t.nrows, [&](int c) { return fmaf(bf, p_old[c], r[c]); }, sh,
-               sh_y);
-    }
-    float rv[4], pv[4], xv[4], pn[4], qv[4];
-    load4(r, t.i0, N, rv);
-    if (!first || !folded) load4(p_old, t.i0, N, pv);
-    if (!folded) load4(xc, t.i0, N, xv);
-    double leaf[1] = {0.0};
+    double leaf[C][1];
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-        pn[kk] = first ? rv[kk] : fmaf(bf, pv[kk], rv[kk]);
-        qv[kk] = sh_y[threadIdx.x * kLeaf + kk];
-        xv[kk] = folded ? 0.0f : fmaf(af, pv[kk], xv[kk]);
-        if (t.i0 + kk < N) leaf[0] = fma((double)pn[kk], (double)qv[kk], leaf[0]);
+    for (int c = 0; c < C; ++c) {
+        leaf[c][0] = 0.0;
+        if (c >= sup.nt) continue;
+        if (c > 0) __syncthreads();   // sh / sh_y reuse
+        const TileRows t(N, sup.tile0 + c);
+        if (first) {
+            P::run(m, N, t.row0, t.nrows, [&](int j) { return r[j]; }, sh, sh_y);
+        } else {
+            P::run(m, N, t.row0, t.nrows, [&](int j) { return fmaf(bf, p_old[j], r[j]); }, sh,
+                   sh_y);
+        }
+        float rv[4], pv[4], xv[4], pn[4], qv[4];
+        load4(r, t.i0, N, rv);
+        if (!first || !folded) load4(p_old, t.i0, N, pv);
+        if (!folded) load4(xc, t.i0, N, xv);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            pn[kk] = first ? rv[kk] : fmaf(bf, pv[kk], rv[kk]);
+            qv[kk] = sh_y[threadIdx.x * kLeaf + kk];
+            xv[kk] = folded ? 0.0f : fmaf(af, pv[kk], xv[kk]);
+            if (t.i0 + kk < N) leaf[c][0] = fma((double)pn[kk], (double)qv[kk], leaf[c][0]);
+        }
+        store4(p_new, t.i0, N, pn);
+        store4(q, t.i0, N, qv);
+        store4(xc, t.i0, N, xv);
     }
-    store4(p_new, t.i0, N, pn);
-    store4(q, t.i0, N, qv);
-    store4(xc, t.i0, N, xv);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+    finish_tiles<1, C>(red, sup.tile0, sup.nt, leaf, shr, [&](const double (&v)[1]) {
         st->alpha = st->rho0 / v[0];
         st->flag = st->flag & ~(F_RESTART | F_FOLDED);
     });
 }
 
+// Split schedule, pass 1: the direction and the deferred correction of k_mx1,
+// streamed (same float arithmetic per row).
+template <int C>
+__global__ __launch_bounds__(kWG) void k_mx_dir(int N, const float* __restrict__ r,
+                                                float* __restrict__ pb0, float* __restrict__ pb1,
+                                                float* __restrict__ xc, const KState* st) {
+    if (st->done || (st->flag & F_UPD)) return;
+    const SuperTile<C> sup(N, true);
+    const int k = st->k + 1;
+    const int flag = st->flag;
+    const bool first = (flag & F_RESTART) != 0;
+    const bool folded = (flag & F_FOLDED) != 0;
+    const float bf = (float)st->beta;
+    const float af = (float)st->alpha;
+    const float* p_old = (k & 1) ? pb0 : pb1;
+    float* p_new = (k & 1) ? pb1 : pb0;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        if (c >= sup.nt) continue;
+        const TileRows t(N, sup.tile0 + c);
+        float rv[4], pv[4] = {0.0f, 0.0f, 0.0f, 0.0f}, xv[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pn[4];
+        load4(r, t.i0, N, rv);
+        if (!first || !folded) load4(p_old, t.i0, N, pv);
+        if (!folded) load4(xc, t.i0, N, xv);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            pn[kk] = first ? rv[kk] : fmaf(bf, pv[kk], rv[kk]);
+            xv[kk] = folded ? 0.0f : fmaf(af, pv[kk], xv[kk]);
+        }
+        store4(p_new, t.i0, N, pn);
+        store4(xc, t.i0, N, xv);
+    }
+}
+
+// Split schedule, pass 2: q = A32 p_k with one gathered vector, (p_k, q) -> alpha_k.
+template <class P, int C>
+__global__ __launch_bounds__(kWG) void k_mx1s(Mat<float> m, int N, const float* __restrict__ pb0,
+                                              const float* __restrict__ pb1,
+                                              float* __restrict__ q, KState* st, RedWs red) {
+    if (st->done || (st->flag & F_UPD)) return;
+    __shared__ typename P::template Shared<float> sh;
+    __shared__ float sh_y[kTile];
+    __shared__ RedShared shr;
+    const SuperTile<C> sup(N, false);
+    const float* p = ((st->k + 1) & 1) ? pb1 : pb0;
+    double leaf[C][1];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        leaf[c][0] = 0.0;
+        if (c >= sup.nt) continue;
+        if (c > 0) __syncthreads();
+        const TileRows t(N, sup.tile0 + c);
+        P::run(m, N, t.row0, t.nrows, [&](int j) { return p[j]; }, sh, sh_y);
+        float pv[4], qv[4];
+        load4(p, t.i0, N, pv);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            qv[kk] = sh_y[threadIdx.x * kLeaf + kk];
+            if (t.i0 + kk < N) leaf[c][0] = fma((double)pv[kk], (double)qv[kk], leaf[c][0]);
+        }
+        store4(q, t.i0, N, qv);
+    }
+    finish_tiles<1, C>(red, sup.tile0, sup.nt, leaf, shr, [&](const double (&v)[1]) {
+        st->alpha = st->rho0 / v[0];
+        st->flag = st->flag & ~(F_RESTART | F_FOLDED);
+    });
+}
+
+template <int C>
 __global__ __launch_bounds__(kWG) void k_mx2(int N, float* __restrict__ r,
                                              const float* __restrict__ q, KState* st,
                                              RedWs red) {
     if (st->done || (st->flag & F_UPD)) return;
     __shared__ RedShared shr;
-    StreamRows t(N);
+    const SuperTile<C> sup(N, true);
     const float af = (float)st->alpha;
-    float rv[4], qv[4];
-    load4(r, t.i0, N, rv);
-    load4(q, t.i0, N, qv);
-    double leaf[1] = {0.0};
+    double leaf[C][1];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        rv[k] = fmaf(-af, qv[k], rv[k]);
-        if (t.i0 + k < N) leaf[0] = fma((double)rv[k], (double)rv[k], leaf[0]);
+    for (int c = 0; c < C; ++c) {
+        leaf[c][0] = 0.0;
+        if (c >= sup.nt) continue;
+        const TileRows t(N, sup.tile0 + c);
+        float rv[4], qv[4];
+        load4(r, t.i0, N, rv);
+        load4(q, t.i0, N, qv);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            rv[kk] = fmaf(-af, qv[kk], rv[kk]);
+            if (t.i0 + kk < N) leaf[c][0] = fma((double)rv[kk], (double)rv[kk], leaf[c][0]);
+        }
+        store4(r, t.i0, N, rv);
     }
-    store4(r, t.i0, N, rv);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+    finish_tiles<1, C>(red, sup.tile0, sup.nt, leaf, shr, [&](const double (&v)[1]) {
         st->k = st->k + 1;
         st->rho1 = st->rho0;
         st->rho0 = v[0];
@@ -308,20 +390,36 @@ int device_solve_mixed(gis_context* h, const MixedArgs& a, double* x, const doub
     // at most one FP64 update launch follows each inner iteration
     GIS_SPMV_DISPATCH(h->sched.spmv32, P32, {
         GIS_SPMV_DISPATCH(h->sched.spmv64, P64, {
-            GIS_TRY(run_blocks(h, max_iter, [&]() -> int {
-                GIS_TRY(timer_begin(h, 0));
-                hipLaunchKernelGGL((k_mx1<P32>), g, dim3(kWG), 0, h->stream, m32, N, r32, pb0,
-                                   pb1, xc, q, h->st, h->red);
-                GIS_TRY(timer_end(h));
-                GIS_TRY(timer_begin(h, 1));
-                hipLaunchKernelGGL(k_mx2, g, dim3(kWG), 0, h->stream, N, r32, q, h->st, h->red);
-                GIS_TRY(timer_end(h));
-                GIS_TRY(timer_begin(h, 2));
-                hipLaunchKernelGGL((k_mxu<P64>), gu, dim3(kWG), 0, h->stream, m64, N, b, x, xb1,
-                                   pb0, pb1, xc, r32, h->st, h->red);
-                GIS_TRY(timer_end(h));
-                return kOK;
-            }));
+            const bool split = h->sched.fp32_direction != 0;
+            GIS_COARSEN_DISPATCH(h->sched.coarsen32, CC, {
+                const dim3 gc = super_grid(N, CC);
+                GIS_TRY(run_blocks(h, max_iter, [&]() -> int {
+                    if (split) {
+                        GIS_TRY(timer_begin(h, 4));
+                        // no reduction to amortise: one tile per workgroup (measured best)
+                        hipLaunchKernelGGL((k_mx_dir<1>), g, dim3(kWG), 0, h->stream, N, r32,
+                                           pb0, pb1, xc, h->st);
+                        GIS_TRY(timer_end(h));
+                        GIS_TRY(timer_begin(h, 0));
+                        hipLaunchKernelGGL((k_mx1s<P32, CC>), gc, dim3(kWG), 0, h->stream, m32,
+                                           N, pb0, pb1, q, h->st, h->red);
+                    } else {
+                        GIS_TRY(timer_begin(h, 0));
+                        hipLaunchKernelGGL((k_mx1<P32, CC>), gc, dim3(kWG), 0, h->stream, m32, N,
+                                           r32, pb0, pb1, xc, q, h->st, h->red);
+                    }
+                    GIS_TRY(timer_end(h));
+                    GIS_TRY(timer_begin(h, 1));
+                    hipLaunchKernelGGL((k_mx2<CC>), gc, dim3(kWG), 0, h->stream, N, r32, q,
+                                       h->st, h->red);
+                    GIS_TRY(timer_end(h));
+                    GIS_TRY(timer_begin(h, 2));
+                    hipLaunchKernelGGL((k_mxu<P64>), gu, dim3(kWG), 0, h->stream, m64, N, b, x,
+                                       xb1, pb0, pb1, xc, r32, h->st, h->red);
+                    GIS_TRY(timer_end(h));
+                    return kOK;
+                }));
+            });
         });
     });
     hipLaunchKernelGGL(k_mx_final, g, dim3(kWG), 0, h->stream, N, x, xb1, pb0, pb1, xc, h->st);

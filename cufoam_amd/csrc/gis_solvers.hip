@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kWG) void k_norm_factor(Mat<double> m, int N,
 //   cg2: r -= alpha_k q, rho = (r, r), norm, history, beta, stopping rule
 // p alternates between two buffers (p_k in buffer k&1).
 // ---------------------------------------------------------------------------
-template <class P>
+template <class P, int C>
 __global__ __launch_bounds__(kWG) void k_cg1(Mat<double> m, int N, const double* __restrict__ r,
                                              double* __restrict__ pb0, double* __restrict__ pb1,
                                              double* __restrict__ x, double* __restrict__ q,
@@ -115,58 +115,141 @@ __global__ __launch_bounds__(kWG) void k_cg1(Mat<double> m, int N, const double*
     __shared__ typename P::template Shared<double> sh;
     __shared__ double sh_y[kTile];
     __shared__ RedShared shr;
-    TileRows t(N);
+    const SuperTile<C> sup(N, false);
     const int k = st->k + 1;
     const bool first = (k == 1);
     const double beta = st->beta;
     const double alpha_prev = st->alpha;
     const double* p_old = (k & 1) ? pb0 : pb1;
     double* p_new = (k & 1) ? pb1 : pb0;
-    if (first) {
-        P::run(m, N, t.row0, t.nrows, [&](int c) { return r[c]; }, sh, sh_y);
-    } else {
-        P::run(m, N, t.row0, t.nrows, [&](int c) { return fma(beta, p_old[c], r[c]); }, sh,
-               sh_y);
-    }
-    double rv[4], pv[4], xv[4], pn[4], qv[4];
-    load4(r, t.i0, N, rv);
-    if (!first) {
-        load4(p_old, t.i0, N, pv);
-        load4(x, t.i0, N, xv);
-    }
-    double leaf[1] = {0.0};
+    double leaf[C][1];
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-        pn[kk] = first ? rv[kk] : fma(beta, pv[kk], rv[kk]);
-        qv[kk] = sh_y[threadIdx.x * kLeaf + kk];
-        if (!first) xv[kk] = fma(alpha_prev, pv[kk], xv[kk]);
-        if (t.i0 + kk < N) leaf[0] = fma(pn[kk], qv[kk], leaf[0]);
+    for (int c = 0; c < C; ++c) {
+        leaf[c][0] = 0.0;
+        if (c >= sup.nt) continue;
+        if (c > 0) __syncthreads();   // sh / sh_y reuse
+        const TileRows t(N, sup.tile0 + c);
+        if (first) {
+            P::run(m, N, t.row0, t.nrows, [&](int j) { return r[j]; }, sh, sh_y);
+        } else {
+            P::run(m, N, t.row0, t.nrows, [&](int j) { return fma(beta, p_old[j], r[j]); }, sh,
+                   sh_y);
+        }
+        double rv[4], pv[4], xv[4], pn[4], qv[4];
+        load4(r, t.i0, N, rv);
+        if (!first) {
+            load4(p_old, t.i0, N, pv);
+            load4(x, t.i0, N, xv);
+        }
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            pn[kk] = first ? rv[kk] : fma(beta, pv[kk], rv[kk]);
+            qv[kk] = sh_y[threadIdx.x * kLeaf + kk];
+            if (!first) xv[kk] = fma(alpha_prev, pv[kk], xv[kk]);
+            if (t.i0 + kk < N) leaf[c][0] = fma(pn[kk], qv[kk], leaf[c][0]);
+        }
+        store4(p_new, t.i0, N, pn);
+        store4(q, t.i0, N, qv);
+        if (!first) store4(x, t.i0, N, xv);
     }
-    store4(p_new, t.i0, N, pn);
-    store4(q, t.i0, N, qv);
-    if (!first) store4(x, t.i0, N, xv);
-    finish_tile<1>(red, t.tile, leaf, shr,
-                   [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
+    finish_tiles<1, C>(red, sup.tile0, sup.nt, leaf, shr,
+                       [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
 }
 
+// Split schedule, pass 1: p_k = r + beta p_{k-1} and the deferred
+// x += alpha_{k-1} p_{k-1}, the same two fma per row as k_cg1, streamed.
+template <int C>
+__global__ __launch_bounds__(kWG) void k_cg_dir(int N, const double* __restrict__ r,
+                                                double* __restrict__ pb0,
+                                                double* __restrict__ pb1,
+                                                double* __restrict__ x, const KState* st) {
+    if (st->done) return;
+    const SuperTile<C> sup(N, true);
+    const int k = st->k + 1;
+    const double* p_old = (k & 1) ? pb0 : pb1;
+    double* p_new = (k & 1) ? pb1 : pb0;
+    const double beta = st->beta;
+    const double alpha_prev = st->alpha;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        if (c >= sup.nt) continue;
+        const TileRows t(N, sup.tile0 + c);
+        double rv[4], pv[4], xv[4];
+        load4(r, t.i0, N, rv);
+        if (k == 1) {
+            store4(p_new, t.i0, N, rv);
+            continue;
+        }
+        load4(p_old, t.i0, N, pv);
+        load4(x, t.i0, N, xv);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            xv[kk] = fma(alpha_prev, pv[kk], xv[kk]);
+            pv[kk] = fma(beta, pv[kk], rv[kk]);
+        }
+        store4(p_new, t.i0, N, pv);
+        store4(x, t.i0, N, xv);
+    }
+}
+
+// Split schedule, pass 2: q = A p_k with one gathered vector, (p_k, q) -> alpha_k.
+template <class P, int C>
+__global__ __launch_bounds__(kWG) void k_cg1s(Mat<double> m, int N,
+                                              const double* __restrict__ pb0,
+                                              const double* __restrict__ pb1,
+                                              double* __restrict__ q, KState* st, RedWs red) {
+    if (st->done) return;
+    __shared__ typename P::template Shared<double> sh;
+    __shared__ double sh_y[kTile];
+    __shared__ RedShared shr;
+    const SuperTile<C> sup(N, false);
+    const double* p = ((st->k + 1) & 1) ? pb1 : pb0;
+    double leaf[C][1];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        leaf[c][0] = 0.0;
+        if (c >= sup.nt) continue;
+        if (c > 0) __syncthreads();
+        const TileRows t(N, sup.tile0 + c);
+        P::run(m, N, t.row0, t.nrows, [&](int j) { return p[j]; }, sh, sh_y);
+        double pv[4], qv[4];
+        load4(p, t.i0, N, pv);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            qv[kk] = sh_y[threadIdx.x * kLeaf + kk];
+            if (t.i0 + kk < N) leaf[c][0] = fma(pv[kk], qv[kk], leaf[c][0]);
+        }
+        store4(q, t.i0, N, qv);
+    }
+    finish_tiles<1, C>(red, sup.tile0, sup.nt, leaf, shr,
+                       [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
+}
+
+template <int C>
 __global__ __launch_bounds__(kWG) void k_cg2(int N, double* __restrict__ r,
                                              const double* __restrict__ q, KState* st,
                                              RedWs red) {
     if (st->done) return;
     __shared__ RedShared shr;
-    StreamRows t(N);
+    const SuperTile<C> sup(N, true);
     const double alpha = st->alpha;
-    double rv[4], qv[4];
-    load4(r, t.i0, N, rv);
-    load4(q, t.i0, N, qv);
-    double leaf[1] = {0.0};
+    double leaf[C][1];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        rv[k] = fma(-alpha, qv[k], rv[k]);
-        if (t.i0 + k < N) leaf[0] = fma(rv[k], rv[k], leaf[0]);
+    for (int c = 0; c < C; ++c) {
+        leaf[c][0] = 0.0;
+        if (c >= sup.nt) continue;
+        const TileRows t(N, sup.tile0 + c);
+        double rv[4], qv[4];
+        load4(r, t.i0, N, rv);
+        load4(q, t.i0, N, qv);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            rv[kk] = fma(-alpha, qv[kk], rv[kk]);
+            if (t.i0 + kk < N) leaf[c][0] = fma(rv[kk], rv[kk], leaf[c][0]);
+        }
+        store4(r, t.i0, N, rv);
     }
-    store4(r, t.i0, N, rv);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+    finish_tiles<1, C>(red, sup.tile0, sup.nt, leaf, shr, [&](const double (&v)[1]) {
         st->rho1 = st->rho0;
         st->rho0 = v[0];
         st->beta = st->rho0 / st->rho1;
@@ -401,16 +484,32 @@ int solve_cg(gis_context* h, const SolveArgs& a, double* x, const double* b) {
         hipLaunchKernelGGL((k_setup<P>), g, dim3(kWG), 0, h->stream, m, N, b, x, nullptr, r,
                            nullptr, h->st, h->red, 0);
         GIS_HIP(h, hipGetLastError());
-        GIS_TRY(run_blocks(h, h->st_host->max_steps, [&]() -> int {
-            GIS_TRY(timer_begin(h, 0));
-            hipLaunchKernelGGL((k_cg1<P>), g, dim3(kWG), 0, h->stream, m, N, r, pb0, pb1, x, q,
-                               h->st, h->red);
-            GIS_TRY(timer_end(h));
-            GIS_TRY(timer_begin(h, 1));
-            hipLaunchKernelGGL(k_cg2, g, dim3(kWG), 0, h->stream, N, r, q, h->st, h->red);
-            GIS_TRY(timer_end(h));
-            return kOK;
-        }));
+        const bool split = h->sched.fp64_direction != 0;
+        GIS_COARSEN_DISPATCH(h->sched.coarsen64, CC, {
+            const dim3 gc = super_grid(N, CC);
+            GIS_TRY(run_blocks(h, h->st_host->max_steps, [&]() -> int {
+                if (split) {
+                    GIS_TRY(timer_begin(h, 4));
+                    // no reduction to amortise: one tile per workgroup (measured best)
+                    hipLaunchKernelGGL((k_cg_dir<1>), g, dim3(kWG), 0, h->stream, N, r, pb0,
+                                       pb1, x, h->st);
+                    GIS_TRY(timer_end(h));
+                    GIS_TRY(timer_begin(h, 0));
+                    hipLaunchKernelGGL((k_cg1s<P, CC>), gc, dim3(kWG), 0, h->stream, m, N, pb0,
+                                       pb1, q, h->st, h->red);
+                } else {
+                    GIS_TRY(timer_begin(h, 0));
+                    hipLaunchKernelGGL((k_cg1<P, CC>), gc, dim3(kWG), 0, h->stream, m, N, r, pb0,
+                                       pb1, x, q, h->st, h->red);
+                }
+                GIS_TRY(timer_end(h));
+                GIS_TRY(timer_begin(h, 1));
+                hipLaunchKernelGGL((k_cg2<CC>), gc, dim3(kWG), 0, h->stream, N, r, q, h->st,
+                                   h->red);
+                GIS_TRY(timer_end(h));
+                return kOK;
+            }));
+        });
     });
     hipLaunchKernelGGL(k_cg_flush, g, dim3(kWG), 0, h->stream, N, x, pb0, pb1, h->st);
     GIS_HIP(h, hipGetLastError());

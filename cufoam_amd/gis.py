@@ -34,6 +34,12 @@ class _Params(C.Structure):
                 ("power_steps", C.c_int), ("int_bound", C.c_double)]
 
 
+class _Schedule(C.Structure):
+    _fields_ = [("spmv64", C.c_int), ("spmv32", C.c_int), ("block_iters", C.c_int),
+                ("fp64_direction", C.c_int), ("fp32_direction", C.c_int),
+                ("coarsen64", C.c_int), ("coarsen32", C.c_int)]
+
+
 class _Perf(C.Structure):
     _fields_ = [("initial_residual", C.c_double), ("final_residual", C.c_double),
                 ("n_iterations", C.c_int), ("status", C.c_int), ("norm_factor", C.c_double)]
@@ -65,8 +71,10 @@ SIGNATURES = {
     "gis_dot_device": (_i, [_vp, _vp, _vp, _dp]),
     "gis_spmv_device_async": (_i, [_vp, _vp, _vp]),
     "gis_spmv32_device_async": (_i, [_vp, _vp, _vp]),
-    "gis_set_schedule": (_i, [_vp, _i, _i, _i]),
-    "gis_get_schedule": (_i, [_vp, _ip, _ip, _ip]),
+    "gis_set_schedule": (_i, [_vp, C.POINTER(_Schedule)]),
+    "gis_get_schedule": (_i, [_vp, C.POINTER(_Schedule)]),
+    "gis_autotune": (_i, [_vp, _i, _i, C.POINTER(_Schedule), _ip, _dp]),
+    "gis_autotune_clear": (_i, []),
     "gis_set_timing": (_i, [_vp, _i]),
     "gis_get_timing": (_i, [_vp, _i, _dp, _llp]),
     "gis_norm_factor": (_i, [_vp, _vp, _vp, _dp]),
@@ -268,17 +276,40 @@ class Gis:
         self._check(st, "norm_factor")
         return out.value
 
-    def set_schedule(self, spmv64: int = -1, spmv32: int = -1, block_iters: int = -1):
-        self._check(self._L.gis_set_schedule(self._h, spmv64, spmv32, block_iters),
-                    "set_schedule")
+    SCHEDULE_FIELDS = ("spmv64", "spmv32", "block_iters", "fp64_direction", "fp32_direction",
+                       "coarsen64", "coarsen32")
 
-    def get_schedule(self) -> tuple[int, int, int]:
-        a, b, c = C.c_int(0), C.c_int(0), C.c_int(0)
-        self._check(self._L.gis_get_schedule(self._h, C.byref(a), C.byref(b), C.byref(c)),
-                    "get_schedule")
-        return a.value, b.value, c.value
+    def set_schedule(self, spmv64: int = -1, spmv32: int = -1, block_iters: int = -1,
+                     fp64_direction: int = -1, fp32_direction: int = -1, coarsen64: int = -1,
+                     coarsen32: int = -1):
+        """Tuning only (bit-identical results); negative values keep the current field."""
+        s = _Schedule(spmv64, spmv32, block_iters, fp64_direction, fp32_direction, coarsen64,
+                      coarsen32)
+        self._check(self._L.gis_set_schedule(self._h, C.byref(s)), "set_schedule")
 
-    TIMER_SLOTS = {"iter_spmv": 0, "iter_update": 1, "fp64_update": 2, "spmv": 3}
+    def get_schedule(self) -> dict:
+        s = _Schedule()
+        self._check(self._L.gis_get_schedule(self._h, C.byref(s)), "get_schedule")
+        return {f: getattr(s, f) for f in self.SCHEDULE_FIELDS}
+
+    def autotune(self, fp64: bool = True, fp32: bool = True, iterations: int = 0):
+        """Time every candidate schedule on this matrix, apply and cache the fastest.
+        Returns (schedule dict, from_cache, best_ms[fp64, fp32])."""
+        s = _Schedule()
+        cached = C.c_int(0)
+        ms = (C.c_double * 2)()
+        targets = (1 if fp64 else 0) | (2 if fp32 else 0)
+        self._check(self._L.gis_autotune(self._h, targets, iterations, C.byref(s),
+                                         C.byref(cached), ms), "autotune")
+        return ({f: getattr(s, f) for f in self.SCHEDULE_FIELDS}, bool(cached.value),
+                (ms[0], ms[1]))
+
+    @staticmethod
+    def autotune_clear():
+        load().gis_autotune_clear()
+
+    TIMER_SLOTS = {"iter_spmv": 0, "iter_update": 1, "fp64_update": 2, "spmv": 3,
+                   "iter_direction": 4}
 
     def set_timing(self, *slots):
         """Record HIP-event durations of the named slots (none: stop timing)."""
@@ -324,6 +355,7 @@ class Gis:
             st = self._L.gis_solve(self._h, m, _ptr(x), _ptr(b), C.byref(p), _ptr(hist),
                                    hist_cap, C.byref(hl))
         self._check(st, f"solve({method})")
+        self.last_iterations = max(hl.value - 1, 0)
         return x, hist[:min(hl.value, hist_cap)].copy(), st
 
     def solve_mixed(self, b, x0=None, rel_tol: float = 1e-10, inner_rel_tol: float = 1e-4,

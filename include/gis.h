@@ -114,19 +114,48 @@ GIS_API int gis_spmv_device_async(gis_handle h, const double *d_x, double *d_y);
 GIS_API int gis_spmv32_device_async(gis_handle h, const float *d_x, float *d_y);
 
 /* ---- launch schedule (tuning only: every schedule gives bit-identical results).
- * spmv64 / spmv32: 0 = sliced rows (64-row slices, one row per lane, default);
- * 1 = CSR with products staged in LDS, 2048 nonzeros per chunk; 2 = the same
- * with 4096; 3 = CSR unstaged.  block_iters: iterations enqueued between host checks
- * of the convergence flag.  Negative values leave a field unchanged. ---- */
-GIS_API int gis_set_schedule(gis_handle h, int spmv64, int spmv32, int block_iters);
-GIS_API int gis_get_schedule(gis_handle h, int *spmv64, int *spmv32, int *block_iters);
+ * Replaces the launch geometry that launch.h:116-175 derives from device
+ * attributes and caches in statics. ---- */
+typedef struct gis_schedule {
+    int spmv64;          /* FP64 SpMV: 0 = sliced rows (64-row slices, one row per lane,
+                            default); 1 = CSR with products staged in LDS, 2048 nonzeros
+                            per chunk; 2 = the same with 4096; 3 = CSR unstaged */
+    int spmv32;          /* FP32 SpMV (mixed-precision inner loop), same ids */
+    int block_iters;     /* iterations enqueued between host checks of the stop flag */
+    int fp64_direction;  /* CG direction p = r + beta p: 0 formed inside the SpMV kernel's
+                            gathers; 1 written by a separate streaming pass first */
+    int fp32_direction;  /* the same for the FP32 inner loop */
+    int coarsen64;       /* tiles (1024 rows each) per workgroup of the FP64 CG
+                            iteration kernels: 1, 2 or 4 */
+    int coarsen32;       /* the same for the FP32 inner loop */
+} gis_schedule;
+/* Negative fields leave the current value unchanged. */
+GIS_API int gis_set_schedule(gis_handle h, const gis_schedule *s);
+GIS_API int gis_get_schedule(gis_handle h, gis_schedule *s);
+
+/* ---- autotuner: times every (SpMV schedule x direction form x tiles per
+ * workgroup) candidate on the
+ * handle's matrix with `iterations` solver iterations per run (<= 0: 24) and
+ * applies the fastest.  A candidate is admitted only if its SpMV equals the
+ * sliced-row schedule bit for bit on a fixed vector and its run ends in the
+ * bit-identical solver state.  The choice is cached per (device, N, nnz):
+ * later calls (any handle) apply it without timing and set *from_cache = 1.
+ * targets: GIS_TUNE_FP64 | GIS_TUNE_FP32.  best_ms[2] (may be NULL): time of
+ * the chosen candidate's run per target (0 when cached).  Requires values. ---- */
+#define GIS_TUNE_FP64 1
+#define GIS_TUNE_FP32 2
+GIS_API int gis_autotune(gis_handle h, int targets, int iterations, gis_schedule *chosen,
+                         int *from_cache, double *best_ms);
+/* Drops every cached autotuner choice (all handles). */
+GIS_API int gis_autotune_clear(void);
 
 /* ---- kernel timing with HIP events on the handle stream (opt-in; the
  * DetailType::Time analogue of support.h:7-9 at kernel granularity).
  * gis_set_timing(h, mask) records the slots whose bit is set in mask (0 stops)
  * and zeroes the counters.  Slots: 0 = the SpMV-carrying kernel of each solver
  * iteration, 1 = the vector-update kernel of each iteration, 2 = the FP64
- * update kernel of gis_solve_mixed, 3 = gis_spmv*_device_async.  Each timed
+ * update kernel of gis_solve_mixed, 3 = gis_spmv*_device_async, 4 = the
+ * separate CG direction pass (direction form 1).  Each timed
  * launch adds two event records to the stream. ---- */
 GIS_API int gis_set_timing(gis_handle h, int slot_mask);
 GIS_API int gis_get_timing(gis_handle h, int slot, double *total_ms, long long *count);

@@ -50,16 +50,26 @@ def test_method_bit_exact_to_convergence(gpu, method):
     assert ref.hist[-1] < err or ref.hist[-1] <= err
 
 
+@pytest.mark.parametrize("method", ["BiCG", "MRJacobi", "Richardson"])
 @pytest.mark.parametrize("schedule", [0, 1, 2, 3])
-@pytest.mark.parametrize("method", ["CG", "BiCG", "MRJacobi", "Richardson"])
 def test_schedules_are_bit_identical(gpu, schedule, method):
-    s = _case(32, 6, method in ("CG", "Richardson"), 0.05)
+    s = _case(32, 6, method == "Richardson", 0.05)
     gpu.set_system(s)
     gpu.set_schedule(schedule, schedule, -1)
-    try:
-        _check(gpu, method, s, s.x0, 0.0, 40)
-    finally:
-        gpu.set_schedule(0, 0, -1)
+    _check(gpu, method, s, s.x0, 0.0, 40)
+
+
+# CG: SpMV schedule x direction form x tiles per workgroup.  48^3 = 108 tiles:
+# two reduction groups and a partial last workgroup for 2 and 4 tiles each.
+@pytest.mark.parametrize("coarsen", [1, 2, 4])
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3])
+@pytest.mark.parametrize("n", [8, 48])
+def test_cg_schedules_are_bit_identical(gpu, n, schedule, direction, coarsen):
+    s = _case(n, 6, True, 0.05)
+    gpu.set_system(s)
+    gpu.set_schedule(schedule, schedule, -1, direction, direction, coarsen, coarsen)
+    _check(gpu, "CG", s, s.x0, 0.0, 30)
 
 
 def test_bicg_breakdown_forced_from_host(gpu):

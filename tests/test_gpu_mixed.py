@@ -18,9 +18,11 @@ def _case(n, seed, shift):
 @pytest.mark.parametrize("n,seed", [(1, 0), (4, 1), (8, 2), (16, 0), (32, 3)])
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("inner", [1e-4, 1e-2])
-def test_mixed_bit_exact(gpu, n, seed, mode, inner):
+@pytest.mark.parametrize("direction,coarsen", [(0, 1), (1, 1), (0, 4), (1, 2)])
+def test_mixed_bit_exact(gpu, n, seed, mode, inner, direction, coarsen):
     s = _case(n, seed, 1e-4)
     gpu.set_system(s)
+    gpu.set_schedule(fp32_direction=direction, coarsen32=coarsen)
     A = O.ldu2csr(s)
     x0 = np.random.default_rng(seed).uniform(-1, 1, s.n_cells) if seed % 2 else s.x0
     rx, rh, rit, rst = O.mixed_cg(A, s.b, x0, 1e-10, inner, 5000, mode)
@@ -31,9 +33,11 @@ def test_mixed_bit_exact(gpu, n, seed, mode, inner):
 
 
 @pytest.mark.parametrize("max_iter", [1, 5, 37])
-def test_mixed_iteration_cap_bit_exact(gpu, max_iter):
-    s = _case(16, 5, 1e-4)
+@pytest.mark.parametrize("direction,coarsen", [(0, 1), (1, 1), (0, 2), (1, 4)])
+def test_mixed_iteration_cap_bit_exact(gpu, max_iter, direction, coarsen):
+    s = _case(48, 5, 1e-4)
     gpu.set_system(s)
+    gpu.set_schedule(fp32_direction=direction, coarsen32=coarsen)
     A = O.ldu2csr(s)
     rx, rh, rit, rst = O.mixed_cg(A, s.b, s.x0, 1e-10, 1e-2, max_iter, 0)
     x, hist, it, rel, st = gpu.solve_mixed(s.b, s.x0, 1e-10, 1e-2, max_iter, 0)
