@@ -81,6 +81,25 @@ def test_adjoint_bit_exact_against_oracle(gpu, method, symmetric):
         np.testing.assert_array_equal(dlo[k], -(ref.x[s.upper_addr] * x[s.lower_addr]))
 
 
+def test_adjoint_dense_identity_small(gpu):
+    """dL/db = A^-T w and dL/dA = -lambda x^T, against dense numpy linear algebra (6^3)."""
+    s = _case(6, 11, False, 0.2)
+    gpu.set_system(s)
+    Ad = mesh.ldu_to_dense(s)
+    x = np.linalg.solve(Ad, s.b)
+    W = np.random.default_rng(12).standard_normal((3, s.n_cells))
+    lam, dd, dlo, dup, it = gpu.adjoint(W, x, method="BiCG", error=1e-11, max_steps=5000)
+    for k in range(3):
+        lam_ref = np.linalg.solve(Ad.T, W[k])
+        tol = dict(rtol=1e-8, atol=1e-10)
+        np.testing.assert_allclose(lam[k], lam_ref, **tol)
+        G = -np.outer(lam_ref, x)             # dL/dA, dense
+        np.testing.assert_allclose(dd[k], np.diag(G), **tol)
+        # A(owner, nbr) = upper[f], A(nbr, owner) = lower[f]
+        np.testing.assert_allclose(dup[k], G[s.lower_addr, s.upper_addr], **tol)
+        np.testing.assert_allclose(dlo[k], G[s.upper_addr, s.lower_addr], **tol)
+
+
 def test_adjoint_finite_differences_64(gpu):
     """64^3 SPD, loss L_k = w_k . x, 8 loss vectors, central differences (step 1e-6)
     on 32 coefficients (16 diagonal, 16 symmetric face pairs) and 32 b entries."""
