@@ -335,27 +335,110 @@ __device__ __forceinline__ T sell_row(const int* __restrict__ scol, const T* __r
     return acc;
 }
 
+// Two rows per lane from two slices at once: both slices' column and value
+// loads are in flight together (twice the memory-level parallelism of
+// sell_row per wave).  Each row is still summed sequentially in slot order.
+template <typename T, class XOp>
+__device__ __forceinline__ void sell_row2(const int* __restrict__ scol,
+                                          const T* __restrict__ sval, int a0, int len0, int a1,
+                                          int len1, int lane, XOp&& xop, T& y0, T& y1) {
+    T acc0 = T(0), acc1 = T(0);
+    const int len = len0 > len1 ? len0 : len1;
+    for (int j = 0; j < len; j += 8) {
+        int c0[8], c1[8];
+        T v0[8], v1[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int s = (j + u) * kSlice + lane;
+            c0[u] = -1;
+            v0[u] = T(0);
+            c1[u] = -1;
+            v1[u] = T(0);
+            if (j + u < len0) {
+                c0[u] = __builtin_nontemporal_load(scol + a0 + s);
+                v0[u] = __builtin_nontemporal_load(sval + a0 + s);
+            }
+            if (j + u < len1) {
+                c1[u] = __builtin_nontemporal_load(scol + a1 + s);
+                v1[u] = __builtin_nontemporal_load(sval + a1 + s);
+            }
+        }
+        T x0[8], x1[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            x0[u] = c0[u] >= 0 ? xop(c0[u]) : T(0);
+            x1[u] = c1[u] >= 0 ? xop(c1[u]) : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (c0[u] >= 0) acc0 = acc0 + v0[u] * x0[u];
+            if (c1[u] >= 0) acc1 = acc1 + v1[u] * x1[u];
+        }
+    }
+    y0 = acc0;
+    y1 = acc1;
+}
+
+// y[row0 .. row0+1024) of one tile into sh_y (tile row order).  Each wave sums
+// 4 slices of 64 rows, one row per lane; KPAIR takes the 4 slices two at a
+// time (sell_row2).
+template <typename T, bool KPAIR, class XOp>
+__device__ __forceinline__ void tile_sell_any(const int* __restrict__ soff,
+                                              const int* __restrict__ scol,
+                                              const T* __restrict__ sval, int N, int row0,
+                                              XOp&& xop, T* __restrict__ sh_y) {
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int nslices = (N + kSlice - 1) / kSlice;
+    const int s0 = row0 / kSlice + w * 4;
+    if (KPAIR) {
+#pragma unroll
+        for (int c = 0; c < 4; c += 2) {
+            const int s = s0 + c;
+            int a0 = 0, len0 = 0, a1 = 0, len1 = 0;
+            if (s < nslices) {
+                a0 = soff[s];
+                len0 = (soff[s + 1] - a0) / kSlice;
+            }
+            if (s + 1 < nslices) {
+                a1 = soff[s + 1];
+                len1 = (soff[s + 2] - a1) / kSlice;
+            }
+            T y0, y1;
+            sell_row2<T>(scol, sval, a0, len0, a1, len1, lane, xop, y0, y1);
+            sh_y[(w * 4 + c) * kSlice + lane] = y0;
+            sh_y[(w * 4 + c + 1) * kSlice + lane] = y1;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int s = s0 + c;
+            T y = T(0);
+            if (s < nslices) {
+                const int a = soff[s];
+                const int len = (soff[s + 1] - a) / kSlice;
+                y = sell_row<T>(scol, sval, a, len, lane, xop);
+            }
+            sh_y[(w * 4 + c) * kSlice + lane] = y;
+        }
+    }
+    wave_sync_lds();
+}
+
 template <typename T, class XOp>
 __device__ __forceinline__ void tile_sell(const int* __restrict__ soff,
                                           const int* __restrict__ scol,
                                           const T* __restrict__ sval, int N, int row0,
                                           XOp&& xop, T* __restrict__ sh_y) {
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int nslices = (N + kSlice - 1) / kSlice;
-    const int s0 = row0 / kSlice + w * 4;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int s = s0 + c;
-        T y = T(0);
-        if (s < nslices) {
-            const int a = soff[s];
-            const int len = (soff[s + 1] - a) / kSlice;
-            y = sell_row<T>(scol, sval, a, len, lane, xop);
-        }
-        sh_y[(w * 4 + c) * kSlice + lane] = y;
-    }
-    wave_sync_lds();
+    tile_sell_any<T, false>(soff, scol, sval, N, row0, xop, sh_y);
+}
+
+template <typename T, class XOp>
+__device__ __forceinline__ void tile_sell2(const int* __restrict__ soff,
+                                           const int* __restrict__ scol,
+                                           const T* __restrict__ sval, int N, int row0,
+                                           XOp&& xop, T* __restrict__ sh_y) {
+    tile_sell_any<T, true>(soff, scol, sval, N, row0, xop, sh_y);
 }
 
 // Unstaged variant: each thread sums its own kLeaf consecutive rows straight

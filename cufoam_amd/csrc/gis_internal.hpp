@@ -45,7 +45,7 @@ struct DevMem {
 
 // SpMV schedule: how a tile's products are formed.  Every schedule yields
 // bit-identical results (same per-row order), so the choice is pure tuning.
-enum class SpmvKind : int { Sell = 0, Staged2K = 1, Staged4K = 2, Direct = 3 };
+enum class SpmvKind : int { Sell = 0, Staged2K = 1, Staged4K = 2, Direct = 3, Sell2 = 4 };
 
 struct Schedule {
     SpmvKind spmv64 = SpmvKind::Sell;       // FP64 kernels

@@ -32,6 +32,17 @@ struct Sell {
     }
 };
 
+// Sliced rows, two slices per lane at a time (more loads in flight per wave).
+struct Sell2 {
+    template <typename T>
+    struct Shared {};
+    template <typename T, class XOp>
+    __device__ static void run(const Mat<T>& m, int N, int row0, int, XOp&& xop, Shared<T>&,
+                               T* y) {
+        tile_sell2<T>(m.soff, m.scol, m.sval, N, row0, xop, y);
+    }
+};
+
 template <int CAP>
 struct Staged {
     template <typename T>
@@ -65,6 +76,10 @@ struct Direct {
         } break;                                      \
         case SpmvKind::Direct: {                      \
             using POLICY = Direct;                    \
+            __VA_ARGS__;                              \
+        } break;                                      \
+        case SpmvKind::Sell2: {                       \
+            using POLICY = Sell2;                     \
             __VA_ARGS__;                              \
         } break;                                      \
         default: {                                    \

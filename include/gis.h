@@ -119,7 +119,8 @@ GIS_API int gis_spmv32_device_async(gis_handle h, const float *d_x, float *d_y);
 typedef struct gis_schedule {
     int spmv64;          /* FP64 SpMV: 0 = sliced rows (64-row slices, one row per lane,
                             default); 1 = CSR with products staged in LDS, 2048 nonzeros
-                            per chunk; 2 = the same with 4096; 3 = CSR unstaged */
+                            per chunk; 2 = the same with 4096; 3 = CSR unstaged;
+                            4 = sliced rows, two slices per lane at a time */
     int spmv32;          /* FP32 SpMV (mixed-precision inner loop), same ids */
     int block_iters;     /* iterations enqueued between host checks of the stop flag */
     int fp64_direction;  /* CG direction p = r + beta p: 0 formed inside the SpMV kernel's
