@@ -210,6 +210,72 @@ __device__ __forceinline__ void finish_tiles(const RedWs& red, int tile0, int nt
     fold_upper<NV>(red, tile0 / kGroup, fin);
 }
 
+// finish_tiles in two phases, so that a kernel's output stores overlap the
+// ticket's round trip instead of delaying it:
+//   begin(): the tiles' subtrees are formed and published as level-0
+//            partials and thread 0 takes the workgroup's ticket (the release
+//            wait covers only what was issued before: loads and partials);
+//   ...      the kernel issues its output stores;
+//   end():   waves 1..3 are done; wave 0 learns whether it arrived last in its
+//            group and, if so, folds the upper levels and calls fin.
+// Partials, tree and totals are those of finish_tiles.
+template <int NV, int C>
+struct TileReduce {
+    double flat[C * NV];
+    unsigned prev;
+    int tile0, nt;
+
+    __device__ void begin(const RedWs& red, int tile0_, int nt_, double (&v)[C][NV],
+                          RedShared& sh) {
+        static_assert(C * NV <= kMaxNV, "RedShared holds kMaxNV values per wave");
+        tile0 = tile0_;
+        nt = nt_;
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) flat[c * NV + i] = v[c][i];
+        block_tree<C * NV>(flat, sh.w);
+        prev = 0;
+        if (red.count[0] != 1 && threadIdx.x == 0) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                if (c < nt) {
+#pragma unroll
+                    for (int i = 0; i < NV; ++i)
+                        st_agent(&red.part[0][(size_t)(tile0 + c) * kMaxNV + i],
+                                 flat[c * NV + i]);
+                }
+            }
+            order_release();
+            prev = ticket_add_n(&red.ticket[0][tile0 / kGroup], (unsigned)nt);
+        }
+    }
+
+    template <class Fin>
+    __device__ void end(const RedWs& red, Fin&& fin) {
+        if (threadIdx.x >= 64) return;
+        if (red.count[0] == 1) {
+            if (threadIdx.x == 0) {
+                double t[NV];
+#pragma unroll
+                for (int i = 0; i < NV; ++i) t[i] = flat[i];
+                fin(t);
+            }
+            return;
+        }
+        int last = 0;
+        if (threadIdx.x == 0) {
+            const int g = tile0 / kGroup;
+            const int members = min(kGroup, red.count[0] - g * kGroup);
+            last = (prev + (unsigned)nt == (unsigned)members);
+            if (last) ticket_reset(&red.ticket[0][g]);
+        }
+        last = __shfl(last, 0, 64);
+        if (!last) return;
+        fold_upper<NV>(red, tile0 / kGroup, fin);
+    }
+};
+
 // One tile per workgroup.
 template <int NV, class Fin>
 __device__ __forceinline__ void finish_tile(const RedWs& red, int tile, double (&v)[NV],
@@ -421,6 +487,73 @@ __device__ __forceinline__ void tile_sell_any(const int* __restrict__ soff,
             }
             sh_y[(w * 4 + c) * kSlice + lane] = y;
         }
+    }
+    wave_sync_lds();
+}
+
+// One row per lane, compressed columns: slot j of the row sits in column
+// row + sbase[a/64 + j] + sres[a + 64 j + lane] (residual -128: padding).  The
+// base is one int per 64 slots, the residual one byte, so the index stream is
+// about a quarter of sell_row's.  Same products, same order, same sum.
+template <typename T, class XOp>
+__device__ __forceinline__ T sell_row_c(const signed char* __restrict__ sres,
+                                        const int* __restrict__ sbase,
+                                        const T* __restrict__ sval, int a, int len, int lane,
+                                        int row, XOp&& xop) {
+    T acc = T(0);
+    const int a64 = a / kSlice;
+    for (int j = 0; j < len; j += 8) {
+        int cc[8];
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = a + (j + u) * kSlice + lane;
+            cc[u] = -1;
+            v[u] = T(0);
+            if (j + u < len) {
+                const int r = sres[idx];
+                const int b = sbase[a64 + j + u];
+                cc[u] = (r == -128) ? -1 : row + b + r;
+                v[u] = __builtin_nontemporal_load(sval + idx);
+            }
+        }
+        T xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = cc[u] >= 0 ? xop(cc[u]) : T(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (cc[u] >= 0) acc = acc + v[u] * xv[u];
+    }
+    return acc;
+}
+
+// tile_sell over compressed columns; slices whose residuals do not fit in a
+// byte (sflag 0) read the int32 columns.
+template <typename T, class XOp>
+__device__ __forceinline__ void tile_sell_c(const int* __restrict__ soff,
+                                            const int* __restrict__ scol,
+                                            const signed char* __restrict__ sres,
+                                            const int* __restrict__ sbase,
+                                            const unsigned char* __restrict__ sflag,
+                                            const T* __restrict__ sval, int N, int row0,
+                                            XOp&& xop, T* __restrict__ sh_y) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nslices = (N + kSlice - 1) / kSlice;
+    const int s0 = row0 / kSlice + w * 4;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int s = s0 + c;
+        T y = T(0);
+        if (s < nslices) {
+            const int a = soff[s];
+            const int len = (soff[s + 1] - a) / kSlice;
+            if (sflag[s])
+                y = sell_row_c<T>(sres, sbase, sval, a, len, lane, s * kSlice + lane, xop);
+            else
+                y = sell_row<T>(scol, sval, a, len, lane, xop);
+        }
+        sh_y[(w * 4 + c) * kSlice + lane] = y;
     }
     wave_sync_lds();
 }

@@ -45,7 +45,9 @@ struct DevMem {
 
 // SpMV schedule: how a tile's products are formed.  Every schedule yields
 // bit-identical results (same per-row order), so the choice is pure tuning.
-enum class SpmvKind : int { Sell = 0, Staged2K = 1, Staged4K = 2, Direct = 3, Sell2 = 4 };
+enum class SpmvKind : int {
+    Sell = 0, Staged2K = 1, Staged4K = 2, Direct = 3, Sell2 = 4, SellC = 5
+};
 
 struct Schedule {
     SpmvKind spmv64 = SpmvKind::Sell;       // FP64 kernels
@@ -119,6 +121,9 @@ struct gis_context {
     long long n_slots = 0;                      // sliced layout (64-row slices)
     gis::DevMem soff, scol, sperm;              // slice offsets, slot columns / codes
     gis::DevMem sval, svalT, sval32;            // slot values: A, A^T, float(A)
+    // compressed slot columns: col = row + sbase[slot row] + sres[slot], int8
+    // residuals (-128 marks padding); sflag[slice] = 1 where every residual fits
+    gis::DevMem sres, sbase, sflag;
     gis::DevMem dinv;                           // per-row helper (1/a_ii or sqrt(1/a_ii))
 
     // workspaces

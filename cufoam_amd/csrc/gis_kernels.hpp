@@ -17,6 +17,9 @@ struct Mat {
     const int* soff;
     const int* scol;
     const T* sval;
+    const signed char* sres;      // compressed slot columns (SellC)
+    const int* sbase;
+    const unsigned char* sflag;
 };
 
 // ---------------------------------------------------------------------------
@@ -29,6 +32,17 @@ struct Sell {
     __device__ static void run(const Mat<T>& m, int N, int row0, int, XOp&& xop, Shared<T>&,
                                T* y) {
         tile_sell<T>(m.soff, m.scol, m.sval, N, row0, xop, y);
+    }
+};
+
+// Sliced rows with compressed columns (1 byte per slot instead of 4).
+struct SellC {
+    template <typename T>
+    struct Shared {};
+    template <typename T, class XOp>
+    __device__ static void run(const Mat<T>& m, int N, int row0, int, XOp&& xop, Shared<T>&,
+                               T* y) {
+        tile_sell_c<T>(m.soff, m.scol, m.sres, m.sbase, m.sflag, m.sval, N, row0, xop, y);
     }
 };
 
@@ -80,6 +94,10 @@ struct Direct {
         } break;                                      \
         case SpmvKind::Sell2: {                       \
             using POLICY = Sell2;                     \
+            __VA_ARGS__;                              \
+        } break;                                      \
+        case SpmvKind::SellC: {                       \
+            using POLICY = SellC;                     \
             __VA_ARGS__;                              \
         } break;                                      \
         default: {                                    \
@@ -207,17 +225,23 @@ inline dim3 super_grid(int N, int C) { return dim3((unsigned)((tiles_of(N) + C -
 
 inline Mat<double> mat64(const gis_context* h) {
     return Mat<double>{h->rowptr.as<int>(), h->col.as<int>(), h->val.as<double>(),
-                       h->soff.as<int>(), h->scol.as<int>(), h->sval.as<double>()};
+                       h->soff.as<int>(), h->scol.as<int>(), h->sval.as<double>(),
+                       h->sres.as<signed char>(), h->sbase.as<int>(),
+                       h->sflag.as<unsigned char>()};
 }
 
 inline Mat<double> mat64T(const gis_context* h) {
     return Mat<double>{h->rowptr.as<int>(), h->col.as<int>(), h->valT.as<double>(),
-                       h->soff.as<int>(), h->scol.as<int>(), h->svalT.as<double>()};
+                       h->soff.as<int>(), h->scol.as<int>(), h->svalT.as<double>(),
+                       h->sres.as<signed char>(), h->sbase.as<int>(),
+                       h->sflag.as<unsigned char>()};
 }
 
 inline Mat<float> mat32(const gis_context* h) {
     return Mat<float>{h->rowptr.as<int>(), h->col.as<int>(), h->val32.as<float>(),
-                      h->soff.as<int>(), h->scol.as<int>(), h->sval32.as<float>()};
+                      h->soff.as<int>(), h->scol.as<int>(), h->sval32.as<float>(),
+                      h->sres.as<signed char>(), h->sbase.as<int>(),
+                      h->sflag.as<unsigned char>()};
 }
 
 // Enqueue iterations in blocks until the device reports done.

@@ -193,6 +193,7 @@ __global__ __launch_bounds__(kWG) void k_mx1s(Mat<float> m, int N, const float* 
     const SuperTile<C> sup(N, false);
     const float* p = ((st->k + 1) & 1) ? pb1 : pb0;
     double leaf[C][1];
+    float qv[C][4];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         leaf[c][0] = 0.0;
@@ -200,16 +201,20 @@ __global__ __launch_bounds__(kWG) void k_mx1s(Mat<float> m, int N, const float* 
         if (c > 0) __syncthreads();
         const TileRows t(N, sup.tile0 + c);
         P::run(m, N, t.row0, t.nrows, [&](int j) { return p[j]; }, sh, sh_y);
-        float pv[4], qv[4];
+        float pv[4];
         load4(p, t.i0, N, pv);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
-            qv[kk] = sh_y[threadIdx.x * kLeaf + kk];
-            if (t.i0 + kk < N) leaf[c][0] = fma((double)pv[kk], (double)qv[kk], leaf[c][0]);
+            qv[c][kk] = sh_y[threadIdx.x * kLeaf + kk];
+            if (t.i0 + kk < N) leaf[c][0] = fma((double)pv[kk], (double)qv[c][kk], leaf[c][0]);
         }
-        store4(q, t.i0, N, qv);
     }
-    finish_tiles<1, C>(red, sup.tile0, sup.nt, leaf, shr, [&](const double (&v)[1]) {
+    TileReduce<1, C> tr;
+    tr.begin(red, sup.tile0, sup.nt, leaf, shr);
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        if (c < sup.nt) store4(q, TileRows(N, sup.tile0 + c).i0, N, qv[c]);
+    tr.end(red, [&](const double (&v)[1]) {
         st->alpha = st->rho0 / v[0];
         st->flag = st->flag & ~(F_RESTART | F_FOLDED);
     });
@@ -224,22 +229,27 @@ __global__ __launch_bounds__(kWG) void k_mx2(int N, float* __restrict__ r,
     const SuperTile<C> sup(N, true);
     const float af = (float)st->alpha;
     double leaf[C][1];
+    float rv[C][4];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         leaf[c][0] = 0.0;
         if (c >= sup.nt) continue;
         const TileRows t(N, sup.tile0 + c);
-        float rv[4], qv[4];
-        load4(r, t.i0, N, rv);
+        float qv[4];
+        load4(r, t.i0, N, rv[c]);
         load4(q, t.i0, N, qv);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
-            rv[kk] = fmaf(-af, qv[kk], rv[kk]);
-            if (t.i0 + kk < N) leaf[c][0] = fma((double)rv[kk], (double)rv[kk], leaf[c][0]);
+            rv[c][kk] = fmaf(-af, qv[kk], rv[c][kk]);
+            if (t.i0 + kk < N) leaf[c][0] = fma((double)rv[c][kk], (double)rv[c][kk], leaf[c][0]);
         }
-        store4(r, t.i0, N, rv);
     }
-    finish_tiles<1, C>(red, sup.tile0, sup.nt, leaf, shr, [&](const double (&v)[1]) {
+    TileReduce<1, C> tr;
+    tr.begin(red, sup.tile0, sup.nt, leaf, shr);
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        if (c < sup.nt) store4(r, TileRows(N, sup.tile0 + c).i0, N, rv[c]);
+    tr.end(red, [&](const double (&v)[1]) {
         st->k = st->k + 1;
         st->rho1 = st->rho0;
         st->rho0 = v[0];

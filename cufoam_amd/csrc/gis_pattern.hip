@@ -14,6 +14,8 @@
 //   perm[p] in [M, M+N)    -> diag[c]
 //   perm[p] in [M+N, 2M+N) -> upper[f]
 // A^T of an LDU matrix has the same pattern with lower and upper swapped.
+#include <climits>
+
 #include "gis_internal.hpp"
 
 namespace gis {
@@ -195,6 +197,48 @@ __global__ __launch_bounds__(kBlk) void k_sell_fill(int N, const int* __restrict
     }
 }
 
+// Compressed slot columns, one wave per slice.  For each slot row u the
+// column offsets d = col - row of the valid lanes get a base B = midpoint of
+// their range, and each lane stores d - B in one byte (-128: padding).  The
+// slice is flagged usable only if every slot row's range spans at most 254.
+__global__ __launch_bounds__(kSlice) void k_sell_compress(int S, const int* __restrict__ soff,
+                                                          const int* __restrict__ scol,
+                                                          signed char* __restrict__ sres,
+                                                          int* __restrict__ sbase,
+                                                          unsigned char* __restrict__ sflag) {
+    const int s = blockIdx.x;
+    if (s >= S) return;
+    const int lane = threadIdx.x;
+    const int a = soff[s];
+    const int len = (soff[s + 1] - a) / kSlice;
+    const long long row = (long long)s * kSlice + lane;
+    int ok = 1;
+    for (int u = 0; u < len; ++u) {
+        const int idx = a + u * kSlice + lane;
+        const int c = scol[idx];
+        const bool valid = c >= 0;
+        const long long d = valid ? (long long)c - row : 0;
+        long long lo = valid ? d : LLONG_MAX;
+        long long hi = valid ? d : LLONG_MIN;
+#pragma unroll
+        for (int m = 1; m < kSlice; m <<= 1) {
+            lo = min(lo, (long long)__shfl_xor(lo, m, kSlice));
+            hi = max(hi, (long long)__shfl_xor(hi, m, kSlice));
+        }
+        long long base = 0;
+        if (lo <= hi) {
+            if (hi - lo > 254) ok = 0;
+            base = lo + (hi - lo) / 2;
+        }
+        const bool fits = ok && base >= INT_MIN && base <= INT_MAX;
+        if (!fits) ok = 0;
+        const long long r = valid ? d - base : -128;
+        sres[idx] = (signed char)(ok ? r : -128);
+        if (lane == 0) sbase[a / kSlice + u] = ok ? (int)base : 0;
+    }
+    if (lane == 0) sflag[s] = (unsigned char)ok;
+}
+
 int scan_exclusive(gis_context* h, const int* in, int* out, int n, int* tmp, size_t tmp_ints) {
     const int nb = (n + kBlk * 4 - 1) / (kBlk * 4);
     if (nb <= 1) {
@@ -288,6 +332,13 @@ int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua
     hipLaunchKernelGGL(k_sell_fill, grid_for(N), dim3(kBlk), 0, h->stream, N,
                        h->rowptr.as<int>(), h->col.as<int>(), h->perm.as<int>(),
                        h->soff.as<int>(), h->scol.as<int>(), h->sperm.as<int>());
+    GIS_HIP(h, hipGetLastError());
+    GIS_HIP(h, h->sres.ensure((size_t)n_slots));
+    GIS_HIP(h, h->sbase.ensure(sizeof(int) * ((size_t)n_slots / kSlice + 1)));
+    GIS_HIP(h, h->sflag.ensure((size_t)S + 1));
+    hipLaunchKernelGGL(k_sell_compress, dim3((unsigned)S), dim3(kSlice), 0, h->stream, S,
+                       h->soff.as<int>(), h->scol.as<int>(), h->sres.as<signed char>(),
+                       h->sbase.as<int>(), h->sflag.as<unsigned char>());
     GIS_HIP(h, hipGetLastError());
     h->n_slots = n_slots;
     h->N = N;

@@ -2,8 +2,9 @@
 // makes once from device attributes (and caches in statics), made here by
 // timing every candidate schedule on the handle's own matrix.
 //
-// Candidates: the five SpMV schedules (sliced rows, LDS-staged CSR 2K/4K,
-// unstaged CSR, sliced rows two slices at a time) times the two CG direction forms (fused into the SpMV
+// Candidates: the six SpMV schedules (sliced rows, LDS-staged CSR 2K/4K,
+// unstaged CSR, sliced rows two slices at a time, sliced rows with compressed
+// columns) times the two CG direction forms (fused into the SpMV
 // gathers, or a separate streaming pass) times 1, 2 or 4 tiles per workgroup
 // of the iteration kernels, separately for the FP64 kernels
 // (plain CG) and the FP32 kernels (mixed-precision CG).  A candidate is
@@ -74,7 +75,8 @@ constexpr Candidate kCandidates[] = {
     {1, 0, 1}, {1, 1, 1}, {1, 0, 2}, {1, 1, 2}, {1, 0, 4}, {1, 1, 4},
     {2, 0, 1}, {2, 1, 1}, {2, 0, 2}, {2, 1, 2}, {2, 0, 4}, {2, 1, 4},
     {3, 0, 1}, {3, 1, 1}, {3, 0, 2}, {3, 1, 2}, {3, 0, 4}, {3, 1, 4},
-    {4, 0, 1}, {4, 1, 1}, {4, 0, 2}, {4, 1, 2}, {4, 0, 4}, {4, 1, 4}};
+    {4, 0, 1}, {4, 1, 1}, {4, 0, 2}, {4, 1, 2}, {4, 0, 4}, {4, 1, 4},
+    {5, 0, 1}, {5, 1, 1}, {5, 0, 2}, {5, 1, 2}, {5, 0, 4}, {5, 1, 4}};
 
 int elapsed(gis_context* h, hipEvent_t a, hipEvent_t b, float* ms) {
     GIS_HIP(h, hipEventSynchronize(b));

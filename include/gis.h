@@ -120,7 +120,10 @@ typedef struct gis_schedule {
     int spmv64;          /* FP64 SpMV: 0 = sliced rows (64-row slices, one row per lane,
                             default); 1 = CSR with products staged in LDS, 2048 nonzeros
                             per chunk; 2 = the same with 4096; 3 = CSR unstaged;
-                            4 = sliced rows, two slices per lane at a time */
+                            4 = sliced rows, two slices per lane at a time;
+                            5 = sliced rows with compressed columns (an int32 base
+                            per 64 slots plus one signed byte per slot; slices
+                            whose offsets do not fit read the int32 columns) */
     int spmv32;          /* FP32 SpMV (mixed-precision inner loop), same ids */
     int block_iters;     /* iterations enqueued between host checks of the stop flag */
     int fp64_direction;  /* CG direction p = r + beta p: 0 formed inside the SpMV kernel's

@@ -76,6 +76,43 @@ def test_spmv_dot_bit_exact(gpu, n, seed):
     assert d == O.dot(x, y)
 
 
+def _mixed_locality_system(seed):
+    """A 16^3 hex block (compressible slot columns) followed by a 4096-cell block
+    with random faces (offsets far beyond a byte): both SpMV column paths."""
+    a = mesh.laplacian(16, seed=seed, shift=0.1, symmetric=False)
+    r = mesh.random_ldu(4096, 12000, seed=seed, symmetric=False)
+    n0 = a.n_cells
+    la = np.concatenate([a.lower_addr, r.lower_addr + n0]).astype(np.int32)
+    ua = np.concatenate([a.upper_addr, r.upper_addr + n0]).astype(np.int32)
+    return mesh.LduSystem(n0 + r.n_cells, la, ua, np.concatenate([a.diag, r.diag]),
+                          np.concatenate([a.lower, r.lower]), np.concatenate([a.upper, r.upper]),
+                          np.concatenate([a.b, r.b]), np.zeros(n0 + r.n_cells))
+
+
+@pytest.mark.parametrize("schedule", [0, 4, 5])
+def test_spmv_schedules_mixed_locality(gpu, schedule):
+    torch = _torch()
+    s = _mixed_locality_system(3)
+    print(f"[case] mixed locality N={s.n_cells} schedule={schedule}", flush=True)
+    gpu.set_system(s)
+    gpu.set_schedule(schedule, schedule, -1)
+    A = O.ldu2csr(s)
+    x = np.random.default_rng(9).standard_normal(s.n_cells)
+    dx = torch.from_numpy(x).cuda()
+    assert np.array_equal(gpu.spmv(dx).cpu().numpy(), O.spmv(A, x))
+    assert np.array_equal(gpu.spmv(dx, transpose=True).cpu().numpy(),
+                          O.spmv(O.transpose(A), x))
+    x32 = dx.float()
+    y32 = torch.empty_like(x32)
+    gpu.spmv_async(x32, y32)
+    gpu.synchronize()
+    assert np.array_equal(y32.cpu().numpy(), O.spmv_f32(A, x.astype(np.float32)))
+    ref = O.solve("BiCG", A, s.b, s.x0, 0.0, 30)
+    xs, hist, st = gpu.solve("BiCG", s.b, s.x0, error=0.0, max_steps=30)
+    np.testing.assert_array_equal(hist, ref.hist)
+    np.testing.assert_array_equal(xs, ref.x)
+
+
 @pytest.mark.parametrize("n,seed", SMALL)
 def test_norm_factor_bit_exact(gpu, n, seed):
     s = _case(n, seed)
