@@ -45,25 +45,34 @@ def mx1_bytes(N, nnz, direction):
 
 
 def cpu_baseline(sys_, iters_fp64: int, sample_iters: int):
-    """The oracle's FP64 CG (the same algorithm, bit-identical iterates) on the host:
-    a bounded sample of `sample_iters` iterations, extrapolated to the iteration count
-    the solve needs."""
-    import numpy as np
+    """FP64 CG of the same system on the host, timed on bounded samples and extrapolated to
+    the iteration count the FP64 solve needs.  The reported value uses every OpenMP thread
+    (CSR matvec and vector loops parallel); one-thread figures are reported beside it: the
+    oracle's CG (the GPU path's algorithm, bit-identical iterates) and the same CG with the
+    matrix applied by OpenFOAM's LDU face loop."""
     from oracle import oracle as O
     A = O.ldu2csr(sys_)
+    threads = O.baseline_threads()
+    omp_iters = 5 * sample_iters
+    t_omp, _ = O.time_cg_csr_omp(A, sys_.b, omp_iters)
+    t_ldu, _ = O.time_cg_ldu(sys_, sample_iters)
     t0 = time.perf_counter()
     O.solve("CG", A, sys_.b, sys_.x0, 0.0, sample_iters, hist_cap=sample_iters + 1)
-    dt = time.perf_counter() - t0
-    per_iter = dt / sample_iters
+    t_orc = time.perf_counter() - t0
+    per = {"omp": t_omp / omp_iters, "ldu": t_ldu / sample_iters, "oracle": t_orc / sample_iters}
     return {
-        "value": per_iter * iters_fp64,
+        "value": per["omp"] * iters_fp64,
         "unit": "s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"{sample_iters} FP64 CG iterations of the same 256^3 system with the C "
-                  f"oracle on 1 host thread ({dt:.1f} s incl. set-up), extrapolated to the "
+        "sample": f"{omp_iters} FP64 CG iterations of the same 256^3 system on {threads} OpenMP "
+                  f"threads ({t_omp:.1f} s incl. set-up residual), extrapolated to the "
                   f"{iters_fp64} iterations the FP64 solve needs",
-        "per_iteration_s": per_iter,
+        "per_iteration_s": per["omp"],
+        "one_thread": {
+            "oracle_cg_s": per["oracle"] * iters_fp64,
+            "ldu_face_loop_cg_s": per["ldu"] * iters_fp64,
+            "sample": f"{sample_iters} iterations each, extrapolated to {iters_fp64}"},
     }
 
 
@@ -125,6 +134,53 @@ def batch_throughput(reps: int, use_graph: bool = True):
     return out
 
 
+def multi_rhs_throughput(reps: int):
+    """128^3 nonsymmetric convection-diffusion system with 3 right-hand sides (seed 0): one
+    SpMM-BiCG solve of all three columns against three single-RHS BiCG solves, each column
+    to ||r||_2 <= 1e-8 ||b_0||_2 from x0 = 0, everything resident in HBM."""
+    import numpy as np
+    import torch
+    from cufoam_amd import mesh
+    from cufoam_amd.gis import Gis
+
+    s = mesh.config_system("convdiff128x3")
+    g = Gis(0)
+    g.set_pattern(s.n_cells, s.lower_addr, s.upper_addr)
+    g.set_values(s.diag, s.lower, s.upper)
+    B = torch.from_numpy(np.ascontiguousarray(s.b)).cuda()
+    err = 1e-8 * float(np.linalg.norm(s.b[0]))
+
+    def multi():
+        return g.solve_multi("BiCG", B, torch.zeros_like(B), error=err, max_steps=20000)
+
+    def single():
+        its = []
+        for k in range(B.shape[0]):
+            g.solve("BiCG", B[k], torch.zeros_like(B[k]), error=err, max_steps=20000, hist_cap=1)
+            its.append(g.last_iterations)
+        return its
+
+    multi()
+    single()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, it_m, _, st_m = multi()
+    torch.cuda.synchronize()
+    t_m = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        it_s = single()
+    torch.cuda.synchronize()
+    t_s = (time.perf_counter() - t0) / reps
+    g.close()
+    return {"n_cells": s.n_cells, "rhs": 3, "spmm_bicg_s": round(t_m, 5),
+            "three_single_bicg_s": round(t_s, 5), "speedup": round(t_s / t_m, 3),
+            "iterations": [int(v) for v in it_m],
+            "iterations_equal_single": [int(v) for v in it_m] == [int(v) for v in it_s],
+            "all_converged": bool(np.all(st_m == 0))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,14 +192,16 @@ def main():
                     help="FP64 update trigger: FP32 residual below inner * last true residual")
     ap.add_argument("--max-iter", type=int, default=100000,
                     help="iteration cap per solve (profiling runs use a small cap)")
-    ap.add_argument("--calib-copy", action="store_true",
-                    help="also run a 256 MiB device copy (counter calibration in PMC runs)")
     ap.add_argument("--no-literal", dest="literal", action="store_false",
                     help="skip the extra restarted-refinement solve with inner relTol 1e-4")
-    ap.add_argument("--cpu-sample", type=int, default=30, help="oracle CG iterations timed")
+    ap.add_argument("--cpu-sample", type=int, default=30,
+                    help="host CG iterations timed per one-thread variant (5x on all threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="keep the default schedule instead of autotuning")
+    ap.add_argument("--schedule", default=None,
+                    help="fixed schedule instead of autotuning: spmv64,spmv32,fp64_direction,"
+                         "fp32_direction,coarsen64,coarsen32 (profiling runs)")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="skip the 64-system batched CG throughput report")
     args = ap.parse_args()
@@ -175,7 +233,12 @@ def main():
     g.set_values(s.diag, s.lower, s.upper)
     t_values = time.perf_counter() - t0
     t0 = time.perf_counter()
-    if args.tune:
+    if args.schedule:
+        v = [int(t) for t in args.schedule.split(",")]
+        g.set_schedule(spmv64=v[0], spmv32=v[1], fp64_direction=v[2], fp32_direction=v[3],
+                       coarsen64=v[4], coarsen32=v[5])
+        sched, tune_ms = g.get_schedule(), (0.0, 0.0)
+    elif args.tune:
         sched, _, tune_ms = g.autotune(iterations=24)
     else:
         sched, tune_ms = g.get_schedule(), (0.0, 0.0)
@@ -184,13 +247,6 @@ def main():
     b = torch.from_numpy(s.b).cuda()
     bnorm = float(np.linalg.norm(s.b))
     max_iter = args.max_iter
-    if args.calib_copy:
-        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
 
     K, W = args.steps, args.warmup
     xs = [torch.zeros_like(b) for _ in range(K)]
@@ -306,6 +362,8 @@ def main():
     if args.batch:
         batched = batch_throughput(3)
         log(f"[bench] batched: {batched}")
+        multi = multi_rhs_throughput(2)
+        log(f"[bench] multi-rhs: {multi}")
 
     cpu = None
     if not args.no_cpu and n == 256:
@@ -365,6 +423,7 @@ def main():
         out["mixed_restarted_inner_1e-4"] = literal
     if batched is not None:
         out["batched_cg_64_systems"] = batched
+        out["spmm_bicg_3_rhs"] = multi
     print(json.dumps(out), flush=True)
 
 

@@ -16,6 +16,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libgis_oracle.so")
+BASELINE_PATH = os.path.join(HERE, "_build", "libcpu_baseline.so")
 REF_PATH = os.path.join(HERE, "_ref", "libref_sparsecpu.so")
 
 METHODS = {"CG": 0, "BiCG": 1, "Jacobi": 2, "MR": 3, "MRJacobi": 4, "JMR": 5, "MRS": 6,
@@ -31,7 +32,7 @@ _lib = None
 
 def build(force: bool = False) -> None:
     """Compile the oracle with its Makefile (gcc only; no GPU toolchain needed)."""
-    if force or not os.path.exists(LIB_PATH):
+    if force or not os.path.exists(LIB_PATH) or not os.path.exists(BASELINE_PATH):
         subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
 
 
@@ -347,3 +348,50 @@ def ref_jacobi_precondition(A: Csr) -> np.ndarray:
     if st != 0:
         raise ValueError("D is not invertible")
     return out
+
+
+# ---- CPU baseline timings (oracle/cpu_baseline.c; bench.py only) -----------------------
+_bl = None
+
+
+def _baseline_lib():
+    global _bl
+    if _bl is None:
+        build()
+        L = C.CDLL(BASELINE_PATH)
+        L.bl_threads.restype = C.c_int
+        L.bl_threads.argtypes = []
+        L.bl_cg_csr_omp.restype = C.c_double
+        L.bl_cg_csr_omp.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int]
+        L.bl_cg_ldu.restype = C.c_double
+        L.bl_cg_ldu.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
+                                _f64p, C.c_int]
+        _bl = L
+    return _bl
+
+
+def baseline_threads() -> int:
+    return int(_baseline_lib().bl_threads())
+
+
+def time_cg_csr_omp(A: Csr, b, iters: int):
+    """Seconds for `iters` FP64 CG iterations (plus set-up residual) on all OpenMP threads."""
+    import time
+    x = np.zeros(A.n)
+    t0 = time.perf_counter()
+    res = _baseline_lib().bl_cg_csr_omp(A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                                        _c(A.val, np.float64), _c(b, np.float64), x, iters)
+    return time.perf_counter() - t0, res
+
+
+def time_cg_ldu(sys, iters: int):
+    """Seconds for `iters` FP64 CG iterations on one thread, matrix applied by the LDU
+    face loop."""
+    import time
+    x = np.zeros(sys.n_cells)
+    t0 = time.perf_counter()
+    res = _baseline_lib().bl_cg_ldu(sys.n_cells, sys.n_faces, _c(sys.lower_addr, np.int32),
+                                    _c(sys.upper_addr, np.int32), _c(sys.diag, np.float64),
+                                    _c(sys.lower, np.float64), _c(sys.upper, np.float64),
+                                    _c(sys.b, np.float64), x, iters)
+    return time.perf_counter() - t0, res
