@@ -127,33 +127,80 @@ __global__ __launch_bounds__(kWG) void k_msetup(Mat<double> m, int N, const doub
     });
 }
 
-// ---- CG ---------------------------------------------------------------------
+// ---- directions (both methods), streamed ----------------------------------------
+// p_k of iteration kk lives in buffer kk & 1.  CG: p = r + beta p_old and the
+// deferred x += alpha_prev p_old (SolverSingleGPU.h:366-445).  BiCG: p = r +
+// beta p_old, p* = r* + beta p* (SolverSingleGPU.h:568-675).  The SpMM that
+// follows then gathers one vector per column instead of forming p at every
+// gathered column from two.
 template <int NR>
-__global__ __launch_bounds__(kWG) void k_mcg1(Mat<double> m, int N, const double* __restrict__ R,
+__global__ __launch_bounds__(kWG) void k_mdir(int N, const double* __restrict__ R,
+                                              const double* __restrict__ RS,
                                               double* __restrict__ P0, double* __restrict__ P1,
-                                              double* __restrict__ X, double* __restrict__ Q,
-                                              KState* st, RedWs red) {
+                                              double* __restrict__ PS, double* __restrict__ X,
+                                              const KState* st, int bicg) {
+    const unsigned act = active_mask<NR>(st);
+    if (!act) return;
+    StreamRows t(N);
+    const size_t n = (size_t)N;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        if (!(act & (1u << k))) continue;
+        const int kk = st[k].k + 1;
+        const double beta = st[k].beta;
+        const double alpha_prev = st[k].alpha;
+        const double* pold = ((kk & 1) ? P0 : P1) + k * n;
+        double* pnew = ((kk & 1) ? P1 : P0) + k * n;
+        double rv[4];
+        load4(R + k * n, t.i0, N, rv);
+        if (kk == 1) {
+            store4(pnew, t.i0, N, rv);
+            if (bicg) {
+                double rsv[4];
+                load4(RS + k * n, t.i0, N, rsv);
+                store4(PS + k * n, t.i0, N, rsv);
+            }
+            continue;
+        }
+        double pv[4], pn[4];
+        load4(pold, t.i0, N, pv);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pn[q] = fma(beta, pv[q], rv[q]);
+        store4(pnew, t.i0, N, pn);
+        if (bicg) {
+            double rsv[4], psv[4];
+            load4(RS + k * n, t.i0, N, rsv);
+            load4(PS + k * n, t.i0, N, psv);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) psv[q] = fma(beta, psv[q], rsv[q]);
+            store4(PS + k * n, t.i0, N, psv);
+        } else {
+            double xv[4];
+            load4(X + k * n, t.i0, N, xv);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) xv[q] = fma(alpha_prev, pv[q], xv[q]);
+            store4(X + k * n, t.i0, N, xv);
+        }
+    }
+}
+
+// ---- CG ---------------------------------------------------------------------
+// after k_mdir: q = A p_k, (p_k, q) -> alpha_k
+template <int NR>
+__global__ __launch_bounds__(kWG) void k_mcg1(Mat<double> m, int N, const double* __restrict__ P0,
+                                              const double* __restrict__ P1,
+                                              double* __restrict__ Q, KState* st, RedWs red) {
     const unsigned act = active_mask<NR>(st);
     if (!act) return;
     __shared__ double sh_w[4][kTile / 4];
     __shared__ RedShared shr;
     TileRows t(N);
     const size_t n = (size_t)N;
-    int kk[NR];
-    double beta[NR], alpha_prev[NR];
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-        kk[k] = st[k].k + 1;
-        beta[k] = st[k].beta;
-        alpha_prev[k] = st[k].alpha;
-    }
-    auto pold = [&](int k) { return ((kk[k] & 1) ? P0 : P1) + k * n; };
-    auto pnew = [&](int k) { return ((kk[k] & 1) ? P1 : P0) + k * n; };
+    // columns advance in lockstep until they stop: every active column is at
+    // the same iteration, hence in the same buffer
+    const double* P = ((st[__ffs(act) - 1].k + 1) & 1) ? P1 : P0;
     double y[4][NR];
-    sell_multi<NR>(m, N, t.row0, [&](int k, int c) {
-        const double r = R[k * n + c];
-        return kk[k] == 1 ? r : fma(beta[k], pold(k)[c], r);
-    }, act, y);
+    sell_multi<NR>(m, N, t.row0, [&](int k, int c) { return P[k * n + c]; }, act, y);
     double leaf[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
@@ -161,22 +208,12 @@ __global__ __launch_bounds__(kWG) void k_mcg1(Mat<double> m, int N, const double
         to_leaf<NR>(y, k, sh_w[threadIdx.x >> 6], yl);
         leaf[k] = 0.0;
         if (!(act & (1u << k))) continue;
-        const bool first = kk[k] == 1;
-        double rv[4], pv[4], xv[4], pn[4];
-        load4(R + k * n, t.i0, N, rv);
-        if (!first) {
-            load4(pold(k), t.i0, N, pv);
-            load4(X + k * n, t.i0, N, xv);
-        }
+        double pv[4];
+        load4(P + k * n, t.i0, N, pv);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            pn[q] = first ? rv[q] : fma(beta[k], pv[q], rv[q]);
-            if (!first) xv[q] = fma(alpha_prev[k], pv[q], xv[q]);
-            if (t.i0 + q < N) leaf[k] = fma(pn[q], yl[q], leaf[k]);
-        }
-        store4(pnew(k), t.i0, N, pn);
+        for (int q = 0; q < 4; ++q)
+            if (t.i0 + q < N) leaf[k] = fma(pv[q], yl[q], leaf[k]);
         store4(Q + k * n, t.i0, N, yl);
-        if (!first) store4(X + k * n, t.i0, N, xv);
     }
     finish_tile<NR>(red, t.tile, leaf, shr, [&](const double (&v)[NR]) {
 #pragma unroll
@@ -248,32 +285,21 @@ __global__ __launch_bounds__(kWG) void k_mcg_flush(int N, double* __restrict__ X
 }
 
 // ---- BiCG (the matrix m is "A", mt is its transpose) -------------------------
+// after k_mdir: q = A p_k, (p*_k, q) -> xi = alpha_k, breakdown test
 template <int NR>
-__global__ __launch_bounds__(kWG) void k_mbi1(Mat<double> m, int N, const double* __restrict__ R,
-                                              const double* __restrict__ RS,
-                                              double* __restrict__ P0, double* __restrict__ P1,
-                                              double* __restrict__ PS, double* __restrict__ Q,
-                                              KState* st, RedWs red) {
+__global__ __launch_bounds__(kWG) void k_mbi1(Mat<double> m, int N, const double* __restrict__ P0,
+                                              const double* __restrict__ P1,
+                                              const double* __restrict__ PS,
+                                              double* __restrict__ Q, KState* st, RedWs red) {
     const unsigned act = active_mask<NR>(st);
     if (!act) return;
     __shared__ double sh_w[4][kTile / 4];
     __shared__ RedShared shr;
     TileRows t(N);
     const size_t n = (size_t)N;
-    int kk[NR];
-    double theta[NR];
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-        kk[k] = st[k].k + 1;
-        theta[k] = st[k].beta;
-    }
-    auto pold = [&](int k) { return ((kk[k] & 1) ? P0 : P1) + k * n; };
-    auto pnew = [&](int k) { return ((kk[k] & 1) ? P1 : P0) + k * n; };
+    const double* P = ((st[__ffs(act) - 1].k + 1) & 1) ? P1 : P0;
     double y[4][NR];
-    sell_multi<NR>(m, N, t.row0, [&](int k, int c) {
-        const double r = R[k * n + c];
-        return kk[k] == 1 ? r : fma(theta[k], pold(k)[c], r);
-    }, act, y);
+    sell_multi<NR>(m, N, t.row0, [&](int k, int c) { return P[k * n + c]; }, act, y);
     double leaf[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
@@ -281,22 +307,11 @@ __global__ __launch_bounds__(kWG) void k_mbi1(Mat<double> m, int N, const double
         to_leaf<NR>(y, k, sh_w[threadIdx.x >> 6], yl);
         leaf[k] = 0.0;
         if (!(act & (1u << k))) continue;
-        const bool first = kk[k] == 1;
-        double rv[4], rsv[4], pv[4], psv[4], pn[4];
-        load4(R + k * n, t.i0, N, rv);
-        load4(RS + k * n, t.i0, N, rsv);
-        if (!first) {
-            load4(pold(k), t.i0, N, pv);
-            load4(PS + k * n, t.i0, N, psv);
-        }
+        double psv[4];
+        load4(PS + k * n, t.i0, N, psv);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            pn[q] = first ? rv[q] : fma(theta[k], pv[q], rv[q]);
-            psv[q] = first ? rsv[q] : fma(theta[k], psv[q], rsv[q]);
+        for (int q = 0; q < 4; ++q)
             if (t.i0 + q < N) leaf[k] = fma(psv[q], yl[q], leaf[k]);
-        }
-        store4(pnew(k), t.i0, N, pn);
-        store4(PS + k * n, t.i0, N, psv);
         store4(Q + k * n, t.i0, N, yl);
     }
     finish_tile<NR>(red, t.tile, leaf, shr, [&](const double (&v)[NR]) {
@@ -477,13 +492,17 @@ int solve_multi_nr(gis_context* h, int method, bool transpose, const SolveArgs& 
     int block = 4;
     for (;;) {
         for (int i = 0; i < block && launched < ms; ++i, ++launched) {
+            GIS_TRY(timer_begin(h, 4));
+            hipLaunchKernelGGL((k_mdir<NR>), g, dim3(kWG), 0, h->stream, N, R, RS, P0, P1, PS,
+                               X, mb.st, bicg ? 1 : 0);
+            GIS_TRY(timer_end(h));
             GIS_TRY(timer_begin(h, 0));
             if (bicg)
-                hipLaunchKernelGGL((k_mbi1<NR>), g, dim3(kWG), 0, h->stream, mA, N, R, RS, P0,
-                                   P1, PS, Q, mb.st, h->red);
+                hipLaunchKernelGGL((k_mbi1<NR>), g, dim3(kWG), 0, h->stream, mA, N, P0, P1, PS,
+                                   Q, mb.st, h->red);
             else
-                hipLaunchKernelGGL((k_mcg1<NR>), g, dim3(kWG), 0, h->stream, mA, N, R, P0, P1,
-                                   X, Q, mb.st, h->red);
+                hipLaunchKernelGGL((k_mcg1<NR>), g, dim3(kWG), 0, h->stream, mA, N, P0, P1, Q,
+                                   mb.st, h->red);
             GIS_TRY(timer_end(h));
             GIS_TRY(timer_begin(h, 1));
             if (bicg)
