@@ -326,6 +326,17 @@ def ref_lib():
         R.ref_transpose.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p]
         R.ref_jacobi_precondition.restype = C.c_int
         R.ref_jacobi_precondition.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p]
+        R.ref_csr_save.restype = C.c_int
+        R.ref_csr_save.argtypes = [C.c_char_p, C.c_int, C.c_int, _i32p, _i32p, _f64p]
+        R.ref_csr_read_sizes.restype = C.c_int
+        R.ref_csr_read_sizes.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int)]
+        R.ref_csr_read.restype = C.c_int
+        R.ref_csr_read.argtypes = [C.c_char_p, _i32p, _i32p, _f64p]
+        R.ref_vec_save.restype = C.c_int
+        R.ref_vec_save.argtypes = [C.c_char_p, C.c_int, _f64p]
+        R.ref_vec_read.restype = C.c_int
+        R.ref_vec_read.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.c_void_p]
         _ref = R
     return _ref
 
@@ -338,6 +349,41 @@ def ref_transpose(A: Csr) -> Csr:
     R.ref_transpose(A.n, A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
                     _c(A.val, np.float64), t_rowptr, t_col, t_val)
     return Csr(A.n, t_rowptr, t_col[:A.nnz], t_val[:A.nnz])
+
+
+def ref_csr_read(path: str):
+    """The reference's SparseCSR::read (suffix dispatch): (n_rows, n_cols, rowptr, col, val),
+    or None when it throws."""
+    R = ref_lib()
+    nr, nc, nnz = C.c_int(0), C.c_int(0), C.c_int(0)
+    if R.ref_csr_read_sizes(path.encode(), C.byref(nr), C.byref(nc), C.byref(nnz)) != 0:
+        return None
+    rp = np.zeros(nr.value + 1, np.int32)
+    col = np.zeros(max(nnz.value, 1), np.int32)
+    val = np.zeros(max(nnz.value, 1))
+    if R.ref_csr_read(path.encode(), rp, col, val) != 0:
+        return None
+    return nr.value, nc.value, rp, col[:nnz.value], val[:nnz.value]
+
+
+def ref_csr_save(path: str, n_rows: int, n_cols: int, rowptr, col, val) -> None:
+    ref_lib().ref_csr_save(path.encode(), n_rows, n_cols, _c(rowptr, np.int32),
+                           _c(col, np.int32), _c(val, np.float64))
+
+
+def ref_vec_save(path: str, v) -> None:
+    v = _c(v, np.float64)
+    ref_lib().ref_vec_save(path.encode(), v.shape[0], v)
+
+
+def ref_vec_read(path: str, mtx: bool):
+    R = ref_lib()
+    n = C.c_int(0)
+    if R.ref_vec_read(path.encode(), int(mtx), C.byref(n), None) != 0:
+        return None
+    out = np.zeros(max(n.value, 1))
+    R.ref_vec_read(path.encode(), int(mtx), C.byref(n), out.ctypes.data_as(C.c_void_p))
+    return out[:n.value]
 
 
 def ref_jacobi_precondition(A: Csr) -> np.ndarray:

@@ -1,4 +1,5 @@
-"""Golden fixtures (tests/golden/*.npz, written by tests/golden/make_golden.py):
+"""Golden solver fixtures (tests/golden/{hex,random}*.npz, written by
+tests/golden/make_golden.py):
 the CPU oracle must reproduce them bit for bit on any host (no GPU), and the
 HIP path through the C ABI must reproduce them bit for bit on the GPU."""
 import glob
@@ -11,7 +12,8 @@ from cufoam_amd import mesh
 from oracle import oracle as O
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "hex*.npz")) +
+                  glob.glob(os.path.join(HERE, "golden", "random*.npz")))
 METHODS = ["CG", "BiCG", "Jacobi", "MR", "MRJacobi", "JMR", "MRS", "Richardson"]
 STEPS = 20
 
