@@ -181,6 +181,66 @@ def multi_rhs_throughput(reps: int):
             "all_converged": bool(np.all(st_m == 0))}
 
 
+def dropin(cpu_sample: int):
+    """The OpenFOAM drop-in path at 128^3 (cuFoam<Name>::solve through gis_foam_solve:
+    host psi/source in and out, error = tolerance * NormFactor, tolerance 1e-8, x0 = 0):
+    CG, Jacobi and Richardson on the SPD system (shift 1e-3), BiCG and MRJacobi on the
+    nonsymmetric one (shift 0.1).  Each solve is timed with and without the per-solve
+    coefficient upload (gis_set_values); the oracle's same method on one host thread is
+    timed on a bounded sample of iterations and extrapolated to the GPU's count."""
+    import numpy as np
+    from cufoam_amd import mesh
+    from cufoam_amd.gis import Gis
+
+    systems = {"poisson128": mesh.config_system("poisson128"),
+               "convdiff128": mesh.config_system("convdiff128")}
+    cases = [("CG", "poisson128", 5000), ("Jacobi", "poisson128", 2000),
+             ("Richardson", "poisson128", 2000), ("BiCG", "convdiff128", 5000),
+             ("MRJacobi", "convdiff128", 5000)]
+    out = {}
+    try:
+        from oracle import oracle as O
+    except Exception:
+        O = None
+    for name, sysname, max_iter in cases:
+        s = systems[sysname]
+        g = Gis(0)
+        g.set_pattern(s.n_cells, s.lower_addr, s.upper_addr)
+        g.set_values(s.diag, s.lower, s.upper)
+        sched = g.autotune(fp32=False)[0]                # once per mesh, cached
+        psi = np.zeros(s.n_cells)
+        g.foam_solve(name, psi, s.b, 1e-8, max_iter)     # warm-up (code load, workspaces)
+        psi = np.zeros(s.n_cells)
+        t0 = time.perf_counter()
+        g.set_values(s.diag, s.lower, s.upper)
+        t1 = time.perf_counter()
+        perf = g.foam_solve(name, psi, s.b, 1e-8, max_iter)
+        t2 = time.perf_counter()
+        g.close()
+        rec = {"system": sysname, "max_iter": max_iter, "iterations": perf.n_iterations,
+               "schedule": sched,
+               "initial_residual": perf.initial_residual,
+               "final_residual": perf.final_residual, "status": perf.status,
+               "converged": bool(perf.final_residual <= 1e-8 * perf.norm_factor),
+               "solve_s": round(t2 - t1, 5), "with_set_values_s": round(t2 - t0, 5),
+               "us_per_iteration": round((t2 - t1) / max(perf.n_iterations, 1) * 1e6, 2)}
+        if O is not None and cpu_sample > 0:
+            A = O.ldu2csr(s)
+            t = []
+            for k in (cpu_sample, 2 * cpu_sample):   # the difference drops set-up work
+                t0 = time.perf_counter()
+                O.solve(name, A, s.b, s.x0, 0.0, k, hist_cap=k + 1)
+                t.append(time.perf_counter() - t0)
+            per = (t[1] - t[0]) / cpu_sample
+            rec["cpu_1thread_s"] = round(t[0] - cpu_sample * per + per * perf.n_iterations, 3)
+            rec["cpu_sample"] = (f"the oracle's {name} for {cpu_sample} and {2 * cpu_sample} "
+                                 f"iterations on one thread: set-up plus per-iteration time, "
+                                 f"extrapolated to the GPU's iteration count")
+        out[f"cuFoam{name}"] = rec
+        log(f"[bench] drop-in cuFoam{name}: {rec}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +262,8 @@ def main():
     ap.add_argument("--schedule", default=None,
                     help="fixed schedule instead of autotuning: spmv64,spmv32,fp64_direction,"
                          "fp32_direction,coarsen64,coarsen32 (profiling runs)")
+    ap.add_argument("--no-dropin", dest="dropin", action="store_false",
+                    help="skip the 128^3 OpenFOAM drop-in wrapper timings")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="skip the 64-system batched CG throughput report")
     args = ap.parse_args()
@@ -365,6 +427,8 @@ def main():
         multi = multi_rhs_throughput(2)
         log(f"[bench] multi-rhs: {multi}")
 
+    drop = dropin(0 if args.no_cpu else 10) if args.dropin else None
+
     cpu = None
     if not args.no_cpu and n == 256:
         try:
@@ -421,6 +485,8 @@ def main():
     }
     if literal is not None:
         out["mixed_restarted_inner_1e-4"] = literal
+    if drop is not None:
+        out["dropin_128"] = drop
     if batched is not None:
         out["batched_cg_64_systems"] = batched
         out["spmm_bicg_3_rhs"] = multi
