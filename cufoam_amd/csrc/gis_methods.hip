@@ -336,6 +336,73 @@ __global__ __launch_bounds__(kWG) void k_bicg1(Mat<double> m, int N, const doubl
     });
 }
 
+// Split schedule, pass 1: the two directions of k_bicg1 (same fma per row), streamed.
+__global__ __launch_bounds__(kWG) void k_bicg_dir(int N, const double* __restrict__ r,
+                                                  const double* __restrict__ rs,
+                                                  double* __restrict__ pb0,
+                                                  double* __restrict__ pb1,
+                                                  double* __restrict__ ps, const KState* st) {
+    if (st->done) return;
+    StreamRows t(N);
+    const int k = st->k + 1;
+    const double* p_old = (k & 1) ? pb0 : pb1;
+    double* p_new = (k & 1) ? pb1 : pb0;
+    double rv[4], rsv[4];
+    load4(r, t.i0, N, rv);
+    load4(rs, t.i0, N, rsv);
+    if (k == 1) {
+        store4(p_new, t.i0, N, rv);
+        store4(ps, t.i0, N, rsv);
+        return;
+    }
+    const double theta = st->beta;
+    double pv[4], psv[4];
+    load4(p_old, t.i0, N, pv);
+    load4(ps, t.i0, N, psv);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        pv[kk] = fma(theta, pv[kk], rv[kk]);
+        psv[kk] = fma(theta, psv[kk], rsv[kk]);
+    }
+    store4(p_new, t.i0, N, pv);
+    store4(ps, t.i0, N, psv);
+}
+
+// Split schedule, pass 2: q = A p_k with one gathered vector, (p*_k, q) -> xi.
+template <class P>
+__global__ __launch_bounds__(kWG) void k_bicg1s(Mat<double> m, int N,
+                                                const double* __restrict__ pb0,
+                                                const double* __restrict__ pb1,
+                                                const double* __restrict__ ps,
+                                                double* __restrict__ q, KState* st, RedWs red) {
+    if (st->done) return;
+    __shared__ typename P::template Shared<double> sh;
+    __shared__ double sh_y[kTile];
+    __shared__ RedShared shr;
+    TileRows t(N);
+    const double* p = ((st->k + 1) & 1) ? pb1 : pb0;
+    P::run(m, N, t.row0, t.nrows, [&](int c) { return p[c]; }, sh, sh_y);
+    double psv[4], qv[4];
+    load4(ps, t.i0, N, psv);
+    double leaf[1][1] = {{0.0}};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        qv[kk] = sh_y[threadIdx.x * kLeaf + kk];
+        if (t.i0 + kk < N) leaf[0][0] = fma(psv[kk], qv[kk], leaf[0][0]);
+    }
+    TileReduce<1, 1> tr;
+    tr.begin(red, t.tile, 1, leaf, shr);
+    store4(q, t.i0, N, qv);
+    tr.end(red, [&](const double (&v)[1]) {
+        const double xi = st->rho0 / v[0];
+        st->alpha = xi;
+        if (xi > -st->int_bound && xi < st->int_bound) {
+            st->status = kBreakdown;
+            st->done = 1;
+        }
+    });
+}
+
 template <class P>
 __global__ __launch_bounds__(kWG) void k_bicg2(Mat<double> mt, int N, double* __restrict__ r,
                                                double* __restrict__ rs,
@@ -519,8 +586,15 @@ int solve_bicg(gis_context* h, const SolveArgs& a, double* x, const double* b) {
     const dim3 g = tiles_grid(N);
     GIS_SPMV_DISPATCH(h->sched.spmv64, P, {
         GIS_TRY(run_blocks(h, h->st_host->max_steps, [&]() -> int {
-            hipLaunchKernelGGL((k_bicg1<P>), g, dim3(kWG), 0, h->stream, m, N, r, rs, pb0, pb1,
-                               ps, q, h->st, h->red);
+            if (h->sched.fp64_direction) {
+                hipLaunchKernelGGL(k_bicg_dir, g, dim3(kWG), 0, h->stream, N, r, rs, pb0, pb1, ps,
+                                   h->st);
+                hipLaunchKernelGGL((k_bicg1s<P>), g, dim3(kWG), 0, h->stream, m, N, pb0, pb1, ps,
+                                   q, h->st, h->red);
+            } else {
+                hipLaunchKernelGGL((k_bicg1<P>), g, dim3(kWG), 0, h->stream, m, N, r, rs, pb0,
+                                   pb1, ps, q, h->st, h->red);
+            }
             hipLaunchKernelGGL((k_bicg2<P>), g, dim3(kWG), 0, h->stream, mt, N, r, rs, pb0,
                                pb1, ps, q, x, h->st, h->red);
             return kOK;

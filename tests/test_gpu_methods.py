@@ -72,6 +72,17 @@ def test_cg_schedules_are_bit_identical(gpu, n, schedule, direction, coarsen):
     _check(gpu, "CG", s, s.x0, 0.0, 30)
 
 
+# BiCG: SpMV schedule x direction form (fused on-the-fly or a streamed pass)
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("schedule", [0, 3, 4, 5])
+@pytest.mark.parametrize("n", [8, 48])
+def test_bicg_direction_forms_are_bit_identical(gpu, n, schedule, direction):
+    s = _case(n, 7, False, 0.1)
+    gpu.set_system(s)
+    gpu.set_schedule(schedule, schedule, -1, direction, direction)
+    _check(gpu, "BiCG", s, s.x0, 0.0, 30)
+
+
 def test_bicg_breakdown_forced_from_host(gpu):
     # A = 1e11 I: xi = 1e-11 < int_bound 1e-10 at the first step (SolverSingleGPU.h:629-630)
     N = 64
