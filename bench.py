@@ -47,9 +47,10 @@ def mx1_bytes(N, nnz, direction):
 def cpu_baseline(sys_, iters_fp64: int, sample_iters: int):
     """FP64 CG of the same system on the host, timed on bounded samples and extrapolated to
     the iteration count the FP64 solve needs.  The reported value uses every OpenMP thread
-    (CSR matvec and vector loops parallel); one-thread figures are reported beside it: the
-    oracle's CG (the GPU path's algorithm, bit-identical iterates) and the same CG with the
-    matrix applied by OpenFOAM's LDU face loop."""
+    (CSR matvec and vector loops parallel); one-thread figures are reported beside it:
+    OpenFOAM's PCG with preconditioner none (LDU face loop, normFactor-scaled L1
+    residuals), the oracle's CG (the GPU path's algorithm, bit-identical iterates) and the
+    same CG with the matrix applied by the LDU face loop."""
     from oracle import oracle as O
     A = O.ldu2csr(sys_)
     threads = O.baseline_threads()
@@ -59,7 +60,9 @@ def cpu_baseline(sys_, iters_fp64: int, sample_iters: int):
     t0 = time.perf_counter()
     O.solve("CG", A, sys_.b, sys_.x0, 0.0, sample_iters, hist_cap=sample_iters + 1)
     t_orc = time.perf_counter() - t0
-    per = {"omp": t_omp / omp_iters, "ldu": t_ldu / sample_iters, "oracle": t_orc / sample_iters}
+    _, _, _, t_of = O.pcg_openfoam(sys_, 0.0, sample_iters)
+    per = {"omp": t_omp / omp_iters, "ldu": t_ldu / sample_iters, "oracle": t_orc / sample_iters,
+           "openfoam": t_of / sample_iters}
     return {
         "value": per["omp"] * iters_fp64,
         "unit": "s",
@@ -70,6 +73,7 @@ def cpu_baseline(sys_, iters_fp64: int, sample_iters: int):
                   f"{iters_fp64} iterations the FP64 solve needs",
         "per_iteration_s": per["omp"],
         "one_thread": {
+            "openfoam_pcg_none_s": per["openfoam"] * iters_fp64,
             "oracle_cg_s": per["oracle"] * iters_fp64,
             "ldu_face_loop_cg_s": per["ldu"] * iters_fp64,
             "sample": f"{sample_iters} iterations each, extrapolated to {iters_fp64}"},
@@ -241,6 +245,39 @@ def dropin(cpu_sample: int):
     return out
 
 
+def plumbing_32():
+    """The CPU-runnable case at 32^3 (SPD, shift 1e-3, x0 = 0, abs tol 1e-6, maxIter 2000):
+    OpenFOAM's PCG (preconditioner none) restated on one host thread, the host ldu2csr the
+    reference wrappers run per solve, and cuFoamCG through gis_foam_solve on the GPU."""
+    import numpy as np
+    from cufoam_amd import mesh
+    from cufoam_amd.gis import Gis
+    s = mesh.config_system("poisson32")
+    out = {"n_cells": s.n_cells}
+    try:
+        from oracle import oracle as O
+        _, it, res, t = O.pcg_openfoam(s, 1e-6, 2000)
+        t0 = time.perf_counter()
+        O.ldu2csr(s)
+        out["cpu_openfoam_pcg"] = {"iterations": it, "scaled_residual": res, "s": round(t, 5)}
+        out["cpu_ldu2csr_s"] = round(time.perf_counter() - t0, 5)
+    except Exception as e:   # the oracle is optional at run time
+        log(f"[bench] 32^3 CPU leg skipped: {e}")
+    g = Gis(0)
+    g.set_system(s)
+    psi = np.zeros(s.n_cells)
+    g.foam_solve("CG", psi, s.b, 1e-6, 2000)
+    psi = np.zeros(s.n_cells)
+    t0 = time.perf_counter()
+    g.set_values(s.diag, s.lower, s.upper)
+    perf = g.foam_solve("CG", psi, s.b, 1e-6, 2000)
+    out["gpu_cuFoamCG"] = {"iterations": perf.n_iterations, "final_residual": perf.final_residual,
+                           "s_with_set_values": round(time.perf_counter() - t0, 5)}
+    g.close()
+    log(f"[bench] 32^3 plumbing: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,7 +356,11 @@ def main():
     torch.cuda.synchronize()
     g.set_timing("iter_spmv")
     t0 = time.perf_counter()
-    res_m = [g.solve_mixed(b, xs[i], 1e-10, args.inner, max_iter, args.mode) for i in range(K)]
+    res_m, step_m = [], []
+    for i in range(K):
+        ts = time.perf_counter()
+        res_m.append(g.solve_mixed(b, xs[i], 1e-10, args.inner, max_iter, args.mode))
+        step_m.append(time.perf_counter() - ts)   # each solve ends with a host sync
     torch.cuda.synchronize()
     t_mixed = (time.perf_counter() - t0) / K
     mx1_ms, mx1_n = g.get_timing("iter_spmv")
@@ -336,8 +377,12 @@ def main():
     g.set_timing("iter_spmv")
     xd = [torch.zeros_like(b) for _ in range(K)]
     t0 = time.perf_counter()
-    res_d = [g.solve("CG", b, xd[i], error=1e-10 * bnorm, max_steps=max_iter,
-                     hist_cap=max_iter + 1) for i in range(K)]
+    res_d, step_d = [], []
+    for i in range(K):
+        ts = time.perf_counter()
+        res_d.append(g.solve("CG", b, xd[i], error=1e-10 * bnorm, max_steps=max_iter,
+                             hist_cap=max_iter + 1))
+        step_d.append(time.perf_counter() - ts)
     torch.cuda.synchronize()
     t_fp64 = (time.perf_counter() - t0) / K
     cg1_ms, cg1_n = g.get_timing("iter_spmv")
@@ -428,6 +473,7 @@ def main():
         log(f"[bench] multi-rhs: {multi}")
 
     drop = dropin(0 if args.no_cpu else 10) if args.dropin else None
+    plumb = plumbing_32() if args.dropin else None
 
     cpu = None
     if not args.no_cpu and n == 256:
@@ -457,6 +503,8 @@ def main():
                    "mixed_mode": "reliable updates" if args.mode == 0 else "restarted refinement",
                    "inner_rel_tol": args.inner},
         "fp64_cg_s": round(t_fp64, 5),
+        "median_step_s": {"mixed": round(float(np.median(step_m)), 5),
+                          "fp64": round(float(np.median(step_d)), 5)},
         "speedup_mixed_vs_fp64": round(t_fp64 / t_mixed, 4),
         "iterations_fp64": it_d,
         "iterations_mixed": it_m,
@@ -487,6 +535,7 @@ def main():
         out["mixed_restarted_inner_1e-4"] = literal
     if drop is not None:
         out["dropin_128"] = drop
+        out["plumbing_32"] = plumb
     if batched is not None:
         out["batched_cg_64_systems"] = batched
         out["spmm_bicg_3_rhs"] = multi

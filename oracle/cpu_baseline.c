@@ -17,6 +17,8 @@
  *                   (cuFoamCG/cuFoamCG.C:45-56 hands lduMatrix to the solver).
  *
  * Both return the final ||r||_2 so that the work cannot be optimised away.
+ * bl_pcg_openfoam (below) is OpenFOAM's own PCG loop, the CPU solver the
+ * wrappers stand in for.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -109,4 +111,75 @@ BL_API double bl_cg_ldu(int n, int m, const int* la, const int* ua, const double
     free(p);
     free(q);
     return sqrt(rho);
+}
+
+/*
+ * OpenFOAM's PCG with preconditioner "none", restated from its published
+ * algorithm (the CPU solver the gis wrappers replace; OpenFOAM is not part of
+ * the cuFoam tree): LDU face-loop matvec, residuals measured as
+ * sum|r| / normFactor with normFactor = sum(|A psi - xRef A 1| + |b - xRef A 1|)
+ * + 1e-20 (xRef = mean(psi)), stop when the scaled residual falls below
+ * `tolerance` (relTol 0) or after max_iter iterations.  One thread.
+ * Returns the iteration count; *final_res receives the last scaled residual.
+ */
+BL_API int bl_pcg_openfoam(int n, int m, const int* la, const int* ua, const double* diag,
+                           const double* lower, const double* upper, const double* b,
+                           double* x, double tolerance, int max_iter, double* final_res) {
+    double* r = malloc(sizeof(double) * (size_t)n);
+    double* p = malloc(sizeof(double) * (size_t)n);
+    double* w = malloc(sizeof(double) * (size_t)n);
+    double* sum_a = malloc(sizeof(double) * (size_t)n);
+    ldu_amul(n, m, la, ua, diag, lower, upper, x, w);
+    for (int i = 0; i < n; ++i) {
+        r[i] = b[i] - w[i];
+        sum_a[i] = diag[i];   /* A 1, by the same face loop */
+    }
+    for (int f = 0; f < m; ++f) {
+        sum_a[ua[f]] += lower[f];
+        sum_a[la[f]] += upper[f];
+    }
+    double xref = 0.0;
+    for (int i = 0; i < n; ++i) xref += x[i];
+    xref /= n;
+    double nf = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double ax = xref * sum_a[i];
+        nf += fabs(w[i] - ax) + fabs(b[i] - ax);
+    }
+    nf += 1e-20;
+    double res = 0.0;
+    for (int i = 0; i < n; ++i) res += fabs(r[i]);
+    res /= nf;
+    int it = 0;
+    double rho_old = 1.0;
+    while (it < max_iter && !(res < tolerance)) {
+        double rho = 0.0;
+        for (int i = 0; i < n; ++i) rho += r[i] * r[i];
+        if (it == 0) {
+            for (int i = 0; i < n; ++i) p[i] = r[i];
+        } else {
+            const double beta = rho / rho_old;
+            for (int i = 0; i < n; ++i) p[i] = r[i] + beta * p[i];
+        }
+        rho_old = rho;
+        ldu_amul(n, m, la, ua, diag, lower, upper, p, w);
+        double pw = 0.0;
+        for (int i = 0; i < n; ++i) pw += w[i] * p[i];
+        if (fabs(pw) / nf < 1e-300) break;
+        const double alpha = rho / pw;
+        res = 0.0;
+        for (int i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * w[i];
+            res += fabs(r[i]);
+        }
+        res /= nf;
+        ++it;
+    }
+    *final_res = res;
+    free(r);
+    free(p);
+    free(w);
+    free(sum_a);
+    return it;
 }

@@ -409,6 +409,9 @@ def _baseline_lib():
         L.bl_threads.argtypes = []
         L.bl_cg_csr_omp.restype = C.c_double
         L.bl_cg_csr_omp.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int]
+        L.bl_pcg_openfoam.restype = C.c_int
+        L.bl_pcg_openfoam.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
+                                      _f64p, C.c_double, C.c_int, C.POINTER(C.c_double)]
         L.bl_cg_ldu.restype = C.c_double
         L.bl_cg_ldu.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
                                 _f64p, C.c_int]
@@ -441,3 +444,18 @@ def time_cg_ldu(sys, iters: int):
                                     _c(sys.lower, np.float64), _c(sys.upper, np.float64),
                                     _c(sys.b, np.float64), x, iters)
     return time.perf_counter() - t0, res
+
+
+def pcg_openfoam(sys, tolerance: float, max_iter: int):
+    """OpenFOAM's PCG (preconditioner none) on one thread: (x, iterations, final scaled
+    residual, seconds)."""
+    import time
+    x = np.zeros(sys.n_cells)
+    res = C.c_double(0.0)
+    t0 = time.perf_counter()
+    it = _baseline_lib().bl_pcg_openfoam(sys.n_cells, sys.n_faces, _c(sys.lower_addr, np.int32),
+                                         _c(sys.upper_addr, np.int32), _c(sys.diag, np.float64),
+                                         _c(sys.lower, np.float64), _c(sys.upper, np.float64),
+                                         _c(sys.b, np.float64), x, tolerance, max_iter,
+                                         C.byref(res))
+    return x, int(it), res.value, time.perf_counter() - t0

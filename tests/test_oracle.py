@@ -407,3 +407,20 @@ def test_mixed_without_updates_tracks_fp64_cg_early():
     t32 = np.linalg.norm(s.b - A.scipy() @ x32)
     t64 = np.linalg.norm(s.b - A.scipy() @ r64.x)
     assert abs(t32 - t64) <= 1e-4 * t64
+
+
+def test_openfoam_pcg_baseline_is_cg():
+    """The CPU baseline's OpenFOAM PCG (preconditioner none) runs the same Krylov
+    recurrence as the oracle's CG: equal iterates up to summation order at a fixed
+    iteration count, and the normFactor closed form for x0 = 0 (nf = sum|b|, so the
+    first scaled residual is 1)."""
+    s = mesh.laplacian(10, seed=4, shift=1e-2)
+    A = O.ldu2csr(s)
+    x, it, res, _ = O.pcg_openfoam(s, 0.0, 25)
+    ref = O.solve("CG", A, s.b, s.x0, 0.0, 25)
+    assert it == 25
+    np.testing.assert_allclose(x, ref.x, rtol=1e-9, atol=1e-12)
+    r = s.b - O.spmv(A, x)
+    assert res == pytest.approx(np.abs(r).sum() / np.abs(s.b).sum(), rel=1e-9)
+    x0, it0, res0, _ = O.pcg_openfoam(s, 2.0, 100)     # tolerance above the first residual
+    assert it0 == 0 and res0 == pytest.approx(1.0, rel=1e-15)
