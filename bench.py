@@ -141,7 +141,9 @@ def batch_throughput(reps: int, use_graph: bool = True):
 def multi_rhs_throughput(reps: int):
     """128^3 nonsymmetric convection-diffusion system with 3 right-hand sides (seed 0): one
     SpMM-BiCG solve of all three columns against three single-RHS BiCG solves, each column
-    to ||r||_2 <= 1e-8 ||b_0||_2 from x0 = 0, everything resident in HBM."""
+    to ||r||_2 <= 1e-8 ||b_0||_2 from x0 = 0, everything resident in HBM; then FP64
+    SpMM-BiCG against FP32/FP64 refinement (FP32 SpMM-BiCG inner solves, inner relTol 1e-4),
+    both to relative residual 1e-10."""
     import numpy as np
     import torch
     from cufoam_amd import mesh
@@ -177,12 +179,41 @@ def multi_rhs_throughput(reps: int):
         it_s = single()
     torch.cuda.synchronize()
     t_s = (time.perf_counter() - t0) / reps
+    # to 1e-10 relative: FP64 SpMM-BiCG against FP32/FP64 refinement (SpMM-BiCG inner solves)
+    bn = np.linalg.norm(s.b, axis=1)
+    err10 = 1e-10 * float(bn.min())
+    g.solve_multi("BiCG", B, torch.zeros_like(B), error=err10, max_steps=20000)
+    g.solve_multi_mixed("BiCG", B, torch.zeros_like(B), rel_tol=1e-10, inner_rel_tol=1e-4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    X64, it64, _, _ = g.solve_multi("BiCG", B, torch.zeros_like(B), error=err10, max_steps=20000)
+    torch.cuda.synchronize()
+    t_64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Xm, hm, itm, relm, stm = g.solve_multi_mixed("BiCG", B, torch.zeros_like(B), rel_tol=1e-10,
+                                                 inner_rel_tol=1e-4)
+    torch.cuda.synchronize()
+    t_mx = time.perf_counter() - t0
+    R = torch.empty_like(X64[0])
+    rel64 = []
+    for k in range(B.shape[0]):
+        g.spmv(X64[k], R)
+        rel64.append(float(torch.linalg.vector_norm(B[k] - R)) / float(bn[k]))
     g.close()
     return {"n_cells": s.n_cells, "rhs": 3, "spmm_bicg_s": round(t_m, 5),
             "three_single_bicg_s": round(t_s, 5), "speedup": round(t_s / t_m, 3),
             "iterations": [int(v) for v in it_m],
             "iterations_equal_single": [int(v) for v in it_m] == [int(v) for v in it_s],
-            "all_converged": bool(np.all(st_m == 0))}
+            "all_converged": bool(np.all(st_m == 0)),
+            "to_1e-10": {"fp64_spmm_bicg_s": round(t_64, 5),
+                         "fp64_iterations": [int(v) for v in it64],
+                         "fp64_rel_residual": rel64,
+                         "mixed_refinement_s": round(t_mx, 5),
+                         "mixed_inner_iterations": [int(v) for v in itm],
+                         "mixed_fp64_steps": [len(h) for h in hm],
+                         "mixed_rel_residual": [float(v) for v in relm],
+                         "mixed_status": [int(v) for v in stm],
+                         "speedup_mixed": round(t_64 / t_mx, 3)}}
 
 
 def dropin(cpu_sample: int):

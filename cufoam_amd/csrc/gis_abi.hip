@@ -543,6 +543,63 @@ GIS_API int gis_solve_multi_device(gis_handle h, int method, int nrhs, double* d
                             final_residual, status, true);
 }
 
+static int solve_multi_mixed_impl(gis_handle h, int method, int nrhs, double* X,
+                                  const double* B, double rel_tol, double inner_rel_tol,
+                                  long long max_outer, long long max_inner, double* hist,
+                                  int hist_cap, int* outer_steps, int* iterations,
+                                  double* rel_residual, int* status, bool dev) {
+    if (!h || !X || !B) return GIS_ERR_ARG;
+    if (nrhs < 1 || nrhs > 8) return fail(h, GIS_ERR_ARG, "nrhs must be in 1..8");
+    if (method != GIS_CG && method != GIS_BICG)
+        return fail(h, GIS_ERR_ARG, "mixed multi-RHS solves support GIS_CG and GIS_BICG");
+    if (!(rel_tol >= 0.0) || !(inner_rel_tol >= 0.0) || hist_cap < 0)
+        return fail(h, GIS_ERR_ARG, "tolerances and hist_cap must be non-negative");
+    if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
+    DeviceGuard g(h->device);
+    const size_t n = (size_t)h->N * nrhs;
+    GIS_HIP(h, h->io2.ensure(sizeof(double) * 2 * n));
+    double* dx = h->io2.as<double>();
+    double* db = dx + n;
+    const hipMemcpyKind kin = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const hipMemcpyKind kout = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    GIS_HIP(h, hipMemcpyAsync(dx, X, sizeof(double) * n, kin, h->stream));
+    GIS_HIP(h, hipMemcpyAsync(db, B, sizeof(double) * n, kin, h->stream));
+    MultiMixedArgs a{};
+    a.rel_tol = rel_tol;
+    a.inner_rel_tol = inner_rel_tol;
+    a.max_outer = max_outer;
+    a.max_inner = max_inner;
+    a.int_bound = 1e-10;
+    a.hist_cap = hist ? hist_cap : 0;
+    const int st = device_solve_multi_mixed(h, method, nrhs, a, dx, db, hist, outer_steps,
+                                            iterations, rel_residual, status);
+    if (st != GIS_OK && st < GIS_BREAKDOWN) return st;
+    GIS_HIP(h, hipMemcpyAsync(X, dx, sizeof(double) * n, kout, h->stream));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    return st;
+}
+
+GIS_API int gis_solve_multi_mixed(gis_handle h, int method, int nrhs, double* X,
+                                  const double* B, double rel_tol, double inner_rel_tol,
+                                  long long max_outer, long long max_inner, double* hist,
+                                  int hist_cap, int* outer_steps, int* iterations,
+                                  double* rel_residual, int* status) {
+    return solve_multi_mixed_impl(h, method, nrhs, X, B, rel_tol, inner_rel_tol, max_outer,
+                                  max_inner, hist, hist_cap, outer_steps, iterations,
+                                  rel_residual, status, false);
+}
+
+GIS_API int gis_solve_multi_mixed_device(gis_handle h, int method, int nrhs, double* d_X,
+                                         const double* d_B, double rel_tol,
+                                         double inner_rel_tol, long long max_outer,
+                                         long long max_inner, double* hist, int hist_cap,
+                                         int* outer_steps, int* iterations,
+                                         double* rel_residual, int* status) {
+    return solve_multi_mixed_impl(h, method, nrhs, d_X, d_B, rel_tol, inner_rel_tol, max_outer,
+                                  max_inner, hist, hist_cap, outer_steps, iterations,
+                                  rel_residual, status, true);
+}
+
 static int adjoint_impl(gis_handle h, int method, int nrhs, const double* W, const double* x,
                         double* lambda, double* d_diag, double* d_lower, double* d_upper,
                         const gis_params* params, int* iterations, int* status, bool dev) {

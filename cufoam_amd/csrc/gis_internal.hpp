@@ -114,6 +114,7 @@ struct gis_context {
     bool has_values = false;
     bool has_val32 = false;
     bool has_valT = false;
+    bool has_val32T = false;
     gis::DevMem rowptr, col, perm, nbr_cnt;     // CSR pattern, value permutation
     gis::DevMem lower_addr, upper_addr;         // LDU addressing copies
     gis::DevMem diag, lower, upper;             // LDU coefficients (device copies)
@@ -121,6 +122,7 @@ struct gis_context {
     long long n_slots = 0;                      // sliced layout (64-row slices)
     gis::DevMem soff, scol, sperm;              // slice offsets, slot columns / codes
     gis::DevMem sval, svalT, sval32;            // slot values: A, A^T, float(A)
+    gis::DevMem val32T, sval32T;                // float(A^T): CSR and slot values
     // compressed slot columns: col = row + sbase[slot row] + sres[slot], int8
     // residuals (-128 marks padding); sflag[slice] = 1 where every residual fits
     gis::DevMem sres, sbase, sflag;
@@ -195,6 +197,7 @@ int build_pattern(gis_context* h, int N, int M, const int* d_lower_addr, const i
 int gather_values(gis_context* h);          // val (and valT / val32 when present) from diag/lower/upper
 int ensure_val32(gis_context* h);
 int ensure_valT(gis_context* h);
+int ensure_val32T(gis_context* h);
 
 // kernels (gis_solvers.hip)
 int launch_spmv(gis_context* h, const double* x, double* y);
@@ -221,6 +224,23 @@ int device_solve(gis_context* h, const SolveArgs& a, double* d_x, const double* 
 int device_solve_multi(gis_context* h, int method, int nrhs, bool transpose, const SolveArgs& a,
                        double* X, const double* B, int* iters, double* norm0, double* normf,
                        int* status);
+// Mixed-precision multi-RHS solve (restarted refinement): FP64 residual
+// R = B - A X, FP32 inner CG/BiCG on A D = R from D = 0 until its recursive
+// residual falls below max(inner_rel_tol ||R||, rel_tol ||b|| / 2), X += D in
+// FP64; per column until
+// ||b - A x||_2 <= rel_tol ||b||_2.  hist_len / iters / rel_res / status per column.
+struct MultiMixedArgs {
+    double rel_tol;
+    double inner_rel_tol;
+    long long max_outer;
+    long long max_inner;
+    double int_bound;
+    int hist_cap;
+};
+int device_solve_multi_mixed(gis_context* h, int method, int nrhs, const MultiMixedArgs& a,
+                             double* X, const double* B, double* hist, int* outer,
+                             int* iters, double* rel_res, int* status);
+
 // dL_k/dA = -lambda_k x^T on the pattern, written into LDU-shaped gradients.
 int device_sddmm_fold(gis_context* h, int nrhs, const double* Lam, const double* x,
                       double* d_diag, double* d_lower, double* d_upper);

@@ -237,6 +237,13 @@ inline Mat<double> mat64T(const gis_context* h) {
                        h->sflag.as<unsigned char>()};
 }
 
+inline Mat<float> mat32T(const gis_context* h) {
+    return Mat<float>{h->rowptr.as<int>(), h->col.as<int>(), h->val32T.as<float>(),
+                      h->soff.as<int>(), h->scol.as<int>(), h->sval32T.as<float>(),
+                      h->sres.as<signed char>(), h->sbase.as<int>(),
+                      h->sflag.as<unsigned char>()};
+}
+
 inline Mat<float> mat32(const gis_context* h) {
     return Mat<float>{h->rowptr.as<int>(), h->col.as<int>(), h->val32.as<float>(),
                       h->soff.as<int>(), h->scol.as<int>(), h->sval32.as<float>(),

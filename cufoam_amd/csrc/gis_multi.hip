@@ -21,10 +21,18 @@ namespace {
 
 constexpr int kMaxRhs = 8;
 
+// Every kernel below is templated on the Krylov value type T: double for the
+// FP64 solvers, float for the inner loop of the mixed-precision refinement.
+// Products, row sums and vector updates run in T (fused multiply-adds only as
+// fmat); every reduction accumulates in double, so T = double is exactly the
+// FP64 arithmetic and T = float that of the oracle's or_mixed_krylov.
+__device__ __forceinline__ double fmat(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float fmat(float a, float b, float c) { return fmaf(a, b, c); }
+
 // y[c][k] for the four 64-row slices of this wave, columns with bit k set in `active`
-template <int NR, class XOp>
-__device__ __forceinline__ void sell_multi(const Mat<double>& m, int N, int row0, XOp&& xop,
-                                           unsigned active, double (&y)[4][NR]) {
+template <int NR, typename T, class XOp>
+__device__ __forceinline__ void sell_multi(const Mat<T>& m, int N, int row0, XOp&& xop,
+                                           unsigned active, T (&y)[4][NR]) {
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int nslices = (N + kSlice - 1) / kSlice;
@@ -32,14 +40,14 @@ __device__ __forceinline__ void sell_multi(const Mat<double>& m, int N, int row0
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
 #pragma unroll
-        for (int k = 0; k < NR; ++k) y[c][k] = 0.0;
+        for (int k = 0; k < NR; ++k) y[c][k] = T(0);
         const int s = s0 + c;
         if (s >= nslices) continue;
         const int a = m.soff[s];
         const int len = (m.soff[s + 1] - a) / kSlice;
         for (int j = 0; j < len; j += 4) {
             int cc[4];
-            double v[4];
+            T v[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int idx = a + (j + u) * kSlice + lane;
@@ -48,7 +56,7 @@ __device__ __forceinline__ void sell_multi(const Mat<double>& m, int N, int row0
                     v[u] = __builtin_nontemporal_load(m.sval + idx);
                 } else {
                     cc[u] = -1;
-                    v[u] = 0.0;
+                    v[u] = T(0);
                 }
             }
 #pragma unroll
@@ -63,9 +71,8 @@ __device__ __forceinline__ void sell_multi(const Mat<double>& m, int N, int row0
 }
 
 // column k of the wave's slices -> this thread's kLeaf consecutive rows
-template <int NR>
-__device__ __forceinline__ void to_leaf(const double (&y)[4][NR], int k, double* wave_lds,
-                                        double (&o)[4]) {
+template <int NR, typename T>
+__device__ __forceinline__ void to_leaf(const T (&y)[4][NR], int k, T* wave_lds, T (&o)[4]) {
     const int lane = threadIdx.x & 63;
     wave_sync_lds();
 #pragma unroll
@@ -84,29 +91,32 @@ __device__ __forceinline__ unsigned active_mask(const KState* st) {
     return a;
 }
 
-// r_k = b_k - A x_k, optional copy, rho0 = (r, r), norm0, hist[0]
-template <int NR>
-__global__ __launch_bounds__(kWG) void k_msetup(Mat<double> m, int N, const double* __restrict__ B,
-                                                const double* __restrict__ X,
-                                                double* __restrict__ R, double* __restrict__ R2,
-                                                KState* st, RedWs red, int done_rule) {
-    __shared__ double sh_w[4][kTile / 4];
+// r_k = b_k - A x_k, optional copy, rho0 = (r, r), norm0, hist[0].  Columns in
+// `skip` are left alone and marked done.
+template <int NR, typename T>
+__global__ __launch_bounds__(kWG) void k_msetup(Mat<T> m, int N, const T* __restrict__ B,
+                                                const T* __restrict__ X, T* __restrict__ R,
+                                                T* __restrict__ R2, KState* st, RedWs red,
+                                                int done_rule, unsigned skip) {
+    __shared__ T sh_w[4][kTile / 4];
     __shared__ RedShared shr;
     TileRows t(N);
     const size_t n = (size_t)N;
-    double y[4][NR];
-    sell_multi<NR>(m, N, t.row0, [&](int k, int c) { return X[k * n + c]; }, (1u << NR) - 1, y);
+    const unsigned act = ((1u << NR) - 1) & ~skip;
+    T y[4][NR];
+    sell_multi<NR, T>(m, N, t.row0, [&](int k, int c) { return X[k * n + c]; }, act, y);
     double leaf[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-        double yl[4], bv[4], rv[4];
-        to_leaf<NR>(y, k, sh_w[threadIdx.x >> 6], yl);
-        load4(B + k * n, t.i0, N, bv);
+        T yl[4], bv[4], rv[4];
+        to_leaf<NR, T>(y, k, sh_w[threadIdx.x >> 6], yl);
         leaf[k] = 0.0;
+        if (!(act & (1u << k))) continue;
+        load4(B + k * n, t.i0, N, bv);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             rv[q] = bv[q] - yl[q];
-            if (t.i0 + q < N) leaf[k] = fma(rv[q], rv[q], leaf[k]);
+            if (t.i0 + q < N) leaf[k] = fma((double)rv[q], (double)rv[q], leaf[k]);
         }
         store4(R + k * n, t.i0, N, rv);
         if (R2) store4(R2 + k * n, t.i0, N, rv);
@@ -114,6 +124,10 @@ __global__ __launch_bounds__(kWG) void k_msetup(Mat<double> m, int N, const doub
     finish_tile<NR>(red, t.tile, leaf, shr, [&](const double (&v)[NR]) {
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
+            if (!(act & (1u << k))) {
+                st[k].done = 1;
+                continue;
+            }
             const double norm = sqrt(v[k]);
             st[k].rho0 = v[k];
             st[k].rho1 = 0.0;
@@ -133,12 +147,11 @@ __global__ __launch_bounds__(kWG) void k_msetup(Mat<double> m, int N, const doub
 // beta p_old, p* = r* + beta p* (SolverSingleGPU.h:568-675).  The SpMM that
 // follows then gathers one vector per column instead of forming p at every
 // gathered column from two.
-template <int NR>
-__global__ __launch_bounds__(kWG) void k_mdir(int N, const double* __restrict__ R,
-                                              const double* __restrict__ RS,
-                                              double* __restrict__ P0, double* __restrict__ P1,
-                                              double* __restrict__ PS, double* __restrict__ X,
-                                              const KState* st, int bicg) {
+template <int NR, typename T>
+__global__ __launch_bounds__(kWG) void k_mdir(int N, const T* __restrict__ R,
+                                              const T* __restrict__ RS, T* __restrict__ P0,
+                                              T* __restrict__ P1, T* __restrict__ PS,
+                                              T* __restrict__ X, const KState* st, int bicg) {
     const unsigned act = active_mask<NR>(st);
     if (!act) return;
     StreamRows t(N);
@@ -147,38 +160,38 @@ __global__ __launch_bounds__(kWG) void k_mdir(int N, const double* __restrict__ 
     for (int k = 0; k < NR; ++k) {
         if (!(act & (1u << k))) continue;
         const int kk = st[k].k + 1;
-        const double beta = st[k].beta;
-        const double alpha_prev = st[k].alpha;
-        const double* pold = ((kk & 1) ? P0 : P1) + k * n;
-        double* pnew = ((kk & 1) ? P1 : P0) + k * n;
-        double rv[4];
+        const T beta = (T)st[k].beta;
+        const T alpha_prev = (T)st[k].alpha;
+        const T* pold = ((kk & 1) ? P0 : P1) + k * n;
+        T* pnew = ((kk & 1) ? P1 : P0) + k * n;
+        T rv[4];
         load4(R + k * n, t.i0, N, rv);
         if (kk == 1) {
             store4(pnew, t.i0, N, rv);
             if (bicg) {
-                double rsv[4];
+                T rsv[4];
                 load4(RS + k * n, t.i0, N, rsv);
                 store4(PS + k * n, t.i0, N, rsv);
             }
             continue;
         }
-        double pv[4], pn[4];
+        T pv[4], pn[4];
         load4(pold, t.i0, N, pv);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) pn[q] = fma(beta, pv[q], rv[q]);
+        for (int q = 0; q < 4; ++q) pn[q] = fmat(beta, pv[q], rv[q]);
         store4(pnew, t.i0, N, pn);
         if (bicg) {
-            double rsv[4], psv[4];
+            T rsv[4], psv[4];
             load4(RS + k * n, t.i0, N, rsv);
             load4(PS + k * n, t.i0, N, psv);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) psv[q] = fma(beta, psv[q], rsv[q]);
+            for (int q = 0; q < 4; ++q) psv[q] = fmat(beta, psv[q], rsv[q]);
             store4(PS + k * n, t.i0, N, psv);
         } else {
-            double xv[4];
+            T xv[4];
             load4(X + k * n, t.i0, N, xv);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) xv[q] = fma(alpha_prev, pv[q], xv[q]);
+            for (int q = 0; q < 4; ++q) xv[q] = fmat(alpha_prev, pv[q], xv[q]);
             store4(X + k * n, t.i0, N, xv);
         }
     }
@@ -186,33 +199,33 @@ __global__ __launch_bounds__(kWG) void k_mdir(int N, const double* __restrict__ 
 
 // ---- CG ---------------------------------------------------------------------
 // after k_mdir: q = A p_k, (p_k, q) -> alpha_k
-template <int NR>
-__global__ __launch_bounds__(kWG) void k_mcg1(Mat<double> m, int N, const double* __restrict__ P0,
-                                              const double* __restrict__ P1,
-                                              double* __restrict__ Q, KState* st, RedWs red) {
+template <int NR, typename T>
+__global__ __launch_bounds__(kWG) void k_mcg1(Mat<T> m, int N, const T* __restrict__ P0,
+                                              const T* __restrict__ P1, T* __restrict__ Q,
+                                              KState* st, RedWs red) {
     const unsigned act = active_mask<NR>(st);
     if (!act) return;
-    __shared__ double sh_w[4][kTile / 4];
+    __shared__ T sh_w[4][kTile / 4];
     __shared__ RedShared shr;
     TileRows t(N);
     const size_t n = (size_t)N;
     // columns advance in lockstep until they stop: every active column is at
     // the same iteration, hence in the same buffer
-    const double* P = ((st[__ffs(act) - 1].k + 1) & 1) ? P1 : P0;
-    double y[4][NR];
-    sell_multi<NR>(m, N, t.row0, [&](int k, int c) { return P[k * n + c]; }, act, y);
+    const T* P = ((st[__ffs(act) - 1].k + 1) & 1) ? P1 : P0;
+    T y[4][NR];
+    sell_multi<NR, T>(m, N, t.row0, [&](int k, int c) { return P[k * n + c]; }, act, y);
     double leaf[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-        double yl[4];
-        to_leaf<NR>(y, k, sh_w[threadIdx.x >> 6], yl);
+        T yl[4];
+        to_leaf<NR, T>(y, k, sh_w[threadIdx.x >> 6], yl);
         leaf[k] = 0.0;
         if (!(act & (1u << k))) continue;
-        double pv[4];
+        T pv[4];
         load4(P + k * n, t.i0, N, pv);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if (t.i0 + q < N) leaf[k] = fma(pv[q], yl[q], leaf[k]);
+            if (t.i0 + q < N) leaf[k] = fma((double)pv[q], (double)yl[q], leaf[k]);
         store4(Q + k * n, t.i0, N, yl);
     }
     finish_tile<NR>(red, t.tile, leaf, shr, [&](const double (&v)[NR]) {
@@ -222,10 +235,9 @@ __global__ __launch_bounds__(kWG) void k_mcg1(Mat<double> m, int N, const double
     });
 }
 
-template <int NR>
-__global__ __launch_bounds__(kWG) void k_mcg2(int N, double* __restrict__ R,
-                                              const double* __restrict__ Q, KState* st,
-                                              RedWs red) {
+template <int NR, typename T>
+__global__ __launch_bounds__(kWG) void k_mcg2(int N, T* __restrict__ R, const T* __restrict__ Q,
+                                              KState* st, RedWs red) {
     const unsigned act = active_mask<NR>(st);
     if (!act) return;
     __shared__ RedShared shr;
@@ -236,14 +248,14 @@ __global__ __launch_bounds__(kWG) void k_mcg2(int N, double* __restrict__ R,
     for (int k = 0; k < NR; ++k) {
         leaf[k] = 0.0;
         if (!(act & (1u << k))) continue;
-        const double alpha = st[k].alpha;
-        double rv[4], qv[4];
+        const T alpha = (T)st[k].alpha;
+        T rv[4], qv[4];
         load4(R + k * n, t.i0, N, rv);
         load4(Q + k * n, t.i0, N, qv);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            rv[q] = fma(-alpha, qv[q], rv[q]);
-            if (t.i0 + q < N) leaf[k] = fma(rv[q], rv[q], leaf[k]);
+            rv[q] = fmat(-alpha, qv[q], rv[q]);
+            if (t.i0 + q < N) leaf[k] = fma((double)rv[q], (double)rv[q], leaf[k]);
         }
         store4(R + k * n, t.i0, N, rv);
     }
@@ -262,56 +274,55 @@ __global__ __launch_bounds__(kWG) void k_mcg2(int N, double* __restrict__ R,
     });
 }
 
-template <int NR>
-__global__ __launch_bounds__(kWG) void k_mcg_flush(int N, double* __restrict__ X,
-                                                   const double* __restrict__ P0,
-                                                   const double* __restrict__ P1,
-                                                   const KState* st) {
+template <int NR, typename T>
+__global__ __launch_bounds__(kWG) void k_mcg_flush(int N, T* __restrict__ X,
+                                                   const T* __restrict__ P0,
+                                                   const T* __restrict__ P1, const KState* st,
+                                                   unsigned skip) {
     StreamRows t(N);
     const size_t n = (size_t)N;
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
         const int it = st[k].k;
-        if (it < 1) continue;
-        const double alpha = st[k].alpha;
-        const double* p = ((it & 1) ? P1 : P0) + k * n;
-        double xv[4], pv[4];
+        if (it < 1 || (skip & (1u << k))) continue;
+        const T alpha = (T)st[k].alpha;
+        const T* p = ((it & 1) ? P1 : P0) + k * n;
+        T xv[4], pv[4];
         load4(X + k * n, t.i0, N, xv);
         load4(p, t.i0, N, pv);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) xv[q] = fma(alpha, pv[q], xv[q]);
+        for (int q = 0; q < 4; ++q) xv[q] = fmat(alpha, pv[q], xv[q]);
         store4(X + k * n, t.i0, N, xv);
     }
 }
 
 // ---- BiCG (the matrix m is "A", mt is its transpose) -------------------------
 // after k_mdir: q = A p_k, (p*_k, q) -> xi = alpha_k, breakdown test
-template <int NR>
-__global__ __launch_bounds__(kWG) void k_mbi1(Mat<double> m, int N, const double* __restrict__ P0,
-                                              const double* __restrict__ P1,
-                                              const double* __restrict__ PS,
-                                              double* __restrict__ Q, KState* st, RedWs red) {
+template <int NR, typename T>
+__global__ __launch_bounds__(kWG) void k_mbi1(Mat<T> m, int N, const T* __restrict__ P0,
+                                              const T* __restrict__ P1, const T* __restrict__ PS,
+                                              T* __restrict__ Q, KState* st, RedWs red) {
     const unsigned act = active_mask<NR>(st);
     if (!act) return;
-    __shared__ double sh_w[4][kTile / 4];
+    __shared__ T sh_w[4][kTile / 4];
     __shared__ RedShared shr;
     TileRows t(N);
     const size_t n = (size_t)N;
-    const double* P = ((st[__ffs(act) - 1].k + 1) & 1) ? P1 : P0;
-    double y[4][NR];
-    sell_multi<NR>(m, N, t.row0, [&](int k, int c) { return P[k * n + c]; }, act, y);
+    const T* P = ((st[__ffs(act) - 1].k + 1) & 1) ? P1 : P0;
+    T y[4][NR];
+    sell_multi<NR, T>(m, N, t.row0, [&](int k, int c) { return P[k * n + c]; }, act, y);
     double leaf[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-        double yl[4];
-        to_leaf<NR>(y, k, sh_w[threadIdx.x >> 6], yl);
+        T yl[4];
+        to_leaf<NR, T>(y, k, sh_w[threadIdx.x >> 6], yl);
         leaf[k] = 0.0;
         if (!(act & (1u << k))) continue;
-        double psv[4];
+        T psv[4];
         load4(PS + k * n, t.i0, N, psv);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if (t.i0 + q < N) leaf[k] = fma(psv[q], yl[q], leaf[k]);
+            if (t.i0 + q < N) leaf[k] = fma((double)psv[q], (double)yl[q], leaf[k]);
         store4(Q + k * n, t.i0, N, yl);
     }
     finish_tile<NR>(red, t.tile, leaf, shr, [&](const double (&v)[NR]) {
@@ -329,33 +340,32 @@ __global__ __launch_bounds__(kWG) void k_mbi1(Mat<double> m, int N, const double
     });
 }
 
-template <int NR>
-__global__ __launch_bounds__(kWG) void k_mbi2(Mat<double> mt, int N, double* __restrict__ R,
-                                              double* __restrict__ RS,
-                                              const double* __restrict__ P0,
-                                              const double* __restrict__ P1,
-                                              const double* __restrict__ PS,
-                                              const double* __restrict__ Q,
-                                              double* __restrict__ X, KState* st, RedWs red) {
+// q* = A^T p*_k; x += xi p, r -= xi q, r* -= xi q*, ||r||, (r, r*) -> beta, stop
+template <int NR, typename T>
+__global__ __launch_bounds__(kWG) void k_mbi2(Mat<T> mt, int N, T* __restrict__ R,
+                                              T* __restrict__ RS, const T* __restrict__ P0,
+                                              const T* __restrict__ P1, const T* __restrict__ PS,
+                                              const T* __restrict__ Q, T* __restrict__ X,
+                                              KState* st, RedWs red) {
     const unsigned act = active_mask<NR>(st);
     if (!act) return;
-    __shared__ double sh_w[4][kTile / 4];
+    __shared__ T sh_w[4][kTile / 4];
     __shared__ RedShared shr;
     TileRows t(N);
     const size_t n = (size_t)N;
-    double y[4][NR];
-    sell_multi<NR>(mt, N, t.row0, [&](int k, int c) { return PS[k * n + c]; }, act, y);
+    T y[4][NR];
+    sell_multi<NR, T>(mt, N, t.row0, [&](int k, int c) { return PS[k * n + c]; }, act, y);
     double leaf[2 * NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-        double yl[4];
-        to_leaf<NR>(y, k, sh_w[threadIdx.x >> 6], yl);
+        T yl[4];
+        to_leaf<NR, T>(y, k, sh_w[threadIdx.x >> 6], yl);
         leaf[2 * k] = 0.0;
         leaf[2 * k + 1] = 0.0;
         if (!(act & (1u << k))) continue;
-        const double xi = st[k].alpha;
-        const double* p = (((st[k].k + 1) & 1) ? P1 : P0) + k * n;
-        double xv[4], pv[4], rv[4], qv[4], rsv[4];
+        const T xi = (T)st[k].alpha;
+        const T* p = (((st[k].k + 1) & 1) ? P1 : P0) + k * n;
+        T xv[4], pv[4], rv[4], qv[4], rsv[4];
         load4(X + k * n, t.i0, N, xv);
         load4(p, t.i0, N, pv);
         load4(R + k * n, t.i0, N, rv);
@@ -363,12 +373,12 @@ __global__ __launch_bounds__(kWG) void k_mbi2(Mat<double> mt, int N, double* __r
         load4(RS + k * n, t.i0, N, rsv);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            xv[q] = fma(xi, pv[q], xv[q]);
-            rv[q] = fma(-xi, qv[q], rv[q]);
-            rsv[q] = fma(-xi, yl[q], rsv[q]);
+            xv[q] = fmat(xi, pv[q], xv[q]);
+            rv[q] = fmat(-xi, qv[q], rv[q]);
+            rsv[q] = fmat(-xi, yl[q], rsv[q]);
             if (t.i0 + q < N) {
-                leaf[2 * k] = fma(rv[q], rv[q], leaf[2 * k]);
-                leaf[2 * k + 1] = fma(rv[q], rsv[q], leaf[2 * k + 1]);
+                leaf[2 * k] = fma((double)rv[q], (double)rv[q], leaf[2 * k]);
+                leaf[2 * k + 1] = fma((double)rv[q], (double)rsv[q], leaf[2 * k + 1]);
             }
         }
         store4(X + k * n, t.i0, N, xv);
@@ -391,6 +401,104 @@ __global__ __launch_bounds__(kWG) void k_mbi2(Mat<double> mt, int N, double* __r
             }
         }
     });
+}
+
+// ---- mixed-precision refinement: the FP64 outer steps ------------------------
+// R64 = B - A X in FP64 (never stored), R32 = (float) R64, ||R64||, and on the
+// first step ||B||; per column outer state in `ost` (norm, error = rel_tol ||b||,
+// done, k = outer steps, hist = outer residual norms).
+template <int NR>
+__global__ __launch_bounds__(kWG) void k_mres(Mat<double> m, int N, const double* __restrict__ B,
+                                              const double* __restrict__ X,
+                                              float* __restrict__ R32, KState* ost, RedWs red,
+                                              double rel_tol, int first) {
+    __shared__ double sh_w[4][kTile / 4];
+    __shared__ RedShared shr;
+    TileRows t(N);
+    const size_t n = (size_t)N;
+    const unsigned act = first ? (1u << NR) - 1 : active_mask<NR>(ost);
+    if (!act) return;
+    double y[4][NR];
+    sell_multi<NR, double>(m, N, t.row0, [&](int k, int c) { return X[k * n + c]; }, act, y);
+    double leaf[2 * NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        double yl[4], bv[4];
+        to_leaf<NR, double>(y, k, sh_w[threadIdx.x >> 6], yl);
+        leaf[2 * k] = 0.0;
+        leaf[2 * k + 1] = 0.0;
+        if (!(act & (1u << k))) continue;
+        load4(B + k * n, t.i0, N, bv);
+        float rf[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double r = bv[q] - yl[q];
+            rf[q] = (float)r;
+            if (t.i0 + q < N) {
+                leaf[2 * k] = fma(r, r, leaf[2 * k]);
+                leaf[2 * k + 1] = fma(bv[q], bv[q], leaf[2 * k + 1]);
+            }
+        }
+        store4(R32 + k * n, t.i0, N, rf);
+    }
+    finish_tile<2 * NR>(red, t.tile, leaf, shr, [&](const double (&v)[2 * NR]) {
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            if (!(act & (1u << k))) continue;
+            KState* s = &ost[k];
+            if (first) {
+                s->aux[3] = sqrt(v[2 * k + 1]);
+                s->error = rel_tol * s->aux[3];
+                s->k = 0;
+                s->nhist = 0;
+                s->status = kOK;
+                s->done = 0;
+                s->aux[0] = 0.0;     // inner iterations, summed
+            }
+            const double rn = sqrt(v[2 * k]);
+            s->norm = rn;
+            if (first) s->norm0 = rn;
+            if (s->nhist < s->hist_cap) s->hist[s->nhist] = rn;
+            s->nhist = s->nhist + 1;
+            if (rn != rn) {
+                s->status = kNaN;
+                s->done = 1;
+            } else if (!(rn > s->error)) {
+                s->done = 1;
+            } else if (s->k >= s->max_steps) {
+                s->status = kNotConverged;
+                s->done = 1;
+            }
+        }
+    });
+}
+
+// X += (double) D for the columns refined in this outer step; counts the step
+template <int NR>
+__global__ __launch_bounds__(kWG) void k_mcorr(int N, double* __restrict__ X,
+                                               const float* __restrict__ D,
+                                               const KState* ist, KState* ost, unsigned refined) {
+    StreamRows t(N);
+    const size_t n = (size_t)N;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        if (!(refined & (1u << k))) continue;
+        double xv[4];
+        float dv[4];
+        load4(X + k * n, t.i0, N, xv);
+        load4(D + k * n, t.i0, N, dv);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xv[q] = xv[q] + (double)dv[q];
+        store4(X + k * n, t.i0, N, xv);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            if (!(refined & (1u << k))) continue;
+            ost[k].k = ost[k].k + 1;
+            ost[k].aux[0] = ost[k].aux[0] + (double)ist[k].k;
+        }
+    }
 }
 
 // dL_k/dA_ij = -lambda_k[i] x[j] on the CSR pattern, folded into the LDU slots
@@ -428,19 +536,27 @@ struct MultiBufs {
     KState* host = nullptr;     // pinned mirror
 };
 
-int multi_prepare(gis_context* h, int nrhs, const SolveArgs& a, int nvec, MultiBufs& mb) {
+// states: h->mstate holds 2 kMaxRhs KStates (the Krylov states, then the outer
+// states of the mixed-precision refinement), mirrored in pinned host memory
+int ensure_multi_states(gis_context* h) {
+    GIS_HIP(h, h->mstate.ensure(sizeof(KState) * 2 * kMaxRhs));
+    if (!h->mstate_host) {
+        void* p = nullptr;
+        GIS_HIP(h, hipHostMalloc(&p, sizeof(KState) * 2 * kMaxRhs, hipHostMallocDefault));
+        h->mstate_host = static_cast<KState*>(p);
+    }
+    return kOK;
+}
+
+int multi_prepare(gis_context* h, int nrhs, const SolveArgs& a, size_t vec_bytes,
+                  MultiBufs& mb) {
     if (!h->has_values) return fail(h, kErrState, "values not set");
     GIS_TRY(ensure_reduction(h, h->N));
     GIS_TRY(ensure_state(h));
-    GIS_HIP(h, h->vec.ensure(sizeof(double) * (size_t)h->N * nvec * nrhs));
+    GIS_HIP(h, h->vec.ensure(vec_bytes));
     const int cap = std::max(1, a.hist_cap);
     GIS_HIP(h, h->hist.ensure(sizeof(double) * (size_t)cap * nrhs));
-    GIS_HIP(h, h->mstate.ensure(sizeof(KState) * kMaxRhs));
-    if (!h->mstate_host) {
-        void* p = nullptr;
-        GIS_HIP(h, hipHostMalloc(&p, sizeof(KState) * kMaxRhs, hipHostMallocDefault));
-        h->mstate_host = static_cast<KState*>(p);
-    }
+    GIS_TRY(ensure_multi_states(h));
     mb.st = h->mstate.as<KState>();
     mb.host = h->mstate_host;
     for (int k = 0; k < nrhs; ++k) {
@@ -448,8 +564,8 @@ int multi_prepare(gis_context* h, int nrhs, const SolveArgs& a, int nvec, MultiB
         s.error = a.error;
         s.max_steps = a.max_steps <= 0 ? (long long)INT_MAX : a.max_steps;
         s.int_bound = a.int_bound;
-        s.hist = h->hist.as<double>() + (size_t)k * cap;
-        s.hist_cap = cap;
+        s.hist = a.hist_cap > 0 ? h->hist.as<double>() + (size_t)k * cap : nullptr;
+        s.hist_cap = a.hist_cap > 0 ? cap : 0;
         mb.host[k] = s;
     }
     GIS_HIP(h, hipMemcpyAsync(mb.st, mb.host, sizeof(KState) * nrhs, hipMemcpyHostToDevice,
@@ -468,48 +584,48 @@ int multi_read(gis_context* h, int nrhs, MultiBufs& mb, bool* all_done) {
     return kOK;
 }
 
-template <int NR>
-int solve_multi_nr(gis_context* h, int method, bool transpose, const SolveArgs& a, double* X,
-                   const double* B, MultiBufs& mb) {
+// The Krylov loop of a multi-RHS CG (method 0) or BiCG (method 1) in value type
+// T on the workspace `base` (6 NR N values of T: R, P0, P1, Q, R*, P*); columns
+// in `skip` take no part.
+template <int NR, typename T>
+int run_multi(gis_context* h, int method, const Mat<T>& mA, const Mat<T>& mAt,
+              long long max_steps, T* X, const T* B, T* base, MultiBufs& mb, unsigned skip) {
     const int N = h->N;
     const size_t n = (size_t)N;
-    const Mat<double> mA = transpose ? mat64T(h) : mat64(h);
-    const Mat<double> mAt = transpose ? mat64(h) : mat64T(h);
     const dim3 g = tiles_grid(N);
-    double* base = h->vec.as<double>();
-    double* R = base;
-    double* P0 = base + 1 * n * NR;
-    double* P1 = base + 2 * n * NR;
-    double* Q = base + 3 * n * NR;
-    double* RS = base + 4 * n * NR;
-    double* PS = base + 5 * n * NR;
+    T* R = base;
+    T* P0 = base + 1 * n * NR;
+    T* P1 = base + 2 * n * NR;
+    T* Q = base + 3 * n * NR;
+    T* RS = base + 4 * n * NR;
+    T* PS = base + 5 * n * NR;
     const bool bicg = method == 1;
-    hipLaunchKernelGGL((k_msetup<NR>), g, dim3(kWG), 0, h->stream, mA, N, B, X, R,
-                       bicg ? RS : nullptr, mb.st, h->red, bicg ? 1 : 0);
+    hipLaunchKernelGGL((k_msetup<NR, T>), g, dim3(kWG), 0, h->stream, mA, N, B, X, R,
+                       bicg ? RS : nullptr, mb.st, h->red, bicg ? 1 : 0, skip);
     GIS_HIP(h, hipGetLastError());
-    const long long ms = a.max_steps <= 0 ? (long long)INT_MAX : a.max_steps;
+    const long long ms = max_steps <= 0 ? (long long)INT_MAX : max_steps;
     long long launched = 0;
     int block = 4;
     for (;;) {
         for (int i = 0; i < block && launched < ms; ++i, ++launched) {
             GIS_TRY(timer_begin(h, 4));
-            hipLaunchKernelGGL((k_mdir<NR>), g, dim3(kWG), 0, h->stream, N, R, RS, P0, P1, PS,
-                               X, mb.st, bicg ? 1 : 0);
+            hipLaunchKernelGGL((k_mdir<NR, T>), g, dim3(kWG), 0, h->stream, N, R, RS, P0, P1,
+                               PS, X, mb.st, bicg ? 1 : 0);
             GIS_TRY(timer_end(h));
             GIS_TRY(timer_begin(h, 0));
             if (bicg)
-                hipLaunchKernelGGL((k_mbi1<NR>), g, dim3(kWG), 0, h->stream, mA, N, P0, P1, PS,
-                                   Q, mb.st, h->red);
+                hipLaunchKernelGGL((k_mbi1<NR, T>), g, dim3(kWG), 0, h->stream, mA, N, P0, P1,
+                                   PS, Q, mb.st, h->red);
             else
-                hipLaunchKernelGGL((k_mcg1<NR>), g, dim3(kWG), 0, h->stream, mA, N, P0, P1, Q,
-                                   mb.st, h->red);
+                hipLaunchKernelGGL((k_mcg1<NR, T>), g, dim3(kWG), 0, h->stream, mA, N, P0, P1,
+                                   Q, mb.st, h->red);
             GIS_TRY(timer_end(h));
             GIS_TRY(timer_begin(h, 1));
             if (bicg)
-                hipLaunchKernelGGL((k_mbi2<NR>), g, dim3(kWG), 0, h->stream, mAt, N, R, RS, P0,
-                                   P1, PS, Q, X, mb.st, h->red);
+                hipLaunchKernelGGL((k_mbi2<NR, T>), g, dim3(kWG), 0, h->stream, mAt, N, R, RS,
+                                   P0, P1, PS, Q, X, mb.st, h->red);
             else
-                hipLaunchKernelGGL((k_mcg2<NR>), g, dim3(kWG), 0, h->stream, N, R, Q, mb.st,
+                hipLaunchKernelGGL((k_mcg2<NR, T>), g, dim3(kWG), 0, h->stream, N, R, Q, mb.st,
                                    h->red);
             GIS_TRY(timer_end(h));
         }
@@ -520,11 +636,81 @@ int solve_multi_nr(gis_context* h, int method, bool transpose, const SolveArgs& 
         block = std::min(block * 2, std::max(1, h->sched.block_iters));
     }
     if (!bicg) {
-        hipLaunchKernelGGL((k_mcg_flush<NR>), g, dim3(kWG), 0, h->stream, N, X, P0, P1, mb.st);
+        hipLaunchKernelGGL((k_mcg_flush<NR, T>), g, dim3(kWG), 0, h->stream, N, X, P0, P1,
+                           mb.st, skip);
         GIS_HIP(h, hipGetLastError());
     }
     return kOK;
 }
+
+template <int NR>
+int solve_multi_nr(gis_context* h, int method, bool transpose, const SolveArgs& a, double* X,
+                   const double* B, MultiBufs& mb) {
+    const Mat<double> mA = transpose ? mat64T(h) : mat64(h);
+    const Mat<double> mAt = transpose ? mat64(h) : mat64T(h);
+    return run_multi<NR, double>(h, method, mA, mAt, a.max_steps, X, B, h->vec.as<double>(), mb,
+                                 0u);
+}
+
+// Restarted refinement: FP64 residual, FP32 Krylov solve for the correction,
+// FP64 correction, per column until ||b - A x|| <= rel_tol ||b|| or max_outer
+// corrections.
+template <int NR>
+int solve_multi_mixed_nr(gis_context* h, int method, const MultiMixedArgs& a, double* X,
+                         const double* B, MultiBufs& mb, KState* ost, KState* ost_host) {
+    const int N = h->N;
+    const size_t n = (size_t)N;
+    const dim3 g = tiles_grid(N);
+    const Mat<double> m64 = mat64(h);
+    const Mat<float> m32 = mat32(h);
+    const Mat<float> m32T = mat32T(h);
+    float* base = h->vec.as<float>();
+    float* R32 = base + 6 * n * NR;
+    float* D = base + 7 * n * NR;
+    const unsigned all = (1u << NR) - 1;
+    for (int o = 0;; ++o) {
+        hipLaunchKernelGGL((k_mres<NR>), g, dim3(kWG), 0, h->stream, m64, N, B, X, R32, ost,
+                           h->red, a.rel_tol, o == 0 ? 1 : 0);
+        GIS_HIP(h, hipGetLastError());
+        GIS_HIP(h, hipMemcpyAsync(ost_host, ost, sizeof(KState) * NR, hipMemcpyDeviceToHost,
+                                  h->stream));
+        GIS_HIP(h, hipStreamSynchronize(h->stream));
+        unsigned skip = 0;
+        for (int k = 0; k < NR; ++k)
+            if (ost_host[k].done) skip |= 1u << k;
+        if (skip == all) break;
+        for (int k = 0; k < NR; ++k) {
+            KState s{};
+            // inner target: inner_rel_tol ||R||, but no further than half the outer
+            // target (the last correction need not overshoot it)
+            s.error = std::max(a.inner_rel_tol * ost_host[k].norm, 0.5 * ost_host[k].error);
+            s.max_steps = a.max_inner <= 0 ? (long long)INT_MAX : a.max_inner;
+            s.int_bound = a.int_bound;
+            mb.host[k] = s;
+        }
+        GIS_HIP(h, hipMemcpyAsync(mb.st, mb.host, sizeof(KState) * NR, hipMemcpyHostToDevice,
+                                  h->stream));
+        GIS_HIP(h, hipMemsetAsync(D, 0, sizeof(float) * n * NR, h->stream));
+        GIS_TRY((run_multi<NR, float>(h, method, m32, m32T, a.max_inner, D, R32, base, mb,
+                                      skip)));
+        hipLaunchKernelGGL((k_mcorr<NR>), g, dim3(kWG), 0, h->stream, N, X, D, mb.st, ost,
+                           all & ~skip);
+        GIS_HIP(h, hipGetLastError());
+    }
+    return kOK;
+}
+
+#define GIS_NR_DISPATCH(nrhs, NRC, ...)                     \
+    switch (nrhs) {                                         \
+        case 1: { constexpr int NRC = 1; __VA_ARGS__; } break; \
+        case 2: { constexpr int NRC = 2; __VA_ARGS__; } break; \
+        case 3: { constexpr int NRC = 3; __VA_ARGS__; } break; \
+        case 4: { constexpr int NRC = 4; __VA_ARGS__; } break; \
+        case 5: { constexpr int NRC = 5; __VA_ARGS__; } break; \
+        case 6: { constexpr int NRC = 6; __VA_ARGS__; } break; \
+        case 7: { constexpr int NRC = 7; __VA_ARGS__; } break; \
+        default: { constexpr int NRC = 8; __VA_ARGS__; } break; \
+    }
 
 }  // namespace
 
@@ -535,18 +721,9 @@ int device_solve_multi(gis_context* h, int method, int nrhs, bool transpose, con
     if (method != 0 && method != 1) return fail(h, kErrArg, "multi-RHS solves support CG and BiCG");
     GIS_TRY(ensure_valT(h));
     MultiBufs mb;
-    GIS_TRY(multi_prepare(h, nrhs, a, 6, mb));
+    GIS_TRY(multi_prepare(h, nrhs, a, sizeof(double) * (size_t)h->N * 6 * nrhs, mb));
     int st = kOK;
-    switch (nrhs) {
-        case 1: st = solve_multi_nr<1>(h, method, transpose, a, X, B, mb); break;
-        case 2: st = solve_multi_nr<2>(h, method, transpose, a, X, B, mb); break;
-        case 3: st = solve_multi_nr<3>(h, method, transpose, a, X, B, mb); break;
-        case 4: st = solve_multi_nr<4>(h, method, transpose, a, X, B, mb); break;
-        case 5: st = solve_multi_nr<5>(h, method, transpose, a, X, B, mb); break;
-        case 6: st = solve_multi_nr<6>(h, method, transpose, a, X, B, mb); break;
-        case 7: st = solve_multi_nr<7>(h, method, transpose, a, X, B, mb); break;
-        default: st = solve_multi_nr<8>(h, method, transpose, a, X, B, mb); break;
-    }
+    GIS_NR_DISPATCH(nrhs, NRC, st = solve_multi_nr<NRC>(h, method, transpose, a, X, B, mb));
     if (st != kOK) return st;
     bool done = false;
     GIS_TRY(multi_read(h, nrhs, mb, &done));
@@ -560,6 +737,53 @@ int device_solve_multi(gis_context* h, int method, int nrhs, bool transpose, con
         if (normf) normf[k] = s.norm;
         if (status) status[k] = sk;
         worst = std::max(worst, sk);
+    }
+    return worst;
+}
+
+int device_solve_multi_mixed(gis_context* h, int method, int nrhs, const MultiMixedArgs& a,
+                             double* X, const double* B, double* hist, int* outer, int* iters,
+                             double* rel_res, int* status) {
+    if (nrhs < 1 || nrhs > kMaxRhs) return fail(h, kErrArg, "nrhs must be in 1..8");
+    if (method != 0 && method != 1) return fail(h, kErrArg, "multi-RHS solves support CG and BiCG");
+    GIS_TRY(ensure_val32(h));
+    if (method == 1) GIS_TRY(ensure_val32T(h));
+    SolveArgs sa{};
+    sa.hist_cap = 0;
+    MultiBufs mb;
+    // float workspace: 6 Krylov vectors, R32 and the correction D, per column
+    GIS_TRY(multi_prepare(h, nrhs, sa, sizeof(float) * (size_t)h->N * 8 * nrhs, mb));
+    const int cap = std::max(1, a.hist_cap);
+    GIS_HIP(h, h->hist.ensure(sizeof(double) * (size_t)cap * nrhs));
+    KState* ost = mb.st + kMaxRhs;
+    KState* ost_host = mb.host + kMaxRhs;
+    for (int k = 0; k < nrhs; ++k) {
+        KState s{};
+        s.max_steps = a.max_outer <= 0 ? (long long)INT_MAX : a.max_outer;
+        s.hist = h->hist.as<double>() + (size_t)k * cap;
+        s.hist_cap = cap;
+        ost_host[k] = s;
+    }
+    GIS_HIP(h, hipMemcpyAsync(ost, ost_host, sizeof(KState) * nrhs, hipMemcpyHostToDevice,
+                              h->stream));
+    int st = kOK;
+    GIS_NR_DISPATCH(nrhs, NRC,
+                    st = solve_multi_mixed_nr<NRC>(h, method, a, X, B, mb, ost, ost_host));
+    if (st != kOK) return st;
+    GIS_HIP(h, hipMemcpyAsync(ost_host, ost, sizeof(KState) * nrhs, hipMemcpyDeviceToHost,
+                              h->stream));
+    if (hist && a.hist_cap > 0)
+        GIS_HIP(h, hipMemcpyAsync(hist, h->hist.p, sizeof(double) * (size_t)cap * nrhs,
+                                  hipMemcpyDeviceToHost, h->stream));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    int worst = kOK;
+    for (int k = 0; k < nrhs; ++k) {
+        const KState& s = ost_host[k];
+        if (outer) outer[k] = s.nhist;
+        if (iters) iters[k] = (int)s.aux[0];
+        if (rel_res) rel_res[k] = s.aux[3] > 0 ? s.norm / s.aux[3] : s.norm;
+        if (status) status[k] = s.status;
+        worst = std::max(worst, s.status);
     }
     return worst;
 }

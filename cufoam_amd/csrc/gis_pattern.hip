@@ -266,7 +266,7 @@ int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua
     if (N < 1 || M < 0 || nnz >= (1LL << 31))
         return fail(h, kErrShape, "n_cells must be >= 1 and n_cells + 2*n_faces < 2^31");
     h->has_pattern = false;
-    h->has_values = h->has_val32 = h->has_valT = false;
+    h->has_values = h->has_val32 = h->has_valT = h->has_val32T = false;
     GIS_HIP(h, h->rowptr.ensure(sizeof(int) * ((size_t)N + 1)));
     GIS_HIP(h, h->col.ensure(sizeof(int) * (size_t)nnz));
     GIS_HIP(h, h->perm.ensure(sizeof(int) * (size_t)nnz));
@@ -371,6 +371,7 @@ int gather_values(gis_context* h) {
     h->has_values = true;
     h->has_val32 = false;
     h->has_valT = false;
+    h->has_val32T = false;
     return kOK;
 }
 
@@ -384,6 +385,16 @@ int ensure_val32(gis_context* h) {
 }
 
 // A^T of an LDU matrix: same pattern, lower <-> upper
+// float(A^T): the mixed-precision transpose solves of BiCG
+int ensure_val32T(gis_context* h) {
+    if (!h->has_values) return fail(h, kErrState, "values not set");
+    if (h->has_val32T) return kOK;
+    GIS_TRY(gather_both<float>(h, h->upper.as<double>(), h->lower.as<double>(), h->val32T,
+                               h->sval32T));
+    h->has_val32T = true;
+    return kOK;
+}
+
 int ensure_valT(gis_context* h) {
     if (!h->has_values) return fail(h, kErrState, "values not set");
     if (h->has_valT) return kOK;

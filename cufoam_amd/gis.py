@@ -85,6 +85,10 @@ SIGNATURES = {
     "gis_solve_multi": (_i, [_vp, _i, _i, _vp, _vp, C.POINTER(_Params), _i, _vp, _vp, _vp]),
     "gis_solve_multi_device": (_i, [_vp, _i, _i, _vp, _vp, C.POINTER(_Params), _i, _vp, _vp,
                                     _vp]),
+    "gis_solve_multi_mixed": (_i, [_vp, _i, _i, _vp, _vp, _d, _d, _ll, _ll, _vp, _i, _vp, _vp,
+                                   _vp, _vp]),
+    "gis_solve_multi_mixed_device": (_i, [_vp, _i, _i, _vp, _vp, _d, _d, _ll, _ll, _vp, _i,
+                                          _vp, _vp, _vp, _vp]),
     "gis_adjoint": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_Params), _vp,
                          _vp]),
     "gis_adjoint_device": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
@@ -400,6 +404,33 @@ class Gis:
                _ptr(it), _ptr(nf), _ptr(sts))
         self._check(st, f"solve_multi({method})")
         return X, it, nf, sts
+
+    def solve_multi_mixed(self, method: str, B, X0=None, rel_tol: float = 1e-10,
+                          inner_rel_tol: float = 1e-4, max_outer: int = 100,
+                          max_inner: int = 0, hist_cap: int = 64):
+        """FP32/FP64 refinement of several right-hand sides (B: nrhs x N) ->
+        (X, outer residual histories, inner iterations, relative residuals, status)."""
+        if _is_dev(B):
+            import torch
+            nrhs = int(B.shape[0])
+            X = X0 if X0 is not None else torch.zeros_like(B)
+            f = self._L.gis_solve_multi_mixed_device
+        else:
+            B = _host(B, np.float64)
+            nrhs = B.shape[0]
+            X = np.array(X0 if X0 is not None else np.zeros_like(B), dtype=np.float64, copy=True)
+            f = self._L.gis_solve_multi_mixed
+        hist = np.zeros((nrhs, max(hist_cap, 1)))
+        outer = np.zeros(nrhs, np.int32)
+        it = np.zeros(nrhs, np.int32)
+        rel = np.zeros(nrhs)
+        sts = np.zeros(nrhs, np.int32)
+        st = f(self._h, METHODS[method], nrhs, _ptr(X), _ptr(B), rel_tol, inner_rel_tol,
+               max_outer, max_inner, _ptr(hist), hist_cap, _ptr(outer), _ptr(it), _ptr(rel),
+               _ptr(sts))
+        self._check(st, f"solve_multi_mixed({method})")
+        hists = [hist[k, :min(int(outer[k]), hist_cap)].copy() for k in range(nrhs)]
+        return X, hists, it, rel, sts
 
     def adjoint(self, W, x, method: str = "CG", error: float = 1e-12, max_steps: int = 0):
         """Gradients of L_k = W[k] . x through x = A^-1 b -> (lambda = dL/db, d_diag,

@@ -200,6 +200,30 @@ GIS_API int gis_solve_multi_device(gis_handle h, int method, int nrhs, double *d
                                    const double *d_B, const gis_params *params, int transpose,
                                    int *iterations, double *final_residual, int *status);
 
+/* ---- mixed-precision refinement of several right-hand sides (no counterpart in
+ * the reference).  Per column, restarted refinement: the FP64 residual
+ * R = B - A X (an FP64 SpMM), an FP32 multi-RHS CG or BiCG on A D = R from D = 0
+ * (FP32 matrix, vectors and updates; FP64 reductions) until its recursive
+ * residual falls below max(inner_rel_tol ||R||_2, rel_tol ||b||_2 / 2) or
+ * max_inner iterations, then
+ * X += D in FP64; until ||b - A x||_2 <= rel_tol ||b||_2 (FP64, recomputed) or
+ * max_outer corrections (<= 0: no cap).  BiCG breakdown bound 1e-10.
+ * Per column: hist[k * hist_cap + j] = ||b - A x||_2 after j corrections (may
+ * be NULL), outer_steps = number of entries of hist, iterations = inner
+ * iterations summed, rel_residual = final ||b - A x|| / ||b||, status GIS_OK /
+ * GIS_NOT_CONVERGED / GIS_NAN. ---- */
+GIS_API int gis_solve_multi_mixed(gis_handle h, int method, int nrhs, double *X,
+                                  const double *B, double rel_tol, double inner_rel_tol,
+                                  long long max_outer, long long max_inner, double *hist,
+                                  int hist_cap, int *outer_steps, int *iterations,
+                                  double *rel_residual, int *status);
+GIS_API int gis_solve_multi_mixed_device(gis_handle h, int method, int nrhs, double *d_X,
+                                         const double *d_B, double rel_tol,
+                                         double inner_rel_tol, long long max_outer,
+                                         long long max_inner, double *hist, int hist_cap,
+                                         int *outer_steps, int *iterations,
+                                         double *rel_residual, int *status);
+
 /* ---- adjoint of x = A^-1 b (no counterpart in the reference; for callers that
  * differentiate through the solve): for nrhs losses with dL_k/dx = W[k],
  * solves A^T lambda_k = W[k] with the same kernels on the transposed matrix

@@ -83,6 +83,11 @@ def lib():
                                C.c_void_p, _f64p, _f64p, C.c_double, C.c_int, C.c_double,
                                C.c_int, C.c_double, _f64p, C.c_int, C.POINTER(C.c_int),
                                C.POINTER(C.c_double)]
+        L.or_mixed_krylov.restype = C.c_int
+        L.or_mixed_krylov.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, _f64p, _f64p, C.c_double,
+                                      C.c_double, C.c_int, C.c_int, _f64p, C.c_int,
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
         if hasattr(L, "or_mixed_cg"):
             L.or_mixed_cg.restype = C.c_int
             L.or_mixed_cg.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double,
@@ -309,6 +314,34 @@ def mixed_cg(A: Csr, b, x0, rel_tol: float = 1e-10, inner_rel_tol: float = 1e-4,
 # the reference's own SparseCPU.h, compiled where the sources exist
 # ---------------------------------------------------------------------------
 _ref = None
+
+
+def mixed_krylov(method: str, A: Csr, b, x0=None, rel_tol: float = 1e-10,
+                 inner_rel_tol: float = 1e-4, max_outer: int = 100, max_inner: int = 0,
+                 hist_cap: int = 64):
+    """Restarted FP32/FP64 refinement with an FP32 CG or BiCG inner solve (one column of
+    gis_solve_multi_mixed) -> (x, outer residual history, inner iterations, status)."""
+    m = {"CG": 0, "BiCG": 1}[method]
+    x = _c(np.array(np.zeros(A.n) if x0 is None else x0, dtype=np.float64, copy=True),
+           np.float64)
+    At = transpose(A) if m == 1 else None
+    hist = np.zeros(max(hist_cap, 1))
+    outer, inner = C.c_int(0), C.c_int(0)
+    keep = []
+
+    def vp(a, dt):
+        a = _c(a, dt)
+        keep.append(a)
+        return a.ctypes.data_as(C.c_void_p)
+
+    st = lib().or_mixed_krylov(m, A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                               _c(A.val, np.float64),
+                               vp(At.rowptr, np.int32) if At else None,
+                               vp(At.col, np.int32) if At else None,
+                               vp(At.val, np.float64) if At else None,
+                               _c(b, np.float64), x, rel_tol, inner_rel_tol, max_outer,
+                               max_inner, hist, hist_cap, C.byref(outer), C.byref(inner))
+    return x, hist[:min(outer.value, hist_cap)].copy(), inner.value, st
 
 
 def ref_lib():

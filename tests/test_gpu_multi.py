@@ -180,3 +180,70 @@ def test_autograd_gradcheck_small_nonsymmetric(gpu):
     f = lambda d_, l_, u_, b_: AG.solve(gpu, d_, l_, u_, b_, method="BiCG", error=1e-14,
                                         max_steps=200)
     assert torch.autograd.gradcheck(f, (d, lo, up, b), eps=1e-6, atol=1e-8, rtol=1e-6)
+
+
+@pytest.mark.parametrize("nrhs", [1, 3, 5])
+@pytest.mark.parametrize("method", ["CG", "BiCG"])
+def test_multi_mixed_columns_equal_oracle(gpu, nrhs, method):
+    """FP32/FP64 refinement of several right-hand sides: each column's solution, outer
+    residual history and inner iteration count equal the oracle's single-column run."""
+    s = _case(12, 20 + nrhs, method == "CG", 0.05)
+    gpu.set_system(s)
+    A = O.ldu2csr(s)
+    rng = np.random.default_rng(nrhs)
+    B = rng.uniform(-1, 1, (nrhs, s.n_cells)) * (1.0 + np.arange(nrhs)[:, None])
+    X, hists, it, rel, sts = gpu.solve_multi_mixed(method, B, rel_tol=1e-11, inner_rel_tol=1e-3)
+    for k in range(nrhs):
+        rx, rh, rit, rst = O.mixed_krylov(method, A, B[k], rel_tol=1e-11, inner_rel_tol=1e-3)
+        assert sts[k] == rst == 0, k
+        assert it[k] == rit, (k, it[k], rit)
+        np.testing.assert_array_equal(hists[k], rh)
+        np.testing.assert_array_equal(X[k], rx)
+        assert rel[k] == rh[-1] / np.sqrt(O.dot(B[k], B[k]))   # the same tree norm of b
+
+
+def test_multi_mixed_caps_and_stopped_columns(gpu):
+    """Column 1 has a zero right-hand side and is converged before any correction; the
+    others hit the outer cap (2 corrections of at most 3 inner iterations): statuses,
+    histories and solutions as the oracle gives them."""
+    s = _case(8, 3, False, 0.2)
+    gpu.set_system(s)
+    A = O.ldu2csr(s)
+    B = np.random.default_rng(2).uniform(-1, 1, (3, s.n_cells))
+    B[1] = 0.0
+    X, hists, it, rel, sts = gpu.solve_multi_mixed("BiCG", B, rel_tol=1e-14, inner_rel_tol=0.5,
+                                                   max_outer=2, max_inner=3)
+    for k in range(3):
+        rx, rh, rit, rst = O.mixed_krylov("BiCG", A, B[k], rel_tol=1e-14, inner_rel_tol=0.5,
+                                          max_outer=2, max_inner=3)
+        assert sts[k] == rst and it[k] == rit
+        np.testing.assert_array_equal(hists[k], rh)
+        np.testing.assert_array_equal(X[k], rx)
+    assert sts[1] == 0 and sts[0] == 11 and sts[2] == 11
+
+
+def test_multi_mixed_3rhs_convdiff_128(gpu):
+    """The batched nonsymmetric workload: 128^3 convection-diffusion, 3 right-hand sides,
+    one SpMM-BiCG per inner step with FP32/FP64 refinement to 1e-10, checked by an
+    independent FP64 residual and against the FP64 SpMM-BiCG solution."""
+    import torch
+    s = mesh.config_system("convdiff128x3")
+    gpu.set_system(mesh.LduSystem(s.n_cells, s.lower_addr, s.upper_addr, s.diag, s.lower,
+                                  s.upper, s.b[0], s.x0[0]))
+    B = torch.from_numpy(np.ascontiguousarray(s.b)).cuda()
+    X, hists, it, rel, sts = gpu.solve_multi_mixed("BiCG", B, rel_tol=1e-10, inner_rel_tol=1e-4)
+    assert np.all(sts == 0)
+    A = O.ldu2csr(mesh.LduSystem(s.n_cells, s.lower_addr, s.upper_addr, s.diag, s.lower,
+                                 s.upper, s.b[0], s.x0[0]))
+    Xh = X.cpu().numpy()
+    for k in range(3):
+        true = np.linalg.norm(s.b[k] - O.spmv(A, Xh[k])) / np.linalg.norm(s.b[k])
+        assert true <= 1e-10 and abs(true - rel[k]) <= 1e-3 * rel[k]
+    X64, it64, nf, st64 = gpu.solve_multi("BiCG", B, torch.zeros_like(B),
+                                          error=1e-11 * float(np.linalg.norm(s.b[0])),
+                                          max_steps=5000)
+    err = [float(torch.linalg.vector_norm(X[k] - X64[k]) / torch.linalg.vector_norm(X64[k]))
+           for k in range(3)]
+    print(f"[mixed3] outer {[len(h) for h in hists]} inner {it.tolist()} fp64 {it64.tolist()} "
+          f"rel.err vs fp64 {err}", flush=True)
+    assert max(err) < 1e-8

@@ -424,3 +424,32 @@ def test_openfoam_pcg_baseline_is_cg():
     assert res == pytest.approx(np.abs(r).sum() / np.abs(s.b).sum(), rel=1e-9)
     x0, it0, res0, _ = O.pcg_openfoam(s, 2.0, 100)     # tolerance above the first residual
     assert it0 == 0 and res0 == pytest.approx(1.0, rel=1e-15)
+
+
+@pytest.mark.parametrize("method,symmetric", [("CG", True), ("BiCG", False)])
+def test_mixed_krylov_refinement_reaches_fp64_target(method, symmetric):
+    """Restarted FP32/FP64 refinement: every outer step shrinks the FP64 residual by
+    about the inner tolerance, and the final x meets the FP64 target recomputed here."""
+    s = mesh.laplacian(12, seed=5, shift=1e-2, symmetric=symmetric)
+    A = O.ldu2csr(s)
+    x, hist, inner, st = O.mixed_krylov(method, A, s.b, rel_tol=1e-12, inner_rel_tol=1e-3)
+    assert st == 0
+    bn = np.linalg.norm(s.b)
+    assert np.linalg.norm(s.b - O.spmv(A, x)) <= 1e-12 * bn
+    assert hist[-1] <= 1e-12 * bn < hist[-2]
+    assert all(hist[j + 1] < 0.05 * hist[j] for j in range(len(hist) - 1))
+    assert inner >= len(hist) - 1
+
+
+def test_mixed_krylov_single_outer_step_is_an_fp32_solve():
+    """One correction from x0 = 0 with a tight inner tolerance: x equals the FP32 Krylov
+    solution, i.e. close to A^-1 b at single-precision accuracy, and the outer cap is
+    reported."""
+    s = mesh.laplacian(6, seed=1, shift=0.5, symmetric=True)
+    A = O.ldu2csr(s)
+    x, hist, inner, st = O.mixed_krylov("CG", A, s.b, rel_tol=1e-14, inner_rel_tol=1e-6,
+                                        max_outer=1)
+    assert st == 11 and len(hist) == 2
+    xe = np.linalg.solve(mesh.ldu_to_dense(s), s.b)
+    assert np.linalg.norm(x - xe) <= 1e-5 * np.linalg.norm(xe)
+    assert hist[1] < 1e-4 * hist[0]
