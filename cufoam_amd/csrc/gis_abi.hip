@@ -694,10 +694,15 @@ GIS_API int gis_batch_solve_device(gis_handle h, double* d_x, const double* d_b,
                        final_residual, status);
 }
 
-GIS_API int gis_foam_solve(gis_handle h, int method, double* psi, const double* source,
-                           double tolerance, long long max_iter, gis_performance* perf) {
+namespace {
+
+int foam_solve_impl(gis_context* h, int method, double* psi, const double* source,
+                    double tolerance, double rel_tol, long long min_iter, long long max_iter,
+                    gis_performance* perf) {
     if (!h || !psi || !source) return GIS_ERR_ARG;
     if (method < GIS_CG || method > GIS_RICHARDSON) return fail(h, GIS_ERR_ARG, "unknown method");
+    if (!(rel_tol >= 0.0) || min_iter < 0)
+        return fail(h, GIS_ERR_ARG, "rel_tol and min_iter must be non-negative");
     if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
     DeviceGuard g(h->device);
     double nf = 0.0;
@@ -710,6 +715,8 @@ GIS_API int gis_foam_solve(gis_handle h, int method, double* psi, const double* 
     p.power_steps = 30;
     p.int_bound = 1e-10;        // cuFoamBiCG.C:68
     SolveArgs a = to_args(method, &p, 1);
+    a.rel_tol = rel_tol;
+    a.min_steps = min_iter;
     int len = 0;
     double n0 = 0.0, nfin = 0.0;
     const int st = solve_staged(h, a, psi, source, false, nullptr, 0, &len, &n0, &nfin);
@@ -722,6 +729,20 @@ GIS_API int gis_foam_solve(gis_handle h, int method, double* psi, const double* 
         perf->norm_factor = nf;
     }
     return st;
+}
+
+}  // namespace
+
+GIS_API int gis_foam_solve(gis_handle h, int method, double* psi, const double* source,
+                           double tolerance, long long max_iter, gis_performance* perf) {
+    return foam_solve_impl(h, method, psi, source, tolerance, 0.0, 0, max_iter, perf);
+}
+
+GIS_API int gis_foam_solve_controls(gis_handle h, int method, double* psi,
+                                    const double* source, double tolerance, double rel_tol,
+                                    long long min_iter, long long max_iter,
+                                    gis_performance* perf) {
+    return foam_solve_impl(h, method, psi, source, tolerance, rel_tol, min_iter, max_iter, perf);
 }
 
 }  // extern "C"

@@ -48,7 +48,15 @@ struct KState {
     int hist_cap;
     int flag;                         // per-solver control word
     int nhist;                        // entries pushed by solvers with sparse histories
+    double rel_tol;                   // > 0: error := max(error, rel_tol * norm0)
+    long long min_steps;              // no stop on convergence before this many steps
 };
+
+// OpenFOAM's loop rule (lduMatrix::solver): iterate while (k < maxIter and not
+// converged) or k < minIter.  With min_steps = 0 it is the reference's rule.
+__device__ __forceinline__ bool stop_after(const KState* st, bool converged) {
+    return (converged || st->k >= st->max_steps) && st->k >= st->min_steps;
+}
 
 // ---------------------------------------------------------------------------
 // cache-bypassing agent-scope accesses for the cross-workgroup partials.

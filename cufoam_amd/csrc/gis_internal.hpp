@@ -214,6 +214,8 @@ struct SolveArgs {
     int power_steps;
     double int_bound;
     int hist_cap;
+    double rel_tol = 0.0;     // > 0: stop also at rel_tol * initial residual
+    long long min_steps = 0;  // no stop on convergence before this many iterations
 };
 
 // x (in/out) and b are device vectors owned by the handle's io workspace.

@@ -198,7 +198,8 @@ __global__ __launch_bounds__(kWG) void k_setup(Mat<double> m, int N,
         st->k = 0;
         st->status = kOK;
         if (st->hist_cap > 0) st->hist[0] = norm;
-        st->done = (done_rule == 0) ? !(norm > st->error) : 0;
+        if (st->rel_tol > 0.0) st->error = fmax(st->error, st->rel_tol * norm);
+        st->done = (done_rule == 0) ? (!(norm > st->error) && st->min_steps <= 0) : 0;
     });
 }
 

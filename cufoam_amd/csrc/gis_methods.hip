@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kWG) void k_mr2(int N, double* __restrict__ r,
     finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
         const double norm = sqrt(v[0]);
         record_norm(st, norm);
-        if (!(norm > st->error) || st->k >= st->max_steps) st->done = 1;
+        if (stop_after(st, !(norm > st->error))) st->done = 1;
     });
 }
 
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kWG) void k_jacobi(Mat<double> m, int N, double* __
     finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
         const double norm = sqrt(v[0]);
         record_norm(st, norm);
-        if (!(norm > st->error) || st->k >= st->max_steps) st->done = 1;
+        if (stop_after(st, !(norm > st->error))) st->done = 1;
     });
 }
 
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kWG) void k_richardson(Mat<double> m, int N,
     finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
         const double norm = sqrt(v[0]);
         record_norm(st, norm);
-        if (!(norm > st->error) || st->k >= st->max_steps) st->done = 1;
+        if (stop_after(st, !(norm > st->error))) st->done = 1;
     });
 }
 
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(kWG) void k_bicg2(Mat<double> mt, int N, double* __
     finish_tile<2>(red, t.tile, leaf, shr, [&](const double (&v)[2]) {
         const double norm = sqrt(v[0]);
         record_norm(st, norm);
-        if (norm < st->error || st->k >= st->max_steps) {
+        if (stop_after(st, norm < st->error)) {
             st->done = 1;
         } else {
             st->rho1 = st->rho0;

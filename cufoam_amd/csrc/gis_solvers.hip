@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kWG) void k_cg2(int N, double* __restrict__ r,
         st->beta = st->rho0 / st->rho1;
         const double norm = sqrt(v[0]);
         record_norm(st, norm);
-        if (!(norm > st->error) || st->k >= st->max_steps) st->done = 1;
+        if (stop_after(st, !(norm > st->error))) st->done = 1;
     });
 }
 
@@ -471,6 +471,8 @@ int prepare_solve(gis_context* h, const SolveArgs& a, int nvec) {
     s.max_steps = a.max_steps <= 0 ? (long long)INT_MAX : a.max_steps;
     s.omega = a.omega;
     s.int_bound = a.int_bound;
+    s.rel_tol = a.rel_tol;
+    s.min_steps = a.min_steps;
     s.hist = h->hist.as<double>();
     s.hist_cap = cap;
     s.aux[4] = INFINITY;   // running min for FindDiagAbsMin

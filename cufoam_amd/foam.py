@@ -13,7 +13,9 @@ Semantics reproduced from the reference (cuFoamCG/cuFoamCG.C:38-89 and the
 seven siblings):
   * ``tolerance`` is absolute on the L2 residual, scaled by NormFactor
     (gis/gis.cu:106-123); ``maxIter`` is the iteration cap (<= 0 unbounded);
-  * ``relTol``, ``minIter`` and ``cmpt`` are accepted and ignored, as there;
+  * ``relTol``, ``minIter`` and ``cmpt`` are accepted and ignored, as there,
+    unless the controls hold ``convergence openfoam``: then relTol and minIter
+    act as in OpenFOAM's lduMatrix::solver (gis_foam_solve_controls);
   * the returned residuals are the unnormalised L2 norms history.front() and
     history.back(), nIterations = history.size() - 1;
   * cuFoamCG and cuFoamRichardson are registered for symmetric matrices only
@@ -86,6 +88,10 @@ class cuFoamSolver:
         self.relTol = float(controls.get("relTol", 0.0))        # ignored, as in the reference
         self.maxIter = int(controls.get("maxIter", 1000))
         self.minIter = int(controls.get("minIter", 0))          # ignored, as in the reference
+        self.convergence = str(controls.get("convergence", "reference"))
+        if self.convergence not in ("reference", "openfoam"):
+            raise FatalError(f"convergence must be reference or openfoam, not "
+                             f"{self.convergence}")
         self._gis = gis if gis is not None else Gis()
         self._pattern_key = None
 
@@ -102,8 +108,13 @@ class cuFoamSolver:
             raise FatalError("psi must be a contiguous float64 field")
         self._upload()
         src = np.ascontiguousarray(source, dtype=np.float64)
-        perf = self._gis.foam_solve(_TABLE[self.typeName][0], psi, src, self.tolerance,
-                                    self.maxIter)
+        if self.convergence == "openfoam":
+            perf = self._gis.foam_solve(_TABLE[self.typeName][0], psi, src, self.tolerance,
+                                        self.maxIter, rel_tol=self.relTol,
+                                        min_iter=self.minIter)
+        else:
+            perf = self._gis.foam_solve(_TABLE[self.typeName][0], psi, src, self.tolerance,
+                                        self.maxIter)
         perf.solver = self.typeName
         return perf
 

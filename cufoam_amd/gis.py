@@ -82,6 +82,7 @@ SIGNATURES = {
     "gis_solve": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
     "gis_solve_device": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
     "gis_foam_solve": (_i, [_vp, _i, _vp, _vp, _d, _ll, C.POINTER(_Perf)]),
+    "gis_foam_solve_controls": (_i, [_vp, _i, _vp, _vp, _d, _d, _ll, _ll, C.POINTER(_Perf)]),
     "gis_solve_multi": (_i, [_vp, _i, _i, _vp, _vp, C.POINTER(_Params), _i, _vp, _vp, _vp]),
     "gis_solve_multi_device": (_i, [_vp, _i, _i, _vp, _vp, C.POINTER(_Params), _i, _vp, _vp,
                                     _vp]),
@@ -511,11 +512,21 @@ class Gis:
         self._check(st, "batch_solve")
         return x, it, nf, sts
 
-    def foam_solve(self, method: str, psi, source, tolerance: float, max_iter: int):
-        """cuFoam<method>::solve: psi (numpy) is updated in place; returns Performance."""
+    def foam_solve(self, method: str, psi, source, tolerance: float, max_iter: int,
+                   rel_tol: float | None = None, min_iter: int = 0):
+        """cuFoam<method>::solve: psi (numpy) is updated in place; returns Performance.
+
+        rel_tol / min_iter (OpenFOAM's relTol / minIter) select
+        gis_foam_solve_controls; left at None / 0 the call is gis_foam_solve."""
         perf = _Perf()
-        st = self._L.gis_foam_solve(self._h, METHODS[method], _ptr(psi), _ptr(source),
-                                    tolerance, max_iter, C.byref(perf))
+        if rel_tol is None and min_iter == 0:
+            st = self._L.gis_foam_solve(self._h, METHODS[method], _ptr(psi), _ptr(source),
+                                        tolerance, max_iter, C.byref(perf))
+        else:
+            st = self._L.gis_foam_solve_controls(self._h, METHODS[method], _ptr(psi),
+                                                 _ptr(source), tolerance,
+                                                 float(rel_tol or 0.0), int(min_iter),
+                                                 max_iter, C.byref(perf))
         self._check(st, f"foam_solve({method})")
         return Performance("cuFoam" + method, perf.initial_residual, perf.final_residual,
                            perf.n_iterations, perf.status, perf.norm_factor)

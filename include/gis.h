@@ -185,6 +185,19 @@ GIS_API int gis_solve_device(gis_handle h, int method, double *d_x, const double
 GIS_API int gis_foam_solve(gis_handle h, int method, double *psi, const double *source,
                            double tolerance, long long max_iter, gis_performance *perf);
 
+/* ---- gis_foam_solve with OpenFOAM's remaining solver controls
+ * (lduMatrix::solver reads tolerance, relTol, minIter and maxIter from
+ * fvSolution; the reference wrappers pass on tolerance and maxIter only,
+ * cuFoamCG.C:68-74).  The residual is the method's own L2 norm and
+ * threshold = max(tolerance * nf, rel_tol * initial residual); iteration k
+ * ends the solve when (the method's convergence test against threshold holds
+ * or k >= max_iter) and k >= min_iter.  rel_tol = 0 and min_iter = 0 give
+ * gis_foam_solve exactly. ---- */
+GIS_API int gis_foam_solve_controls(gis_handle h, int method, double *psi,
+                                    const double *source, double tolerance, double rel_tol,
+                                    long long min_iter, long long max_iter,
+                                    gis_performance *perf);
+
 /* ---- several right-hand sides sharing the matrix (no counterpart in the
  * reference, which solves one field component per call; for callers with the
  * three velocity components of one momentum matrix, or batched adjoint loads):

@@ -346,10 +346,27 @@ static inline void hpush(hist_t *h, double v) {
 
 static inline int64_t unbounded(int max_steps) { return max_steps <= 0 ? INT_MAX : max_steps; }
 
+/* OpenFOAM's remaining solver controls (lduMatrix::solver: relTol, minIter).
+ * rel_tol = 0 and min_steps = 0 leave every loop below as the reference has it. */
+typedef struct {
+    double rel_tol;    /* > 0: threshold = max(error, rel_tol * initial residual) */
+    int64_t min_steps; /* iterations run whatever the residual */
+} ctl_t;
+
+static inline double ctl_error(const ctl_t *c, double error, double norm0) {
+    return c->rel_tol > 0.0 ? fmax(error, c->rel_tol * norm0) : error;
+}
+
+/* whether iteration k runs: (k <= ms and not converged) or k <= min_steps */
+static inline int ctl_next(const ctl_t *c, int64_t k, int64_t ms, double norm_r, double error) {
+    return (k <= ms && norm_r > error) || k <= c->min_steps;
+}
+
 /* ------------------------------------------------------------------------ */
 /* CG  (SolverSingleGPU.h:366-445)                                           */
 /* ------------------------------------------------------------------------ */
-static int cg(const csr_t *A, const double *b, double *x, double error, int max_steps, hist_t *h) {
+static int cg(const csr_t *A, const double *b, double *x, double error, int max_steps,
+              const ctl_t *c, hist_t *h) {
     const int N = A->n;
     double *r = (double *)malloc(sizeof(double) * (size_t)N * 3);
     if (!r) return OR_ERR_NOMEM;
@@ -361,7 +378,8 @@ static int cg(const csr_t *A, const double *b, double *x, double error, int max_
     double norm_r = sqrt(rho0);
     int64_t ms = unbounded(max_steps);
     hpush(h, norm_r);
-    for (int64_t k = 1; k <= ms && norm_r > error; ++k) {
+    error = ctl_error(c, error, norm_r);
+    for (int64_t k = 1; ctl_next(c, k, ms, norm_r, error); ++k) {
         if (k > 1) axpy(N, rho0 / rho1, p, r, p);
         spmv(A, p, q);
         double xi = rho0 / or_dot(N, p, q);
@@ -380,7 +398,7 @@ static int cg(const csr_t *A, const double *b, double *x, double error, int max_
 /* BiCG with an explicit transpose  (SolverSingleGPU.h:568-675)              */
 /* ------------------------------------------------------------------------ */
 static int bicg(const csr_t *A, const csr_t *At, const double *b, double *x, double error,
-                int max_steps, double int_bound, hist_t *h) {
+                int max_steps, double int_bound, const ctl_t *c, hist_t *h) {
     const int N = A->n;
     double *r = (double *)malloc(sizeof(double) * (size_t)N * 6);
     if (!r) return OR_ERR_NOMEM;
@@ -395,8 +413,9 @@ static int bicg(const csr_t *A, const csr_t *At, const double *b, double *x, dou
     double norm_r = norm2(N, r);
     int64_t ms = unbounded(max_steps);
     hpush(h, norm_r);
+    error = ctl_error(c, error, norm_r);
     int status = OR_OK;
-    for (int64_t k = 1; k <= ms; ++k) {
+    for (int64_t k = 1; k <= ms || k <= c->min_steps; ++k) {
         spmv(A, p, q);
         double xi = rho0 / or_dot(N, ps, q);
         if (xi > -int_bound && xi < int_bound) {
@@ -407,7 +426,7 @@ static int bicg(const csr_t *A, const csr_t *At, const double *b, double *x, dou
         axpy(N, -xi, q, r, r);
         norm_r = norm2(N, r);
         hpush(h, norm_r);
-        if (norm_r < error) break;
+        if (norm_r < error && k >= c->min_steps) break;
         spmv(At, ps, qs);
         axpy(N, -xi, qs, rs, rs);
         rho1 = rho0;
@@ -460,7 +479,8 @@ OR_API double or_optimize_power(int N, const int *rowptr, const int *col, const 
 }
 
 static int richardson(const csr_t *A, const double *b, double *x, double error, double omega,
-                      int power_steps, int max_steps, hist_t *h, double *omega_used) {
+                      int power_steps, int max_steps, const ctl_t *c, hist_t *h,
+                      double *omega_used) {
     const int N = A->n;
     if (-1e-15 < omega && omega < 1e-15)
         omega = or_optimize_power(N, A->rowptr, A->col, A->val, power_steps);
@@ -471,8 +491,8 @@ static int richardson(const csr_t *A, const double *b, double *x, double error, 
     double norm_r = norm2(N, r);
     int64_t ms = unbounded(max_steps);
     hpush(h, norm_r);
-    int64_t k = 1;
-    while (k++ <= ms && norm_r > error) {
+    error = ctl_error(c, error, norm_r);
+    for (int64_t k = 1; ctl_next(c, k, ms, norm_r, error); ++k) {
         axpy(N, omega, r, x, x);
         residual(A, b, x, r);
         norm_r = norm2(N, r);
@@ -485,7 +505,8 @@ static int richardson(const csr_t *A, const double *b, double *x, double error, 
 /* ------------------------------------------------------------------------ */
 /* MR  (SolverSingleGPU.h:825-890)                                           */
 /* ------------------------------------------------------------------------ */
-static int mr(const csr_t *A, const double *b, double *x, double error, int max_steps, hist_t *h) {
+static int mr(const csr_t *A, const double *b, double *x, double error, int max_steps,
+              const ctl_t *c, hist_t *h) {
     const int N = A->n;
     double *r = (double *)malloc(sizeof(double) * (size_t)N * 2);
     if (!r) return OR_ERR_NOMEM;
@@ -494,8 +515,8 @@ static int mr(const csr_t *A, const double *b, double *x, double error, int max_
     double norm_r = norm2(N, r);
     int64_t ms = unbounded(max_steps);
     hpush(h, norm_r);
-    int64_t k = 1;
-    while (k++ <= ms && norm_r > error) {
+    error = ctl_error(c, error, norm_r);
+    for (int64_t k = 1; ctl_next(c, k, ms, norm_r, error); ++k) {
         spmv(A, r, Ar);
         double omega = or_dot(N, r, Ar) / or_dot(N, Ar, Ar);
         axpy(N, omega, r, x, x);
@@ -511,7 +532,7 @@ static int mr(const csr_t *A, const double *b, double *x, double error, int max_
 /* Jacobi (residual form)  (SolverSingleGPU.h:924-994)                       */
 /* ------------------------------------------------------------------------ */
 static int jacobi(const csr_t *A, const double *b, double *x, double error, int max_steps,
-                  hist_t *h) {
+                  const ctl_t *c, hist_t *h) {
     const int N = A->n;
     double *r = (double *)malloc(sizeof(double) * (size_t)N * 4);
     if (!r) return OR_ERR_NOMEM;
@@ -522,8 +543,8 @@ static int jacobi(const csr_t *A, const double *b, double *x, double error, int 
     for (int i = 0; i < N; ++i) D_1[i] = 1.0 / D_1[i];
     int64_t ms = unbounded(max_steps);
     hpush(h, norm_r);
-    int64_t k = 1;
-    while (k++ <= ms && norm_r > error) {
+    error = ctl_error(c, error, norm_r);
+    for (int64_t k = 1; ctl_next(c, k, ms, norm_r, error); ++k) {
         for (int i = 0; i < N; ++i) z[i] = D_1[i] * r[i];
         spmv(A, z, w);
         axpy(N, 1.0, z, x, x);
@@ -539,7 +560,7 @@ static int jacobi(const csr_t *A, const double *b, double *x, double error, int 
 /* MRJacobi  (SolverSingleGPU.h:1000-1073)                                   */
 /* ------------------------------------------------------------------------ */
 static int mrjacobi(const csr_t *A, const double *b, double *x, double error, int max_steps,
-                    hist_t *h) {
+                    const ctl_t *c, hist_t *h) {
     const int N = A->n;
     double *r = (double *)malloc(sizeof(double) * (size_t)N * 4);
     if (!r) return OR_ERR_NOMEM;
@@ -550,8 +571,8 @@ static int mrjacobi(const csr_t *A, const double *b, double *x, double error, in
     for (int i = 0; i < N; ++i) D_1[i] = 1.0 / D_1[i];
     int64_t ms = unbounded(max_steps);
     hpush(h, norm_r);
-    int64_t k = 1;
-    while (k++ <= ms && norm_r > error) {
+    error = ctl_error(c, error, norm_r);
+    for (int64_t k = 1; ctl_next(c, k, ms, norm_r, error); ++k) {
         for (int i = 0; i < N; ++i) z[i] = D_1[i] * r[i];
         spmv(A, z, w);
         double alpha = or_dot(N, r, w) / or_dot(N, w, w);
@@ -568,7 +589,7 @@ static int mrjacobi(const csr_t *A, const double *b, double *x, double error, in
 /* JMR = PreconditionedMRJacobi  (SolverSingleGPU.h:1079-1155)               */
 /* ------------------------------------------------------------------------ */
 static int jmr(const csr_t *A, const double *b, double *x, double error, int max_steps,
-               hist_t *h) {
+               const ctl_t *c, hist_t *h) {
     const int N = A->n;
     double *r = (double *)malloc(sizeof(double) * (size_t)N * 4);
     if (!r) return OR_ERR_NOMEM;
@@ -580,8 +601,8 @@ static int jmr(const csr_t *A, const double *b, double *x, double error, int max
     double norm_r = norm2(N, r);
     int64_t ms = unbounded(max_steps);
     hpush(h, norm_r);
-    int64_t k = 1;
-    while (k++ <= ms && norm_r > error) {
+    error = ctl_error(c, error, norm_r);
+    for (int64_t k = 1; ctl_next(c, k, ms, norm_r, error); ++k) {
         for (int i = 0; i < N; ++i) z[i] = S[i] * r[i];
         spmv(A, z, w);
         for (int i = 0; i < N; ++i) w[i] = S[i] * w[i];
@@ -599,7 +620,7 @@ static int jmr(const csr_t *A, const double *b, double *x, double error, int max
 /* MRS with invM = D^-1 as a diagonal CSR  (SolverSingleGPU.h:1236-1303)     */
 /* ------------------------------------------------------------------------ */
 static int mrs(const csr_t *A, const double *b, double *x, double error, int max_steps,
-               hist_t *h) {
+               const ctl_t *c, hist_t *h) {
     const int N = A->n;
     /* JacobiPrecondition (SparseCPU.h:428-446): invD(i,i) = 1 / a_ii */
     int *inv_rp = (int *)malloc(sizeof(int) * ((size_t)N + 1) + sizeof(int) * (size_t)N);
@@ -625,8 +646,8 @@ static int mrs(const csr_t *A, const double *b, double *x, double error, int max
     double norm_r = norm2(N, r);
     int64_t ms = unbounded(max_steps);
     hpush(h, norm_r);
-    int64_t k = 1;
-    while (k++ <= ms && norm_r > error) {
+    error = ctl_error(c, error, norm_r);
+    for (int64_t k = 1; ctl_next(c, k, ms, norm_r, error); ++k) {
         spmv(&invM, r, z);
         spmv(A, z, w);
         double alpha = or_dot(N, r, w) / or_dot(N, w, w);
@@ -648,32 +669,44 @@ static int mrs(const csr_t *A, const double *b, double *x, double error, int max
  * residual norms; *hist_len gets the full count (iterations = len - 1).
  * omega/power_steps: Richardson; int_bound: BiCG breakdown bound.
  * *omega_used receives Richardson's relaxation factor (may be NULL).
+ * rel_tol / min_steps: OpenFOAM's relTol and minIter (0 and 0: the reference).
  */
-OR_API int or_solve(int method, int N, const int *rowptr, const int *col, const double *val,
-                    const int *t_rowptr, const int *t_col, const double *t_val,
-                    const double *b, double *x, double error, int max_steps, double omega,
-                    int power_steps, double int_bound, double *hist, int hist_cap,
-                    int *hist_len, double *omega_used) {
+OR_API int or_solve_ctl(int method, int N, const int *rowptr, const int *col, const double *val,
+                        const int *t_rowptr, const int *t_col, const double *t_val,
+                        const double *b, double *x, double error, int max_steps, double omega,
+                        int power_steps, double int_bound, double rel_tol, int64_t min_steps,
+                        double *hist, int hist_cap, int *hist_len, double *omega_used) {
     csr_t A = {N, rowptr, col, val};
     csr_t At = {N, t_rowptr, t_col, t_val};
     hist_t h = {hist, hist_cap, 0};
+    const ctl_t c = {rel_tol, min_steps};
     int st;
     switch (method) {
-        case OR_CG: st = cg(&A, b, x, error, max_steps, &h); break;
+        case OR_CG: st = cg(&A, b, x, error, max_steps, &c, &h); break;
         case OR_BICG:
             if (!t_rowptr || !t_col || !t_val) return OR_ERR_ARG;
-            st = bicg(&A, &At, b, x, error, max_steps, int_bound, &h);
+            st = bicg(&A, &At, b, x, error, max_steps, int_bound, &c, &h);
             break;
-        case OR_JACOBI: st = jacobi(&A, b, x, error, max_steps, &h); break;
-        case OR_MR: st = mr(&A, b, x, error, max_steps, &h); break;
-        case OR_MRJACOBI: st = mrjacobi(&A, b, x, error, max_steps, &h); break;
-        case OR_JMR: st = jmr(&A, b, x, error, max_steps, &h); break;
-        case OR_MRS: st = mrs(&A, b, x, error, max_steps, &h); break;
+        case OR_JACOBI: st = jacobi(&A, b, x, error, max_steps, &c, &h); break;
+        case OR_MR: st = mr(&A, b, x, error, max_steps, &c, &h); break;
+        case OR_MRJACOBI: st = mrjacobi(&A, b, x, error, max_steps, &c, &h); break;
+        case OR_JMR: st = jmr(&A, b, x, error, max_steps, &c, &h); break;
+        case OR_MRS: st = mrs(&A, b, x, error, max_steps, &c, &h); break;
         case OR_RICHARDSON:
-            st = richardson(&A, b, x, error, omega, power_steps, max_steps, &h, omega_used);
+            st = richardson(&A, b, x, error, omega, power_steps, max_steps, &c, &h, omega_used);
             break;
         default: return OR_ERR_ARG;
     }
     if (hist_len) *hist_len = h.len;
     return st;
+}
+
+OR_API int or_solve(int method, int N, const int *rowptr, const int *col, const double *val,
+                    const int *t_rowptr, const int *t_col, const double *t_val,
+                    const double *b, double *x, double error, int max_steps, double omega,
+                    int power_steps, double int_bound, double *hist, int hist_cap,
+                    int *hist_len, double *omega_used) {
+    return or_solve_ctl(method, N, rowptr, col, val, t_rowptr, t_col, t_val, b, x, error,
+                        max_steps, omega, power_steps, int_bound, 0.0, 0, hist, hist_cap,
+                        hist_len, omega_used);
 }

@@ -78,6 +78,11 @@ def lib():
         L.or_find_diag_abs_min.argtypes = [C.c_int, _i32p, _i32p, _f64p]
         L.or_optimize_power.restype = C.c_double
         L.or_optimize_power.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_int]
+        L.or_solve_ctl.restype = C.c_int
+        L.or_solve_ctl.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, _f64p, _f64p, C.c_double, C.c_int,
+                                   C.c_double, C.c_int, C.c_double, C.c_double, C.c_int64, _f64p,
+                                   C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.or_solve.restype = C.c_int
         L.or_solve.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, C.c_void_p, C.c_void_p,
                                C.c_void_p, _f64p, _f64p, C.c_double, C.c_int, C.c_double,
@@ -235,12 +240,16 @@ class SolveResult:
 
 def solve(method: str, A: Csr, b, x0, error: float, max_steps: int, At: Csr | None = None,
           omega: float = 0.0, power_steps: int = 30, int_bound: float = 1e-10,
-          hist_cap: int | None = None) -> SolveResult:
-    """One gis solver (SingleGPU::<method> semantics) on the CPU; x0 is copied."""
+          hist_cap: int | None = None, rel_tol: float = 0.0,
+          min_steps: int = 0) -> SolveResult:
+    """One gis solver (SingleGPU::<method> semantics) on the CPU; x0 is copied.
+
+    rel_tol / min_steps add OpenFOAM's relTol and minIter (lduMatrix::solver);
+    0 and 0 are the reference's loops."""
     m = METHODS[method]
     x = _c(np.array(x0, dtype=np.float64, copy=True), np.float64)
     if hist_cap is None:
-        hist_cap = (max_steps if max_steps > 0 else 100000) + 1
+        hist_cap = max(max_steps if max_steps > 0 else 100000, min_steps) + 1
     hist = np.zeros(max(hist_cap, 1))
     hl = C.c_int(0)
     om = C.c_double(0.0)
@@ -252,10 +261,11 @@ def solve(method: str, A: Csr, b, x0, error: float, max_steps: int, At: Csr | No
         tptrs = [k.ctypes.data for k in keep]
     else:
         tptrs = [None, None, None]
-    st = lib().or_solve(m, A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
-                        _c(A.val, np.float64), tptrs[0], tptrs[1], tptrs[2], _c(b, np.float64),
-                        x, error, max_steps, omega, power_steps, int_bound, hist, hist_cap,
-                        C.byref(hl), C.byref(om))
+    st = lib().or_solve_ctl(m, A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                            _c(A.val, np.float64), tptrs[0], tptrs[1], tptrs[2],
+                            _c(b, np.float64), x, error, max_steps, omega, power_steps,
+                            int_bound, rel_tol, min_steps, hist, hist_cap, C.byref(hl),
+                            C.byref(om))
     del keep
     if st not in (0, BREAKDOWN):
         raise RuntimeError(f"or_solve({method}) failed with status {st}")
@@ -276,17 +286,21 @@ class Performance:
 
 
 def foam_solve(method: str, sys, psi, source, tolerance: float, max_iter: int,
-               A: Csr | None = None) -> tuple[np.ndarray, Performance]:
+               A: Csr | None = None, rel_tol: float = 0.0,
+               min_iter: int = 0) -> tuple[np.ndarray, Performance]:
     """cuFoam<method>::solve on the CPU (cuFoamCG/cuFoamCG.C:45-83 and siblings).
 
     error = tolerance * NormFactor(A, b, x); the solver runs with max_steps = maxIter;
     initialResidual = hist.front(), finalResidual = hist.back(), nIterations = size-1.
+    rel_tol / min_iter: OpenFOAM's relTol and minIter, which the reference
+    wrappers do not pass on (0 and 0 reproduce them).
     """
     if A is None:
         A = ldu2csr(sys)
     nf = norm_factor(A, source, psi)
     res = solve(method, A, source, psi, tolerance * nf, max_iter,
-                At=transpose(A) if method == "BiCG" else None)
+                At=transpose(A) if method == "BiCG" else None, rel_tol=rel_tol,
+                min_steps=min_iter)
     perf = Performance("cuFoam" + method, float(res.hist[0]), float(res.hist[-1]),
                        res.hist_len - 1, res.status)
     return res.x, perf

@@ -60,3 +60,11 @@ def test_create_without_gpu_reports_error():
     lib = G.load()
     h = ctypes.c_void_p()
     assert lib.gis_create(ctypes.byref(h), 0) != 0
+
+
+def test_foam_convergence_control_is_validated_before_any_gpu_call():
+    from cufoam_amd import foam, mesh
+    s = mesh.laplacian(4, seed=0, shift=0.1)
+    m = foam.lduMatrix.from_system(s)
+    with pytest.raises(foam.FatalError):
+        foam.New("cuFoamCG", "p", m, {"convergence": "strict"})

@@ -134,6 +134,50 @@ def test_foam_wrappers_bit_exact(gpu, name):
     np.testing.assert_array_equal(psi, ref_x)
 
 
+FOAM_CONTROLS = [(1e-12, 0.05, 0, 300), (1e-12, 1e-3, 4, 300), (1e30, 0.0, 6, 300),
+                 (1e-12, 0.0, 9, 5)]
+
+
+@pytest.mark.parametrize("name", ["cuFoamCG", "cuFoamBiCG", "cuFoamJacobi", "cuFoamMR",
+                                  "cuFoamMRJacobi", "cuFoamJMR", "cuFoamMRS",
+                                  "cuFoamRichardson"])
+@pytest.mark.parametrize("ctl", FOAM_CONTROLS)
+def test_foam_wrappers_openfoam_controls_bit_exact(gpu, name, ctl):
+    """relTol / minIter through gis_foam_solve_controls, against the oracle."""
+    from cufoam_amd import foam
+    tol, rel_tol, min_iter, max_iter = ctl
+    sym = name in ("cuFoamCG", "cuFoamRichardson")
+    s = _case(16, 5, sym, 0.2)
+    m = foam.lduMatrix.from_system(s)
+    psi = np.random.default_rng(4).uniform(-1, 1, s.n_cells)
+    psi_ref = psi.copy()
+    solver = foam.New(name, "U", m, {"tolerance": tol, "relTol": rel_tol, "minIter": min_iter,
+                                     "maxIter": max_iter, "convergence": "openfoam"}, gis=gpu)
+    perf = solver.solve(psi, s.b)
+    ref_x, ref = O.foam_solve(name[len("cuFoam"):], s, psi_ref, s.b, tol, max_iter,
+                              rel_tol=rel_tol, min_iter=min_iter)
+    assert perf.n_iterations == ref.n_iterations
+    assert perf.n_iterations >= min_iter
+    assert perf.initial_residual == ref.initial_residual
+    assert perf.final_residual == ref.final_residual
+    np.testing.assert_array_equal(psi, ref_x)
+
+
+def test_foam_controls_zero_equal_the_reference_entry_point(gpu):
+    s = _case(16, 6, False, 0.2)
+    gpu.set_system(s)
+    psi0 = np.random.default_rng(5).uniform(-1, 1, s.n_cells)
+    a, b = psi0.copy(), psi0.copy()
+    pa = gpu.foam_solve("BiCG", a, s.b, 1e-9, 300)
+    pb = gpu.foam_solve("BiCG", b, s.b, 1e-9, 300, rel_tol=0.0, min_iter=0)
+    assert (pa.n_iterations, pa.final_residual) == (pb.n_iterations, pb.final_residual)
+    np.testing.assert_array_equal(a, b)
+    with pytest.raises(RuntimeError):
+        gpu.foam_solve("CG", a, s.b, 1e-9, 300, rel_tol=-1.0)
+    with pytest.raises(RuntimeError):
+        gpu.foam_solve("CG", a, s.b, 1e-9, 300, rel_tol=0.0, min_iter=-1)
+
+
 def test_foam_selection_rules():
     from cufoam_amd import foam
     s = _case(4, 0, False, 0.1)

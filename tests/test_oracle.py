@@ -384,6 +384,66 @@ def test_foam_wrapper_semantics():
 
 
 # ---------------------------------------------------------------------------
+# OpenFOAM's relTol / minIter (lduMatrix::solver; not passed on by the
+# reference wrappers): pinned against a brute-force stop predictor.  Every
+# method is deterministic, so a run with more iterations extends the same
+# residual history; the stop index of any (error, relTol, minIter, maxIter) is
+# then read off one long history.
+# ---------------------------------------------------------------------------
+def _predict_stop(hist, method, error, rel_tol, min_iter, max_iter):
+    thr = max(error, rel_tol * hist[0]) if rel_tol > 0 else error
+    conv = (lambda v: v < thr) if method == "BiCG" else (lambda v: not v > thr)
+    for k in range(0 if method != "BiCG" else 1, len(hist)):
+        if (conv(hist[k]) or k >= max_iter) and k >= min_iter:
+            return k
+    raise AssertionError("history too short for the prediction")
+
+
+CONTROLS = [(1e-30, 0.1, 0, 500), (1e-30, 1e-3, 4, 500), (1e30, 0.0, 6, 500),
+            (1e-30, 0.0, 9, 5), (1e-4, 0.5, 0, 500), (1e-30, 0.0, 0, 7)]
+
+
+@pytest.mark.parametrize("method", list(O.METHODS))
+@pytest.mark.parametrize("ctl", CONTROLS)
+def test_rel_tol_and_min_iter_stop_where_predicted(method, ctl):
+    error, rel_tol, min_iter, max_iter = ctl
+    sym = method in ("CG", "Richardson")
+    s = mesh.laplacian(7, seed=21, shift=0.3, symmetric=sym)
+    A = O.ldu2csr(s)
+    long = O.solve(method, A, s.b, s.x0, 0.0, 400)
+    assert long.status == 0 and long.hist_len == 401
+    k = _predict_stop(long.hist, method, error, rel_tol, min_iter, max_iter)
+    r = O.solve(method, A, s.b, s.x0, error, max_iter, rel_tol=rel_tol, min_steps=min_iter)
+    assert r.iterations == k
+    np.testing.assert_array_equal(r.hist, long.hist[:k + 1])
+
+
+def test_zero_controls_are_the_reference_loops():
+    s = mesh.laplacian(7, seed=22, shift=0.3, symmetric=False)
+    A = O.ldu2csr(s)
+    for m in O.METHODS:
+        if m in ("CG", "Richardson"):
+            continue
+        a = O.solve(m, A, s.b, s.x0, 1e-6, 200)
+        b = O.solve(m, A, s.b, s.x0, 1e-6, 200, rel_tol=0.0, min_steps=0)
+        np.testing.assert_array_equal(a.x, b.x)
+        np.testing.assert_array_equal(a.hist, b.hist)
+
+
+def test_foam_controls_rel_tol_counts_from_the_initial_residual():
+    s = mesh.laplacian(8, seed=12, shift=1e-2)
+    A = O.ldu2csr(s)
+    psi = np.random.default_rng(3).uniform(-1, 1, s.n_cells)
+    x, perf = O.foam_solve("CG", s, psi.copy(), s.b, 1e-12, 1000, A=A, rel_tol=1e-2)
+    assert perf.final_residual <= 1e-2 * perf.initial_residual
+    res = O.solve("CG", A, s.b, psi, 0.0, perf.n_iterations)
+    assert res.hist[-2] > 1e-2 * perf.initial_residual
+    _, perf2 = O.foam_solve("CG", s, psi.copy(), s.b, 1e-12, 1000, A=A, min_iter=3)
+    _, perf3 = O.foam_solve("CG", s, psi.copy(), s.b, 1e-12, 1000, A=A)
+    assert perf2.n_iterations == perf3.n_iterations > 3
+
+
+# ---------------------------------------------------------------------------
 # mixed precision (no reference counterpart): pinned by properties
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("mode", [0, 1])
