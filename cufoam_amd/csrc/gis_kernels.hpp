@@ -252,6 +252,13 @@ inline Mat<float> mat32(const gis_context* h) {
                       h->sflag.as<unsigned char>()};
 }
 
+// Iterations the host may launch: the cap, or min_steps when that is larger
+// (the kernels end the solve themselves through KState::done).
+inline long long iteration_budget(const gis_context* h) {
+    return h->st_host->max_steps > h->st_host->min_steps ? h->st_host->max_steps
+                                                         : h->st_host->min_steps;
+}
+
 // Enqueue iterations in blocks until the device reports done.
 template <class Iter>
 int run_blocks(gis_context* h, long long max_steps, Iter&& launch_iteration) {

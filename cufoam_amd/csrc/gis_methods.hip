@@ -496,7 +496,7 @@ int solve_mr_family(gis_context* h, const SolveArgs& a, double* x, const double*
     GIS_TRY(launch_setup(h, m, b, x, rscale, r, nullptr, 0));
     const dim3 g = tiles_grid(N);
     GIS_SPMV_DISPATCH(h->sched.spmv64, P, {
-        GIS_TRY(run_blocks(h, h->st_host->max_steps, [&]() -> int {
+        GIS_TRY(run_blocks(h, iteration_budget(h), [&]() -> int {
             hipLaunchKernelGGL((k_mr1<P>), g, dim3(kWG), 0, h->stream, m, N, r, dvec, wscale,
                                zero_add, w, h->st, h->red);
             hipLaunchKernelGGL(k_mr2, g, dim3(kWG), 0, h->stream, N, r, w, x, dvec, zero_add,
@@ -518,7 +518,7 @@ int solve_jacobi(gis_context* h, const SolveArgs& a, double* x, const double* b)
     const dim3 g = tiles_grid(N);
     const double* dinv = h->dinv.as<double>();
     GIS_SPMV_DISPATCH(h->sched.spmv64, P, {
-        GIS_TRY(run_blocks(h, h->st_host->max_steps, [&]() -> int {
+        GIS_TRY(run_blocks(h, iteration_budget(h), [&]() -> int {
             hipLaunchKernelGGL((k_jacobi<P>), g, dim3(kWG), 0, h->stream, m, N, rb0, rb1, dinv,
                                x, h->st, h->red);
             return kOK;
@@ -558,7 +558,7 @@ int solve_richardson(gis_context* h, const SolveArgs& a, double* x, const double
     }
     GIS_TRY(launch_setup(h, m, b, x, nullptr, rb0, nullptr, 0));
     GIS_SPMV_DISPATCH(h->sched.spmv64, P, {
-        GIS_TRY(run_blocks(h, h->st_host->max_steps, [&]() -> int {
+        GIS_TRY(run_blocks(h, iteration_budget(h), [&]() -> int {
             hipLaunchKernelGGL((k_richardson<P>), g, dim3(kWG), 0, h->stream, m, N, b, x, xb1,
                                rb0, rb1, h->st, h->red);
             return kOK;
@@ -585,7 +585,7 @@ int solve_bicg(gis_context* h, const SolveArgs& a, double* x, const double* b) {
     GIS_TRY(launch_setup(h, m, b, x, nullptr, r, rs, 1));
     const dim3 g = tiles_grid(N);
     GIS_SPMV_DISPATCH(h->sched.spmv64, P, {
-        GIS_TRY(run_blocks(h, h->st_host->max_steps, [&]() -> int {
+        GIS_TRY(run_blocks(h, iteration_budget(h), [&]() -> int {
             if (h->sched.fp64_direction) {
                 hipLaunchKernelGGL(k_bicg_dir, g, dim3(kWG), 0, h->stream, N, r, rs, pb0, pb1, ps,
                                    h->st);

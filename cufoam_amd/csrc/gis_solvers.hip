@@ -498,7 +498,7 @@ int solve_cg(gis_context* h, const SolveArgs& a, double* x, const double* b) {
         const bool split = h->sched.fp64_direction != 0;
         GIS_COARSEN_DISPATCH(h->sched.coarsen64, CC, {
             const dim3 gc = super_grid(N, CC);
-            GIS_TRY(run_blocks(h, h->st_host->max_steps, [&]() -> int {
+            GIS_TRY(run_blocks(h, iteration_budget(h), [&]() -> int {
                 if (split) {
                     GIS_TRY(timer_begin(h, 4));
                     // no reduction to amortise: one tile per workgroup (measured best)
