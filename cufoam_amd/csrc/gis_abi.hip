@@ -4,6 +4,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/gis.h"
 #include "gis_internal.hpp"
@@ -199,6 +200,9 @@ static int set_pattern_impl(gis_handle h, int n_cells, int n_faces, const int* l
     if (n_cells < 1 || n_faces < 0 || (n_faces > 0 && (!la || !ua)))
         return fail(h, GIS_ERR_ARG, "set_pattern: need n_cells >= 1, n_faces >= 0, addressing");
     DeviceGuard g(h->device);
+    h->batch.ready = false;   // the handle now holds one system
+    h->batch.graph_valid = false;
+    h->n_cpl = 0;
     const size_t b = sizeof(int) * (size_t)n_faces;
     if (dev) {
         GIS_TRY(copy_d2d(h, h->lower_addr, la, b));
@@ -251,6 +255,78 @@ GIS_API int gis_set_values(gis_handle h, const double* diag, const double* lower
 GIS_API int gis_set_values_device(gis_handle h, const double* d_diag, const double* d_lower,
                                   const double* d_upper) {
     return set_values_impl(h, d_diag, d_lower, d_upper, true);
+}
+
+GIS_API int gis_set_interfaces(gis_handle h, int n_couples, const int* row, const int* col) {
+    if (!h) return GIS_ERR_ARG;
+    if (!h->has_pattern || h->batch.ready)
+        return fail(h, GIS_ERR_STATE, "set_interfaces needs a single-system pattern");
+    if (n_couples < 0 || (n_couples > 0 && (!row || !col)))
+        return fail(h, GIS_ERR_ARG, "set_interfaces: need n_couples >= 0 and row/col arrays");
+    const int N = h->N;
+    for (int j = 0; j < n_couples; ++j)
+        if (row[j] < 0 || row[j] >= N || col[j] < 0 || col[j] >= N || row[j] == col[j])
+            return fail(h, GIS_ERR_ARG,
+                        "interface entries need 0 <= row, col < n_cells and row != col");
+    // mirrored entry of each (r, c): the k-th (r, c) in entry order pairs with the
+    // k-th (c, r), as the two halves of a cyclic patch list them
+    std::vector<long long> key(n_couples), mkey(n_couples);
+    std::vector<int> a(n_couples), b(n_couples), partner(n_couples);
+    for (int j = 0; j < n_couples; ++j) {
+        key[j] = (long long)row[j] * N + col[j];
+        mkey[j] = (long long)col[j] * N + row[j];
+        a[j] = b[j] = j;
+    }
+    std::stable_sort(a.begin(), a.end(), [&](int x, int y) { return key[x] < key[y]; });
+    std::stable_sort(b.begin(), b.end(), [&](int x, int y) { return mkey[x] < mkey[y]; });
+    for (int i = 0; i < n_couples; ++i) {
+        if (key[a[i]] != mkey[b[i]])
+            return fail(h, GIS_ERR_ARG,
+                        "interface entries must be structurally symmetric: every (row, col) "
+                        "needs a (col, row)");
+        partner[a[i]] = b[i];
+    }
+    DeviceGuard g(h->device);
+    const size_t bi = sizeof(int) * (size_t)n_couples;
+    h->n_cpl = 0;
+    if (n_couples > 0) {
+        GIS_TRY(upload(h, h->cpl_row, row, bi));
+        GIS_TRY(upload(h, h->cpl_col, col, bi));
+        GIS_TRY(upload(h, h->cpl_partner, partner.data(), bi));
+        GIS_HIP(h, h->cpl_val.ensure(sizeof(double) * (size_t)n_couples));
+        GIS_HIP(h, hipMemsetAsync(h->cpl_val.p, 0, sizeof(double) * (size_t)n_couples,
+                                  h->stream));
+        GIS_HIP(h, hipStreamSynchronize(h->stream));   // partner is a host temporary
+    }
+    h->n_cpl = n_couples;
+    GIS_TRY(build_pattern(h, N, h->M, h->lower_addr.as<int>(), h->upper_addr.as<int>()));
+    GIS_TRY(ensure_reduction(h, N));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    return GIS_OK;
+}
+
+static int set_coupled_values_impl(gis_handle h, const double* diag, const double* lower,
+                                   const double* upper, const double* coeffs, bool dev) {
+    if (!h) return GIS_ERR_ARG;
+    if (h->n_cpl > 0 && !coeffs) return fail(h, GIS_ERR_ARG, "null interface coefficients");
+    if (h->has_pattern && h->n_cpl > 0) {
+        DeviceGuard g(h->device);
+        const size_t bc = sizeof(double) * (size_t)h->n_cpl;
+        if (dev) GIS_TRY(copy_d2d(h, h->cpl_val, coeffs, bc));
+        else GIS_TRY(upload(h, h->cpl_val, coeffs, bc));
+    }
+    return set_values_impl(h, diag, lower, upper, dev);
+}
+
+GIS_API int gis_set_coupled_values(gis_handle h, const double* diag, const double* lower,
+                                   const double* upper, const double* coeffs) {
+    return set_coupled_values_impl(h, diag, lower, upper, coeffs, false);
+}
+
+GIS_API int gis_set_coupled_values_device(gis_handle h, const double* d_diag,
+                                          const double* d_lower, const double* d_upper,
+                                          const double* d_coeffs) {
+    return set_coupled_values_impl(h, d_diag, d_lower, d_upper, d_coeffs, true);
 }
 
 GIS_API int gis_get_size(gis_handle h, int* n_rows, long long* nnz) {

@@ -190,6 +190,7 @@ int batch_set_pattern(gis_context* h, int S, const int* n_cells, const int* n_fa
     B.graph_valid = false;
     if (B.graph_exec) (void)hipGraphExecDestroy(B.graph_exec);
     B.graph_exec = nullptr;
+    h->n_cpl = 0;   // the stacked systems carry no interface entries
     B.S = S;
     B.n.assign(n_cells, n_cells + S);
     B.m.assign(n_faces, n_faces + S);

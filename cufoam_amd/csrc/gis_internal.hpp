@@ -118,6 +118,10 @@ struct gis_context {
     gis::DevMem rowptr, col, perm, nbr_cnt;     // CSR pattern, value permutation
     gis::DevMem lower_addr, upper_addr;         // LDU addressing copies
     gis::DevMem diag, lower, upper;             // LDU coefficients (device copies)
+    // coupled interfaces: entry j adds A(cpl_row[j], cpl_col[j]) = cpl_val[j] after the
+    // LDU entries of its row; cpl_partner[j] is the entry at the mirrored position
+    int n_cpl = 0;
+    gis::DevMem cpl_row, cpl_col, cpl_val, cpl_partner;
     gis::DevMem val, valT, val32;               // CSR values: A, A^T, float(A)
     long long n_slots = 0;                      // sliced layout (64-row slices)
     gis::DevMem soff, scol, sperm;              // slice offsets, slot columns / codes

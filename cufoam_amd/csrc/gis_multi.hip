@@ -523,7 +523,8 @@ __global__ __launch_bounds__(kWG) void k_sddmm_fold(int N, int M, int nrhs,
             const double g = -(Lam[k * n + i] * xj);
             if (code < M) d_lower[k * mm + code] = g;
             else if (code < M + N) d_diag[k * n + (code - M)] = g;
-            else d_upper[k * mm + (code - M - N)] = g;
+            else if (code < 2 * M + N) d_upper[k * mm + (code - M - N)] = g;
+            // coupling entries (codes >= 2M + N) have no LDU slot to fold into
         }
     }
 }

@@ -43,10 +43,26 @@ __global__ __launch_bounds__(kBlk) void k_count(int N, int M, const int* __restr
     if (lp != l) first_own[l] = f;
 }
 
+// coupling entries per row; row == col or an index out of range is an error
+__global__ __launch_bounds__(kBlk) void k_count_cpl(int N, int C, const int* __restrict__ cr,
+                                                    const int* __restrict__ cc, int* cpl_cnt,
+                                                    int* err) {
+    const int j = blockIdx.x * kBlk + threadIdx.x;
+    if (j >= C) return;
+    const int r = cr[j];
+    const int c = cc[j];
+    if (r < 0 || r >= N || c < 0 || c >= N || r == c) {
+        atomicOr(err, 2);
+        return;
+    }
+    atomicAdd(&cpl_cnt[r], 1);
+}
+
 __global__ __launch_bounds__(kBlk) void k_rowlen(int N, const int* __restrict__ own_cnt,
-                                                 const int* __restrict__ nbr_cnt, int* rowlen) {
+                                                 const int* __restrict__ nbr_cnt,
+                                                 const int* __restrict__ cpl_cnt, int* rowlen) {
     const int c = blockIdx.x * kBlk + threadIdx.x;
-    if (c < N) rowlen[c] = 1 + own_cnt[c] + nbr_cnt[c];
+    if (c < N) rowlen[c] = 1 + own_cnt[c] + nbr_cnt[c] + cpl_cnt[c];
     else if (c == N) rowlen[c] = 0;
 }
 
@@ -146,13 +162,50 @@ __global__ __launch_bounds__(kBlk) void k_sort_lower(int N, const int* __restric
     for (int i = 0; i < n; ++i) col[a + i] = la[perm[a + i]];
 }
 
-// value of one CSR / slot position from its code (-1: padding slot)
+// coupling entries at the end of their row, then ordered by entry index
+__global__ __launch_bounds__(kBlk) void k_fill_cpl(int N, int M, int C,
+                                                   const int* __restrict__ cr,
+                                                   const int* __restrict__ rowptr,
+                                                   const int* __restrict__ cpl_cnt, int* cfill,
+                                                   int* perm) {
+    const int j = blockIdx.x * kBlk + threadIdx.x;
+    if (j >= C) return;
+    const int r = cr[j];
+    const int slot = atomicAdd(&cfill[r], 1);
+    perm[rowptr[r + 1] - cpl_cnt[r] + slot] = 2 * M + N + j;
+}
+
+__global__ __launch_bounds__(kBlk) void k_sort_cpl(int N, int M, const int* __restrict__ cc,
+                                                   const int* __restrict__ rowptr,
+                                                   const int* __restrict__ cpl_cnt, int* col,
+                                                   int* perm) {
+    const int r = blockIdx.x * kBlk + threadIdx.x;
+    if (r >= N) return;
+    const int n = cpl_cnt[r];
+    const int a = rowptr[r + 1] - n;
+    for (int i = 1; i < n; ++i) {
+        const int v = perm[a + i];
+        int j = i - 1;
+        while (j >= 0 && perm[a + j] > v) {
+            perm[a + j + 1] = perm[a + j];
+            --j;
+        }
+        perm[a + j + 1] = v;
+    }
+    for (int i = 0; i < n; ++i) col[a + i] = cc[perm[a + i] - 2 * M - N];
+}
+
+// value of one CSR / slot position from its code (-1: padding slot); codes
+// [0, M) lower, [M, M+N) diag, [M+N, 2M+N) upper, then coupling entries, read
+// through cpl_map when it is given (the mirrored entry, for A^T)
 template <typename T>
 __global__ __launch_bounds__(kBlk) void k_gather(long long nnz, int N, int M,
                                                  const int* __restrict__ perm,
                                                  const double* __restrict__ diag,
                                                  const double* __restrict__ lower,
-                                                 const double* __restrict__ upper, T* val) {
+                                                 const double* __restrict__ upper,
+                                                 const double* __restrict__ cpl,
+                                                 const int* __restrict__ cpl_map, T* val) {
     const long long p = (long long)blockIdx.x * kBlk + threadIdx.x;
     if (p >= nnz) return;
     const int code = perm[p];
@@ -160,7 +213,11 @@ __global__ __launch_bounds__(kBlk) void k_gather(long long nnz, int N, int M,
     if (code < 0) v = 0.0;
     else if (code < M) v = lower[code];
     else if (code < M + N) v = diag[code - M];
-    else v = upper[code - M - N];
+    else if (code < 2 * M + N) v = upper[code - M - N];
+    else {
+        const int j = code - 2 * M - N;
+        v = cpl[cpl_map ? cpl_map[j] : j];
+    }
     val[p] = static_cast<T>(v);
 }
 
@@ -262,45 +319,57 @@ inline dim3 grid_for(long long n) { return dim3((unsigned)((n + kBlk - 1) / kBlk
 }  // namespace
 
 int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua) {
-    const long long nnz = (long long)N + 2LL * M;
-    if (N < 1 || M < 0 || nnz >= (1LL << 31))
-        return fail(h, kErrShape, "n_cells must be >= 1 and n_cells + 2*n_faces < 2^31");
+    const int C = h->n_cpl;
+    const long long nnz = (long long)N + 2LL * M + C;
+    if (N < 1 || M < 0 || C < 0 || nnz >= (1LL << 31))
+        return fail(h, kErrShape,
+                    "n_cells must be >= 1 and n_cells + 2*n_faces + n_couples < 2^31");
     h->has_pattern = false;
     h->has_values = h->has_val32 = h->has_valT = h->has_val32T = false;
     GIS_HIP(h, h->rowptr.ensure(sizeof(int) * ((size_t)N + 1)));
     GIS_HIP(h, h->col.ensure(sizeof(int) * (size_t)nnz));
     GIS_HIP(h, h->perm.ensure(sizeof(int) * (size_t)nnz));
     GIS_HIP(h, h->nbr_cnt.ensure(sizeof(int) * (size_t)N));
-    // scratch: own_cnt[N] first_own[N] fill[N] rowlen[N+1] err[1] + scan temporaries
+    // scratch: own_cnt[N] first_own[N] fill[N] rowlen[N+1] err[1] cpl_cnt[N] cfill[N]
+    // + scan temporaries
     const size_t scan_tmp = 4 * ((size_t)N / (kBlk * 4) + 64);
-    const size_t scratch_ints = 4 * (size_t)N + 2 + scan_tmp;
+    const size_t scratch_ints = 6 * (size_t)N + 2 + scan_tmp;
     GIS_HIP(h, h->scratch.ensure(sizeof(int) * scratch_ints));
     int* own_cnt = h->scratch.as<int>();
     int* first_own = own_cnt + N;
     int* fill = first_own + N;
     int* rowlen = fill + N;
     int* err = rowlen + N + 1;
-    int* tmp = err + 1;
+    int* cpl_cnt = err + 1;
+    int* cfill = cpl_cnt + N;
+    int* tmp = cfill + N;
     int* nbr_cnt = h->nbr_cnt.as<int>();
     GIS_HIP(h, hipMemsetAsync(own_cnt, 0, sizeof(int) * (3 * (size_t)N + 2), h->stream));
-    GIS_HIP(h, hipMemsetAsync(err, 0, sizeof(int), h->stream));
+    GIS_HIP(h, hipMemsetAsync(err, 0, sizeof(int) * (2 * (size_t)N + 1), h->stream));
     GIS_HIP(h, hipMemsetAsync(nbr_cnt, 0, sizeof(int) * (size_t)N, h->stream));
     if (M > 0) {
         hipLaunchKernelGGL(k_count, grid_for(M), dim3(kBlk), 0, h->stream, N, M, d_la, d_ua,
                            own_cnt, nbr_cnt, first_own, err);
         GIS_HIP(h, hipGetLastError());
     }
+    if (C > 0) {
+        hipLaunchKernelGGL(k_count_cpl, grid_for(C), dim3(kBlk), 0, h->stream, N, C,
+                           h->cpl_row.as<int>(), h->cpl_col.as<int>(), cpl_cnt, err);
+        GIS_HIP(h, hipGetLastError());
+    }
     hipLaunchKernelGGL(k_rowlen, grid_for((long long)N + 1), dim3(kBlk), 0, h->stream, N, own_cnt,
-                       nbr_cnt, rowlen);
+                       nbr_cnt, cpl_cnt, rowlen);
     GIS_HIP(h, hipGetLastError());
     GIS_TRY(scan_exclusive(h, rowlen, h->rowptr.as<int>(), N + 1, tmp, scan_tmp));
     int herr = 0;
     GIS_HIP(h, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     GIS_HIP(h, hipStreamSynchronize(h->stream));
-    if (herr)
+    if (herr & 1)
         return fail(h, kErrArg,
                     "face addressing invalid: need 0 <= lowerAddr, upperAddr < n_cells and "
                     "lowerAddr sorted ascending");
+    if (herr & 2)
+        return fail(h, kErrArg, "interface entries need 0 <= row, col < n_cells and row != col");
     hipLaunchKernelGGL(k_fill_diag, grid_for(N), dim3(kBlk), 0, h->stream, N, M,
                        h->rowptr.as<int>(), nbr_cnt, h->col.as<int>(), h->perm.as<int>());
     GIS_HIP(h, hipGetLastError());
@@ -311,6 +380,16 @@ int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua
         GIS_HIP(h, hipGetLastError());
         hipLaunchKernelGGL(k_sort_lower, grid_for(N), dim3(kBlk), 0, h->stream, N, d_la,
                            h->rowptr.as<int>(), nbr_cnt, h->col.as<int>(), h->perm.as<int>());
+        GIS_HIP(h, hipGetLastError());
+    }
+    if (C > 0) {
+        hipLaunchKernelGGL(k_fill_cpl, grid_for(C), dim3(kBlk), 0, h->stream, N, M, C,
+                           h->cpl_row.as<int>(), h->rowptr.as<int>(), cpl_cnt, cfill,
+                           h->perm.as<int>());
+        GIS_HIP(h, hipGetLastError());
+        hipLaunchKernelGGL(k_sort_cpl, grid_for(N), dim3(kBlk), 0, h->stream, N, M,
+                           h->cpl_col.as<int>(), h->rowptr.as<int>(), cpl_cnt, h->col.as<int>(),
+                           h->perm.as<int>());
         GIS_HIP(h, hipGetLastError());
     }
     // sliced copy of the pattern (64-row slices, slot-major)
@@ -348,26 +427,30 @@ int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua
     return kOK;
 }
 
-// CSR values and slot values of one matrix: A (lower, upper) or A^T (swapped)
+// CSR values and slot values of one matrix: A, or A^T (lower and upper swapped,
+// each coupling entry taking the value of its mirrored entry)
 template <typename T>
-static int gather_both(gis_context* h, const double* lower, const double* upper, DevMem& csr,
-                       DevMem& sell) {
+static int gather_both(gis_context* h, bool transpose, DevMem& csr, DevMem& sell) {
+    const double* lower = (transpose ? h->upper : h->lower).as<double>();
+    const double* upper = (transpose ? h->lower : h->upper).as<double>();
+    const double* cpl = h->n_cpl ? h->cpl_val.as<double>() : nullptr;
+    const int* map = (h->n_cpl && transpose) ? h->cpl_partner.as<int>() : nullptr;
     GIS_HIP(h, csr.ensure(sizeof(T) * (size_t)h->nnz));
     GIS_HIP(h, sell.ensure(sizeof(T) * (size_t)h->n_slots));
     hipLaunchKernelGGL(k_gather<T>, grid_for(h->nnz), dim3(kBlk), 0, h->stream, h->nnz, h->N,
-                       h->M, h->perm.as<int>(), h->diag.as<double>(), lower, upper, csr.as<T>());
+                       h->M, h->perm.as<int>(), h->diag.as<double>(), lower, upper, cpl, map,
+                       csr.as<T>());
     GIS_HIP(h, hipGetLastError());
     hipLaunchKernelGGL(k_gather<T>, grid_for(h->n_slots), dim3(kBlk), 0, h->stream, h->n_slots,
-                       h->N, h->M, h->sperm.as<int>(), h->diag.as<double>(), lower, upper,
-                       sell.as<T>());
+                       h->N, h->M, h->sperm.as<int>(), h->diag.as<double>(), lower, upper, cpl,
+                       map, sell.as<T>());
     GIS_HIP(h, hipGetLastError());
     return kOK;
 }
 
 int gather_values(gis_context* h) {
     if (!h->has_pattern) return fail(h, kErrState, "pattern not set");
-    GIS_TRY(gather_both<double>(h, h->lower.as<double>(), h->upper.as<double>(), h->val,
-                                h->sval));
+    GIS_TRY(gather_both<double>(h, false, h->val, h->sval));
     h->has_values = true;
     h->has_val32 = false;
     h->has_valT = false;
@@ -378,19 +461,17 @@ int gather_values(gis_context* h) {
 int ensure_val32(gis_context* h) {
     if (!h->has_values) return fail(h, kErrState, "values not set");
     if (h->has_val32) return kOK;
-    GIS_TRY(gather_both<float>(h, h->lower.as<double>(), h->upper.as<double>(), h->val32,
-                               h->sval32));
+    GIS_TRY(gather_both<float>(h, false, h->val32, h->sval32));
     h->has_val32 = true;
     return kOK;
 }
 
-// A^T of an LDU matrix: same pattern, lower <-> upper
+// A^T of an LDU matrix: same pattern, lower <-> upper (coupling entries mirrored)
 // float(A^T): the mixed-precision transpose solves of BiCG
 int ensure_val32T(gis_context* h) {
     if (!h->has_values) return fail(h, kErrState, "values not set");
     if (h->has_val32T) return kOK;
-    GIS_TRY(gather_both<float>(h, h->upper.as<double>(), h->lower.as<double>(), h->val32T,
-                               h->sval32T));
+    GIS_TRY(gather_both<float>(h, true, h->val32T, h->sval32T));
     h->has_val32T = true;
     return kOK;
 }
@@ -398,8 +479,7 @@ int ensure_val32T(gis_context* h) {
 int ensure_valT(gis_context* h) {
     if (!h->has_values) return fail(h, kErrState, "values not set");
     if (h->has_valT) return kOK;
-    GIS_TRY(gather_both<double>(h, h->upper.as<double>(), h->lower.as<double>(), h->valT,
-                                h->svalT));
+    GIS_TRY(gather_both<double>(h, true, h->valT, h->svalT));
     h->has_valT = true;
     return kOK;
 }

@@ -22,7 +22,11 @@ seven siblings):
     (cuFoamCG.C:14-15, cuFoamRichardson.C:11-12); the others for both.
 
 The coupling arguments (interfaces, coupleBouCoeffs) are accepted and, as in
-the reference, not used: processor and cyclic patches are not coupled.
+the reference, not used: processor and cyclic patches are not coupled.  With
+``interfaces coupled`` in the controls, cyclic interfaces whose both sides are
+cells of this mesh (``cyclicInterface``) become matrix entries
+A(faceCells, nbrFaceCells) = -coupleBouCoeffs (gis_set_interfaces), the sign
+of OpenFOAM's interface update in Amul.
 """
 from __future__ import annotations
 
@@ -31,6 +35,14 @@ from dataclasses import dataclass
 import numpy as np
 
 from .gis import Gis, Performance
+
+
+@dataclass
+class cyclicInterface:
+    """One half of a cyclic patch pair: its face cells and, face by face, the
+    cells on the other half (cyclicLduInterface::faceCells / neighbour faceCells)."""
+    faceCells: np.ndarray
+    nbrFaceCells: np.ndarray
 
 
 @dataclass
@@ -92,16 +104,39 @@ class cuFoamSolver:
         if self.convergence not in ("reference", "openfoam"):
             raise FatalError(f"convergence must be reference or openfoam, not "
                              f"{self.convergence}")
+        self.couple = str(controls.get("interfaces", "ignored"))
+        if self.couple not in ("ignored", "coupled"):
+            raise FatalError(f"interfaces must be ignored or coupled, not {self.couple}")
+        self._cpl = None
+        if self.couple == "coupled" and interfaces:
+            rows, cols, bou = [], [], []
+            for i, itf in enumerate(interfaces):
+                if itf is None:
+                    continue
+                if not isinstance(itf, cyclicInterface):
+                    raise FatalError("only cyclic interfaces within this mesh can be coupled")
+                rows.append(np.asarray(itf.faceCells, np.int32))
+                cols.append(np.asarray(itf.nbrFaceCells, np.int32))
+                bou.append(np.asarray(coupleBouCoeffs[i], np.float64))
+            if rows:
+                self._cpl = (np.concatenate(rows), np.concatenate(cols), np.concatenate(bou))
         self._gis = gis if gis is not None else Gis()
         self._pattern_key = None
 
     def _upload(self):
-        key = (id(self.matrix.lowerAddr), self.matrix.size())
+        key = (id(self.matrix.lowerAddr), self.matrix.size(),
+               None if self._cpl is None else id(self._cpl[0]))
         if key != self._pattern_key:
             self._gis.set_pattern(self.matrix.size(), self.matrix.lowerAddr,
                                   self.matrix.upperAddr)
+            if self._cpl is not None:
+                self._gis.set_interfaces(self._cpl[0], self._cpl[1])
             self._pattern_key = key
-        self._gis.set_values(self.matrix.diag, self.matrix.lower, self.matrix.upper)
+        if self._cpl is not None:
+            self._gis.set_coupled_values(self.matrix.diag, self.matrix.lower, self.matrix.upper,
+                                         -self._cpl[2])
+        else:
+            self._gis.set_values(self.matrix.diag, self.matrix.lower, self.matrix.upper)
 
     def solve(self, psi: np.ndarray, source: np.ndarray, cmpt: int = 0) -> Performance:
         if psi.dtype != np.float64 or not psi.flags.c_contiguous:
@@ -137,7 +172,8 @@ _CLASSES = {c.typeName: c for c in (cuFoamCG, cuFoamBiCG, cuFoamJacobi, cuFoamRi
 
 
 def New(name: str, fieldName: str, matrix: lduMatrix, solverControls: dict,
-        gis: Gis | None = None) -> cuFoamSolver:
+        gis: Gis | None = None, interfaceBouCoeffs=None, interfaceIntCoeffs=None,
+        interfaces=None) -> cuFoamSolver:
     """lduMatrix::solver::New: runtime selection by name and matrix symmetry."""
     if name not in _TABLE:
         raise FatalError(f"Unknown {'symmetric' if matrix.symmetric() else 'asymmetric'} "
@@ -146,4 +182,5 @@ def New(name: str, fieldName: str, matrix: lduMatrix, solverControls: dict,
     if matrix.symmetric() and not sym_ok or matrix.asymmetric() and not asym_ok:
         kind = "symmetric" if matrix.symmetric() else "asymmetric"
         raise FatalError(f"Unknown {kind} matrix solver {name}")
-    return _CLASSES[name](fieldName, matrix, None, None, None, solverControls, gis=gis)
+    return _CLASSES[name](fieldName, matrix, interfaceBouCoeffs, interfaceIntCoeffs, interfaces,
+                          solverControls, gis=gis)

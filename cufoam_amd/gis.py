@@ -63,6 +63,9 @@ SIGNATURES = {
     "gis_set_pattern_device": (_i, [_vp, _i, _i, _vp, _vp]),
     "gis_set_values": (_i, [_vp, _vp, _vp, _vp]),
     "gis_set_values_device": (_i, [_vp, _vp, _vp, _vp]),
+    "gis_set_interfaces": (_i, [_vp, _i, _vp, _vp]),
+    "gis_set_coupled_values": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "gis_set_coupled_values_device": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "gis_get_size": (_i, [_vp, _ip, _llp]),
     "gis_get_csr": (_i, [_vp, _vp, _vp, _vp]),
     "gis_get_csr_transpose": (_i, [_vp, _vp, _vp, _vp]),
@@ -241,9 +244,37 @@ class Gis:
             st = self._L.gis_set_values(self._h, _ptr(d), _ptr(lo), _ptr(up))
         self._check(st, "set_values")
 
+    def set_interfaces(self, row, col):
+        """Coupled-interface entries A(row[j], col[j]) (structurally symmetric)."""
+        r = _host(row, np.int32)
+        c = _host(col, np.int32)
+        if r.shape != c.shape:
+            raise ValueError("row and col must have the same length")
+        self._check(self._L.gis_set_interfaces(self._h, r.shape[0], _ptr(r), _ptr(c)),
+                    "set_interfaces")
+        n = C.c_int(0)
+        nnz = C.c_longlong(0)
+        self._check(self._L.gis_get_size(self._h, C.byref(n), C.byref(nnz)), "get_size")
+        self.n, self.nnz = n.value, nnz.value
+
+    def set_coupled_values(self, diag, lower, upper, coeffs):
+        if _is_dev(diag):
+            st = self._L.gis_set_coupled_values_device(self._h, _ptr(diag), _ptr(lower),
+                                                       _ptr(upper), _ptr(coeffs))
+        else:
+            d, lo, up = _host(diag, np.float64), _host(lower, np.float64), _host(upper, np.float64)
+            cv = None if coeffs is None else _host(coeffs, np.float64)
+            st = self._L.gis_set_coupled_values(self._h, _ptr(d), _ptr(lo), _ptr(up),
+                                                None if cv is None else _ptr(cv))
+        self._check(st, "set_coupled_values")
+
     def set_system(self, sys):
         self.set_pattern(sys.n_cells, sys.lower_addr, sys.upper_addr)
-        self.set_values(sys.diag, sys.lower, sys.upper)
+        if getattr(sys, "cpl_row", None) is not None and len(sys.cpl_row):
+            self.set_interfaces(sys.cpl_row, sys.cpl_col)
+            self.set_coupled_values(sys.diag, sys.lower, sys.upper, sys.cpl_val)
+        else:
+            self.set_values(sys.diag, sys.lower, sys.upper)
 
     def get_csr(self, transpose: bool = False):
         rp = np.zeros(self.n + 1, np.int32)

@@ -41,14 +41,23 @@ class LduSystem:
     b: np.ndarray           # float64[N] or float64[nrhs, N]
     x0: np.ndarray          # float64[N] or float64[nrhs, N]
     meta: dict = field(default_factory=dict)
+    # coupled interfaces (cyclic patches): extra entries A(cpl_row[j], cpl_col[j]) =
+    # cpl_val[j], i.e. -coupleBouCoeffs of the patch face; None when uncoupled
+    cpl_row: np.ndarray | None = None
+    cpl_col: np.ndarray | None = None
+    cpl_val: np.ndarray | None = None
 
     @property
     def n_faces(self) -> int:
         return int(self.lower_addr.shape[0])
 
     @property
+    def n_couples(self) -> int:
+        return 0 if self.cpl_row is None else int(self.cpl_row.shape[0])
+
+    @property
     def nnz(self) -> int:
-        return self.n_cells + 2 * self.n_faces
+        return self.n_cells + 2 * self.n_faces + self.n_couples
 
     @property
     def symmetric(self) -> bool:
@@ -102,6 +111,65 @@ def laplacian(n: int, seed: int = 0, shift: float = 1e-3, symmetric: bool = True
     return LduSystem(N, owner, neigh, diag, lower, upper, b, x0,
                      meta={"n": n, "seed": seed, "shift": shift, "symmetric": symmetric,
                            "nrhs": nrhs})
+
+
+def periodic_laplacian(n: int, seed: int = 0, shift: float = 1e-3,
+                       symmetric: bool = True) -> LduSystem:
+    """laplacian(n) on a mesh that is periodic in x: the faces between i = n-1 and
+    i = 0 are a cyclic patch pair, held as interface entries rather than faces.
+
+    The internal faces and their coefficients are those of laplacian(n, seed,
+    symmetric=...).  Then, per cyclic face (j, k ascending), w ~ U[0,1) and (for
+    the nonsymmetric case) v ~ U[0,1) give the coefficient -(1 + 0.5 w) of the
+    half at i = 0 (row (0,j,k), column (n-1,j,k)) and -(1 + 0.5 w)(1 + 0.5 v) of the
+    half at i = n-1.  Entries list the i = 0 half first, then the i = n-1 half,
+    as OpenFOAM orders a cyclic pair's interfaces.  diag includes |coupling|
+    terms, so the matrix stays diagonally dominant; b is drawn last.
+    """
+    if n < 2:
+        raise ValueError("a periodic mesh needs n >= 2")
+    s = laplacian(n, seed, shift=shift, symmetric=symmetric)
+    rng = np.random.default_rng([seed, 7919])
+    jk = np.arange(n * n)
+    c0 = (n * jk).astype(np.int32)            # cells (0, j, k)
+    c1 = (n * jk + n - 1).astype(np.int32)    # cells (n-1, j, k)
+    w = -(1.0 + 0.5 * rng.random(n * n))
+    w1 = w.copy() if symmetric else w * (1.0 + 0.5 * rng.random(n * n))
+    row = np.concatenate([c0, c1])
+    col = np.concatenate([c1, c0])
+    val = np.concatenate([w, w1])
+    diag = row_abs_sum(s.n_cells, s.lower_addr, s.upper_addr, s.lower, s.upper)
+    diag = diag + np.bincount(row, weights=np.abs(val), minlength=s.n_cells) + shift
+    s.diag = diag
+    s.b = rng.uniform(-1.0, 1.0, s.n_cells)
+    s.cpl_row, s.cpl_col, s.cpl_val = row, col, val
+    s.meta = dict(s.meta, periodic="x")
+    return s
+
+
+def couplings_as_faces(sys: LduSystem) -> LduSystem:
+    """The same matrix with each mirrored pair of coupling entries made an ordinary
+    face (owner = lower cell index), faces re-sorted to upper-triangular order.
+    Needs at most one coupling entry per (row, col)."""
+    r, c, v = sys.cpl_row, sys.cpl_col, sys.cpl_val
+    keys = {(int(a), int(b)): float(x) for a, b, x in zip(r, c, v)}
+    if len(keys) != len(r):
+        raise ValueError("duplicate coupling entries")
+    own, nei, up, lo = [], [], [], []
+    for (a, b), x in keys.items():
+        if a < b:
+            own.append(a)
+            nei.append(b)
+            up.append(x)
+            lo.append(keys[(b, a)])
+    la = np.concatenate([sys.lower_addr, np.array(own, np.int32)])
+    ua = np.concatenate([sys.upper_addr, np.array(nei, np.int32)])
+    lower = np.concatenate([sys.lower, np.array(lo)])
+    upper = np.concatenate([sys.upper, np.array(up)])
+    order = np.lexsort((ua, la))
+    return LduSystem(sys.n_cells, la[order].astype(np.int32), ua[order].astype(np.int32),
+                     sys.diag.copy(), lower[order], upper[order], sys.b, sys.x0,
+                     meta=dict(sys.meta, couplings="as faces"))
 
 
 def config_system(name: str, seed: int = 0) -> LduSystem:
@@ -162,10 +230,13 @@ def batch_edges(count: int = 64, lo: int = 16, hi: int = 64, seed: int = 0) -> n
 
 
 def ldu_matvec(sys: LduSystem, x: np.ndarray) -> np.ndarray:
-    """y = A x by OpenFOAM's face loop (``lduMatrix::Amul`` without interfaces)."""
+    """y = A x by OpenFOAM's face loop (``lduMatrix::Amul``), then the interface
+    entries (the interface update subtracts coupleBouCoeffs * psi(neighbour))."""
     y = sys.diag * x
     np.add.at(y, sys.upper_addr, sys.lower * x[sys.lower_addr])
     np.add.at(y, sys.lower_addr, sys.upper * x[sys.upper_addr])
+    if sys.n_couples:
+        np.add.at(y, sys.cpl_row, sys.cpl_val * x[sys.cpl_col])
     return y
 
 
@@ -173,4 +244,6 @@ def ldu_to_dense(sys: LduSystem) -> np.ndarray:
     A = np.diag(sys.diag.astype(np.float64))
     A[sys.lower_addr, sys.upper_addr] = sys.upper
     A[sys.upper_addr, sys.lower_addr] = sys.lower
+    if sys.n_couples:
+        np.add.at(A, (sys.cpl_row, sys.cpl_col), sys.cpl_val)
     return A

@@ -185,6 +185,26 @@ GIS_API int gis_solve_device(gis_handle h, int method, double *d_x, const double
 GIS_API int gis_foam_solve(gis_handle h, int method, double *psi, const double *source,
                            double tolerance, long long max_iter, gis_performance *perf);
 
+/* ---- coupled interfaces on one device (no counterpart in the reference,
+ * whose wrappers ignore interfaces and coupleBouCoeffs, cuFoamCG.C:30-31).
+ * n_couples extra matrix entries A(row[j], col[j]), for couplings whose two
+ * sides are cells of this mesh (cyclic patches): row != col, and the set is
+ * structurally symmetric (the k-th (r, c) in entry order is mirrored by the
+ * k-th (c, r)).  Each row keeps its LDU entries in the reference's order and
+ * gets its coupling entries appended, in entry order.  Every solver, SpMV,
+ * NormFactor and A^T then includes them.  Call after gis_set_pattern, which
+ * clears them; n_couples = 0 clears them too.  Values are set with
+ * gis_set_coupled_values; gis_set_values keeps the last coupling values
+ * (zero after gis_set_interfaces).  OpenFOAM's Amul subtracts
+ * coupleBouCoeffs * psi(neighbour), so coeffs[j] = -interfaceBouCoeffs of that face.
+ * gis_adjoint returns no gradient for coupling entries. ---- */
+GIS_API int gis_set_interfaces(gis_handle h, int n_couples, const int *row, const int *col);
+GIS_API int gis_set_coupled_values(gis_handle h, const double *diag, const double *lower,
+                                   const double *upper, const double *coeffs);
+GIS_API int gis_set_coupled_values_device(gis_handle h, const double *d_diag,
+                                          const double *d_lower, const double *d_upper,
+                                          const double *d_coeffs);
+
 /* ---- gis_foam_solve with OpenFOAM's remaining solver controls
  * (lduMatrix::solver reads tolerance, relTol, minIter and maxIter from
  * fvSolution; the reference wrappers pass on tolerance and maxIter only,

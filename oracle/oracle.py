@@ -141,14 +141,50 @@ def ldu2csr_pattern(n_cells, lower_addr, upper_addr):
 
 
 def ldu2csr(sys) -> Csr:
-    """CPU CSR of an LduSystem, as getSparseCSRMat builds it (gis/ldu2csr.h:7-42)."""
+    """CPU CSR of an LduSystem, as getSparseCSRMat builds it (gis/ldu2csr.h:7-42).
+
+    Interface entries (sys.cpl_*; no counterpart in the reference, which drops
+    them) follow each row's LDU entries, in entry order (append_entries)."""
     rowptr, col, perm = ldu2csr_pattern(sys.n_cells, sys.lower_addr, sys.upper_addr)
     nnz = perm.shape[0]
     val = np.zeros(max(nnz, 1))
     lib().or_ldu2csr_values(sys.n_cells, sys.n_faces, nnz, _c(perm if nnz else np.zeros(1), np.int32),
                             _c(sys.diag, np.float64), _c(sys.lower if sys.n_faces else np.zeros(1), np.float64),
                             _c(sys.upper if sys.n_faces else np.zeros(1), np.float64), val)
-    return Csr(sys.n_cells, rowptr, col, val[:nnz])
+    A = Csr(sys.n_cells, rowptr, col, val[:nnz])
+    if getattr(sys, "cpl_row", None) is not None and len(sys.cpl_row):
+        A = append_entries(A, sys.cpl_row, sys.cpl_col, sys.cpl_val)
+    return A
+
+
+def append_entries(A: Csr, row, col, val) -> Csr:
+    """A with extra entries (row[j], col[j], val[j]) placed after the existing
+    entries of their row, in j order."""
+    rows = np.repeat(np.arange(A.n), np.diff(A.rowptr))
+    all_rows = np.concatenate([rows, np.asarray(row, np.int64)])
+    order = np.argsort(all_rows, kind="stable")
+    counts = np.bincount(all_rows, minlength=A.n)
+    rowptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    c = np.concatenate([A.col, np.asarray(col, np.int32)])[order].astype(np.int32)
+    v = np.concatenate([A.val, np.asarray(val, np.float64)])[order]
+    return Csr(A.n, rowptr, c, v)
+
+
+def mirror_transpose(A: Csr) -> Csr:
+    """A^T on A's own pattern, for a structurally symmetric A: the k-th entry of
+    column c in row r takes the value of the k-th entry of column r in row c.
+    (For an LDU matrix whose rows ascend in column this is transpose(A).)"""
+    pos: dict[tuple[int, int], list[int]] = {}
+    for r in range(A.n):
+        for p in range(A.rowptr[r], A.rowptr[r + 1]):
+            pos.setdefault((r, int(A.col[p])), []).append(p)
+    val = np.empty_like(A.val)
+    for (r, c), ps in pos.items():
+        qs = pos.get((c, r))
+        if qs is None or len(qs) != len(ps):
+            raise ValueError("mirror_transpose needs a structurally symmetric pattern")
+        val[ps] = A.val[qs]
+    return Csr(A.n, A.rowptr.copy(), A.col.copy(), val)
 
 
 def transpose(A: Csr) -> Csr:
@@ -298,8 +334,11 @@ def foam_solve(method: str, sys, psi, source, tolerance: float, max_iter: int,
     if A is None:
         A = ldu2csr(sys)
     nf = norm_factor(A, source, psi)
-    res = solve(method, A, source, psi, tolerance * nf, max_iter,
-                At=transpose(A) if method == "BiCG" else None, rel_tol=rel_tol,
+    At = None
+    if method == "BiCG":
+        coupled = getattr(sys, "cpl_row", None) is not None and len(sys.cpl_row)
+        At = mirror_transpose(A) if coupled else transpose(A)
+    res = solve(method, A, source, psi, tolerance * nf, max_iter, At=At, rel_tol=rel_tol,
                 min_steps=min_iter)
     perf = Performance("cuFoam" + method, float(res.hist[0]), float(res.hist[-1]),
                        res.hist_len - 1, res.status)
