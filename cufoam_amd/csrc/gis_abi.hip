@@ -159,6 +159,8 @@ GIS_API int gis_destroy(gis_handle h) {
         h->mstate_host = nullptr;
         timer_release(h);
         batch_release(h);
+        delete h->amg;
+        h->amg = nullptr;
         if (h->own_stream) (void)hipStreamDestroy(h->stream);
         delete h;   // DevMem destructors free device memory with the device current
     }
@@ -528,6 +530,60 @@ GIS_API int gis_solve_device(gis_handle h, int method, double* d_x, const double
     return solve_impl(h, method, d_x, d_b, params, hist, hist_cap, hist_len, true);
 }
 
+static int pcg_impl(gis_handle h, int precond, double* x, const double* b,
+                    const gis_params* params, double* hist, int hist_cap, int* hist_len,
+                    bool dev) {
+    if (!h || !x || !b) return GIS_ERR_ARG;
+    if (precond < GIS_PRECOND_NONE || precond > GIS_PRECOND_GAMG)
+        return fail(h, GIS_ERR_ARG, "unknown preconditioner");
+    if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
+    DeviceGuard g(h->device);
+    SolveArgs a = to_args(8 + precond, params, std::max(1, hist_cap));
+    return solve_staged(h, a, x, b, dev, hist, hist_cap, hist_len, nullptr, nullptr);
+}
+
+GIS_API int gis_solve_pcg(gis_handle h, int precond, double* x, const double* b,
+                          const gis_params* params, double* hist, int hist_cap, int* hist_len) {
+    return pcg_impl(h, precond, x, b, params, hist, hist_cap, hist_len, false);
+}
+
+GIS_API int gis_solve_pcg_device(gis_handle h, int precond, double* d_x, const double* d_b,
+                                 const gis_params* params, double* hist, int hist_cap,
+                                 int* hist_len) {
+    return pcg_impl(h, precond, d_x, d_b, params, hist, hist_cap, hist_len, true);
+}
+
+GIS_API int gis_set_gamg(gis_handle h, double omega, int nu, double cscale) {
+    if (!h) return GIS_ERR_ARG;
+    if (!(omega > 0.0 && omega < 2.0) || nu < 1 || nu > 8 || !(cscale > 0.0 && cscale < 2.0))
+        return fail(h, GIS_ERR_ARG, "GAMG needs 0 < omega < 2, 1 <= nu <= 8, 0 < cscale < 2");
+    if (!h->amg) {
+        h->amg = new (std::nothrow) gis::AmgData();
+        if (!h->amg) return GIS_ERR_NOMEM;
+    }
+    h->amg->omega = omega;
+    h->amg->nu = nu;
+    h->amg->cscale = cscale;
+    h->amg->values_ok = false;
+    return GIS_OK;
+}
+
+GIS_API int gis_gamg_levels(gis_handle h, int* n_levels) {
+    if (!h || !n_levels) return GIS_ERR_ARG;
+    if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
+    DeviceGuard g(h->device);
+    return amg_levels(h, n_levels);
+}
+
+GIS_API int gis_gamg_level(gis_handle h, int level, int* n_cells, int* n_faces, int* agg,
+                           int* lower_addr, int* upper_addr, double* diag, double* lower,
+                           double* upper) {
+    if (!h) return GIS_ERR_ARG;
+    if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
+    DeviceGuard g(h->device);
+    return amg_level(h, level, n_cells, n_faces, agg, lower_addr, upper_addr, diag, lower, upper);
+}
+
 static int solve_mixed_impl(gis_handle h, double* x, const double* b, double rel_tol,
                             double inner_rel_tol, long long max_iter, int mode, double* hist,
                             int hist_cap, int* hist_len, int* iterations, double* rel_residual,
@@ -772,11 +828,13 @@ GIS_API int gis_batch_solve_device(gis_handle h, double* d_x, const double* d_b,
 
 namespace {
 
+// method: GIS_CG .. GIS_RICHARDSON, or 8 + preconditioner for PCG
 int foam_solve_impl(gis_context* h, int method, double* psi, const double* source,
                     double tolerance, double rel_tol, long long min_iter, long long max_iter,
                     gis_performance* perf) {
     if (!h || !psi || !source) return GIS_ERR_ARG;
-    if (method < GIS_CG || method > GIS_RICHARDSON) return fail(h, GIS_ERR_ARG, "unknown method");
+    if (method < GIS_CG || method > 8 + GIS_PRECOND_GAMG)
+        return fail(h, GIS_ERR_ARG, "unknown method");
     if (!(rel_tol >= 0.0) || min_iter < 0)
         return fail(h, GIS_ERR_ARG, "rel_tol and min_iter must be non-negative");
     if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
@@ -812,6 +870,15 @@ int foam_solve_impl(gis_context* h, int method, double* psi, const double* sourc
 GIS_API int gis_foam_solve(gis_handle h, int method, double* psi, const double* source,
                            double tolerance, long long max_iter, gis_performance* perf) {
     return foam_solve_impl(h, method, psi, source, tolerance, 0.0, 0, max_iter, perf);
+}
+
+GIS_API int gis_foam_solve_pcg(gis_handle h, int precond, double* psi, const double* source,
+                               double tolerance, double rel_tol, long long min_iter,
+                               long long max_iter, gis_performance* perf) {
+    if (h && (precond < GIS_PRECOND_NONE || precond > GIS_PRECOND_GAMG))
+        return fail(h, GIS_ERR_ARG, "unknown preconditioner");
+    return foam_solve_impl(h, 8 + precond, psi, source, tolerance, rel_tol, min_iter, max_iter,
+                           perf);
 }
 
 GIS_API int gis_foam_solve_controls(gis_handle h, int method, double* psi,

@@ -1,0 +1,873 @@
+// Preconditioned CG on gfx950: preconditioner none, diagonal, or GAMG
+// (pairwise algebraic agglomeration, Galerkin coarse LDU levels, symmetric
+// V-cycle with damped Jacobi).  The reference has no preconditioners (its
+// roadmap, README.md:71-73); the loop keeps the stopping rule and history of
+// its CG (SolverSingleGPU.h:366-445), and with preconditioner none the
+// iterates are those of gis CG, bit for bit.
+//
+// Per iteration (device state as in CG, no host round-trip):
+//   dir:   p = z (k = 1) or fma(beta, p, z); deferred x = fma(alpha, p_prev, x)
+//   spmv:  q = A p, alpha = rho / (p, q)                         (k_cg1s)
+//   upd:   r = fma(-alpha, q, r), ||r||, history, stop test; diagonal: z = r / a_ii
+//          and rho = (r, z) in the same pass
+//   GAMG:  z = V-cycle(r); its last sweep also forms rho = (r, z)
+//
+// The hierarchy is built on the host once per pattern, from the values
+// present at that time (agglomeration weights |upper| + |lower|): two
+// pairwise passes per level until a level has <= kDense cells, whose dense
+// inverse (Gauss-Jordan without pivoting) is formed on the device.  The
+// Galerkin values of every coarse level are recomputed on the device after
+// each change of values, in a fixed order (members ascending, then faces
+// ascending), so results do not depend on the launch.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+#include "gis_kernels.hpp"
+
+namespace gis {
+
+int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const double* b);
+
+namespace {
+
+constexpr int kBlk = 256;
+constexpr int kDense = 64;        // the last level is inverted densely at or below this size
+constexpr int kMaxLevels = 32;
+
+inline dim3 grid_of(long long n) { return dim3((unsigned)((n + kBlk - 1) / kBlk)); }
+
+// ---------------------------------------------------------------------------
+// host agglomeration (same steps as the oracle's restatement)
+// ---------------------------------------------------------------------------
+struct HostLevel {
+    int n = 0, m = 0;
+    std::vector<int> la, ua;
+    std::vector<double> diag, lower, upper;
+};
+
+// rows of the face graph: faces where the cell is neighbour (face order), then
+// faces where it is owner (face order)
+void graph_rows(int n, int m, const int* la, const int* ua, std::vector<int>& rp,
+                std::vector<int>& face, std::vector<int>& nb) {
+    rp.assign((size_t)n + 1, 0);
+    face.resize(2 * (size_t)m);
+    nb.resize(2 * (size_t)m);
+    std::vector<int> lo_cnt(n, 0), fill(n, 0);
+    for (int f = 0; f < m; ++f) {
+        rp[ua[f] + 1]++;
+        rp[la[f] + 1]++;
+        lo_cnt[ua[f]]++;
+    }
+    for (int i = 0; i < n; ++i) rp[i + 1] += rp[i];
+    for (int f = 0; f < m; ++f) {
+        const int c = ua[f];
+        const int p = rp[c] + fill[c]++;
+        face[p] = f;
+        nb[p] = la[f];
+    }
+    std::fill(fill.begin(), fill.end(), 0);
+    for (int f = 0; f < m; ++f) {
+        const int c = la[f];
+        const int p = rp[c] + lo_cnt[c] + fill[c]++;
+        face[p] = f;
+        nb[p] = ua[f];
+    }
+}
+
+int pair_pass(int n, int m, const int* la, const int* ua, const double* w, std::vector<int>& agg) {
+    std::vector<int> rp, face, nb;
+    graph_rows(n, m, la, ua, rp, face, nb);
+    agg.assign(n, -1);
+    int nc = 0;
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        int best = -1;
+        double bw = -1.0;
+        for (int p = rp[i]; p < rp[i + 1]; ++p)
+            if (agg[nb[p]] < 0 && w[face[p]] > bw) {
+                bw = w[face[p]];
+                best = nb[p];
+            }
+        if (best >= 0) {
+            agg[i] = agg[best] = nc++;
+            continue;
+        }
+        for (int p = rp[i]; p < rp[i + 1]; ++p)
+            if (w[face[p]] > bw) {
+                bw = w[face[p]];
+                best = nb[p];
+            }
+        agg[i] = best >= 0 ? agg[best] : nc++;
+    }
+    return nc;
+}
+
+// Distinct aggregate pairs of the fine faces in upper-triangular order
+// (sorted by owner, then neighbour); fcode[f] = 2 F + flip, or -1 - C for a
+// face inside aggregate C.
+int coarse_faces(int m, const int* la, const int* ua, const int* agg, int nc,
+                 std::vector<int>& fcode, std::vector<int>& cla, std::vector<int>& cua) {
+    std::vector<int> start((size_t)nc + 1, 0);
+    for (int f = 0; f < m; ++f) {
+        const int a = agg[la[f]], b = agg[ua[f]];
+        if (a != b) start[std::min(a, b) + 1]++;
+    }
+    for (int c = 0; c < nc; ++c) start[c + 1] += start[c];
+    std::vector<int> nbr(start[nc]), fill(nc, 0);
+    for (int f = 0; f < m; ++f) {
+        const int a = agg[la[f]], b = agg[ua[f]];
+        if (a != b) {
+            const int o = std::min(a, b);
+            nbr[start[o] + fill[o]++] = std::max(a, b);
+        }
+    }
+    // per owner: sorted distinct neighbours; uniq[c] .. uniq[c+1] their face ids
+    std::vector<int> ustart((size_t)nc + 1, 0);
+    cla.clear();
+    cua.clear();
+    for (int c = 0; c < nc; ++c) {
+        auto b0 = nbr.begin() + start[c], b1 = nbr.begin() + start[c + 1];
+        std::sort(b0, b1);
+        auto e = std::unique(b0, b1);
+        ustart[c] = (int)cla.size();
+        for (auto it = b0; it != e; ++it) {
+            cla.push_back(c);
+            cua.push_back(*it);
+        }
+    }
+    ustart[nc] = (int)cla.size();
+    fcode.resize(m);
+    for (int f = 0; f < m; ++f) {
+        const int a = agg[la[f]], b = agg[ua[f]];
+        if (a == b) {
+            fcode[f] = -1 - a;
+            continue;
+        }
+        const int o = std::min(a, b), t = std::max(a, b);
+        const auto it = std::lower_bound(cua.begin() + ustart[o], cua.begin() + ustart[o + 1], t);
+        fcode[f] = 2 * (int)(it - cua.begin()) + (a > b ? 1 : 0);
+    }
+    return (int)cla.size();
+}
+
+// Galerkin values of the next level on the host (for the next level's weights)
+void restrict_host(const HostLevel& F, const std::vector<int>& agg,
+                   const std::vector<int>& fcode, HostLevel& C) {
+    C.diag.assign(C.n, 0.0);
+    C.upper.assign(C.m, 0.0);
+    C.lower.assign(C.m, 0.0);
+    for (int i = 0; i < F.n; ++i) C.diag[agg[i]] = C.diag[agg[i]] + F.diag[i];
+    for (int f = 0; f < F.m; ++f) {
+        const int code = fcode[f];
+        if (code < 0) {
+            const int c = -1 - code;
+            C.diag[c] = C.diag[c] + F.upper[f];
+            C.diag[c] = C.diag[c] + F.lower[f];
+        } else {
+            const int g = code >> 1;
+            const bool flip = code & 1;
+            C.upper[g] = C.upper[g] + (flip ? F.lower[f] : F.upper[f]);
+            C.lower[g] = C.lower[g] + (flip ? F.upper[f] : F.lower[f]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// device: coarse values
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlk) void k_amg_cdiag(int nc, const int* __restrict__ mptr,
+                                                    const int* __restrict__ mlist,
+                                                    const int* __restrict__ iptr,
+                                                    const int* __restrict__ ilist,
+                                                    const double* __restrict__ diag,
+                                                    const double* __restrict__ lower,
+                                                    const double* __restrict__ upper,
+                                                    double* __restrict__ cdiag) {
+    const int c = blockIdx.x * kBlk + threadIdx.x;
+    if (c >= nc) return;
+    double s = 0.0;
+    for (int p = mptr[c]; p < mptr[c + 1]; ++p) s = s + diag[mlist[p]];
+    for (int p = iptr[c]; p < iptr[c + 1]; ++p) {
+        const int f = ilist[p];
+        s = s + upper[f];
+        s = s + lower[f];
+    }
+    cdiag[c] = s;
+}
+
+__global__ __launch_bounds__(kBlk) void k_amg_cface(int mc, const int* __restrict__ fptr,
+                                                    const int* __restrict__ flist,
+                                                    const double* __restrict__ lower,
+                                                    const double* __restrict__ upper,
+                                                    double* __restrict__ clower,
+                                                    double* __restrict__ cupper) {
+    const int g = blockIdx.x * kBlk + threadIdx.x;
+    if (g >= mc) return;
+    double su = 0.0, sl = 0.0;
+    for (int p = fptr[g]; p < fptr[g + 1]; ++p) {
+        const int code = flist[p];
+        const int f = code >> 1;
+        const bool flip = code & 1;
+        su = su + (flip ? lower[f] : upper[f]);
+        sl = sl + (flip ? upper[f] : lower[f]);
+    }
+    cupper[g] = su;
+    clower[g] = sl;
+}
+
+__global__ __launch_bounds__(kBlk) void k_amg_sm(int n, double omega,
+                                                 const double* __restrict__ diag,
+                                                 double* __restrict__ sm) {
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    if (i < n) sm[i] = omega / diag[i];
+}
+
+__global__ __launch_bounds__(kBlk) void k_recip(int n, const double* __restrict__ diag,
+                                                double* __restrict__ out) {
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    if (i < n) out[i] = 1.0 / diag[i];
+}
+
+// Dense inverse of the last level by Gauss-Jordan without pivoting on the
+// augmented [A | I] (n x 2n in M); thread j owns column j.
+__global__ __launch_bounds__(2 * kDense) void k_amg_dense_inv(int n, int m,
+                                                              const int* __restrict__ la,
+                                                              const int* __restrict__ ua,
+                                                              const double* __restrict__ diag,
+                                                              const double* __restrict__ lower,
+                                                              const double* __restrict__ upper,
+                                                              double* __restrict__ M,
+                                                              double* __restrict__ ainv) {
+    __shared__ double fac[kDense];
+    const int w = 2 * n;
+    const int j = threadIdx.x;
+    for (int idx = j; idx < n * w; idx += blockDim.x) M[idx] = 0.0;
+    __syncthreads();
+    if (j < n) {
+        M[j * w + j] = diag[j];
+        M[j * w + n + j] = 1.0;
+    }
+    __syncthreads();
+    if (j == 0) {   // face order (faces may repeat a cell pair on a fine level)
+        for (int f = 0; f < m; ++f) {
+            M[la[f] * w + ua[f]] = M[la[f] * w + ua[f]] + upper[f];
+            M[ua[f] * w + la[f]] = M[ua[f] * w + la[f]] + lower[f];
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        const double piv = M[k * w + k];
+        __syncthreads();
+        if (j < w) M[k * w + j] = M[k * w + j] / piv;
+        __syncthreads();
+        if (j < n) fac[j] = M[j * w + k];
+        __syncthreads();
+        if (j < w) {
+            const double pk = M[k * w + j];
+            for (int i = 0; i < n; ++i)
+                if (i != k) M[i * w + j] = fma(-fac[i], pk, M[i * w + j]);
+        }
+        __syncthreads();
+    }
+    if (j < n)
+        for (int i = 0; i < n; ++i) ainv[i * n + j] = M[i * w + n + j];
+}
+
+// ---------------------------------------------------------------------------
+// device: V-cycle pieces (every kernel is a no-op once the solve is done)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kWG) void k_vc_pre(int N, const double* __restrict__ sm,
+                                                const double* __restrict__ r,
+                                                double* __restrict__ z, const KState* st) {
+    if (st->done) return;
+    StreamRows t(N);
+    double sv[4], rv[4], zv[4];
+    load4(sm, t.i0, N, sv);
+    load4(r, t.i0, N, rv);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) zv[k] = sv[k] * rv[k];
+    store4(z, t.i0, N, zv);
+}
+
+// zn = fma(sm, r - A z, z)
+template <class P>
+__global__ __launch_bounds__(kWG) void k_vc_smooth(Mat<double> m, int N,
+                                                   const double* __restrict__ sm,
+                                                   const double* __restrict__ r,
+                                                   const double* __restrict__ z,
+                                                   double* __restrict__ zn, const KState* st) {
+    if (st->done) return;
+    __shared__ typename P::template Shared<double> sh;
+    __shared__ double sh_y[kTile];
+    TileRows t(N);
+    P::run(m, N, t.row0, t.nrows, [&](int c) { return z[c]; }, sh, sh_y);
+    double sv[4], rv[4], zv[4];
+    load4(sm, t.i0, N, sv);
+    load4(r, t.i0, N, rv);
+    load4(z, t.i0, N, zv);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) zv[k] = fma(sv[k], rv[k] - sh_y[threadIdx.x * kLeaf + k], zv[k]);
+    store4(zn, t.i0, N, zv);
+}
+
+// t = r - A z
+template <class P>
+__global__ __launch_bounds__(kWG) void k_vc_resid(Mat<double> m, int N,
+                                                  const double* __restrict__ r,
+                                                  const double* __restrict__ z,
+                                                  double* __restrict__ tr, const KState* st) {
+    if (st->done) return;
+    __shared__ typename P::template Shared<double> sh;
+    __shared__ double sh_y[kTile];
+    TileRows t(N);
+    P::run(m, N, t.row0, t.nrows, [&](int c) { return z[c]; }, sh, sh_y);
+    double rv[4], tv[4];
+    load4(r, t.i0, N, rv);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) tv[k] = rv[k] - sh_y[threadIdx.x * kLeaf + k];
+    store4(tr, t.i0, N, tv);
+}
+
+// rc[C] = sum of t over C's cells, ascending
+__global__ __launch_bounds__(kBlk) void k_vc_restrict(int nc, const int* __restrict__ mptr,
+                                                      const int* __restrict__ mlist,
+                                                      const double* __restrict__ t,
+                                                      double* __restrict__ rc,
+                                                      const KState* st) {
+    if (st->done) return;
+    const int c = blockIdx.x * kBlk + threadIdx.x;
+    if (c >= nc) return;
+    double s = 0.0;
+    for (int p = mptr[c]; p < mptr[c + 1]; ++p) s = s + t[mlist[p]];
+    rc[c] = s;
+}
+
+// z = fma(cscale, zc[agg], z)
+__global__ __launch_bounds__(kWG) void k_vc_prolong(int N, double cscale,
+                                                    const int* __restrict__ agg,
+                                                    const double* __restrict__ zc,
+                                                    double* __restrict__ z, const KState* st) {
+    if (st->done) return;
+    StreamRows t(N);
+    double zv[4];
+    load4(z, t.i0, N, zv);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (t.i0 + k < N) zv[k] = fma(cscale, zc[agg[t.i0 + k]], zv[k]);
+    store4(z, t.i0, N, zv);
+}
+
+// z = Ainv r on the last level (n <= kDense)
+__global__ __launch_bounds__(kDense) void k_vc_dense(int n, const double* __restrict__ ainv,
+                                                     const double* __restrict__ r,
+                                                     double* __restrict__ z, const KState* st) {
+    if (st->done) return;
+    const int i = threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = 0; j < n; ++j) s = s + ainv[i * n + j] * r[j];
+    z[i] = s;
+}
+
+// Last sweep on the finest level: zn = fma(sm, r - A z, z), rho = (r, zn),
+// beta = rho / rho_prev.
+template <class P>
+__global__ __launch_bounds__(kWG) void k_pcg_post(Mat<double> m, int N,
+                                                  const double* __restrict__ sm,
+                                                  const double* __restrict__ r,
+                                                  const double* __restrict__ z,
+                                                  double* __restrict__ zn, KState* st,
+                                                  RedWs red) {
+    if (st->done) return;
+    __shared__ typename P::template Shared<double> sh;
+    __shared__ double sh_y[kTile];
+    __shared__ RedShared shr;
+    TileRows t(N);
+    P::run(m, N, t.row0, t.nrows, [&](int c) { return z[c]; }, sh, sh_y);
+    double sv[4], rv[4], zv[4];
+    load4(sm, t.i0, N, sv);
+    load4(r, t.i0, N, rv);
+    load4(z, t.i0, N, zv);
+    double leaf[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        zv[k] = fma(sv[k], rv[k] - sh_y[threadIdx.x * kLeaf + k], zv[k]);
+        if (t.i0 + k < N) leaf[0] = fma(rv[k], zv[k], leaf[0]);
+    }
+    store4(zn, t.i0, N, zv);
+    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+        st->rho1 = st->rho0;
+        st->rho0 = v[0];
+        st->beta = st->rho0 / st->rho1;
+    });
+}
+
+// rho = (r, z), beta = rho / rho_prev (after a one-level V-cycle)
+__global__ __launch_bounds__(kWG) void k_pcg_rho(int N, const double* __restrict__ r,
+                                                 const double* __restrict__ z, KState* st,
+                                                 RedWs red) {
+    if (st->done) return;
+    __shared__ RedShared shr;
+    TileRows t(N);
+    double rv[4], zv[4];
+    load4(r, t.i0, N, rv);
+    load4(z, t.i0, N, zv);
+    double leaf[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (t.i0 + k < N) leaf[0] = fma(rv[k], zv[k], leaf[0]);
+    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+        st->rho1 = st->rho0;
+        st->rho0 = v[0];
+        st->beta = st->rho0 / st->rho1;
+    });
+}
+
+// z = dinv r and rho = (r, z) (initial diagonal preconditioning)
+__global__ __launch_bounds__(kWG) void k_pcg_diag0(int N, const double* __restrict__ dinv,
+                                                   const double* __restrict__ r,
+                                                   double* __restrict__ z, KState* st,
+                                                   RedWs red) {
+    if (st->done) return;
+    __shared__ RedShared shr;
+    TileRows t(N);
+    double dv[4], rv[4], zv[4];
+    load4(dinv, t.i0, N, dv);
+    load4(r, t.i0, N, rv);
+    double leaf[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        zv[k] = dv[k] * rv[k];
+        if (t.i0 + k < N) leaf[0] = fma(rv[k], zv[k], leaf[0]);
+    }
+    store4(z, t.i0, N, zv);
+    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) { st->rho0 = v[0]; });
+}
+
+// p = z (k = 1) or fma(beta, p_prev, z); x = fma(alpha_prev, p_prev, x) (deferred)
+__global__ __launch_bounds__(kWG) void k_pcg_dir(int N, const double* __restrict__ z,
+                                                 double* __restrict__ pb0,
+                                                 double* __restrict__ pb1,
+                                                 double* __restrict__ x, const KState* st) {
+    if (st->done) return;
+    StreamRows t(N);
+    const int k = st->k + 1;
+    const double* p_old = (k & 1) ? pb0 : pb1;
+    double* p_new = (k & 1) ? pb1 : pb0;
+    double zv[4], pv[4], xv[4];
+    load4(z, t.i0, N, zv);
+    if (k == 1) {
+        store4(p_new, t.i0, N, zv);
+        return;
+    }
+    const double beta = st->beta;
+    const double alpha_prev = st->alpha;
+    load4(p_old, t.i0, N, pv);
+    load4(x, t.i0, N, xv);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        xv[kk] = fma(alpha_prev, pv[kk], xv[kk]);
+        pv[kk] = fma(beta, pv[kk], zv[kk]);
+    }
+    store4(p_new, t.i0, N, pv);
+    store4(x, t.i0, N, xv);
+}
+
+// r = fma(-alpha, q, r); ||r||, history, stop test.  PC 0: rho = (r, r);
+// PC 1: z = dinv r, rho = (r, z); PC 2: rho is left to the V-cycle.
+template <int C, int PC>
+__global__ __launch_bounds__(kWG) void k_pcg2(int N, double* __restrict__ r,
+                                              const double* __restrict__ q,
+                                              const double* __restrict__ dinv,
+                                              double* __restrict__ z, KState* st, RedWs red) {
+    if (st->done) return;
+    constexpr int NV = PC == 1 ? 2 : 1;
+    __shared__ RedShared shr;
+    const SuperTile<C> sup(N, true);
+    const double alpha = st->alpha;
+    double leaf[C][NV];
+    double rv[C][4], zv[C][4];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) leaf[c][v] = 0.0;
+        if (c >= sup.nt) continue;
+        const TileRows t(N, sup.tile0 + c);
+        double qv[4], dv[4];
+        load4(r, t.i0, N, rv[c]);
+        load4(q, t.i0, N, qv);
+        if (PC == 1) load4(dinv, t.i0, N, dv);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            rv[c][kk] = fma(-alpha, qv[kk], rv[c][kk]);
+            if (PC == 1) zv[c][kk] = dv[kk] * rv[c][kk];
+            if (t.i0 + kk < N) {
+                leaf[c][0] = fma(rv[c][kk], rv[c][kk], leaf[c][0]);
+                if (PC == 1) leaf[c][NV - 1] = fma(rv[c][kk], zv[c][kk], leaf[c][NV - 1]);
+            }
+        }
+    }
+    TileReduce<NV, C> tr;
+    tr.begin(red, sup.tile0, sup.nt, leaf, shr);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        if (c >= sup.nt) continue;
+        const int i0 = TileRows(N, sup.tile0 + c).i0;
+        store4(r, i0, N, rv[c]);
+        if (PC == 1) store4(z, i0, N, zv[c]);
+    }
+    tr.end(red, [&](const double (&v)[NV]) {
+        const double norm = sqrt(v[0]);
+        record_norm(st, norm);
+        if (PC != 2) {
+            st->rho1 = st->rho0;
+            st->rho0 = v[NV - 1];
+            st->beta = st->rho0 / st->rho1;
+        }
+        if (stop_after(st, !(norm > st->error))) st->done = 1;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// host: hierarchy on the device
+// ---------------------------------------------------------------------------
+template <typename T>
+int upload_vec(gis_context* h, DevMem& d, const std::vector<T>& v) {
+    GIS_HIP(h, d.ensure(sizeof(T) * std::max<size_t>(v.size(), 1)));
+    if (!v.empty())
+        GIS_HIP(h, hipMemcpyAsync(d.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice,
+                                  h->stream));
+    return kOK;
+}
+
+// counting-sort lists: for key k, the items i (ascending) with key[i] == k
+void bucket(int nkeys, const std::vector<int>& key, const std::vector<int>& item,
+            std::vector<int>& ptr, std::vector<int>& list) {
+    ptr.assign((size_t)nkeys + 1, 0);
+    for (int k : key)
+        if (k >= 0) ptr[k + 1]++;
+    for (int c = 0; c < nkeys; ++c) ptr[c + 1] += ptr[c];
+    list.resize(ptr[nkeys]);
+    std::vector<int> fill(nkeys, 0);
+    for (size_t i = 0; i < key.size(); ++i)
+        if (key[i] >= 0) list[ptr[key[i]] + fill[key[i]]++] = item[i];
+}
+
+template <typename T>
+int download(gis_context* h, std::vector<T>& out, const DevMem& d, size_t n) {
+    out.resize(n);
+    if (n)
+        GIS_HIP(h, hipMemcpyAsync(out.data(), d.p, sizeof(T) * n, hipMemcpyDeviceToHost,
+                                  h->stream));
+    return kOK;
+}
+
+gis_context* level_ctx(gis_context* h, int l) { return l == 0 ? h : h->amg->lv[l].ctx; }
+
+int amg_build(gis_context* h) {
+    AmgData& A = *h->amg;
+    A.clear();
+    HostLevel fine;
+    fine.n = h->N;
+    fine.m = h->M;
+    GIS_TRY(download(h, fine.la, h->lower_addr, (size_t)h->M));
+    GIS_TRY(download(h, fine.ua, h->upper_addr, (size_t)h->M));
+    GIS_TRY(download(h, fine.lower, h->lower, (size_t)h->M));
+    GIS_TRY(download(h, fine.upper, h->upper, (size_t)h->M));
+    GIS_TRY(download(h, fine.diag, h->diag, (size_t)h->N));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    A.lv.emplace_back();
+    A.lv[0].n = fine.n;
+    A.lv[0].m = fine.m;
+    HostLevel cur = std::move(fine);
+    while (cur.n > kDense && (int)A.lv.size() < kMaxLevels) {
+        std::vector<double> w(cur.m);
+        for (int f = 0; f < cur.m; ++f) w[f] = std::fabs(cur.upper[f]) + std::fabs(cur.lower[f]);
+        std::vector<int> agg1, fc1, la1, ua1, agg2, fcode, cla, cua;
+        const int n1 = pair_pass(cur.n, cur.m, cur.la.data(), cur.ua.data(), w.data(), agg1);
+        const int m1 = coarse_faces(cur.m, cur.la.data(), cur.ua.data(), agg1.data(), n1, fc1, la1,
+                                    ua1);
+        std::vector<double> w1(m1, 0.0);
+        for (int f = 0; f < cur.m; ++f)
+            if (fc1[f] >= 0) w1[fc1[f] >> 1] = w1[fc1[f] >> 1] + w[f];
+        const int n2 = pair_pass(n1, m1, la1.data(), ua1.data(), w1.data(), agg2);
+        if ((long long)n2 * 10 > (long long)cur.n * 9) break;   // stagnation
+        std::vector<int> agg(cur.n);
+        for (int i = 0; i < cur.n; ++i) agg[i] = agg2[agg1[i]];
+        const int mc = coarse_faces(cur.m, cur.la.data(), cur.ua.data(), agg.data(), n2, fcode,
+                                    cla, cua);
+        HostLevel next;
+        next.n = n2;
+        next.m = mc;
+        restrict_host(cur, agg, fcode, next);
+        next.la = cla;
+        next.ua = cua;
+        // device data of the transfer cur -> next
+        AmgLevel& L = A.lv.back();
+        L.nc = n2;
+        std::vector<int> cells(cur.n), ptr, list;
+        for (int i = 0; i < cur.n; ++i) cells[i] = i;
+        GIS_TRY(upload_vec(h, L.agg, agg));
+        bucket(n2, agg, cells, ptr, list);
+        GIS_TRY(upload_vec(h, L.mptr, ptr));
+        GIS_TRY(upload_vec(h, L.mlist, list));
+        std::vector<int> inside(cur.m), faces(cur.m), coarse_face(cur.m), code(cur.m);
+        for (int f = 0; f < cur.m; ++f) {
+            inside[f] = fcode[f] < 0 ? -1 - fcode[f] : -1;
+            coarse_face[f] = fcode[f] >= 0 ? fcode[f] >> 1 : -1;
+            faces[f] = f;
+            code[f] = fcode[f] >= 0 ? 2 * f + (fcode[f] & 1) : 0;
+        }
+        bucket(n2, inside, faces, ptr, list);
+        GIS_TRY(upload_vec(h, L.iptr, ptr));
+        GIS_TRY(upload_vec(h, L.ilist, list));
+        bucket(mc, coarse_face, code, ptr, list);
+        GIS_TRY(upload_vec(h, L.fptr, ptr));
+        GIS_TRY(upload_vec(h, L.flist, list));
+        // the next level as a pattern of its own
+        A.lv.emplace_back();
+        AmgLevel& C = A.lv.back();
+        C.n = n2;
+        C.m = mc;
+        C.ctx = new (std::nothrow) gis_context();
+        if (!C.ctx) return fail(h, kErrNoMem, "GAMG level context");
+        C.ctx->device = h->device;
+        C.ctx->stream = h->stream;
+        C.ctx->own_stream = false;
+        GIS_TRY(upload_vec(h, C.ctx->lower_addr, cla));
+        GIS_TRY(upload_vec(h, C.ctx->upper_addr, cua));
+        GIS_HIP(h, hipStreamSynchronize(h->stream));
+        if (build_pattern(C.ctx, n2, mc, C.ctx->lower_addr.as<int>(),
+                          C.ctx->upper_addr.as<int>()) != kOK)
+            return fail(h, kErrHip, "GAMG level pattern: " + C.ctx->err);
+        cur = std::move(next);
+    }
+    A.dense_n = cur.n <= kDense ? cur.n : 0;
+    for (size_t l = 0; l < A.lv.size(); ++l) {
+        AmgLevel& L = A.lv[l];
+        const size_t n = (size_t)L.n;
+        GIS_HIP(h, L.sm.ensure(sizeof(double) * n));
+        GIS_HIP(h, L.vecs.ensure(sizeof(double) * 4 * n));
+    }
+    if (A.dense_n) {
+        GIS_HIP(h, A.ainv.ensure(sizeof(double) * A.dense_n * A.dense_n));
+        GIS_HIP(h, A.dense_work.ensure(sizeof(double) * 2 * A.dense_n * A.dense_n));
+    }
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    A.built = true;
+    A.values_ok = false;
+    return kOK;
+}
+
+int amg_values(gis_context* h) {
+    AmgData& A = *h->amg;
+    for (size_t l = 0; l < A.lv.size(); ++l) {
+        AmgLevel& L = A.lv[l];
+        gis_context* c = level_ctx(h, (int)l);
+        if (l > 0) GIS_TRY(gather_values(c));
+        hipLaunchKernelGGL(k_amg_sm, grid_of(L.n), dim3(kBlk), 0, h->stream, L.n, A.omega,
+                           c->diag.as<double>(), L.sm.as<double>());
+        GIS_HIP(h, hipGetLastError());
+        if (l + 1 < A.lv.size()) {
+            gis_context* n = A.lv[l + 1].ctx;
+            const AmgLevel& C = A.lv[l + 1];
+            GIS_HIP(h, n->diag.ensure(sizeof(double) * std::max(C.n, 1)));
+            GIS_HIP(h, n->lower.ensure(sizeof(double) * std::max(C.m, 1)));
+            GIS_HIP(h, n->upper.ensure(sizeof(double) * std::max(C.m, 1)));
+            hipLaunchKernelGGL(k_amg_cdiag, grid_of(C.n), dim3(kBlk), 0, h->stream, C.n,
+                               L.mptr.as<int>(), L.mlist.as<int>(), L.iptr.as<int>(),
+                               L.ilist.as<int>(), c->diag.as<double>(), c->lower.as<double>(),
+                               c->upper.as<double>(), n->diag.as<double>());
+            GIS_HIP(h, hipGetLastError());
+            if (C.m > 0) {
+                hipLaunchKernelGGL(k_amg_cface, grid_of(C.m), dim3(kBlk), 0, h->stream, C.m,
+                                   L.fptr.as<int>(), L.flist.as<int>(), c->lower.as<double>(),
+                                   c->upper.as<double>(), n->lower.as<double>(),
+                                   n->upper.as<double>());
+                GIS_HIP(h, hipGetLastError());
+            }
+        }
+    }
+    if (A.dense_n) {
+        gis_context* c = level_ctx(h, (int)A.lv.size() - 1);
+        hipLaunchKernelGGL(k_amg_dense_inv, dim3(1), dim3(2 * kDense), 0, h->stream, A.dense_n,
+                           c->M, c->lower_addr.as<int>(), c->upper_addr.as<int>(),
+                           c->diag.as<double>(), c->lower.as<double>(), c->upper.as<double>(),
+                           A.dense_work.as<double>(), A.ainv.as<double>());
+        GIS_HIP(h, hipGetLastError());
+    }
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    A.values_ok = true;
+    return kOK;
+}
+
+int amg_prepare(gis_context* h) {
+    if (h->n_cpl > 0)
+        return fail(h, kErrState, "GAMG does not take interface entries (gis_set_interfaces)");
+    if (h->batch.ready) return fail(h, kErrState, "GAMG needs a single-system pattern");
+    if (!h->amg) {
+        h->amg = new (std::nothrow) AmgData();
+        if (!h->amg) return fail(h, kErrNoMem, "GAMG data");
+    }
+    if (!h->amg->built) GIS_TRY(amg_build(h));
+    if (!h->amg->values_ok) GIS_TRY(amg_values(h));
+    return kOK;
+}
+
+// V-cycle of level l on rhs r; returns the buffer holding z.  On the finest
+// level the last sweep goes to `out` together with rho = (r, z).
+template <class P>
+const double* vcycle(gis_context* h, int l, const double* r, double* out, int* err) {
+    AmgData& A = *h->amg;
+    AmgLevel& L = A.lv[l];
+    gis_context* c = level_ctx(h, l);
+    const int N = L.n;
+    double* z = L.vecs.as<double>();
+    double* z2 = z + N;
+    double* t = z + 2 * (size_t)N;
+    const dim3 g = tiles_grid(N);
+    const Mat<double> m = mat64(c);
+    if (l + 1 == (int)A.lv.size()) {
+        if (l == 0) z = out;
+        if (A.dense_n)
+            hipLaunchKernelGGL(k_vc_dense, dim3(1), dim3(kDense), 0, h->stream, N,
+                               A.ainv.as<double>(), r, z, h->st);
+        else
+            hipLaunchKernelGGL(k_vc_pre, g, dim3(kWG), 0, h->stream, N, L.sm.as<double>(), r, z,
+                               h->st);
+        if (l == 0)   // a one-level hierarchy: rho = (r, z) by a dot pass
+            hipLaunchKernelGGL(k_pcg_rho, g, dim3(kWG), 0, h->stream, N, r, z, h->st, h->red);
+        return z;
+    }
+    AmgLevel& C = A.lv[l + 1];
+    double* rc = C.vecs.as<double>() + 3 * (size_t)C.n;
+    hipLaunchKernelGGL(k_vc_pre, g, dim3(kWG), 0, h->stream, N, L.sm.as<double>(), r, z, h->st);
+    for (int s = 1; s < A.nu; ++s) {
+        hipLaunchKernelGGL((k_vc_smooth<P>), g, dim3(kWG), 0, h->stream, m, N, L.sm.as<double>(),
+                           r, z, z2, h->st);
+        std::swap(z, z2);
+    }
+    hipLaunchKernelGGL((k_vc_resid<P>), g, dim3(kWG), 0, h->stream, m, N, r, z, t, h->st);
+    hipLaunchKernelGGL(k_vc_restrict, grid_of(C.n), dim3(kBlk), 0, h->stream, C.n,
+                       L.mptr.as<int>(), L.mlist.as<int>(), t, rc, h->st);
+    const double* zc = vcycle<P>(h, l + 1, rc, nullptr, err);
+    hipLaunchKernelGGL(k_vc_prolong, g, dim3(kWG), 0, h->stream, N, A.cscale, L.agg.as<int>(), zc,
+                       z, h->st);
+    for (int s = 0; s < A.nu; ++s) {
+        if (l == 0 && s + 1 == A.nu) {
+            hipLaunchKernelGGL((k_pcg_post<P>), g, dim3(kWG), 0, h->stream, m, N,
+                               L.sm.as<double>(), r, z, out, h->st, h->red);
+            return out;
+        }
+        hipLaunchKernelGGL((k_vc_smooth<P>), g, dim3(kWG), 0, h->stream, m, N, L.sm.as<double>(),
+                           r, z, z2, h->st);
+        std::swap(z, z2);
+    }
+    if (hipGetLastError() != hipSuccess) *err = 1;
+    return z;
+}
+
+}  // namespace
+
+AmgData::~AmgData() { clear(); }
+
+void AmgData::clear() {
+    for (auto& L : lv) delete L.ctx;
+    lv.clear();
+    ainv.release();
+    dense_work.release();
+    dense_n = 0;
+    built = false;
+    values_ok = false;
+}
+
+int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const double* b) {
+    if (pc < 0 || pc > 2) return fail(h, kErrArg, "unknown preconditioner");
+    if (pc == 2) GIS_TRY(amg_prepare(h));
+    GIS_TRY(prepare_solve(h, a, 5));
+    const int N = h->N;
+    double* r = h->vec.as<double>();
+    double* pb0 = r + N;
+    double* pb1 = r + 2 * (size_t)N;
+    double* q = r + 3 * (size_t)N;
+    double* z = r + 4 * (size_t)N;
+    if (pc == 1) {
+        GIS_HIP(h, h->dinv.ensure(sizeof(double) * (size_t)N));
+        hipLaunchKernelGGL(k_recip, grid_of(N), dim3(kBlk), 0, h->stream, N,
+                           h->diag.as<double>(), h->dinv.as<double>());
+        GIS_HIP(h, hipGetLastError());
+    }
+    const double* dinv = pc == 1 ? h->dinv.as<double>() : nullptr;
+    const double* zin = pc == 0 ? r : z;
+    const dim3 g = tiles_grid(N);
+    const Mat<double> m = mat64(h);
+    int err = 0;
+    GIS_SPMV_DISPATCH(h->sched.spmv64, P, {
+        hipLaunchKernelGGL((k_setup<P>), g, dim3(kWG), 0, h->stream, m, N, b, x, nullptr, r,
+                           nullptr, h->st, h->red, 0);
+        GIS_HIP(h, hipGetLastError());
+        if (pc == 1)
+            hipLaunchKernelGGL(k_pcg_diag0, g, dim3(kWG), 0, h->stream, N, dinv, r, z, h->st,
+                               h->red);
+        if (pc == 2) (void)vcycle<P>(h, 0, r, z, &err);
+        GIS_COARSEN_DISPATCH(h->sched.coarsen64, CC, {
+            const dim3 gc = super_grid(N, CC);
+            GIS_TRY(run_blocks(h, iteration_budget(h), [&]() -> int {
+                hipLaunchKernelGGL(k_pcg_dir, g, dim3(kWG), 0, h->stream, N, zin, pb0, pb1, x,
+                                   h->st);
+                hipLaunchKernelGGL((k_cg1s<P, CC>), gc, dim3(kWG), 0, h->stream, m, N, pb0, pb1,
+                                   q, h->st, h->red);
+                if (pc == 0)
+                    hipLaunchKernelGGL((k_pcg2<CC, 0>), gc, dim3(kWG), 0, h->stream, N, r, q,
+                                       dinv, z, h->st, h->red);
+                else if (pc == 1)
+                    hipLaunchKernelGGL((k_pcg2<CC, 1>), gc, dim3(kWG), 0, h->stream, N, r, q,
+                                       dinv, z, h->st, h->red);
+                else {
+                    hipLaunchKernelGGL((k_pcg2<CC, 2>), gc, dim3(kWG), 0, h->stream, N, r, q,
+                                       dinv, z, h->st, h->red);
+                    (void)vcycle<P>(h, 0, r, z, &err);
+                }
+                return err ? fail(h, kErrHip, "V-cycle launch") : kOK;
+            }));
+        });
+    });
+    hipLaunchKernelGGL(k_cg_flush, g, dim3(kWG), 0, h->stream, N, x, pb0, pb1, h->st);
+    GIS_HIP(h, hipGetLastError());
+    return kOK;
+}
+
+int amg_levels(gis_context* h, int* n_levels) {
+    GIS_TRY(amg_prepare(h));
+    *n_levels = (int)h->amg->lv.size();
+    return kOK;
+}
+
+int amg_level(gis_context* h, int l, int* n, int* m, int* agg, int* la, int* ua, double* diag,
+              double* lower, double* upper) {
+    GIS_TRY(amg_prepare(h));
+    AmgData& A = *h->amg;
+    if (l < 0 || l >= (int)A.lv.size()) return fail(h, kErrArg, "no such GAMG level");
+    const AmgLevel& L = A.lv[l];
+    gis_context* c = level_ctx(h, l);
+    if (n) *n = L.n;
+    if (m) *m = L.m;
+    auto get = [&](void* dst, const DevMem& src, size_t bytes) -> int {
+        if (dst && bytes)
+            GIS_HIP(h, hipMemcpyAsync(dst, src.p, bytes, hipMemcpyDeviceToHost, h->stream));
+        return kOK;
+    };
+    const size_t bm = (size_t)L.m, bn = (size_t)L.n;
+    GIS_TRY(get(la, c->lower_addr, sizeof(int) * bm));
+    GIS_TRY(get(ua, c->upper_addr, sizeof(int) * bm));
+    GIS_TRY(get(diag, c->diag, sizeof(double) * bn));
+    GIS_TRY(get(lower, c->lower, sizeof(double) * bm));
+    GIS_TRY(get(upper, c->upper, sizeof(double) * bm));
+    if (l + 1 < (int)A.lv.size()) GIS_TRY(get(agg, L.agg, sizeof(int) * bn));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    return kOK;
+}
+
+}  // namespace gis

@@ -6,10 +6,13 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <deque>
 #include <string>
 #include <vector>
 
 #include "gis_device.hpp"
+
+struct gis_context;
 
 namespace gis {
 
@@ -99,6 +102,32 @@ struct BatchInfo {
     void* graph_key = nullptr;
 };
 
+// GAMG hierarchy (preconditioned CG).  Level 0 is the handle's own matrix;
+// every coarser level is a context of its own (pattern + values) on the
+// handle's stream.
+struct AmgLevel {
+    gis_context* ctx = nullptr;   // levels >= 1
+    int n = 0, m = 0, nc = 0;     // cells, faces, cells of the next level
+    // transfer to the next level: cell -> aggregate; aggregate -> its cells and
+    // the faces inside it; next-level face -> fine faces (2 f + flip); all ascending
+    DevMem agg, mptr, mlist, iptr, ilist, fptr, flist;
+    DevMem sm;                    // omega / diag
+    DevMem vecs;                  // z, z2, t, r (4 n)
+};
+
+struct AmgData {
+    bool built = false;           // hierarchy of the current pattern
+    bool values_ok = false;       // coarse values of the current values
+    double omega = 0.8;           // Jacobi damping
+    int nu = 1;                   // sweeps before and after the coarse correction
+    double cscale = 1.8;          // coarse-correction factor
+    std::deque<AmgLevel> lv;
+    DevMem ainv, dense_work;      // dense inverse of the last level (dense_n > 0)
+    int dense_n = 0;
+    void clear();
+    ~AmgData();
+};
+
 }  // namespace gis
 
 struct gis_context {
@@ -131,6 +160,7 @@ struct gis_context {
     // residuals (-128 marks padding); sflag[slice] = 1 where every residual fits
     gis::DevMem sres, sbase, sflag;
     gis::DevMem dinv;                           // per-row helper (1/a_ii or sqrt(1/a_ii))
+    gis::AmgData* amg = nullptr;                // GAMG hierarchy (built on first use)
 
     // workspaces
     gis::DevMem vec;                            // solver vectors
@@ -221,6 +251,13 @@ struct SolveArgs {
     double rel_tol = 0.0;     // > 0: stop also at rel_tol * initial residual
     long long min_steps = 0;  // no stop on convergence before this many iterations
 };
+
+// Preconditioned CG (gis_amg.hip): pc 0 none, 1 diagonal, 2 GAMG.  As
+// device_solve methods 8, 9, 10.
+int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const double* b);
+int amg_levels(gis_context* h, int* n_levels);
+int amg_level(gis_context* h, int l, int* n, int* m, int* agg, int* la, int* ua, double* diag,
+              double* lower, double* upper);
 
 // x (in/out) and b are device vectors owned by the handle's io workspace.
 int device_solve(gis_context* h, const SolveArgs& a, double* d_x, const double* d_b,

@@ -617,6 +617,9 @@ int device_solve(gis_context* h, const SolveArgs& a, double* d_x, const double* 
         case 5:
         case 6: st = solve_mr_family(h, a, d_x, d_b); break;
         case 7: st = solve_richardson(h, a, d_x, d_b); break;
+        case 8:
+        case 9:
+        case 10: st = solve_pcg(h, a, a.method - 8, d_x, d_b); break;
         default: return fail(h, kErrArg, "unknown solver method");
     }
     if (st != kOK) return st;

@@ -326,6 +326,7 @@ int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua
                     "n_cells must be >= 1 and n_cells + 2*n_faces + n_couples < 2^31");
     h->has_pattern = false;
     h->has_values = h->has_val32 = h->has_valT = h->has_val32T = false;
+    if (h->amg) h->amg->clear();
     GIS_HIP(h, h->rowptr.ensure(sizeof(int) * ((size_t)N + 1)));
     GIS_HIP(h, h->col.ensure(sizeof(int) * (size_t)nnz));
     GIS_HIP(h, h->perm.ensure(sizeof(int) * (size_t)nnz));
@@ -451,6 +452,7 @@ static int gather_both(gis_context* h, bool transpose, DevMem& csr, DevMem& sell
 int gather_values(gis_context* h) {
     if (!h->has_pattern) return fail(h, kErrState, "pattern not set");
     GIS_TRY(gather_both<double>(h, false, h->val, h->sval));
+    if (h->amg) h->amg->values_ok = false;
     h->has_values = true;
     h->has_val32 = false;
     h->has_valT = false;

@@ -192,43 +192,6 @@ __global__ __launch_bounds__(kWG) void k_cg_dir(int N, const double* __restrict_
     }
 }
 
-// Split schedule, pass 2: q = A p_k with one gathered vector, (p_k, q) -> alpha_k.
-template <class P, int C>
-__global__ __launch_bounds__(kWG) void k_cg1s(Mat<double> m, int N,
-                                              const double* __restrict__ pb0,
-                                              const double* __restrict__ pb1,
-                                              double* __restrict__ q, KState* st, RedWs red) {
-    if (st->done) return;
-    __shared__ typename P::template Shared<double> sh;
-    __shared__ double sh_y[kTile];
-    __shared__ RedShared shr;
-    const SuperTile<C> sup(N, false);
-    const double* p = ((st->k + 1) & 1) ? pb1 : pb0;
-    double leaf[C][1];
-    double qv[C][4];
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-        leaf[c][0] = 0.0;
-        if (c >= sup.nt) continue;
-        if (c > 0) __syncthreads();
-        const TileRows t(N, sup.tile0 + c);
-        P::run(m, N, t.row0, t.nrows, [&](int j) { return p[j]; }, sh, sh_y);
-        double pv[4];
-        load4(p, t.i0, N, pv);
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-            qv[c][kk] = sh_y[threadIdx.x * kLeaf + kk];
-            if (t.i0 + kk < N) leaf[c][0] = fma(pv[kk], qv[c][kk], leaf[c][0]);
-        }
-    }
-    TileReduce<1, C> tr;
-    tr.begin(red, sup.tile0, sup.nt, leaf, shr);
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-        if (c < sup.nt) store4(q, TileRows(N, sup.tile0 + c).i0, N, qv[c]);
-    tr.end(red, [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
-}
-
 template <int C>
 __global__ __launch_bounds__(kWG) void k_cg2(int N, double* __restrict__ r,
                                              const double* __restrict__ q, KState* st,
@@ -266,24 +229,6 @@ __global__ __launch_bounds__(kWG) void k_cg2(int N, double* __restrict__ r,
         record_norm(st, norm);
         if (stop_after(st, !(norm > st->error))) st->done = 1;
     });
-}
-
-// x += alpha_k p_k for the last iteration (the update cg1 defers)
-__global__ __launch_bounds__(kWG) void k_cg_flush(int N, double* __restrict__ x,
-                                                  const double* __restrict__ pb0,
-                                                  const double* __restrict__ pb1,
-                                                  const KState* st) {
-    const int k = st->k;
-    if (k < 1) return;
-    const double alpha = st->alpha;
-    const double* p = (k & 1) ? pb1 : pb0;
-    StreamRows t(N);
-    double xv[4], pv[4];
-    load4(x, t.i0, N, xv);
-    load4(p, t.i0, N, pv);
-#pragma unroll
-    for (int kk = 0; kk < 4; ++kk) xv[kk] = fma(alpha, pv[kk], xv[kk]);
-    store4(x, t.i0, N, xv);
 }
 
 // ---------------------------------------------------------------------------

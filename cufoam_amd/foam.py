@@ -78,6 +78,9 @@ _TABLE = {
     "cuFoamMRJacobi": ("MRJacobi", True, True),
     "cuFoamJMR": ("JMR", True, True),
     "cuFoamMRS": ("MRS", True, True),
+    # no counterpart in the reference: CG with a preconditioner (controls key
+    # "preconditioner": none / diagonal / GAMG), for symmetric matrices
+    "cuFoamPCG": ("PCG", True, False),
 }
 
 
@@ -104,6 +107,10 @@ class cuFoamSolver:
         if self.convergence not in ("reference", "openfoam"):
             raise FatalError(f"convergence must be reference or openfoam, not "
                              f"{self.convergence}")
+        self.preconditioner = str(controls.get("preconditioner", "none"))
+        if self.preconditioner not in ("none", "diagonal", "GAMG"):
+            raise FatalError(f"preconditioner must be none, diagonal or GAMG, not "
+                             f"{self.preconditioner}")
         self.couple = str(controls.get("interfaces", "ignored"))
         if self.couple not in ("ignored", "coupled"):
             raise FatalError(f"interfaces must be ignored or coupled, not {self.couple}")
@@ -143,7 +150,13 @@ class cuFoamSolver:
             raise FatalError("psi must be a contiguous float64 field")
         self._upload()
         src = np.ascontiguousarray(source, dtype=np.float64)
-        if self.convergence == "openfoam":
+        if self.typeName == "cuFoamPCG":
+            of = self.convergence == "openfoam"
+            perf = self._gis.foam_solve("PCG", psi, src, self.tolerance, self.maxIter,
+                                        rel_tol=self.relTol if of else 0.0,
+                                        min_iter=self.minIter if of else 0,
+                                        preconditioner=self.preconditioner)
+        elif self.convergence == "openfoam":
             perf = self._gis.foam_solve(_TABLE[self.typeName][0], psi, src, self.tolerance,
                                         self.maxIter, rel_tol=self.relTol,
                                         min_iter=self.minIter)
@@ -166,9 +179,11 @@ cuFoamMR = _make("cuFoamMR")
 cuFoamMRJacobi = _make("cuFoamMRJacobi")
 cuFoamJMR = _make("cuFoamJMR")
 cuFoamMRS = _make("cuFoamMRS")
+cuFoamPCG = _make("cuFoamPCG")
 
 _CLASSES = {c.typeName: c for c in (cuFoamCG, cuFoamBiCG, cuFoamJacobi, cuFoamRichardson,
-                                      cuFoamMR, cuFoamMRJacobi, cuFoamJMR, cuFoamMRS)}
+                                      cuFoamMR, cuFoamMRJacobi, cuFoamJMR, cuFoamMRS,
+                                      cuFoamPCG)}
 
 
 def New(name: str, fieldName: str, matrix: lduMatrix, solverControls: dict,

@@ -21,6 +21,7 @@ BREAKDOWN, NOT_CONVERGED, NAN = 10, 11, 12
 
 METHODS = {"CG": 0, "BiCG": 1, "Jacobi": 2, "MR": 3, "MRJacobi": 4, "JMR": 5, "MRS": 6,
            "Richardson": 7}
+PRECONDITIONERS = {"none": 0, "diagonal": 1, "GAMG": 2}
 
 
 class GisError(RuntimeError):
@@ -86,6 +87,12 @@ SIGNATURES = {
     "gis_solve_device": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
     "gis_foam_solve": (_i, [_vp, _i, _vp, _vp, _d, _ll, C.POINTER(_Perf)]),
     "gis_foam_solve_controls": (_i, [_vp, _i, _vp, _vp, _d, _d, _ll, _ll, C.POINTER(_Perf)]),
+    "gis_solve_pcg": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
+    "gis_solve_pcg_device": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
+    "gis_foam_solve_pcg": (_i, [_vp, _i, _vp, _vp, _d, _d, _ll, _ll, C.POINTER(_Perf)]),
+    "gis_set_gamg": (_i, [_vp, _d, _i, _d]),
+    "gis_gamg_levels": (_i, [_vp, _ip]),
+    "gis_gamg_level": (_i, [_vp, _i, _ip, _ip, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gis_solve_multi": (_i, [_vp, _i, _i, _vp, _vp, C.POINTER(_Params), _i, _vp, _vp, _vp]),
     "gis_solve_multi_device": (_i, [_vp, _i, _i, _vp, _vp, C.POINTER(_Params), _i, _vp, _vp,
                                     _vp]),
@@ -366,6 +373,57 @@ class Gis:
         self._check(self._L.gis_synchronize(self._h), "synchronize")
 
     # -- solvers ---------------------------------------------------------------
+    def solve_pcg(self, precond: str, b, x0=None, error: float = 1e-4, max_steps: int = 0,
+                  hist_cap: int | None = None):
+        """Preconditioned CG (precond none / diagonal / GAMG) -> (x, hist, status)."""
+        p = _Params(error, max_steps, 0.0, 30, 1e-10)
+        if hist_cap is None:
+            hist_cap = (max_steps if max_steps > 0 else 10000) + 1
+        hist = np.zeros(max(hist_cap, 1))
+        hl = C.c_int(0)
+        pc = PRECONDITIONERS[precond]
+        if _is_dev(b):
+            import torch
+            x = x0 if x0 is not None else torch.zeros_like(b)
+            st = self._L.gis_solve_pcg_device(self._h, pc, _ptr(x), _ptr(b), C.byref(p),
+                                              _ptr(hist), hist_cap, C.byref(hl))
+        else:
+            b = _host(b, np.float64)
+            x = np.array(x0 if x0 is not None else np.zeros_like(b), dtype=np.float64, copy=True)
+            st = self._L.gis_solve_pcg(self._h, pc, _ptr(x), _ptr(b), C.byref(p), _ptr(hist),
+                                       hist_cap, C.byref(hl))
+        self._check(st, f"solve_pcg({precond})")
+        self.last_iterations = max(hl.value - 1, 0)
+        return x, hist[:min(hl.value, hist_cap)].copy(), st
+
+    def set_gamg(self, omega: float = 0.8, nu: int = 1, cscale: float = 1.8):
+        self._check(self._L.gis_set_gamg(self._h, omega, nu, cscale), "set_gamg")
+
+    def gamg_levels(self) -> int:
+        n = C.c_int(0)
+        self._check(self._L.gis_gamg_levels(self._h, C.byref(n)), "gamg_levels")
+        return n.value
+
+    def gamg_level(self, level: int) -> dict:
+        n, m = C.c_int(0), C.c_int(0)
+        self._check(self._L.gis_gamg_level(self._h, level, C.byref(n), C.byref(m), None, None,
+                                           None, None, None, None), "gamg_level")
+        n, m = n.value, m.value
+        out = {"n": n, "m": m, "agg": np.zeros(max(n, 1), np.int32),
+               "la": np.zeros(max(m, 1), np.int32), "ua": np.zeros(max(m, 1), np.int32),
+               "diag": np.zeros(max(n, 1)), "lower": np.zeros(max(m, 1)),
+               "upper": np.zeros(max(m, 1))}
+        last = level + 1 == self.gamg_levels()
+        self._check(self._L.gis_gamg_level(self._h, level, None, None,
+                                           None if last else _ptr(out["agg"]), _ptr(out["la"]),
+                                           _ptr(out["ua"]), _ptr(out["diag"]),
+                                           _ptr(out["lower"]), _ptr(out["upper"])), "gamg_level")
+        for k in ("la", "ua", "lower", "upper"):
+            out[k] = out[k][:m]
+        out["diag"] = out["diag"][:n]
+        out["agg"] = None if last else out["agg"][:n]
+        return out
+
     def solve(self, method: str, b, x0=None, error: float = 1e-4, max_steps: int = 0,
               omega: float = 0.0, power_steps: int = 30, int_bound: float = 1e-10,
               hist_cap: int | None = None):
@@ -544,13 +602,20 @@ class Gis:
         return x, it, nf, sts
 
     def foam_solve(self, method: str, psi, source, tolerance: float, max_iter: int,
-                   rel_tol: float | None = None, min_iter: int = 0):
+                   rel_tol: float | None = None, min_iter: int = 0,
+                   preconditioner: str | None = None):
         """cuFoam<method>::solve: psi (numpy) is updated in place; returns Performance.
 
         rel_tol / min_iter (OpenFOAM's relTol / minIter) select
-        gis_foam_solve_controls; left at None / 0 the call is gis_foam_solve."""
+        gis_foam_solve_controls; left at None / 0 the call is gis_foam_solve.
+        method "PCG" with a preconditioner calls gis_foam_solve_pcg."""
         perf = _Perf()
-        if rel_tol is None and min_iter == 0:
+        if method == "PCG":
+            st = self._L.gis_foam_solve_pcg(self._h, PRECONDITIONERS[preconditioner or "none"],
+                                            _ptr(psi), _ptr(source), tolerance,
+                                            float(rel_tol or 0.0), int(min_iter), max_iter,
+                                            C.byref(perf))
+        elif rel_tol is None and min_iter == 0:
             st = self._L.gis_foam_solve(self._h, METHODS[method], _ptr(psi), _ptr(source),
                                         tolerance, max_iter, C.byref(perf))
         else:

@@ -205,6 +205,41 @@ GIS_API int gis_set_coupled_values_device(gis_handle h, const double *d_diag,
                                           const double *d_lower, const double *d_upper,
                                           const double *d_coeffs);
 
+/* ---- preconditioned CG (no counterpart in the reference, whose roadmap lists
+ * preconditioners as future work, README.md:71-73).  The loop, stopping rule
+ * (||r||_2 <= error) and history are those of GIS_CG; z = M r, rho = (r, z),
+ * p = z + (rho / rho_prev) p.  precond GIS_PRECOND_NONE gives GIS_CG bit for
+ * bit; GIS_PRECOND_DIAGONAL uses M = D^-1; GIS_PRECOND_GAMG uses one
+ * symmetric V-cycle of an algebraic multigrid hierarchy: two pairwise
+ * agglomeration passes per level (weights |upper| + |lower|), Galerkin
+ * coarse LDU matrices, nu damped-Jacobi sweeps (factor omega) before and
+ * after the coarse correction (scaled by cscale), dense inverse at <= 64
+ * cells.  The hierarchy is built on first use after gis_set_pattern (from
+ * the values then present) and its coarse values are refreshed after every
+ * gis_set_values.  For symmetric positive definite A; GAMG does not take
+ * interface entries. ---- */
+#define GIS_PRECOND_NONE 0
+#define GIS_PRECOND_DIAGONAL 1
+#define GIS_PRECOND_GAMG 2
+GIS_API int gis_solve_pcg(gis_handle h, int precond, double *x, const double *b,
+                          const gis_params *params, double *hist, int hist_cap, int *hist_len);
+GIS_API int gis_solve_pcg_device(gis_handle h, int precond, double *d_x, const double *d_b,
+                                 const gis_params *params, double *hist, int hist_cap,
+                                 int *hist_len);
+/* OpenFOAM's PCG entry with controls, as gis_foam_solve_controls. */
+GIS_API int gis_foam_solve_pcg(gis_handle h, int precond, double *psi, const double *source,
+                               double tolerance, double rel_tol, long long min_iter,
+                               long long max_iter, gis_performance *perf);
+/* GAMG settings (defaults omega 0.8, nu 1, cscale 1.8). */
+GIS_API int gis_set_gamg(gis_handle h, double omega, int nu, double cscale);
+/* The hierarchy (built if needed): level count, then one level's sizes,
+ * aggregation map (cell -> next-level cell; levels below the last) and LDU
+ * matrix (any output pointer may be NULL). */
+GIS_API int gis_gamg_levels(gis_handle h, int *n_levels);
+GIS_API int gis_gamg_level(gis_handle h, int level, int *n_cells, int *n_faces, int *agg,
+                           int *lower_addr, int *upper_addr, double *diag, double *lower,
+                           double *upper);
+
 /* ---- gis_foam_solve with OpenFOAM's remaining solver controls
  * (lduMatrix::solver reads tolerance, relTol, minIter and maxIter from
  * fvSolution; the reference wrappers pass on tolerance and maxIter only,

@@ -322,18 +322,25 @@ class Performance:
 
 
 def foam_solve(method: str, sys, psi, source, tolerance: float, max_iter: int,
-               A: Csr | None = None, rel_tol: float = 0.0,
-               min_iter: int = 0) -> tuple[np.ndarray, Performance]:
+               A: Csr | None = None, rel_tol: float = 0.0, min_iter: int = 0,
+               preconditioner: str = "none") -> tuple[np.ndarray, Performance]:
     """cuFoam<method>::solve on the CPU (cuFoamCG/cuFoamCG.C:45-83 and siblings).
 
     error = tolerance * NormFactor(A, b, x); the solver runs with max_steps = maxIter;
     initialResidual = hist.front(), finalResidual = hist.back(), nIterations = size-1.
     rel_tol / min_iter: OpenFOAM's relTol and minIter, which the reference
-    wrappers do not pass on (0 and 0 reproduce them).
+    wrappers do not pass on (0 and 0 reproduce them).  method "PCG": CG with
+    `preconditioner` (no counterpart in the reference).
     """
     if A is None:
         A = ldu2csr(sys)
     nf = norm_factor(A, source, psi)
+    if method == "PCG":
+        res = pcg(preconditioner, sys, source, psi, tolerance * nf, max_iter, rel_tol=rel_tol,
+                  min_steps=min_iter)
+        perf = Performance("cuFoamPCG", float(res.hist[0]), float(res.hist[-1]),
+                           res.hist_len - 1, res.status)
+        return res.x, perf
     At = None
     if method == "BiCG":
         coupled = getattr(sys, "cpl_row", None) is not None and len(sys.cpl_row)
@@ -343,6 +350,131 @@ def foam_solve(method: str, sys, psi, source, tolerance: float, max_iter: int,
     perf = Performance("cuFoam" + method, float(res.hist[0]), float(res.hist[-1]),
                        res.hist_len - 1, res.status)
     return res.x, perf
+
+
+# ---------------------------------------------------------------------------
+# preconditioned CG (none / diagonal / GAMG), gis_oracle_amg.c
+# ---------------------------------------------------------------------------
+PRECONDITIONERS = {"none": 0, "diagonal": 1, "GAMG": 2}
+GAMG_OMEGA, GAMG_NU, GAMG_CSCALE = 0.8, 1, 1.8
+
+
+def _amg_sigs(L):
+    L.or_pcg.restype = C.c_int
+    L.or_pcg.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
+                         _f64p, C.c_double, C.c_int, C.c_double, C.c_int64, C.c_double, C.c_int,
+                         C.c_double, _f64p, C.c_int, C.POINTER(C.c_int)]
+    L.or_amg_build.restype = C.c_void_p
+    L.or_amg_build.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double,
+                               C.c_int, C.c_double]
+    L.or_amg_free.argtypes = [C.c_void_p]
+    L.or_amg_levels.argtypes = [C.c_void_p]
+    L.or_amg_level_size.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.or_amg_level_get.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 7
+    L.or_amg_dense_inverse.argtypes = [C.c_void_p, _f64p]
+    L.or_amg_vcycle.argtypes = [C.c_void_p, _f64p, _f64p]
+    L.or_pair_pass.restype = C.c_int
+    L.or_pair_pass.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _i32p]
+
+
+def pair_pass(n: int, la, ua, w):
+    """One pairwise agglomeration pass -> (aggregate count, cell -> aggregate)."""
+    L = lib()
+    if not hasattr(L, "_amg_sigs"):
+        _amg_sigs(L)
+        L._amg_sigs = True
+    agg = np.zeros(n, np.int32)
+    la = np.asarray(la, np.int32)
+    nc = L.or_pair_pass(n, la.shape[0], _z(la, np.int32), _z(np.asarray(ua, np.int32), np.int32),
+                        _z(np.asarray(w, np.float64), np.float64), agg)
+    return nc, agg
+
+
+def _z(a, dt):
+    return _c(a if len(a) else np.zeros(1, dt), dt)
+
+
+def pcg(precond: str, sys, b, x0, error: float, max_steps: int, rel_tol: float = 0.0,
+        min_steps: int = 0, omega: float = GAMG_OMEGA, nu: int = GAMG_NU,
+        cscale: float = GAMG_CSCALE, hist_cap: int | None = None) -> SolveResult:
+    """Preconditioned CG on an LduSystem (gis_oracle_amg.c); x0 is copied."""
+    L = lib()
+    if not hasattr(L, "_amg_sigs"):
+        _amg_sigs(L)
+        L._amg_sigs = True
+    x = _c(np.array(x0, dtype=np.float64, copy=True), np.float64)
+    if hist_cap is None:
+        hist_cap = max(max_steps if max_steps > 0 else 100000, min_steps) + 1
+    hist = np.zeros(max(hist_cap, 1))
+    hl = C.c_int(0)
+    st = L.or_pcg(PRECONDITIONERS[precond], sys.n_cells, sys.n_faces,
+                  _z(sys.lower_addr, np.int32), _z(sys.upper_addr, np.int32),
+                  _c(sys.diag, np.float64), _z(sys.lower, np.float64), _z(sys.upper, np.float64),
+                  _c(b, np.float64), x, error, max_steps, rel_tol, min_steps, omega, nu, cscale,
+                  hist, hist_cap, C.byref(hl))
+    if st != 0:
+        raise RuntimeError(f"or_pcg({precond}) failed with status {st}")
+    n = min(hl.value, hist_cap)
+    return SolveResult(x, hist[:n].copy(), hl.value, 0)
+
+
+class Amg:
+    """The GAMG hierarchy of an LduSystem (inspection and V-cycle)."""
+
+    def __init__(self, sys, omega: float = GAMG_OMEGA, nu: int = GAMG_NU,
+                 cscale: float = GAMG_CSCALE):
+        L = lib()
+        if not hasattr(L, "_amg_sigs"):
+            _amg_sigs(L)
+            L._amg_sigs = True
+        self._L = L
+        self.n = sys.n_cells
+        self._h = L.or_amg_build(sys.n_cells, sys.n_faces, _z(sys.lower_addr, np.int32),
+                                 _z(sys.upper_addr, np.int32), _c(sys.diag, np.float64),
+                                 _z(sys.lower, np.float64), _z(sys.upper, np.float64), omega,
+                                 nu, cscale)
+        if not self._h:
+            raise RuntimeError("or_amg_build failed")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.or_amg_free(self._h)
+            self._h = None
+
+    @property
+    def n_levels(self) -> int:
+        return self._L.or_amg_levels(self._h)
+
+    def level(self, l: int) -> dict:
+        n, m, nc = C.c_int(0), C.c_int(0), C.c_int(0)
+        self._L.or_amg_level_size(self._h, l, C.byref(n), C.byref(m), C.byref(nc))
+        n, m, nc = n.value, m.value, nc.value
+        out = {"n": n, "m": m, "nc": nc,
+               "la": np.zeros(max(m, 1), np.int32), "ua": np.zeros(max(m, 1), np.int32),
+               "diag": np.zeros(n), "lower": np.zeros(max(m, 1)), "upper": np.zeros(max(m, 1)),
+               "agg": np.zeros(n, np.int32), "fcode": np.zeros(max(m, 1), np.int32)}
+        self._L.or_amg_level_get(self._h, l, *[out[k].ctypes.data for k in
+                                               ("la", "ua", "diag", "lower", "upper", "agg",
+                                                "fcode")])
+        for k in ("la", "ua", "lower", "upper", "fcode"):
+            out[k] = out[k][:m]
+        if nc == 0:
+            out["agg"] = None
+            out["fcode"] = None
+        return out
+
+    def dense_inverse(self):
+        n = self.level(self.n_levels - 1)["n"]
+        out = np.zeros(n * n)
+        if self._L.or_amg_dense_inverse(self._h, out) != 0:
+            return None
+        return out.reshape(n, n)
+
+    def vcycle(self, r):
+        z = np.zeros(self.n)
+        self._L.or_amg_vcycle(self._h, _c(r, np.float64), z)
+        return z
 
 
 def mixed_cg(A: Csr, b, x0, rel_tol: float = 1e-10, inner_rel_tol: float = 1e-4,
