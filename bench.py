@@ -312,7 +312,7 @@ def plumbing_32():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="mesh edge (256 = the named workload)")
     ap.add_argument("--mode", type=int, default=0, help="mixed mode: 0 reliable updates, 1 restart")
@@ -334,6 +334,8 @@ def main():
                     help="skip the 128^3 OpenFOAM drop-in wrapper timings")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="skip the 64-system batched CG throughput report")
+    ap.add_argument("--no-pcg", dest="pcg", action="store_false",
+                    help="skip the GAMG-preconditioned CG solve of the same system")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -421,6 +423,35 @@ def main():
     _, hist_d, st_d = res_d[-1]
     it_d = len(hist_d) - 1
     log(f"[bench] fp64: {t_fp64:.3f}s/solve iters={it_d} status={st_d}")
+
+    # ---- GAMG-preconditioned CG on the same system (FP64 throughout) ---------
+    pcg = None
+    if args.pcg:
+        t0 = time.perf_counter()
+        levels = g.gamg_levels()               # builds the hierarchy (once per mesh)
+        t_build = time.perf_counter() - t0
+        sizes = [g.gamg_level(l)["n"] for l in range(levels)]
+        for _ in range(W):
+            g.solve_pcg("GAMG", b, torch.zeros_like(b), error=1e-10 * bnorm,
+                        max_steps=max_iter, hist_cap=max_iter + 1)
+        torch.cuda.synchronize()
+        xp = [torch.zeros_like(b) for _ in range(K)]
+        step_p = []
+        for i in range(K):
+            ts = time.perf_counter()
+            _, hist_p, st_p = g.solve_pcg("GAMG", b, xp[i], error=1e-10 * bnorm,
+                                          max_steps=max_iter, hist_cap=max_iter + 1)
+            step_p.append(time.perf_counter() - ts)
+        t_pcg = float(np.median(step_p))
+        rp = torch.empty_like(b)
+        g.spmv(xp[-1], rp)
+        it_p = len(hist_p) - 1
+        pcg = {"preconditioner": "GAMG", "levels": levels, "level_cells": sizes,
+               "hierarchy_build_s": round(t_build, 3), "solve_s": round(t_pcg, 5),
+               "iterations": it_p, "iteration_ms": round(t_pcg / max(it_p, 1) * 1e3, 3),
+               "rel_residual": float(torch.linalg.vector_norm(b - rp)) / bnorm,
+               "status": st_p}
+        log(f"[bench] pcg GAMG: {pcg}")
 
     # ---- per-kernel breakdown: one extra solve of each with every slot timed ----
     slots = ("iter_spmv", "iter_update", "fp64_update", "iter_direction")
@@ -569,6 +600,10 @@ def main():
         out["plumbing_32"] = plumb
     if batched is not None:
         out["batched_cg_64_systems"] = batched
+    if pcg is not None:
+        pcg["speedup_vs_fp64_cg"] = round(t_fp64 / pcg["solve_s"], 3)
+        pcg["speedup_vs_mixed"] = round(t_mixed / pcg["solve_s"], 3)
+        out["pcg_gamg_to_1e-10"] = pcg
         out["spmm_bicg_3_rhs"] = multi
     print(json.dumps(out), flush=True)
 
