@@ -34,7 +34,7 @@ namespace {
 
 constexpr int kBlk = 256;
 constexpr int kDense = 64;        // the last level is inverted densely at or below this size
-constexpr int kMaxLevels = 32;
+constexpr int kAmgLevels = 32;
 
 inline dim3 grid_of(long long n) { return dim3((unsigned)((n + kBlk - 1) / kBlk)); }
 
@@ -291,25 +291,49 @@ __global__ __launch_bounds__(kWG) void k_vc_pre(int N, const double* __restrict_
     store4(z, t.i0, N, zv);
 }
 
-// zn = fma(sm, r - A z, z)
-template <class P>
-__global__ __launch_bounds__(kWG) void k_vc_smooth(Mat<double> m, int N,
-                                                   const double* __restrict__ sm,
-                                                   const double* __restrict__ r,
-                                                   const double* __restrict__ z,
-                                                   double* __restrict__ zn, const KState* st) {
+// One damped-Jacobi sweep zn = fma(sm, r - A x, x).  PROLONG: x is the
+// prolongated iterate fma(cscale, zc[agg], z), formed on the fly (also at the
+// gathered columns), which is the prolongation pass and the first sweep after
+// it in one.  RED (finest level): rho = (r, zn), beta = rho / rho_prev.
+template <class P, bool PROLONG, bool RED>
+__global__ __launch_bounds__(kWG) void k_vc_sweep(Mat<double> m, int N,
+                                                  const double* __restrict__ sm,
+                                                  const double* __restrict__ r,
+                                                  const double* __restrict__ z, double cscale,
+                                                  const int* __restrict__ agg,
+                                                  const double* __restrict__ zc,
+                                                  double* __restrict__ zn, KState* st,
+                                                  RedWs red) {
     if (st->done) return;
     __shared__ typename P::template Shared<double> sh;
     __shared__ double sh_y[kTile];
+    __shared__ RedShared shr;
     TileRows t(N);
-    P::run(m, N, t.row0, t.nrows, [&](int c) { return z[c]; }, sh, sh_y);
+    if constexpr (PROLONG) {
+        P::run(m, N, t.row0, t.nrows, [&](int c) { return fma(cscale, zc[agg[c]], z[c]); }, sh,
+               sh_y);
+    } else {
+        P::run(m, N, t.row0, t.nrows, [&](int c) { return z[c]; }, sh, sh_y);
+    }
     double sv[4], rv[4], zv[4];
     load4(sm, t.i0, N, sv);
     load4(r, t.i0, N, rv);
     load4(z, t.i0, N, zv);
+    double leaf[1] = {0.0};
 #pragma unroll
-    for (int k = 0; k < 4; ++k) zv[k] = fma(sv[k], rv[k] - sh_y[threadIdx.x * kLeaf + k], zv[k]);
+    for (int k = 0; k < 4; ++k) {
+        if (PROLONG && t.i0 + k < N) zv[k] = fma(cscale, zc[agg[t.i0 + k]], zv[k]);
+        zv[k] = fma(sv[k], rv[k] - sh_y[threadIdx.x * kLeaf + k], zv[k]);
+        if (RED && t.i0 + k < N) leaf[0] = fma(rv[k], zv[k], leaf[0]);
+    }
     store4(zn, t.i0, N, zv);
+    if constexpr (RED) {
+        finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+            st->rho1 = st->rho0;
+            st->rho0 = v[0];
+            st->beta = st->rho0 / st->rho1;
+        });
+    }
 }
 
 // t = r - A z
@@ -330,11 +354,14 @@ __global__ __launch_bounds__(kWG) void k_vc_resid(Mat<double> m, int N,
     store4(tr, t.i0, N, tv);
 }
 
-// rc[C] = sum of t over C's cells, ascending
+// rc[C] = sum of t over C's cells, ascending, and the next level's first
+// sweep from zero, zc0 = smc .* rc
 __global__ __launch_bounds__(kBlk) void k_vc_restrict(int nc, const int* __restrict__ mptr,
                                                       const int* __restrict__ mlist,
                                                       const double* __restrict__ t,
+                                                      const double* __restrict__ smc,
                                                       double* __restrict__ rc,
+                                                      double* __restrict__ zc0,
                                                       const KState* st) {
     if (st->done) return;
     const int c = blockIdx.x * kBlk + threadIdx.x;
@@ -342,21 +369,111 @@ __global__ __launch_bounds__(kBlk) void k_vc_restrict(int nc, const int* __restr
     double s = 0.0;
     for (int p = mptr[c]; p < mptr[c + 1]; ++p) s = s + t[mlist[p]];
     rc[c] = s;
+    zc0[c] = smc[c] * s;
 }
 
-// z = fma(cscale, zc[agg], z)
-__global__ __launch_bounds__(kWG) void k_vc_prolong(int N, double cscale,
-                                                    const int* __restrict__ agg,
-                                                    const double* __restrict__ zc,
-                                                    double* __restrict__ z, const KState* st) {
+// The levels from the first one of at most kTail cells down to the last, as
+// one workgroup: the same steps and arithmetic as the per-level kernels (CSR
+// rows summed in storage order), separated by workgroup barriers.
+struct TailLevel {
+    int n;
+    const int* rowptr;
+    const int* col;
+    const double* val;
+    const double* sm;
+    const int* agg;     // to the next tail level
+    const int* mptr;
+    const int* mlist;
+    double* r;
+    double* z;
+    double* z2;
+    double* t;
+};
+
+constexpr int kTail = 4096;
+constexpr int kTailThreads = 1024;
+
+__device__ __forceinline__ double csr_row(const TailLevel& L, int i, const double* x) {
+    double s = 0.0;
+    for (int j = L.rowptr[i]; j < L.rowptr[i + 1]; ++j) s = s + L.val[j] * x[L.col[j]];
+    return s;
+}
+
+// On entry lv[0].r holds the rhs and lv[0].z its first sweep from zero; on
+// exit lv[0].z holds the cycle's result.
+__global__ __launch_bounds__(kTailThreads) void k_vc_tail(const TailLevel* __restrict__ lv,
+                                                          int nl, int nu, double cscale,
+                                                          const double* __restrict__ ainv,
+                                                          int dense_n, const KState* st) {
     if (st->done) return;
-    StreamRows t(N);
-    double zv[4];
-    load4(z, t.i0, N, zv);
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (t.i0 + k < N) zv[k] = fma(cscale, zc[agg[t.i0 + k]], zv[k]);
-    store4(z, t.i0, N, zv);
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    double* cur[kAmgLevels];
+    for (int l = 0; l + 1 < nl; ++l) {
+        const TailLevel& L = lv[l];
+        double* z = L.z;
+        double* z2 = L.z2;
+        for (int s = 1; s < nu; ++s) {
+            for (int i = tid; i < L.n; i += nt)
+                z2[i] = fma(L.sm[i], L.r[i] - csr_row(L, i, z), z[i]);
+            __syncthreads();
+            double* tmp = z;
+            z = z2;
+            z2 = tmp;
+        }
+        for (int i = tid; i < L.n; i += nt) L.t[i] = L.r[i] - csr_row(L, i, z);
+        __syncthreads();
+        const TailLevel& C = lv[l + 1];
+        for (int c = tid; c < C.n; c += nt) {
+            double s = 0.0;
+            for (int p = L.mptr[c]; p < L.mptr[c + 1]; ++p) s = s + L.t[L.mlist[p]];
+            C.r[c] = s;
+            C.z[c] = C.sm[c] * s;
+        }
+        __syncthreads();
+        cur[l] = z;
+    }
+    const TailLevel& E = lv[nl - 1];
+    if (dense_n) {
+        for (int i = tid; i < E.n; i += nt) {
+            double s = 0.0;
+            for (int j = 0; j < E.n; ++j) s = s + ainv[i * E.n + j] * E.r[j];
+            E.z[i] = s;
+        }
+        __syncthreads();
+    }
+    const double* zc = E.z;
+    for (int l = nl - 2; l >= 0; --l) {
+        const TailLevel& L = lv[l];
+        double* z = cur[l];
+        double* z2 = (z == L.z) ? L.z2 : L.z;
+        for (int i = tid; i < L.n; i += nt) {
+            double y = 0.0;
+            for (int j = L.rowptr[i]; j < L.rowptr[i + 1]; ++j) {
+                const int c = L.col[j];
+                y = y + L.val[j] * fma(cscale, zc[L.agg[c]], z[c]);
+            }
+            const double x = fma(cscale, zc[L.agg[i]], z[i]);
+            z2[i] = fma(L.sm[i], L.r[i] - y, x);
+        }
+        __syncthreads();
+        double* tmp = z;
+        z = z2;
+        z2 = tmp;
+        for (int s = 1; s < nu; ++s) {
+            for (int i = tid; i < L.n; i += nt)
+                z2[i] = fma(L.sm[i], L.r[i] - csr_row(L, i, z), z[i]);
+            __syncthreads();
+            tmp = z;
+            z = z2;
+            z2 = tmp;
+        }
+        if (z != L.z) {
+            for (int i = tid; i < L.n; i += nt) L.z[i] = z[i];
+            __syncthreads();
+        }
+        zc = L.z;
+    }
 }
 
 // z = Ainv r on the last level (n <= kDense)
@@ -369,39 +486,6 @@ __global__ __launch_bounds__(kDense) void k_vc_dense(int n, const double* __rest
     double s = 0.0;
     for (int j = 0; j < n; ++j) s = s + ainv[i * n + j] * r[j];
     z[i] = s;
-}
-
-// Last sweep on the finest level: zn = fma(sm, r - A z, z), rho = (r, zn),
-// beta = rho / rho_prev.
-template <class P>
-__global__ __launch_bounds__(kWG) void k_pcg_post(Mat<double> m, int N,
-                                                  const double* __restrict__ sm,
-                                                  const double* __restrict__ r,
-                                                  const double* __restrict__ z,
-                                                  double* __restrict__ zn, KState* st,
-                                                  RedWs red) {
-    if (st->done) return;
-    __shared__ typename P::template Shared<double> sh;
-    __shared__ double sh_y[kTile];
-    __shared__ RedShared shr;
-    TileRows t(N);
-    P::run(m, N, t.row0, t.nrows, [&](int c) { return z[c]; }, sh, sh_y);
-    double sv[4], rv[4], zv[4];
-    load4(sm, t.i0, N, sv);
-    load4(r, t.i0, N, rv);
-    load4(z, t.i0, N, zv);
-    double leaf[1] = {0.0};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        zv[k] = fma(sv[k], rv[k] - sh_y[threadIdx.x * kLeaf + k], zv[k]);
-        if (t.i0 + k < N) leaf[0] = fma(rv[k], zv[k], leaf[0]);
-    }
-    store4(zn, t.i0, N, zv);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
-        st->rho1 = st->rho0;
-        st->rho0 = v[0];
-        st->beta = st->rho0 / st->rho1;
-    });
 }
 
 // rho = (r, z), beta = rho / rho_prev (after a one-level V-cycle)
@@ -477,6 +561,7 @@ __global__ __launch_bounds__(kWG) void k_pcg_dir(int N, const double* __restrict
 
 // r = fma(-alpha, q, r); ||r||, history, stop test.  PC 0: rho = (r, r);
 // PC 1: z = dinv r, rho = (r, z); PC 2: rho is left to the V-cycle.
+// PC 2 also writes the V-cycle's first sweep from zero, z = sm .* r (dinv = sm).
 template <int C, int PC>
 __global__ __launch_bounds__(kWG) void k_pcg2(int N, double* __restrict__ r,
                                               const double* __restrict__ q,
@@ -498,11 +583,11 @@ __global__ __launch_bounds__(kWG) void k_pcg2(int N, double* __restrict__ r,
         double qv[4], dv[4];
         load4(r, t.i0, N, rv[c]);
         load4(q, t.i0, N, qv);
-        if (PC == 1) load4(dinv, t.i0, N, dv);
+        if (PC != 0) load4(dinv, t.i0, N, dv);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             rv[c][kk] = fma(-alpha, qv[kk], rv[c][kk]);
-            if (PC == 1) zv[c][kk] = dv[kk] * rv[c][kk];
+            if (PC != 0) zv[c][kk] = dv[kk] * rv[c][kk];
             if (t.i0 + kk < N) {
                 leaf[c][0] = fma(rv[c][kk], rv[c][kk], leaf[c][0]);
                 if (PC == 1) leaf[c][NV - 1] = fma(rv[c][kk], zv[c][kk], leaf[c][NV - 1]);
@@ -516,7 +601,7 @@ __global__ __launch_bounds__(kWG) void k_pcg2(int N, double* __restrict__ r,
         if (c >= sup.nt) continue;
         const int i0 = TileRows(N, sup.tile0 + c).i0;
         store4(r, i0, N, rv[c]);
-        if (PC == 1) store4(z, i0, N, zv[c]);
+        if (PC != 0) store4(z, i0, N, zv[c]);
     }
     tr.end(red, [&](const double (&v)[NV]) {
         const double norm = sqrt(v[0]);
@@ -582,7 +667,7 @@ int amg_build(gis_context* h) {
     A.lv[0].n = fine.n;
     A.lv[0].m = fine.m;
     HostLevel cur = std::move(fine);
-    while (cur.n > kDense && (int)A.lv.size() < kMaxLevels) {
+    while (cur.n > kDense && (int)A.lv.size() < kAmgLevels) {
         std::vector<double> w(cur.m);
         for (int f = 0; f < cur.m; ++f) w[f] = std::fabs(cur.upper[f]) + std::fabs(cur.lower[f]);
         std::vector<int> agg1, fc1, la1, ua1, agg2, fcode, cla, cua;
@@ -698,6 +783,32 @@ int amg_values(gis_context* h) {
                            A.dense_work.as<double>(), A.ainv.as<double>());
         GIS_HIP(h, hipGetLastError());
     }
+    // the coarse tail (levels of at most kTail cells below the finest) as one workgroup
+    A.tail_start = 0;
+    A.tail_levels = 0;
+    for (size_t l = 1; l < A.lv.size(); ++l)
+        if (A.lv[l].n <= kTail) {
+            A.tail_start = (int)l;
+            break;
+        }
+    if (A.tail_start) {
+        std::vector<TailLevel> meta;
+        for (size_t l = A.tail_start; l < A.lv.size(); ++l) {
+            AmgLevel& L = A.lv[l];
+            gis_context* c = level_ctx(h, (int)l);
+            double* v = L.vecs.as<double>();
+            const size_t n = (size_t)L.n;
+            const bool last = l + 1 == A.lv.size();
+            meta.push_back(TailLevel{L.n, c->rowptr.as<int>(), c->col.as<int>(),
+                                     c->val.as<double>(), L.sm.as<double>(),
+                                     last ? nullptr : L.agg.as<int>(),
+                                     last ? nullptr : L.mptr.as<int>(),
+                                     last ? nullptr : L.mlist.as<int>(), v + 3 * n, v, v + n,
+                                     v + 2 * n});
+        }
+        A.tail_levels = (int)meta.size();
+        GIS_TRY(upload_vec(h, A.tail, meta));
+    }
     GIS_HIP(h, hipStreamSynchronize(h->stream));
     A.values_ok = true;
     return kOK;
@@ -716,57 +827,77 @@ int amg_prepare(gis_context* h) {
     return kOK;
 }
 
-// V-cycle of level l on rhs r; returns the buffer holding z.  On the finest
-// level the last sweep goes to `out` together with rho = (r, z).
+// V-cycle of level l on rhs r; returns the buffer holding z.  On entry L.z
+// holds the first sweep from zero, sm .* r (written by whoever formed r).  On
+// the finest level the last sweep goes to `out` together with rho = (r, z).
 template <class P>
-const double* vcycle(gis_context* h, int l, const double* r, double* out, int* err) {
+const double* vcycle(gis_context* h, int l, const double* r, double* out) {
     AmgData& A = *h->amg;
     AmgLevel& L = A.lv[l];
     gis_context* c = level_ctx(h, l);
     const int N = L.n;
-    double* z = L.vecs.as<double>();
-    double* z2 = z + N;
-    double* t = z + 2 * (size_t)N;
+    double* za = L.vecs.as<double>();
+    double* zb = za + N;
+    double* t = za + 2 * (size_t)N;
     const dim3 g = tiles_grid(N);
     const Mat<double> m = mat64(c);
+    const double* sm = L.sm.as<double>();
     if (l + 1 == (int)A.lv.size()) {
-        if (l == 0) z = out;
-        if (A.dense_n)
-            hipLaunchKernelGGL(k_vc_dense, dim3(1), dim3(kDense), 0, h->stream, N,
-                               A.ainv.as<double>(), r, z, h->st);
-        else
-            hipLaunchKernelGGL(k_vc_pre, g, dim3(kWG), 0, h->stream, N, L.sm.as<double>(), r, z,
-                               h->st);
-        if (l == 0)   // a one-level hierarchy: rho = (r, z) by a dot pass
-            hipLaunchKernelGGL(k_pcg_rho, g, dim3(kWG), 0, h->stream, N, r, z, h->st, h->red);
-        return z;
-    }
-    AmgLevel& C = A.lv[l + 1];
-    double* rc = C.vecs.as<double>() + 3 * (size_t)C.n;
-    hipLaunchKernelGGL(k_vc_pre, g, dim3(kWG), 0, h->stream, N, L.sm.as<double>(), r, z, h->st);
-    for (int s = 1; s < A.nu; ++s) {
-        hipLaunchKernelGGL((k_vc_smooth<P>), g, dim3(kWG), 0, h->stream, m, N, L.sm.as<double>(),
-                           r, z, z2, h->st);
-        std::swap(z, z2);
-    }
-    hipLaunchKernelGGL((k_vc_resid<P>), g, dim3(kWG), 0, h->stream, m, N, r, z, t, h->st);
-    hipLaunchKernelGGL(k_vc_restrict, grid_of(C.n), dim3(kBlk), 0, h->stream, C.n,
-                       L.mptr.as<int>(), L.mlist.as<int>(), t, rc, h->st);
-    const double* zc = vcycle<P>(h, l + 1, rc, nullptr, err);
-    hipLaunchKernelGGL(k_vc_prolong, g, dim3(kWG), 0, h->stream, N, A.cscale, L.agg.as<int>(), zc,
-                       z, h->st);
-    for (int s = 0; s < A.nu; ++s) {
-        if (l == 0 && s + 1 == A.nu) {
-            hipLaunchKernelGGL((k_pcg_post<P>), g, dim3(kWG), 0, h->stream, m, N,
-                               L.sm.as<double>(), r, z, out, h->st, h->red);
+        if (l == 0) {   // a one-level hierarchy: z into out, then rho = (r, z)
+            if (A.dense_n)
+                hipLaunchKernelGGL(k_vc_dense, dim3(1), dim3(kDense), 0, h->stream, N,
+                                   A.ainv.as<double>(), r, out, h->st);
+            else
+                hipLaunchKernelGGL(k_vc_pre, g, dim3(kWG), 0, h->stream, N, sm, r, out, h->st);
+            hipLaunchKernelGGL(k_pcg_rho, g, dim3(kWG), 0, h->stream, N, r, out, h->st, h->red);
             return out;
         }
-        hipLaunchKernelGGL((k_vc_smooth<P>), g, dim3(kWG), 0, h->stream, m, N, L.sm.as<double>(),
-                           r, z, z2, h->st);
-        std::swap(z, z2);
+        if (A.dense_n)
+            hipLaunchKernelGGL(k_vc_dense, dim3(1), dim3(kDense), 0, h->stream, N,
+                               A.ainv.as<double>(), r, za, h->st);
+        return za;
     }
-    if (hipGetLastError() != hipSuccess) *err = 1;
-    return z;
+    for (int s = 1; s < A.nu; ++s) {
+        hipLaunchKernelGGL((k_vc_sweep<P, false, false>), g, dim3(kWG), 0, h->stream, m, N, sm,
+                           r, za, 0.0, nullptr, nullptr, zb, h->st, h->red);
+        std::swap(za, zb);
+    }
+    hipLaunchKernelGGL((k_vc_resid<P>), g, dim3(kWG), 0, h->stream, m, N, r, za, t, h->st);
+    AmgLevel& C = A.lv[l + 1];
+    double* zc0 = C.vecs.as<double>();
+    double* rc = zc0 + 3 * (size_t)C.n;
+    hipLaunchKernelGGL(k_vc_restrict, grid_of(C.n), dim3(kBlk), 0, h->stream, C.n,
+                       L.mptr.as<int>(), L.mlist.as<int>(), t, C.sm.as<double>(), rc, zc0,
+                       h->st);
+    const double* zc;
+    if (l + 1 == A.tail_start) {
+        hipLaunchKernelGGL(k_vc_tail, dim3(1), dim3(kTailThreads), 0, h->stream,
+                           A.tail.as<TailLevel>(), A.tail_levels, A.nu, A.cscale,
+                           A.dense_n ? A.ainv.as<double>() : nullptr, A.dense_n, h->st);
+        zc = zc0;
+    } else {
+        zc = vcycle<P>(h, l + 1, rc, nullptr);
+    }
+    const int* agg = L.agg.as<int>();
+    for (int s = 0; s < A.nu; ++s) {
+        const bool red = (l == 0 && s + 1 == A.nu);
+        double* dst = red ? out : zb;
+        if (s == 0 && red)
+            hipLaunchKernelGGL((k_vc_sweep<P, true, true>), g, dim3(kWG), 0, h->stream, m, N, sm,
+                               r, za, A.cscale, agg, zc, dst, h->st, h->red);
+        else if (s == 0)
+            hipLaunchKernelGGL((k_vc_sweep<P, true, false>), g, dim3(kWG), 0, h->stream, m, N,
+                               sm, r, za, A.cscale, agg, zc, dst, h->st, h->red);
+        else if (red)
+            hipLaunchKernelGGL((k_vc_sweep<P, false, true>), g, dim3(kWG), 0, h->stream, m, N,
+                               sm, r, za, 0.0, nullptr, nullptr, dst, h->st, h->red);
+        else
+            hipLaunchKernelGGL((k_vc_sweep<P, false, false>), g, dim3(kWG), 0, h->stream, m, N,
+                               sm, r, za, 0.0, nullptr, nullptr, dst, h->st, h->red);
+        zb = za;
+        za = dst;
+    }
+    return za;
 }
 
 }  // namespace
@@ -778,7 +909,9 @@ void AmgData::clear() {
     lv.clear();
     ainv.release();
     dense_work.release();
+    tail.release();
     dense_n = 0;
+    tail_start = tail_levels = 0;
     built = false;
     values_ok = false;
 }
@@ -799,11 +932,14 @@ int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const doubl
                            h->diag.as<double>(), h->dinv.as<double>());
         GIS_HIP(h, hipGetLastError());
     }
-    const double* dinv = pc == 1 ? h->dinv.as<double>() : nullptr;
+    // PC 1: dinv = 1 / diag; PC 2: the finest level's sm (k_pcg2 writes the
+    // V-cycle's first sweep sm .* r into z0)
+    const double* dinv = pc == 1 ? h->dinv.as<double>()
+                                 : (pc == 2 ? h->amg->lv[0].sm.as<double>() : nullptr);
+    double* z0 = pc == 2 ? h->amg->lv[0].vecs.as<double>() : z;
     const double* zin = pc == 0 ? r : z;
     const dim3 g = tiles_grid(N);
     const Mat<double> m = mat64(h);
-    int err = 0;
     GIS_SPMV_DISPATCH(h->sched.spmv64, P, {
         hipLaunchKernelGGL((k_setup<P>), g, dim3(kWG), 0, h->stream, m, N, b, x, nullptr, r,
                            nullptr, h->st, h->red, 0);
@@ -811,7 +947,11 @@ int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const doubl
         if (pc == 1)
             hipLaunchKernelGGL(k_pcg_diag0, g, dim3(kWG), 0, h->stream, N, dinv, r, z, h->st,
                                h->red);
-        if (pc == 2) (void)vcycle<P>(h, 0, r, z, &err);
+        if (pc == 2) {
+            hipLaunchKernelGGL(k_vc_pre, g, dim3(kWG), 0, h->stream, N, dinv, r, z0, h->st);
+            (void)vcycle<P>(h, 0, r, z);
+        }
+        GIS_HIP(h, hipGetLastError());
         GIS_COARSEN_DISPATCH(h->sched.coarsen64, CC, {
             const dim3 gc = super_grid(N, CC);
             GIS_TRY(run_blocks(h, iteration_budget(h), [&]() -> int {
@@ -827,11 +967,11 @@ int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const doubl
                                        dinv, z, h->st, h->red);
                 else {
                     hipLaunchKernelGGL((k_pcg2<CC, 2>), gc, dim3(kWG), 0, h->stream, N, r, q,
-                                       dinv, z, h->st, h->red);
-                    (void)vcycle<P>(h, 0, r, z, &err);
+                                       dinv, z0, h->st, h->red);
+                    (void)vcycle<P>(h, 0, r, z);
                 }
-                return err ? fail(h, kErrHip, "V-cycle launch") : kOK;
-            }));
+                return kOK;
+            }, pc == 2));
         });
     });
     hipLaunchKernelGGL(k_cg_flush, g, dim3(kWG), 0, h->stream, N, x, pb0, pb1, h->st);

@@ -124,6 +124,9 @@ struct AmgData {
     std::deque<AmgLevel> lv;
     DevMem ainv, dense_work;      // dense inverse of the last level (dense_n > 0)
     int dense_n = 0;
+    DevMem tail;                  // per-level pointers of the single-workgroup tail
+    int tail_start = 0;           // first tail level (0: none)
+    int tail_levels = 0;
     void clear();
     ~AmgData();
 };

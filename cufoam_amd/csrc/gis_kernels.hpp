@@ -321,9 +321,14 @@ inline long long iteration_budget(const gis_context* h) {
                                                          : h->st_host->min_steps;
 }
 
-// Enqueue iterations in blocks until the device reports done.
+// Enqueue iterations in blocks until the device reports done.  predict:
+// iterations are costly, so a block is cut to the iterations that the mean
+// contraction so far says are left (plus 10%): fewer launches run after
+// convergence.  Block sizes never change results (launches after `done` are
+// no-ops).
 template <class Iter>
-int run_blocks(gis_context* h, long long max_steps, Iter&& launch_iteration) {
+int run_blocks(gis_context* h, long long max_steps, Iter&& launch_iteration,
+               bool predict = false) {
     long long launched = 0;
     int block = 4;
     for (;;) {
@@ -333,6 +338,13 @@ int run_blocks(gis_context* h, long long max_steps, Iter&& launch_iteration) {
         GIS_TRY(read_state(h));
         if (h->st_host->done || launched >= max_steps) break;
         block = block * 2 < h->sched.block_iters ? block * 2 : h->sched.block_iters;
+        const KState& s = *h->st_host;
+        if (predict && s.k >= 2 && s.norm > s.error && s.error > 0.0 && s.norm0 > s.norm &&
+            s.k >= s.min_steps) {
+            const double rate = std::log(s.norm / s.norm0) / s.k;
+            const double left = std::log(s.error / s.norm) / rate;
+            if (left > 0.0 && left * 1.1 + 1.0 < block) block = (int)(left * 1.1) + 1;
+        }
         if (block < 1) block = 1;
     }
     return kOK;
