@@ -291,17 +291,13 @@ __global__ __launch_bounds__(kWG) void k_vc_pre(int N, const double* __restrict_
     store4(z, t.i0, N, zv);
 }
 
-// One damped-Jacobi sweep zn = fma(sm, r - A x, x).  PROLONG: x is the
-// prolongated iterate fma(cscale, zc[agg], z), formed on the fly (also at the
-// gathered columns), which is the prolongation pass and the first sweep after
-// it in one.  RED (finest level): rho = (r, zn), beta = rho / rho_prev.
-template <class P, bool PROLONG, bool RED>
+// One damped-Jacobi sweep zn = fma(sm, r - A z, z).  RED (finest level):
+// rho = (r, zn), beta = rho / rho_prev.
+template <class P, bool RED>
 __global__ __launch_bounds__(kWG) void k_vc_sweep(Mat<double> m, int N,
                                                   const double* __restrict__ sm,
                                                   const double* __restrict__ r,
-                                                  const double* __restrict__ z, double cscale,
-                                                  const int* __restrict__ agg,
-                                                  const double* __restrict__ zc,
+                                                  const double* __restrict__ z,
                                                   double* __restrict__ zn, KState* st,
                                                   RedWs red) {
     if (st->done) return;
@@ -309,12 +305,7 @@ __global__ __launch_bounds__(kWG) void k_vc_sweep(Mat<double> m, int N,
     __shared__ double sh_y[kTile];
     __shared__ RedShared shr;
     TileRows t(N);
-    if constexpr (PROLONG) {
-        P::run(m, N, t.row0, t.nrows, [&](int c) { return fma(cscale, zc[agg[c]], z[c]); }, sh,
-               sh_y);
-    } else {
-        P::run(m, N, t.row0, t.nrows, [&](int c) { return z[c]; }, sh, sh_y);
-    }
+    P::run(m, N, t.row0, t.nrows, [&](int c) { return z[c]; }, sh, sh_y);
     double sv[4], rv[4], zv[4];
     load4(sm, t.i0, N, sv);
     load4(r, t.i0, N, rv);
@@ -322,7 +313,6 @@ __global__ __launch_bounds__(kWG) void k_vc_sweep(Mat<double> m, int N,
     double leaf[1] = {0.0};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (PROLONG && t.i0 + k < N) zv[k] = fma(cscale, zc[agg[t.i0 + k]], zv[k]);
         zv[k] = fma(sv[k], rv[k] - sh_y[threadIdx.x * kLeaf + k], zv[k]);
         if (RED && t.i0 + k < N) leaf[0] = fma(rv[k], zv[k], leaf[0]);
     }
@@ -372,6 +362,22 @@ __global__ __launch_bounds__(kBlk) void k_vc_restrict(int nc, const int* __restr
     zc0[c] = smc[c] * s;
 }
 
+// z = fma(cscale, zc[agg], z)   (a separate pass: folding it into the next
+// sweep's gathers costs two dependent loads per nonzero, measured slower)
+__global__ __launch_bounds__(kWG) void k_vc_prolong(int N, double cscale,
+                                                    const int* __restrict__ agg,
+                                                    const double* __restrict__ zc,
+                                                    double* __restrict__ z, const KState* st) {
+    if (st->done) return;
+    StreamRows t(N);
+    double zv[4];
+    load4(z, t.i0, N, zv);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (t.i0 + k < N) zv[k] = fma(cscale, zc[agg[t.i0 + k]], zv[k]);
+    store4(z, t.i0, N, zv);
+}
+
 // The levels from the first one of at most kTail cells down to the last, as
 // one workgroup: the same steps and arithmetic as the per-level kernels (CSR
 // rows summed in storage order), separated by workgroup barriers.
@@ -390,7 +396,7 @@ struct TailLevel {
     double* t;
 };
 
-constexpr int kTail = 4096;
+constexpr int kTail = 256;        // one workgroup runs a level in ~10 us per step: only tiny levels
 constexpr int kTailThreads = 1024;
 
 __device__ __forceinline__ double csr_row(const TailLevel& L, int i, const double* x) {
@@ -858,8 +864,8 @@ const double* vcycle(gis_context* h, int l, const double* r, double* out) {
         return za;
     }
     for (int s = 1; s < A.nu; ++s) {
-        hipLaunchKernelGGL((k_vc_sweep<P, false, false>), g, dim3(kWG), 0, h->stream, m, N, sm,
-                           r, za, 0.0, nullptr, nullptr, zb, h->st, h->red);
+        hipLaunchKernelGGL((k_vc_sweep<P, false>), g, dim3(kWG), 0, h->stream, m, N, sm, r, za,
+                           zb, h->st, h->red);
         std::swap(za, zb);
     }
     hipLaunchKernelGGL((k_vc_resid<P>), g, dim3(kWG), 0, h->stream, m, N, r, za, t, h->st);
@@ -878,22 +884,17 @@ const double* vcycle(gis_context* h, int l, const double* r, double* out) {
     } else {
         zc = vcycle<P>(h, l + 1, rc, nullptr);
     }
-    const int* agg = L.agg.as<int>();
+    hipLaunchKernelGGL(k_vc_prolong, g, dim3(kWG), 0, h->stream, N, A.cscale, L.agg.as<int>(),
+                       zc, za, h->st);
     for (int s = 0; s < A.nu; ++s) {
         const bool red = (l == 0 && s + 1 == A.nu);
         double* dst = red ? out : zb;
-        if (s == 0 && red)
-            hipLaunchKernelGGL((k_vc_sweep<P, true, true>), g, dim3(kWG), 0, h->stream, m, N, sm,
-                               r, za, A.cscale, agg, zc, dst, h->st, h->red);
-        else if (s == 0)
-            hipLaunchKernelGGL((k_vc_sweep<P, true, false>), g, dim3(kWG), 0, h->stream, m, N,
-                               sm, r, za, A.cscale, agg, zc, dst, h->st, h->red);
-        else if (red)
-            hipLaunchKernelGGL((k_vc_sweep<P, false, true>), g, dim3(kWG), 0, h->stream, m, N,
-                               sm, r, za, 0.0, nullptr, nullptr, dst, h->st, h->red);
+        if (red)
+            hipLaunchKernelGGL((k_vc_sweep<P, true>), g, dim3(kWG), 0, h->stream, m, N, sm, r,
+                               za, dst, h->st, h->red);
         else
-            hipLaunchKernelGGL((k_vc_sweep<P, false, false>), g, dim3(kWG), 0, h->stream, m, N,
-                               sm, r, za, 0.0, nullptr, nullptr, dst, h->st, h->red);
+            hipLaunchKernelGGL((k_vc_sweep<P, false>), g, dim3(kWG), 0, h->stream, m, N, sm, r,
+                               za, dst, h->st, h->red);
         zb = za;
         za = dst;
     }
