@@ -600,11 +600,11 @@ def main():
         out["plumbing_32"] = plumb
     if batched is not None:
         out["batched_cg_64_systems"] = batched
+        out["spmm_bicg_3_rhs"] = multi
     if pcg is not None:
         pcg["speedup_vs_fp64_cg"] = round(t_fp64 / pcg["solve_s"], 3)
         pcg["speedup_vs_mixed"] = round(t_mixed / pcg["solve_s"], 3)
         out["pcg_gamg_to_1e-10"] = pcg
-        out["spmm_bicg_3_rhs"] = multi
     print(json.dumps(out), flush=True)
 
 
