@@ -431,26 +431,31 @@ def main():
         levels = g.gamg_levels()               # builds the hierarchy (once per mesh)
         t_build = time.perf_counter() - t0
         sizes = [g.gamg_level(l)["n"] for l in range(levels)]
-        for _ in range(W):
-            g.solve_pcg("GAMG", b, torch.zeros_like(b), error=1e-10 * bnorm,
-                        max_steps=max_iter, hist_cap=max_iter + 1)
-        torch.cuda.synchronize()
-        xp = [torch.zeros_like(b) for _ in range(K)]
-        step_p = []
-        for i in range(K):
-            ts = time.perf_counter()
-            _, hist_p, st_p = g.solve_pcg("GAMG", b, xp[i], error=1e-10 * bnorm,
-                                          max_steps=max_iter, hist_cap=max_iter + 1)
-            step_p.append(time.perf_counter() - ts)
-        t_pcg = float(np.median(step_p))
-        rp = torch.empty_like(b)
-        g.spmv(xp[-1], rp)
-        it_p = len(hist_p) - 1
+        def time_pcg(pc):
+            for _ in range(W):
+                g.solve_pcg(pc, b, torch.zeros_like(b), error=1e-10 * bnorm,
+                            max_steps=max_iter, hist_cap=max_iter + 1)
+            torch.cuda.synchronize()
+            xp = [torch.zeros_like(b) for _ in range(K)]
+            step_p = []
+            for i in range(K):
+                ts = time.perf_counter()
+                _, hist_p, st_p = g.solve_pcg(pc, b, xp[i], error=1e-10 * bnorm,
+                                              max_steps=max_iter, hist_cap=max_iter + 1)
+                step_p.append(time.perf_counter() - ts)
+            t_pcg = float(np.median(step_p))
+            rp = torch.empty_like(b)
+            g.spmv(xp[-1], rp)
+            it_p = len(hist_p) - 1
+            return {"solve_s": round(t_pcg, 5), "iterations": it_p,
+                    "iteration_ms": round(t_pcg / max(it_p, 1) * 1e3, 3),
+                    "rel_residual": float(torch.linalg.vector_norm(b - rp)) / bnorm,
+                    "status": st_p}
+
         pcg = {"preconditioner": "GAMG", "levels": levels, "level_cells": sizes,
-               "hierarchy_build_s": round(t_build, 3), "solve_s": round(t_pcg, 5),
-               "iterations": it_p, "iteration_ms": round(t_pcg / max(it_p, 1) * 1e3, 3),
-               "rel_residual": float(torch.linalg.vector_norm(b - rp)) / bnorm,
-               "status": st_p}
+               "hierarchy_build_s": round(t_build, 3)}
+        pcg.update(time_pcg("GAMG"))
+        pcg["fp32_cycle"] = time_pcg("GAMG32")
         log(f"[bench] pcg GAMG: {pcg}")
 
     # ---- per-kernel breakdown: one extra solve of each with every slot timed ----
@@ -602,8 +607,9 @@ def main():
         out["batched_cg_64_systems"] = batched
         out["spmm_bicg_3_rhs"] = multi
     if pcg is not None:
-        pcg["speedup_vs_fp64_cg"] = round(t_fp64 / pcg["solve_s"], 3)
-        pcg["speedup_vs_mixed"] = round(t_mixed / pcg["solve_s"], 3)
+        for rec in (pcg, pcg["fp32_cycle"]):
+            rec["speedup_vs_fp64_cg"] = round(t_fp64 / rec["solve_s"], 3)
+            rec["speedup_vs_mixed"] = round(t_mixed / rec["solve_s"], 3)
         out["pcg_gamg_to_1e-10"] = pcg
     print(json.dumps(out), flush=True)
 

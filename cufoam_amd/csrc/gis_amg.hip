@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "gis_kernels.hpp"
@@ -276,14 +277,19 @@ __global__ __launch_bounds__(2 * kDense) void k_amg_dense_inv(int n, int m,
 }
 
 // ---------------------------------------------------------------------------
-// device: V-cycle pieces (every kernel is a no-op once the solve is done)
+// device: V-cycle pieces (every kernel is a no-op once the solve is done),
+// for T = double (GAMG) and T = float (GAMG32: the same steps in FP32)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kWG) void k_vc_pre(int N, const double* __restrict__ sm,
-                                                const double* __restrict__ r,
-                                                double* __restrict__ z, const KState* st) {
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return fmaf(a, b, c); }
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_vc_pre(int N, const T* __restrict__ sm,
+                                                const T* __restrict__ r, T* __restrict__ z,
+                                                const KState* st) {
     if (st->done) return;
     StreamRows t(N);
-    double sv[4], rv[4], zv[4];
+    T sv[4], rv[4], zv[4];
     load4(sm, t.i0, N, sv);
     load4(r, t.i0, N, rv);
 #pragma unroll
@@ -291,30 +297,49 @@ __global__ __launch_bounds__(kWG) void k_vc_pre(int N, const double* __restrict_
     store4(z, t.i0, N, zv);
 }
 
-// One damped-Jacobi sweep zn = fma(sm, r - A z, z).  RED (finest level):
-// rho = (r, zn), beta = rho / rho_prev.
-template <class P, bool RED>
-__global__ __launch_bounds__(kWG) void k_vc_sweep(Mat<double> m, int N,
-                                                  const double* __restrict__ sm,
-                                                  const double* __restrict__ r,
-                                                  const double* __restrict__ z,
-                                                  double* __restrict__ zn, KState* st,
-                                                  RedWs red) {
+// GAMG32 entry on the finest level: r32 = (float) r, z32 = sm32 .* r32
+__global__ __launch_bounds__(kWG) void k_vc_narrow_pre(int N, const float* __restrict__ sm,
+                                                       const double* __restrict__ r,
+                                                       float* __restrict__ r32,
+                                                       float* __restrict__ z32,
+                                                       const KState* st) {
     if (st->done) return;
-    __shared__ typename P::template Shared<double> sh;
-    __shared__ double sh_y[kTile];
+    StreamRows t(N);
+    double rv[4];
+    float sv[4], fv[4], zv[4];
+    load4(sm, t.i0, N, sv);
+    load4(r, t.i0, N, rv);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        fv[k] = (float)rv[k];
+        zv[k] = sv[k] * fv[k];
+    }
+    store4(r32, t.i0, N, fv);
+    store4(z32, t.i0, N, zv);
+}
+
+// One damped-Jacobi sweep zn = fma(sm, r - A z, z).  RED (finest level, FP64):
+// rho = (r, zn), beta = rho / rho_prev.
+template <class P, bool RED, typename T>
+__global__ __launch_bounds__(kWG) void k_vc_sweep(Mat<T> m, int N, const T* __restrict__ sm,
+                                                  const T* __restrict__ r,
+                                                  const T* __restrict__ z, T* __restrict__ zn,
+                                                  KState* st, RedWs red) {
+    if (st->done) return;
+    __shared__ typename P::template Shared<T> sh;
+    __shared__ T sh_y[kTile];
     __shared__ RedShared shr;
     TileRows t(N);
     P::run(m, N, t.row0, t.nrows, [&](int c) { return z[c]; }, sh, sh_y);
-    double sv[4], rv[4], zv[4];
+    T sv[4], rv[4], zv[4];
     load4(sm, t.i0, N, sv);
     load4(r, t.i0, N, rv);
     load4(z, t.i0, N, zv);
     double leaf[1] = {0.0};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        zv[k] = fma(sv[k], rv[k] - sh_y[threadIdx.x * kLeaf + k], zv[k]);
-        if (RED && t.i0 + k < N) leaf[0] = fma(rv[k], zv[k], leaf[0]);
+        zv[k] = fmaT(sv[k], rv[k] - sh_y[threadIdx.x * kLeaf + k], zv[k]);
+        if (RED && t.i0 + k < N) leaf[0] = fma((double)rv[k], (double)zv[k], leaf[0]);
     }
     store4(zn, t.i0, N, zv);
     if constexpr (RED) {
@@ -326,18 +351,52 @@ __global__ __launch_bounds__(kWG) void k_vc_sweep(Mat<double> m, int N,
     }
 }
 
-// t = r - A z
+// GAMG32's last sweep on the finest level: zn = fmaf(sm, (float) r - A z, z),
+// out = (double) zn, rho = (r, out) in FP64, beta = rho / rho_prev.
 template <class P>
-__global__ __launch_bounds__(kWG) void k_vc_resid(Mat<double> m, int N,
-                                                  const double* __restrict__ r,
-                                                  const double* __restrict__ z,
-                                                  double* __restrict__ tr, const KState* st) {
+__global__ __launch_bounds__(kWG) void k_vc_sweep_out(Mat<float> m, int N,
+                                                      const float* __restrict__ sm,
+                                                      const double* __restrict__ r,
+                                                      const float* __restrict__ z,
+                                                      double* __restrict__ out, KState* st,
+                                                      RedWs red) {
     if (st->done) return;
-    __shared__ typename P::template Shared<double> sh;
-    __shared__ double sh_y[kTile];
+    __shared__ typename P::template Shared<float> sh;
+    __shared__ float sh_y[kTile];
+    __shared__ RedShared shr;
     TileRows t(N);
     P::run(m, N, t.row0, t.nrows, [&](int c) { return z[c]; }, sh, sh_y);
-    double rv[4], tv[4];
+    float sv[4], zv[4];
+    double rv[4], ov[4];
+    load4(sm, t.i0, N, sv);
+    load4(r, t.i0, N, rv);
+    load4(z, t.i0, N, zv);
+    double leaf[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        zv[k] = fmaf(sv[k], (float)rv[k] - sh_y[threadIdx.x * kLeaf + k], zv[k]);
+        ov[k] = (double)zv[k];
+        if (t.i0 + k < N) leaf[0] = fma(rv[k], ov[k], leaf[0]);
+    }
+    store4(out, t.i0, N, ov);
+    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+        st->rho1 = st->rho0;
+        st->rho0 = v[0];
+        st->beta = st->rho0 / st->rho1;
+    });
+}
+
+// t = r - A z
+template <class P, typename T>
+__global__ __launch_bounds__(kWG) void k_vc_resid(Mat<T> m, int N, const T* __restrict__ r,
+                                                  const T* __restrict__ z, T* __restrict__ tr,
+                                                  const KState* st) {
+    if (st->done) return;
+    __shared__ typename P::template Shared<T> sh;
+    __shared__ T sh_y[kTile];
+    TileRows t(N);
+    P::run(m, N, t.row0, t.nrows, [&](int c) { return z[c]; }, sh, sh_y);
+    T rv[4], tv[4];
     load4(r, t.i0, N, rv);
 #pragma unroll
     for (int k = 0; k < 4; ++k) tv[k] = rv[k] - sh_y[threadIdx.x * kLeaf + k];
@@ -346,17 +405,17 @@ __global__ __launch_bounds__(kWG) void k_vc_resid(Mat<double> m, int N,
 
 // rc[C] = sum of t over C's cells, ascending, and the next level's first
 // sweep from zero, zc0 = smc .* rc
+template <typename T>
 __global__ __launch_bounds__(kBlk) void k_vc_restrict(int nc, const int* __restrict__ mptr,
                                                       const int* __restrict__ mlist,
-                                                      const double* __restrict__ t,
-                                                      const double* __restrict__ smc,
-                                                      double* __restrict__ rc,
-                                                      double* __restrict__ zc0,
+                                                      const T* __restrict__ t,
+                                                      const T* __restrict__ smc,
+                                                      T* __restrict__ rc, T* __restrict__ zc0,
                                                       const KState* st) {
     if (st->done) return;
     const int c = blockIdx.x * kBlk + threadIdx.x;
     if (c >= nc) return;
-    double s = 0.0;
+    T s = T(0);
     for (int p = mptr[c]; p < mptr[c + 1]; ++p) s = s + t[mlist[p]];
     rc[c] = s;
     zc0[c] = smc[c] * s;
@@ -364,74 +423,77 @@ __global__ __launch_bounds__(kBlk) void k_vc_restrict(int nc, const int* __restr
 
 // z = fma(cscale, zc[agg], z)   (a separate pass: folding it into the next
 // sweep's gathers costs two dependent loads per nonzero, measured slower)
-__global__ __launch_bounds__(kWG) void k_vc_prolong(int N, double cscale,
-                                                    const int* __restrict__ agg,
-                                                    const double* __restrict__ zc,
-                                                    double* __restrict__ z, const KState* st) {
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_vc_prolong(int N, T cscale, const int* __restrict__ agg,
+                                                    const T* __restrict__ zc,
+                                                    T* __restrict__ z, const KState* st) {
     if (st->done) return;
     StreamRows t(N);
-    double zv[4];
+    T zv[4];
     load4(z, t.i0, N, zv);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        if (t.i0 + k < N) zv[k] = fma(cscale, zc[agg[t.i0 + k]], zv[k]);
+        if (t.i0 + k < N) zv[k] = fmaT(cscale, zc[agg[t.i0 + k]], zv[k]);
     store4(z, t.i0, N, zv);
 }
 
 // The levels from the first one of at most kTail cells down to the last, as
 // one workgroup: the same steps and arithmetic as the per-level kernels (CSR
 // rows summed in storage order), separated by workgroup barriers.
+template <typename T>
 struct TailLevel {
     int n;
     const int* rowptr;
     const int* col;
-    const double* val;
-    const double* sm;
+    const T* val;
+    const T* sm;
     const int* agg;     // to the next tail level
     const int* mptr;
     const int* mlist;
-    double* r;
-    double* z;
-    double* z2;
-    double* t;
+    T* r;
+    T* z;
+    T* z2;
+    T* t;
 };
 
 constexpr int kTail = 256;        // one workgroup runs a level in ~10 us per step: only tiny levels
 constexpr int kTailThreads = 1024;
 
-__device__ __forceinline__ double csr_row(const TailLevel& L, int i, const double* x) {
-    double s = 0.0;
+template <typename T>
+__device__ __forceinline__ T csr_row(const TailLevel<T>& L, int i, const T* x) {
+    T s = T(0);
     for (int j = L.rowptr[i]; j < L.rowptr[i + 1]; ++j) s = s + L.val[j] * x[L.col[j]];
     return s;
 }
 
 // On entry lv[0].r holds the rhs and lv[0].z its first sweep from zero; on
 // exit lv[0].z holds the cycle's result.
-__global__ __launch_bounds__(kTailThreads) void k_vc_tail(const TailLevel* __restrict__ lv,
-                                                          int nl, int nu, double cscale,
-                                                          const double* __restrict__ ainv,
+template <typename T>
+__global__ __launch_bounds__(kTailThreads) void k_vc_tail(const TailLevel<T>* __restrict__ lv,
+                                                          int nl, int nu, T cscale,
+                                                          const T* __restrict__ ainv,
                                                           int dense_n, const KState* st) {
     if (st->done) return;
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
-    double* cur[kAmgLevels];
+    T* cur[kAmgLevels];
     for (int l = 0; l + 1 < nl; ++l) {
-        const TailLevel& L = lv[l];
-        double* z = L.z;
-        double* z2 = L.z2;
+        const TailLevel<T>& L = lv[l];
+        T* z = L.z;
+        T* z2 = L.z2;
         for (int s = 1; s < nu; ++s) {
             for (int i = tid; i < L.n; i += nt)
-                z2[i] = fma(L.sm[i], L.r[i] - csr_row(L, i, z), z[i]);
+                z2[i] = fmaT(L.sm[i], L.r[i] - csr_row(L, i, z), z[i]);
             __syncthreads();
-            double* tmp = z;
+            T* tmp = z;
             z = z2;
             z2 = tmp;
         }
         for (int i = tid; i < L.n; i += nt) L.t[i] = L.r[i] - csr_row(L, i, z);
         __syncthreads();
-        const TailLevel& C = lv[l + 1];
+        const TailLevel<T>& C = lv[l + 1];
         for (int c = tid; c < C.n; c += nt) {
-            double s = 0.0;
+            T s = T(0);
             for (int p = L.mptr[c]; p < L.mptr[c + 1]; ++p) s = s + L.t[L.mlist[p]];
             C.r[c] = s;
             C.z[c] = C.sm[c] * s;
@@ -439,38 +501,27 @@ __global__ __launch_bounds__(kTailThreads) void k_vc_tail(const TailLevel* __res
         __syncthreads();
         cur[l] = z;
     }
-    const TailLevel& E = lv[nl - 1];
+    const TailLevel<T>& E = lv[nl - 1];
     if (dense_n) {
         for (int i = tid; i < E.n; i += nt) {
-            double s = 0.0;
+            T s = T(0);
             for (int j = 0; j < E.n; ++j) s = s + ainv[i * E.n + j] * E.r[j];
             E.z[i] = s;
         }
         __syncthreads();
     }
-    const double* zc = E.z;
+    const T* zc = E.z;
     for (int l = nl - 2; l >= 0; --l) {
-        const TailLevel& L = lv[l];
-        double* z = cur[l];
-        double* z2 = (z == L.z) ? L.z2 : L.z;
-        for (int i = tid; i < L.n; i += nt) {
-            double y = 0.0;
-            for (int j = L.rowptr[i]; j < L.rowptr[i + 1]; ++j) {
-                const int c = L.col[j];
-                y = y + L.val[j] * fma(cscale, zc[L.agg[c]], z[c]);
-            }
-            const double x = fma(cscale, zc[L.agg[i]], z[i]);
-            z2[i] = fma(L.sm[i], L.r[i] - y, x);
-        }
+        const TailLevel<T>& L = lv[l];
+        T* z = cur[l];
+        T* z2 = (z == L.z) ? L.z2 : L.z;
+        for (int i = tid; i < L.n; i += nt) z[i] = fmaT(cscale, zc[L.agg[i]], z[i]);
         __syncthreads();
-        double* tmp = z;
-        z = z2;
-        z2 = tmp;
-        for (int s = 1; s < nu; ++s) {
+        for (int s = 0; s < nu; ++s) {
             for (int i = tid; i < L.n; i += nt)
-                z2[i] = fma(L.sm[i], L.r[i] - csr_row(L, i, z), z[i]);
+                z2[i] = fmaT(L.sm[i], L.r[i] - csr_row(L, i, z), z[i]);
             __syncthreads();
-            tmp = z;
+            T* tmp = z;
             z = z2;
             z2 = tmp;
         }
@@ -483,15 +534,22 @@ __global__ __launch_bounds__(kTailThreads) void k_vc_tail(const TailLevel* __res
 }
 
 // z = Ainv r on the last level (n <= kDense)
-__global__ __launch_bounds__(kDense) void k_vc_dense(int n, const double* __restrict__ ainv,
-                                                     const double* __restrict__ r,
-                                                     double* __restrict__ z, const KState* st) {
+template <typename T>
+__global__ __launch_bounds__(kDense) void k_vc_dense(int n, const T* __restrict__ ainv,
+                                                     const T* __restrict__ r, T* __restrict__ z,
+                                                     const KState* st) {
     if (st->done) return;
     const int i = threadIdx.x;
     if (i >= n) return;
-    double s = 0.0;
+    T s = T(0);
     for (int j = 0; j < n; ++j) s = s + ainv[i * n + j] * r[j];
     z[i] = s;
+}
+
+__global__ __launch_bounds__(kBlk) void k_narrow(long long n, const double* __restrict__ a,
+                                                 float* __restrict__ b) {
+    const long long i = (long long)blockIdx.x * kBlk + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
 }
 
 // rho = (r, z), beta = rho / rho_prev (after a one-level V-cycle)
@@ -508,6 +566,32 @@ __global__ __launch_bounds__(kWG) void k_pcg_rho(int N, const double* __restrict
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         if (t.i0 + k < N) leaf[0] = fma(rv[k], zv[k], leaf[0]);
+    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+        st->rho1 = st->rho0;
+        st->rho0 = v[0];
+        st->beta = st->rho0 / st->rho1;
+    });
+}
+
+// one-level GAMG32: out = (double) z32, rho = (r, out), beta = rho / rho_prev
+__global__ __launch_bounds__(kWG) void k_pcg_rho32(int N, const double* __restrict__ r,
+                                                   const float* __restrict__ z32,
+                                                   double* __restrict__ out, KState* st,
+                                                   RedWs red) {
+    if (st->done) return;
+    __shared__ RedShared shr;
+    TileRows t(N);
+    double rv[4], ov[4];
+    float zv[4];
+    load4(r, t.i0, N, rv);
+    load4(z32, t.i0, N, zv);
+    double leaf[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        ov[k] = (double)zv[k];
+        if (t.i0 + k < N) leaf[0] = fma(rv[k], ov[k], leaf[0]);
+    }
+    store4(out, t.i0, N, ov);
     finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
         st->rho1 = st->rho0;
         st->rho0 = v[0];
@@ -566,13 +650,17 @@ __global__ __launch_bounds__(kWG) void k_pcg_dir(int N, const double* __restrict
 }
 
 // r = fma(-alpha, q, r); ||r||, history, stop test.  PC 0: rho = (r, r);
-// PC 1: z = dinv r, rho = (r, z); PC 2: rho is left to the V-cycle.
-// PC 2 also writes the V-cycle's first sweep from zero, z = sm .* r (dinv = sm).
+// PC 1: z = dinv r, rho = (r, z); PC 2, 3: rho is left to the V-cycle.
+// PC 2 also writes the V-cycle's first sweep from zero, z = sm .* r (dinv = sm);
+// PC 3 writes r32 = (float) r and z32 = sm32 .* r32.
 template <int C, int PC>
 __global__ __launch_bounds__(kWG) void k_pcg2(int N, double* __restrict__ r,
                                               const double* __restrict__ q,
                                               const double* __restrict__ dinv,
-                                              double* __restrict__ z, KState* st, RedWs red) {
+                                              double* __restrict__ z,
+                                              const float* __restrict__ sm32,
+                                              float* __restrict__ r32, float* __restrict__ z32,
+                                              KState* st, RedWs red) {
     if (st->done) return;
     constexpr int NV = PC == 1 ? 2 : 1;
     __shared__ RedShared shr;
@@ -589,11 +677,11 @@ __global__ __launch_bounds__(kWG) void k_pcg2(int N, double* __restrict__ r,
         double qv[4], dv[4];
         load4(r, t.i0, N, rv[c]);
         load4(q, t.i0, N, qv);
-        if (PC != 0) load4(dinv, t.i0, N, dv);
+        if (PC == 1 || PC == 2) load4(dinv, t.i0, N, dv);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             rv[c][kk] = fma(-alpha, qv[kk], rv[c][kk]);
-            if (PC != 0) zv[c][kk] = dv[kk] * rv[c][kk];
+            if (PC == 1 || PC == 2) zv[c][kk] = dv[kk] * rv[c][kk];
             if (t.i0 + kk < N) {
                 leaf[c][0] = fma(rv[c][kk], rv[c][kk], leaf[c][0]);
                 if (PC == 1) leaf[c][NV - 1] = fma(rv[c][kk], zv[c][kk], leaf[c][NV - 1]);
@@ -607,12 +695,23 @@ __global__ __launch_bounds__(kWG) void k_pcg2(int N, double* __restrict__ r,
         if (c >= sup.nt) continue;
         const int i0 = TileRows(N, sup.tile0 + c).i0;
         store4(r, i0, N, rv[c]);
-        if (PC != 0) store4(z, i0, N, zv[c]);
+        if (PC == 1 || PC == 2) store4(z, i0, N, zv[c]);
+        if (PC == 3) {
+            float sv[4], fv[4], gv[4];
+            load4(sm32, i0, N, sv);
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                fv[kk] = (float)rv[c][kk];
+                gv[kk] = sv[kk] * fv[kk];
+            }
+            store4(r32, i0, N, fv);
+            store4(z32, i0, N, gv);
+        }
     }
     tr.end(red, [&](const double (&v)[NV]) {
         const double norm = sqrt(v[0]);
         record_norm(st, norm);
-        if (PC != 2) {
+        if (PC < 2) {
             st->rho1 = st->rho0;
             st->rho0 = v[NV - 1];
             st->beta = st->rho0 / st->rho1;
@@ -798,14 +897,14 @@ int amg_values(gis_context* h) {
             break;
         }
     if (A.tail_start) {
-        std::vector<TailLevel> meta;
+        std::vector<TailLevel<double>> meta;
         for (size_t l = A.tail_start; l < A.lv.size(); ++l) {
             AmgLevel& L = A.lv[l];
             gis_context* c = level_ctx(h, (int)l);
             double* v = L.vecs.as<double>();
             const size_t n = (size_t)L.n;
             const bool last = l + 1 == A.lv.size();
-            meta.push_back(TailLevel{L.n, c->rowptr.as<int>(), c->col.as<int>(),
+            meta.push_back(TailLevel<double>{L.n, c->rowptr.as<int>(), c->col.as<int>(),
                                      c->val.as<double>(), L.sm.as<double>(),
                                      last ? nullptr : L.agg.as<int>(),
                                      last ? nullptr : L.mptr.as<int>(),
@@ -817,6 +916,48 @@ int amg_values(gis_context* h) {
     }
     GIS_HIP(h, hipStreamSynchronize(h->stream));
     A.values_ok = true;
+    A.values32_ok = false;
+    return kOK;
+}
+
+// FP32 copies for GAMG32: level values, sm32 = (float) sm, (float) ainv
+int amg_values32(gis_context* h) {
+    AmgData& A = *h->amg;
+    for (size_t l = 0; l < A.lv.size(); ++l) {
+        AmgLevel& L = A.lv[l];
+        GIS_TRY(ensure_val32(level_ctx(h, (int)l)));
+        GIS_HIP(h, L.sm32.ensure(sizeof(float) * std::max(L.n, 1)));
+        GIS_HIP(h, L.vecs32.ensure(sizeof(float) * 4 * std::max(L.n, 1)));
+        hipLaunchKernelGGL(k_narrow, grid_of(L.n), dim3(kBlk), 0, h->stream, (long long)L.n,
+                           L.sm.as<double>(), L.sm32.as<float>());
+        GIS_HIP(h, hipGetLastError());
+    }
+    if (A.dense_n) {
+        const long long nn = (long long)A.dense_n * A.dense_n;
+        GIS_HIP(h, A.ainv32.ensure(sizeof(float) * nn));
+        hipLaunchKernelGGL(k_narrow, grid_of(nn), dim3(kBlk), 0, h->stream, nn,
+                           A.ainv.as<double>(), A.ainv32.as<float>());
+        GIS_HIP(h, hipGetLastError());
+    }
+    if (A.tail_start) {
+        std::vector<TailLevel<float>> meta;
+        for (size_t l = A.tail_start; l < A.lv.size(); ++l) {
+            AmgLevel& L = A.lv[l];
+            gis_context* c = level_ctx(h, (int)l);
+            float* v = L.vecs32.as<float>();
+            const size_t n = (size_t)L.n;
+            const bool last = l + 1 == A.lv.size();
+            meta.push_back(TailLevel<float>{L.n, c->rowptr.as<int>(), c->col.as<int>(),
+                                            c->val32.as<float>(), L.sm32.as<float>(),
+                                            last ? nullptr : L.agg.as<int>(),
+                                            last ? nullptr : L.mptr.as<int>(),
+                                            last ? nullptr : L.mlist.as<int>(), v + 3 * n, v,
+                                            v + n, v + 2 * n});
+        }
+        GIS_TRY(upload_vec(h, A.tail32, meta));
+    }
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    A.values32_ok = true;
     return kOK;
 }
 
@@ -833,70 +974,106 @@ int amg_prepare(gis_context* h) {
     return kOK;
 }
 
-// V-cycle of level l on rhs r; returns the buffer holding z.  On entry L.z
-// holds the first sweep from zero, sm .* r (written by whoever formed r).  On
-// the finest level the last sweep goes to `out` together with rho = (r, z).
-template <class P>
-const double* vcycle(gis_context* h, int l, const double* r, double* out) {
+template <typename T>
+struct LevelBufs {
+    static T* vecs(AmgLevel& L);
+    static T* sm(AmgLevel& L);
+};
+template <>
+double* LevelBufs<double>::vecs(AmgLevel& L) { return L.vecs.as<double>(); }
+template <>
+double* LevelBufs<double>::sm(AmgLevel& L) { return L.sm.as<double>(); }
+template <>
+float* LevelBufs<float>::vecs(AmgLevel& L) { return L.vecs32.as<float>(); }
+template <>
+float* LevelBufs<float>::sm(AmgLevel& L) { return L.sm32.as<float>(); }
+
+template <typename T>
+Mat<T> level_mat(const gis_context* c);
+template <>
+Mat<double> level_mat<double>(const gis_context* c) { return mat64(c); }
+template <>
+Mat<float> level_mat<float>(const gis_context* c) { return mat32(c); }
+
+// V-cycle of level l on rhs r (type T); returns the buffer holding z.  On
+// entry the level's z holds the first sweep from zero, sm .* r (written by
+// whoever formed r).  On the finest level the last sweep goes to `out` (FP64)
+// together with rho = (r64, out); r64 is the FP64 residual (T = float).
+template <class P, typename T>
+const T* vcycle(gis_context* h, int l, const T* r, double* out, const double* r64) {
     AmgData& A = *h->amg;
     AmgLevel& L = A.lv[l];
     gis_context* c = level_ctx(h, l);
     const int N = L.n;
-    double* za = L.vecs.as<double>();
-    double* zb = za + N;
-    double* t = za + 2 * (size_t)N;
+    T* za = LevelBufs<T>::vecs(L);
+    T* zb = za + N;
+    T* t = za + 2 * (size_t)N;
     const dim3 g = tiles_grid(N);
-    const Mat<double> m = mat64(c);
-    const double* sm = L.sm.as<double>();
+    const Mat<T> m = level_mat<T>(c);
+    const T* sm = LevelBufs<T>::sm(L);
+    const T* ainv = nullptr;
+    if constexpr (std::is_same<T, double>::value) ainv = A.ainv.as<double>();
+    else ainv = A.ainv32.as<float>();
     if (l + 1 == (int)A.lv.size()) {
-        if (l == 0) {   // a one-level hierarchy: z into out, then rho = (r, z)
-            if (A.dense_n)
-                hipLaunchKernelGGL(k_vc_dense, dim3(1), dim3(kDense), 0, h->stream, N,
-                                   A.ainv.as<double>(), r, out, h->st);
-            else
-                hipLaunchKernelGGL(k_vc_pre, g, dim3(kWG), 0, h->stream, N, sm, r, out, h->st);
-            hipLaunchKernelGGL(k_pcg_rho, g, dim3(kWG), 0, h->stream, N, r, out, h->st, h->red);
-            return out;
+        T* dst = za;
+        if constexpr (std::is_same<T, double>::value) {
+            if (l == 0) dst = out;   // a one-level hierarchy writes z straight into out
         }
         if (A.dense_n)
-            hipLaunchKernelGGL(k_vc_dense, dim3(1), dim3(kDense), 0, h->stream, N,
-                               A.ainv.as<double>(), r, za, h->st);
-        return za;
+            hipLaunchKernelGGL((k_vc_dense<T>), dim3(1), dim3(kDense), 0, h->stream, N, ainv, r,
+                               dst, h->st);
+        else
+            hipLaunchKernelGGL((k_vc_pre<T>), g, dim3(kWG), 0, h->stream, N, sm, r, dst, h->st);
+        if (l == 0) {   // rho = (r, z) for a one-level hierarchy
+            if constexpr (std::is_same<T, double>::value)
+                hipLaunchKernelGGL(k_pcg_rho, g, dim3(kWG), 0, h->stream, N, r64, out, h->st,
+                                   h->red);
+            else
+                hipLaunchKernelGGL(k_pcg_rho32, g, dim3(kWG), 0, h->stream, N, r64, za, out,
+                                   h->st, h->red);
+        }
+        return dst;
     }
     for (int s = 1; s < A.nu; ++s) {
-        hipLaunchKernelGGL((k_vc_sweep<P, false>), g, dim3(kWG), 0, h->stream, m, N, sm, r, za,
+        hipLaunchKernelGGL((k_vc_sweep<P, false, T>), g, dim3(kWG), 0, h->stream, m, N, sm, r, za,
                            zb, h->st, h->red);
         std::swap(za, zb);
     }
-    hipLaunchKernelGGL((k_vc_resid<P>), g, dim3(kWG), 0, h->stream, m, N, r, za, t, h->st);
+    hipLaunchKernelGGL((k_vc_resid<P, T>), g, dim3(kWG), 0, h->stream, m, N, r, za, t, h->st);
     AmgLevel& C = A.lv[l + 1];
-    double* zc0 = C.vecs.as<double>();
-    double* rc = zc0 + 3 * (size_t)C.n;
-    hipLaunchKernelGGL(k_vc_restrict, grid_of(C.n), dim3(kBlk), 0, h->stream, C.n,
-                       L.mptr.as<int>(), L.mlist.as<int>(), t, C.sm.as<double>(), rc, zc0,
+    T* zc0 = LevelBufs<T>::vecs(C);
+    T* rc = zc0 + 3 * (size_t)C.n;
+    hipLaunchKernelGGL((k_vc_restrict<T>), grid_of(C.n), dim3(kBlk), 0, h->stream, C.n,
+                       L.mptr.as<int>(), L.mlist.as<int>(), t, LevelBufs<T>::sm(C), rc, zc0,
                        h->st);
-    const double* zc;
+    const T* zc;
     if (l + 1 == A.tail_start) {
-        hipLaunchKernelGGL(k_vc_tail, dim3(1), dim3(kTailThreads), 0, h->stream,
-                           A.tail.as<TailLevel>(), A.tail_levels, A.nu, A.cscale,
-                           A.dense_n ? A.ainv.as<double>() : nullptr, A.dense_n, h->st);
+        const TailLevel<T>* meta;
+        if constexpr (std::is_same<T, double>::value) meta = A.tail.as<TailLevel<double>>();
+        else meta = A.tail32.as<TailLevel<float>>();
+        hipLaunchKernelGGL((k_vc_tail<T>), dim3(1), dim3(kTailThreads), 0, h->stream, meta,
+                           A.tail_levels, A.nu, (T)A.cscale, A.dense_n ? ainv : nullptr,
+                           A.dense_n, h->st);
         zc = zc0;
     } else {
-        zc = vcycle<P>(h, l + 1, rc, nullptr);
+        zc = vcycle<P, T>(h, l + 1, rc, nullptr, nullptr);
     }
-    hipLaunchKernelGGL(k_vc_prolong, g, dim3(kWG), 0, h->stream, N, A.cscale, L.agg.as<int>(),
-                       zc, za, h->st);
+    hipLaunchKernelGGL((k_vc_prolong<T>), g, dim3(kWG), 0, h->stream, N, (T)A.cscale,
+                       L.agg.as<int>(), zc, za, h->st);
     for (int s = 0; s < A.nu; ++s) {
-        const bool red = (l == 0 && s + 1 == A.nu);
-        double* dst = red ? out : zb;
-        if (red)
-            hipLaunchKernelGGL((k_vc_sweep<P, true>), g, dim3(kWG), 0, h->stream, m, N, sm, r,
-                               za, dst, h->st, h->red);
-        else
-            hipLaunchKernelGGL((k_vc_sweep<P, false>), g, dim3(kWG), 0, h->stream, m, N, sm, r,
-                               za, dst, h->st, h->red);
-        zb = za;
-        za = dst;
+        const bool last_fine = (l == 0 && s + 1 == A.nu);
+        if (last_fine) {
+            if constexpr (std::is_same<T, double>::value)
+                hipLaunchKernelGGL((k_vc_sweep<P, true, double>), g, dim3(kWG), 0, h->stream, m,
+                                   N, sm, r, za, out, h->st, h->red);
+            else
+                hipLaunchKernelGGL((k_vc_sweep_out<P>), g, dim3(kWG), 0, h->stream, m, N, sm,
+                                   r64, za, out, h->st, h->red);
+            return za;
+        }
+        hipLaunchKernelGGL((k_vc_sweep<P, false, T>), g, dim3(kWG), 0, h->stream, m, N, sm, r, za,
+                           zb, h->st, h->red);
+        std::swap(za, zb);
     }
     return za;
 }
@@ -909,17 +1086,21 @@ void AmgData::clear() {
     for (auto& L : lv) delete L.ctx;
     lv.clear();
     ainv.release();
+    ainv32.release();
     dense_work.release();
     tail.release();
+    tail32.release();
     dense_n = 0;
     tail_start = tail_levels = 0;
     built = false;
     values_ok = false;
+    values32_ok = false;
 }
 
 int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const double* b) {
-    if (pc < 0 || pc > 2) return fail(h, kErrArg, "unknown preconditioner");
-    if (pc == 2) GIS_TRY(amg_prepare(h));
+    if (pc < 0 || pc > 3) return fail(h, kErrArg, "unknown preconditioner");
+    if (pc >= 2) GIS_TRY(amg_prepare(h));
+    if (pc == 3 && !h->amg->values32_ok) GIS_TRY(amg_values32(h));
     GIS_TRY(prepare_solve(h, a, 5));
     const int N = h->N;
     double* r = h->vec.as<double>();
@@ -938,6 +1119,10 @@ int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const doubl
     const double* dinv = pc == 1 ? h->dinv.as<double>()
                                  : (pc == 2 ? h->amg->lv[0].sm.as<double>() : nullptr);
     double* z0 = pc == 2 ? h->amg->lv[0].vecs.as<double>() : z;
+    // PC 3: the finest level's FP32 rhs (r32) and first sweep (z32)
+    float* z32 = pc == 3 ? h->amg->lv[0].vecs32.as<float>() : nullptr;
+    float* r32 = pc == 3 ? z32 + 3 * (size_t)N : nullptr;
+    const float* sm32 = pc == 3 ? h->amg->lv[0].sm32.as<float>() : nullptr;
     const double* zin = pc == 0 ? r : z;
     const dim3 g = tiles_grid(N);
     const Mat<double> m = mat64(h);
@@ -949,8 +1134,14 @@ int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const doubl
             hipLaunchKernelGGL(k_pcg_diag0, g, dim3(kWG), 0, h->stream, N, dinv, r, z, h->st,
                                h->red);
         if (pc == 2) {
-            hipLaunchKernelGGL(k_vc_pre, g, dim3(kWG), 0, h->stream, N, dinv, r, z0, h->st);
-            (void)vcycle<P>(h, 0, r, z);
+            hipLaunchKernelGGL((k_vc_pre<double>), g, dim3(kWG), 0, h->stream, N, dinv, r, z0,
+                               h->st);
+            (void)vcycle<P, double>(h, 0, r, z, r);
+        }
+        if (pc == 3) {
+            hipLaunchKernelGGL(k_vc_narrow_pre, g, dim3(kWG), 0, h->stream, N, sm32, r, r32, z32,
+                               h->st);
+            (void)vcycle<P, float>(h, 0, r32, z, r);
         }
         GIS_HIP(h, hipGetLastError());
         GIS_COARSEN_DISPATCH(h->sched.coarsen64, CC, {
@@ -960,16 +1151,20 @@ int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const doubl
                                    h->st);
                 hipLaunchKernelGGL((k_cg1s<P, CC>), gc, dim3(kWG), 0, h->stream, m, N, pb0, pb1,
                                    q, h->st, h->red);
-                if (pc == 0)
+                if (pc == 0) {
                     hipLaunchKernelGGL((k_pcg2<CC, 0>), gc, dim3(kWG), 0, h->stream, N, r, q,
-                                       dinv, z, h->st, h->red);
-                else if (pc == 1)
+                                       dinv, z, sm32, r32, z32, h->st, h->red);
+                } else if (pc == 1) {
                     hipLaunchKernelGGL((k_pcg2<CC, 1>), gc, dim3(kWG), 0, h->stream, N, r, q,
-                                       dinv, z, h->st, h->red);
-                else {
+                                       dinv, z, sm32, r32, z32, h->st, h->red);
+                } else if (pc == 2) {
                     hipLaunchKernelGGL((k_pcg2<CC, 2>), gc, dim3(kWG), 0, h->stream, N, r, q,
-                                       dinv, z0, h->st, h->red);
-                    (void)vcycle<P>(h, 0, r, z);
+                                       dinv, z0, sm32, r32, z32, h->st, h->red);
+                    (void)vcycle<P, double>(h, 0, r, z, r);
+                } else {
+                    hipLaunchKernelGGL((k_pcg2<CC, 3>), gc, dim3(kWG), 0, h->stream, N, r, q,
+                                       dinv, z, sm32, r32, z32, h->st, h->red);
+                    (void)vcycle<P, float>(h, 0, r32, z, r);
                 }
                 return kOK;
             }, pc == 2));

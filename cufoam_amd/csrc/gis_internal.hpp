@@ -113,18 +113,21 @@ struct AmgLevel {
     DevMem agg, mptr, mlist, iptr, ilist, fptr, flist;
     DevMem sm;                    // omega / diag
     DevMem vecs;                  // z, z2, t, r (4 n)
+    DevMem sm32, vecs32;          // the same for the FP32 cycle (GAMG32)
 };
 
 struct AmgData {
     bool built = false;           // hierarchy of the current pattern
     bool values_ok = false;       // coarse values of the current values
+    bool values32_ok = false;     // FP32 copies (GAMG32) of the current values
     double omega = 0.8;           // Jacobi damping
     int nu = 1;                   // sweeps before and after the coarse correction
     double cscale = 1.8;          // coarse-correction factor
     std::deque<AmgLevel> lv;
     DevMem ainv, dense_work;      // dense inverse of the last level (dense_n > 0)
     int dense_n = 0;
-    DevMem tail;                  // per-level pointers of the single-workgroup tail
+    DevMem ainv32;                // (float) ainv
+    DevMem tail, tail32;          // per-level pointers of the single-workgroup tail
     int tail_start = 0;           // first tail level (0: none)
     int tail_levels = 0;
     void clear();

@@ -21,7 +21,7 @@ BREAKDOWN, NOT_CONVERGED, NAN = 10, 11, 12
 
 METHODS = {"CG": 0, "BiCG": 1, "Jacobi": 2, "MR": 3, "MRJacobi": 4, "JMR": 5, "MRS": 6,
            "Richardson": 7}
-PRECONDITIONERS = {"none": 0, "diagonal": 1, "GAMG": 2}
+PRECONDITIONERS = {"none": 0, "diagonal": 1, "GAMG": 2, "GAMG32": 3}
 
 
 class GisError(RuntimeError):

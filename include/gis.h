@@ -221,6 +221,10 @@ GIS_API int gis_set_coupled_values_device(gis_handle h, const double *d_diag,
 #define GIS_PRECOND_NONE 0
 #define GIS_PRECOND_DIAGONAL 1
 #define GIS_PRECOND_GAMG 2
+/* The same V-cycle in FP32 (level matrices and vectors in float, the rhs
+ * rounded to float, the result widened); the Krylov loop, residuals and
+ * reductions stay FP64, so the attainable residual is unchanged. */
+#define GIS_PRECOND_GAMG32 3
 GIS_API int gis_solve_pcg(gis_handle h, int precond, double *x, const double *b,
                           const gis_params *params, double *hist, int hist_cap, int *hist_len);
 GIS_API int gis_solve_pcg_device(gis_handle h, int precond, double *d_x, const double *d_b,

@@ -43,6 +43,11 @@
  *     z = fma(cscale, zc[agg], z) with zc the V-cycle of level L + 1 on rc;
  *     nu sweeps z = fma(sm, r - A z, z).
  *   Symmetric (same Jacobi sweeps before and after, R = P^T), so PCG applies.
+ * GAMG32: the same cycle in FP32 on the same hierarchy: level values
+ *   (float) of the FP64 values, sm32 = (float)(omega / diag), the dense
+ *   inverse rounded to float, float vectors and float arithmetic (fmaf
+ *   where the FP64 cycle has fma, cscale as float); the finest rhs is
+ *   (float) r and the result is widened to double; rho = (r, z) in FP64.
  *
  * Arithmetic contract as in gis_oracle.c (-ffp-contract=off, fma only where
  * written, sequential CSR row sums, the fixed reduction tree of or_dot).
@@ -58,7 +63,7 @@
 #define OR_AMG_DENSE 64
 
 enum { OR_OK = 0, OR_ERR_ARG = 1, OR_ERR_NOMEM = 5 };
-enum { PC_NONE = 0, PC_DIAGONAL = 1, PC_GAMG = 2 };
+enum { PC_NONE = 0, PC_DIAGONAL = 1, PC_GAMG = 2, PC_GAMG32 = 3 };
 
 /* from gis_oracle.c */
 double or_dot(int64_t n, const double *a, const double *b);
@@ -68,6 +73,8 @@ void or_ldu2csr_values(int N, int M, int64_t nnz, const int *perm, const double 
                        const double *lower, const double *upper, double *val);
 void or_spmv(int N, const int *rowptr, const int *col, const double *val, const double *x,
              double *y);
+void or_spmv_f32(int N, const int *rowptr, const int *col, const float *val, const float *x,
+                 float *y);
 
 typedef struct {
     int n, m;
@@ -80,12 +87,14 @@ typedef struct {
     int *agg;    /* cell -> next-level cell */
     int *fcode;  /* face -> 2 F + flip (F a next-level face) or -1 - C (inside next-level cell C) */
     double *r, *z, *z2, *t;  /* V-cycle work vectors */
+    float *val32, *sm32, *r32, *z32, *z232, *t32;   /* the FP32 cycle (GAMG32) */
 } level_t;
 
 typedef struct {
     int nl;
     level_t lv[OR_AMG_LEVELS];
     double *ainv;  /* dense inverse of the last level, or NULL */
+    float *ainv32; /* (float) ainv, made with the FP32 cycle's data */
     double omega, cscale;
     int nu;
 } amg_t;
@@ -257,6 +266,7 @@ static void free_level(level_t *L) {
     free(L->la); free(L->ua); free(L->diag); free(L->lower); free(L->upper);
     free(L->rowptr); free(L->col); free(L->perm); free(L->val); free(L->sm);
     free(L->agg); free(L->fcode); free(L->r); free(L->z); free(L->z2); free(L->t);
+    free(L->val32); free(L->sm32); free(L->r32); free(L->z32); free(L->z232); free(L->t32);
 }
 
 static int alloc_level(level_t *L, int n, int m) {
@@ -341,6 +351,7 @@ OR_API void or_amg_free(void *p) {
     if (!a) return;
     for (int l = 0; l < a->nl; ++l) free_level(&a->lv[l]);
     free(a->ainv);
+    free(a->ainv32);
     free(a);
 }
 
@@ -476,6 +487,88 @@ static void vcycle(const amg_t *a, int l, const double *r, double *z) {
 
 OR_API void or_amg_vcycle(void *p, const double *r, double *z) { vcycle((amg_t *)p, 0, r, z); }
 
+/* FP32 data of the hierarchy (made once) */
+static int make32(amg_t *a) {
+    if (a->lv[0].val32) return OR_OK;
+    for (int l = 0; l < a->nl; ++l) {
+        level_t *L = &a->lv[l];
+        const int64_t nnz = (int64_t)L->n + 2 * (int64_t)L->m;
+        L->val32 = xcalloc((size_t)nnz, sizeof(float));
+        L->sm32 = xcalloc((size_t)L->n, sizeof(float));
+        L->r32 = xcalloc((size_t)L->n, sizeof(float));
+        L->z32 = xcalloc((size_t)L->n, sizeof(float));
+        L->z232 = xcalloc((size_t)L->n, sizeof(float));
+        L->t32 = xcalloc((size_t)L->n, sizeof(float));
+        if (!L->val32 || !L->sm32 || !L->r32 || !L->z32 || !L->z232 || !L->t32)
+            return OR_ERR_NOMEM;
+        for (int64_t j = 0; j < nnz; ++j) L->val32[j] = (float)L->val[j];
+        for (int i = 0; i < L->n; ++i) L->sm32[i] = (float)L->sm[i];
+    }
+    if (a->ainv) {
+        const int n = a->lv[a->nl - 1].n;
+        a->ainv32 = xcalloc((size_t)n * n, sizeof(float));
+        if (!a->ainv32) return OR_ERR_NOMEM;
+        for (int64_t k = 0; k < (int64_t)n * n; ++k) a->ainv32[k] = (float)a->ainv[k];
+    }
+    return OR_OK;
+}
+
+static void smooth32(const level_t *L, const float *r, const float *z, float *zn) {
+    or_spmv_f32(L->n, L->rowptr, L->col, L->val32, z, L->t32);
+    for (int i = 0; i < L->n; ++i) zn[i] = fmaf(L->sm32[i], r[i] - L->t32[i], z[i]);
+}
+
+static void vcycle32(const amg_t *a, int l, const float *r, float *z) {
+    const level_t *L = &a->lv[l];
+    if (l == a->nl - 1) {
+        if (a->ainv32) {
+            for (int i = 0; i < L->n; ++i) {
+                float s = 0.0f;
+                for (int j = 0; j < L->n; ++j) s = s + a->ainv32[(size_t)i * L->n + j] * r[j];
+                z[i] = s;
+            }
+        } else {
+            for (int i = 0; i < L->n; ++i) z[i] = L->sm32[i] * r[i];
+        }
+        return;
+    }
+    level_t *C = (level_t *)&a->lv[l + 1];
+    float *za = L->z32, *zb = L->z232;
+    for (int i = 0; i < L->n; ++i) za[i] = L->sm32[i] * r[i];
+    for (int s = 1; s < a->nu; ++s) {
+        smooth32(L, r, za, zb);
+        float *tmp = za; za = zb; zb = tmp;
+    }
+    or_spmv_f32(L->n, L->rowptr, L->col, L->val32, za, L->t32);
+    for (int i = 0; i < L->n; ++i) L->t32[i] = r[i] - L->t32[i];
+    for (int c = 0; c < C->n; ++c) C->r32[c] = 0.0f;
+    for (int i = 0; i < L->n; ++i) C->r32[L->agg[i]] = C->r32[L->agg[i]] + L->t32[i];
+    float *zc = xcalloc((size_t)C->n, sizeof(float));
+    vcycle32(a, l + 1, C->r32, zc);
+    const float cs = (float)a->cscale;
+    for (int i = 0; i < L->n; ++i) za[i] = fmaf(cs, zc[L->agg[i]], za[i]);
+    free(zc);
+    for (int s = 0; s < a->nu; ++s) {
+        smooth32(L, r, za, zb);
+        float *tmp = za; za = zb; zb = tmp;
+    }
+    memcpy(z, za, sizeof(float) * (size_t)L->n);
+}
+
+/* the FP32 cycle on a double rhs: (float) r in, (double) z out */
+OR_API int or_amg_vcycle32(void *p, const double *r, double *z) {
+    amg_t *a = (amg_t *)p;
+    if (make32(a) != OR_OK) return OR_ERR_NOMEM;
+    const int n = a->lv[0].n;
+    float *r32 = xcalloc((size_t)n, sizeof(float)), *z32 = xcalloc((size_t)n, sizeof(float));
+    for (int i = 0; i < n; ++i) r32[i] = (float)r[i];
+    vcycle32(a, 0, r32, z32);
+    for (int i = 0; i < n; ++i) z[i] = (double)z32[i];
+    free(r32);
+    free(z32);
+    return OR_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* PCG                                                                        */
 /* ------------------------------------------------------------------------ */
@@ -485,8 +578,10 @@ static void precond(int pc, const amg_t *amg, int n, const double *dinv, const d
         memcpy(z, r, sizeof(double) * (size_t)n);
     } else if (pc == PC_DIAGONAL) {
         for (int i = 0; i < n; ++i) z[i] = dinv[i] * r[i];
-    } else {
+    } else if (pc == PC_GAMG) {
         vcycle(amg, 0, r, z);
+    } else {
+        (void)or_amg_vcycle32((void *)amg, r, z);
     }
 }
 
@@ -499,7 +594,7 @@ OR_API int or_pcg(int pc, int N, int M, const int *la, const int *ua, const doub
                   const double *lower, const double *upper, const double *b, double *x,
                   double error, int max_steps, double rel_tol, int64_t min_steps, double omega,
                   int nu, double cscale, double *hist, int hist_cap, int *hist_len) {
-    if (pc < PC_NONE || pc > PC_GAMG || N < 1) return OR_ERR_ARG;
+    if (pc < PC_NONE || pc > PC_GAMG32 || N < 1) return OR_ERR_ARG;
     amg_t *amg = NULL;
     level_t A;
     memset(&A, 0, sizeof(A));
@@ -512,9 +607,10 @@ OR_API int or_pcg(int pc, int N, int M, const int *la, const int *ua, const doub
     A.upper = (double *)upper;
     if (level_csr(&A) != OR_OK) return OR_ERR_ARG;
     or_ldu2csr_values(N, M, (int64_t)N + 2 * (int64_t)M, A.perm, diag, lower, upper, A.val);
-    if (pc == PC_GAMG) {
+    if (pc >= PC_GAMG) {
         amg = or_amg_build(N, M, la, ua, diag, lower, upper, omega, nu, cscale);
-        if (!amg) {
+        if (!amg || (pc == PC_GAMG32 && make32(amg) != OR_OK)) {
+            or_amg_free(amg);
             free(A.rowptr); free(A.col); free(A.perm); free(A.val);
             return OR_ERR_NOMEM;
         }

@@ -355,7 +355,7 @@ def foam_solve(method: str, sys, psi, source, tolerance: float, max_iter: int,
 # ---------------------------------------------------------------------------
 # preconditioned CG (none / diagonal / GAMG), gis_oracle_amg.c
 # ---------------------------------------------------------------------------
-PRECONDITIONERS = {"none": 0, "diagonal": 1, "GAMG": 2}
+PRECONDITIONERS = {"none": 0, "diagonal": 1, "GAMG": 2, "GAMG32": 3}
 GAMG_OMEGA, GAMG_NU, GAMG_CSCALE = 0.8, 1, 1.8
 
 
@@ -374,6 +374,8 @@ def _amg_sigs(L):
     L.or_amg_level_get.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 7
     L.or_amg_dense_inverse.argtypes = [C.c_void_p, _f64p]
     L.or_amg_vcycle.argtypes = [C.c_void_p, _f64p, _f64p]
+    L.or_amg_vcycle32.restype = C.c_int
+    L.or_amg_vcycle32.argtypes = [C.c_void_p, _f64p, _f64p]
     L.or_pair_pass.restype = C.c_int
     L.or_pair_pass.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _i32p]
 
@@ -474,6 +476,13 @@ class Amg:
     def vcycle(self, r):
         z = np.zeros(self.n)
         self._L.or_amg_vcycle(self._h, _c(r, np.float64), z)
+        return z
+
+    def vcycle32(self, r):
+        """The FP32 cycle: (float) r in, (double) z out."""
+        z = np.zeros(self.n)
+        if self._L.or_amg_vcycle32(self._h, _c(r, np.float64), z) != 0:
+            raise RuntimeError("or_amg_vcycle32 failed")
         return z
 
 

@@ -63,12 +63,13 @@ def test_gamg_hierarchy_matches_the_oracle(gpu, n, symmetric):
             np.testing.assert_array_equal(a["agg"], b["agg"], err_msg=f"level {l} agg")
 
 
+@pytest.mark.parametrize("pc", ["GAMG", "GAMG32"])
 @pytest.mark.parametrize("n,seed", [(4, 0), (8, 1), (16, 2), (24, 3), (40, 4)])
-def test_pcg_gamg_bit_exact(gpu, n, seed):
+def test_pcg_gamg_bit_exact(gpu, pc, n, seed):
     s = _case(n, seed, shift=1e-3)
     gpu.set_system(s)
-    _check(gpu, "GAMG", s, s.x0, 0.0, 12)
-    ref = _check(gpu, "GAMG", s, s.x0, 1e-10 * np.linalg.norm(s.b), 1000)
+    _check(gpu, pc, s, s.x0, 0.0, 12)
+    ref = _check(gpu, pc, s, s.x0, 1e-10 * np.linalg.norm(s.b), 1000)
     assert ref.hist[-1] <= 1e-10 * np.linalg.norm(s.b)
 
 
@@ -79,6 +80,7 @@ def test_pcg_gamg_schedules(gpu, schedule, coarsen):
     gpu.set_system(s)
     gpu.set_schedule(schedule, -1, -1, -1, -1, coarsen)
     _check(gpu, "GAMG", s, s.x0, 0.0, 10)
+    _check(gpu, "GAMG32", s, s.x0, 0.0, 10)
     _check(gpu, "diagonal", s, s.x0, 0.0, 10)
     gpu.set_schedule(0, -1, -1, -1, -1, 1)
 
@@ -89,10 +91,11 @@ def test_pcg_gamg_settings(gpu, omega, nu, cscale):
     gpu.set_system(s)
     gpu.set_gamg(omega, nu, cscale)
     try:
-        ref = O.pcg("GAMG", s, s.b, s.x0, 0.0, 10, omega=omega, nu=nu, cscale=cscale)
-        x, hist, st = gpu.solve_pcg("GAMG", s.b, s.x0, error=0.0, max_steps=10)
-        np.testing.assert_array_equal(hist, ref.hist)
-        np.testing.assert_array_equal(x, ref.x)
+        for pc in ("GAMG", "GAMG32"):
+            ref = O.pcg(pc, s, s.b, s.x0, 0.0, 10, omega=omega, nu=nu, cscale=cscale)
+            x, hist, st = gpu.solve_pcg(pc, s.b, s.x0, error=0.0, max_steps=10)
+            np.testing.assert_array_equal(hist, ref.hist)
+            np.testing.assert_array_equal(x, ref.x)
     finally:
         gpu.set_gamg()
 
@@ -105,8 +108,10 @@ def test_gamg_values_refresh_keeps_the_agglomeration(gpu):
     # the coarse values must follow the new values
     t = mesh.LduSystem(s.n_cells, s.lower_addr, s.upper_addr, 2.0 * s.diag, 2.0 * s.lower,
                        2.0 * s.upper, s.b, s.x0)
+    _check(gpu, "GAMG32", s, s.x0, 0.0, 8)
     gpu.set_values(t.diag, t.lower, t.upper)
     _check(gpu, "GAMG", t, t.x0, 0.0, 8)
+    _check(gpu, "GAMG32", t, t.x0, 0.0, 8)
 
 
 def test_gamg_single_level_and_no_faces(gpu):
@@ -115,6 +120,7 @@ def test_gamg_single_level_and_no_faces(gpu):
     assert gpu.gamg_levels() == 1
     ref = _check(gpu, "GAMG", s, s.x0, 1e-9 * np.linalg.norm(s.b), 50)
     assert ref.iterations == 1
+    _check(gpu, "GAMG32", s, s.x0, 0.0, 6)
     n = 300
     d = mesh.LduSystem(n, np.zeros(0, np.int32), np.zeros(0, np.int32),
                        np.random.default_rng(3).uniform(1, 2, n), np.zeros(0), np.zeros(0),
@@ -122,9 +128,10 @@ def test_gamg_single_level_and_no_faces(gpu):
     gpu.set_system(d)
     assert gpu.gamg_levels() == 1
     _check(gpu, "GAMG", d, d.x0, 1e-12, 50)
+    _check(gpu, "GAMG32", d, d.x0, 0.0, 5)
 
 
-@pytest.mark.parametrize("pc", ["none", "diagonal", "GAMG"])
+@pytest.mark.parametrize("pc", ["none", "diagonal", "GAMG", "GAMG32"])
 def test_foam_pcg_wrapper(gpu, pc):
     from cufoam_amd import foam
     s = _case(16, 10, shift=1e-3)

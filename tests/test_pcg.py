@@ -189,3 +189,20 @@ def test_pcg_rel_tol_and_min_steps():
     r = O.pcg("GAMG", s, s.b, s.x0, 1e30, 100, min_steps=4)
     assert r.iterations == 4
     np.testing.assert_array_equal(r.hist, h[:5])
+
+
+def test_fp32_cycle_tracks_the_fp64_cycle_and_converges():
+    s = mesh.laplacian(20, seed=11, shift=1e-3)
+    a = O.Amg(s)
+    r = np.random.default_rng(2).standard_normal(s.n_cells)
+    z64, z32 = a.vcycle(r), a.vcycle32(r)
+    assert 1e-12 < np.linalg.norm(z32 - z64) / np.linalg.norm(z64) < 1e-5
+    # it is the FP64 cycle applied in float: the same operator up to float rounding
+    z32b = a.vcycle32(2.0 * r)
+    np.testing.assert_array_equal(z32b, 2.0 * z32)   # scaling by 2 is exact in float
+    err = 1e-10 * np.linalg.norm(s.b)
+    g64 = O.pcg("GAMG", s, s.b, s.x0, err, 1000)
+    g32 = O.pcg("GAMG32", s, s.b, s.x0, err, 1000)
+    assert g32.hist[-1] <= err and g32.iterations <= g64.iterations + 3
+    A = O.ldu2csr(s).scipy()
+    assert np.linalg.norm(s.b - A @ g32.x) <= 2 * err
