@@ -36,6 +36,7 @@ namespace {
 constexpr int kBlk = 256;
 constexpr int kDense = 64;        // the last level is inverted densely at or below this size
 constexpr int kAmgLevels = 32;
+constexpr int kGammaSlot = 7;     // KState::aux slot of (r, z_prev) for GAMG32's beta
 
 inline dim3 grid_of(long long n) { return dim3((unsigned)((n + kBlk - 1) / kBlk)); }
 
@@ -352,7 +353,8 @@ __global__ __launch_bounds__(kWG) void k_vc_sweep(Mat<T> m, int N, const T* __re
 }
 
 // GAMG32's last sweep on the finest level: zn = fmaf(sm, (float) r - A z, z),
-// out = (double) zn, rho = (r, out) in FP64, beta = rho / rho_prev.
+// out = (double) zn, rho = (r, out) in FP64 and the flexible direction
+// factor beta = (rho - gamma) / rho_prev, gamma = (r, z_prev) from k_pcg2.
 template <class P>
 __global__ __launch_bounds__(kWG) void k_vc_sweep_out(Mat<float> m, int N,
                                                       const float* __restrict__ sm,
@@ -382,7 +384,7 @@ __global__ __launch_bounds__(kWG) void k_vc_sweep_out(Mat<float> m, int N,
     finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
         st->rho1 = st->rho0;
         st->rho0 = v[0];
-        st->beta = st->rho0 / st->rho1;
+        st->beta = (st->rho0 - st->aux[kGammaSlot]) / st->rho1;
     });
 }
 
@@ -573,7 +575,7 @@ __global__ __launch_bounds__(kWG) void k_pcg_rho(int N, const double* __restrict
     });
 }
 
-// one-level GAMG32: out = (double) z32, rho = (r, out), beta = rho / rho_prev
+// one-level GAMG32: out = (double) z32, rho = (r, out), flexible beta as k_vc_sweep_out
 __global__ __launch_bounds__(kWG) void k_pcg_rho32(int N, const double* __restrict__ r,
                                                    const float* __restrict__ z32,
                                                    double* __restrict__ out, KState* st,
@@ -595,7 +597,7 @@ __global__ __launch_bounds__(kWG) void k_pcg_rho32(int N, const double* __restri
     finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
         st->rho1 = st->rho0;
         st->rho0 = v[0];
-        st->beta = st->rho0 / st->rho1;
+        st->beta = (st->rho0 - st->aux[kGammaSlot]) / st->rho1;
     });
 }
 
@@ -662,7 +664,7 @@ __global__ __launch_bounds__(kWG) void k_pcg2(int N, double* __restrict__ r,
                                               float* __restrict__ r32, float* __restrict__ z32,
                                               KState* st, RedWs red) {
     if (st->done) return;
-    constexpr int NV = PC == 1 ? 2 : 1;
+    constexpr int NV = (PC == 1 || PC == 3) ? 2 : 1;
     __shared__ RedShared shr;
     const SuperTile<C> sup(N, true);
     const double alpha = st->alpha;
@@ -678,13 +680,15 @@ __global__ __launch_bounds__(kWG) void k_pcg2(int N, double* __restrict__ r,
         load4(r, t.i0, N, rv[c]);
         load4(q, t.i0, N, qv);
         if (PC == 1 || PC == 2) load4(dinv, t.i0, N, dv);
+        if (PC == 3) load4(z, t.i0, N, zv[c]);   // the previous preconditioned residual
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             rv[c][kk] = fma(-alpha, qv[kk], rv[c][kk]);
             if (PC == 1 || PC == 2) zv[c][kk] = dv[kk] * rv[c][kk];
             if (t.i0 + kk < N) {
                 leaf[c][0] = fma(rv[c][kk], rv[c][kk], leaf[c][0]);
-                if (PC == 1) leaf[c][NV - 1] = fma(rv[c][kk], zv[c][kk], leaf[c][NV - 1]);
+                if (PC == 1 || PC == 3)
+                    leaf[c][NV - 1] = fma(rv[c][kk], zv[c][kk], leaf[c][NV - 1]);
             }
         }
     }
@@ -716,6 +720,7 @@ __global__ __launch_bounds__(kWG) void k_pcg2(int N, double* __restrict__ r,
             st->rho0 = v[NV - 1];
             st->beta = st->rho0 / st->rho1;
         }
+        if (PC == 3) st->aux[kGammaSlot] = v[1];
         if (stop_after(st, !(norm > st->error))) st->done = 1;
     });
 }

@@ -48,6 +48,10 @@
  *   inverse rounded to float, float vectors and float arithmetic (fmaf
  *   where the FP64 cycle has fma, cscale as float); the finest rhs is
  *   (float) r and the result is widened to double; rho = (r, z) in FP64.
+ *   The rounding makes the preconditioner vary slightly from one application
+ *   to the next, so GAMG32 takes the flexible (Polak-Ribiere) direction
+ *   update beta = (rho' - (r', z)) / rho with z the previous preconditioned
+ *   residual; (r', z) is 0 in exact arithmetic, where it is CG's beta.
  *
  * Arithmetic contract as in gis_oracle.c (-ffp-contract=off, fma only where
  * written, sequential CSR row sums, the fixed reduction tree of or_dot).
@@ -644,9 +648,10 @@ OR_API int or_pcg(int pc, int N, int M, const int *la, const int *ua, const doub
         if (hist && len < hist_cap) hist[len] = norm;
         len++;
         if (!((k + 1 <= ms && norm > error) || k + 1 <= min_steps)) break;
+        const double gamma = pc == PC_GAMG32 ? or_dot(N, r, z) : 0.0;
         precond(pc, amg, N, dinv, r, z);
         const double rho_new = or_dot(N, r, z);
-        beta = rho_new / rho;
+        beta = pc == PC_GAMG32 ? (rho_new - gamma) / rho : rho_new / rho;
         rho = rho_new;
     }
     if (hist_len) *hist_len = len;
