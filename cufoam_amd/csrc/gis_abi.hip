@@ -434,7 +434,7 @@ GIS_API int gis_dot_device(gis_handle h, const double* d_a, const double* d_b, d
 GIS_API int gis_set_schedule(gis_handle h, const gis_schedule* s) {
     if (!h || !s) return GIS_ERR_ARG;
     auto coarsen_ok = [](int c) { return c <= 0 || c == 1 || c == 2 || c == 4; };
-    if (s->spmv64 > 5 || s->spmv32 > 5 || s->fp64_direction > 1 || s->fp32_direction > 1 ||
+    if (s->spmv64 > 6 || s->spmv32 > 6 || s->fp64_direction > 1 || s->fp32_direction > 1 ||
         !coarsen_ok(s->coarsen64) || !coarsen_ok(s->coarsen32))
         return fail(h, GIS_ERR_ARG, "schedule field out of range");
     if (s->spmv64 >= 0) h->sched.spmv64 = static_cast<SpmvKind>(s->spmv64);

@@ -566,6 +566,127 @@ __device__ __forceinline__ void tile_sell_c(const int* __restrict__ soff,
     wave_sync_lds();
 }
 
+// Slices with a column-offset table (SellD): the slice's rows share a sorted
+// list of U column offsets (doffs[ooff[s] ..]); slot k of a row holds column
+// row + doffs[ooff[s] + k] when bit k of the row's mask is set, and is a
+// padding slot otherwise.  A row's entries ascend in column, so the set bits
+// visit them in storage order: same products, same order, same sum as
+// sell_row, with no column index per slot.  Values: slot-major from 64 * ooff[s].
+template <typename T, class XOp>
+__device__ __forceinline__ T sell_row_d(const int* __restrict__ offs, unsigned mask,
+                                        const T* __restrict__ dval, int a, int len, int lane,
+                                        int row, XOp&& xop) {
+    T acc = T(0);
+    for (int j = 0; j < len; j += 8) {
+        T v[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            v[u] = T(0);
+            if (j + u < len) v[u] = __builtin_nontemporal_load(dval + a + (j + u) * kSlice + lane);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            xv[u] = (j + u < len && ((mask >> (j + u)) & 1u)) ? xop(row + offs[j + u]) : T(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (j + u < len && ((mask >> (j + u)) & 1u)) acc = acc + v[u] * xv[u];
+    }
+    return acc;
+}
+
+// Two offset-table slices per lane at a time (their loads in flight together).
+template <typename T, class XOp>
+__device__ __forceinline__ void sell_row2_d(const int* __restrict__ offs0, unsigned m0, int a0,
+                                            int len0, int row0, const int* __restrict__ offs1,
+                                            unsigned m1, int a1, int len1, int row1,
+                                            const T* __restrict__ dval, int lane, XOp&& xop,
+                                            T& y0, T& y1) {
+    T acc0 = T(0), acc1 = T(0);
+    const int len = len0 > len1 ? len0 : len1;
+    for (int j = 0; j < len; j += 8) {
+        T v0[8], v1[8], x0[8], x1[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            v0[u] = T(0);
+            v1[u] = T(0);
+            if (j + u < len0)
+                v0[u] = __builtin_nontemporal_load(dval + a0 + (j + u) * kSlice + lane);
+            if (j + u < len1)
+                v1[u] = __builtin_nontemporal_load(dval + a1 + (j + u) * kSlice + lane);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            x0[u] = (j + u < len0 && ((m0 >> (j + u)) & 1u)) ? xop(row0 + offs0[j + u]) : T(0);
+            x1[u] = (j + u < len1 && ((m1 >> (j + u)) & 1u)) ? xop(row1 + offs1[j + u]) : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (j + u < len0 && ((m0 >> (j + u)) & 1u)) acc0 = acc0 + v0[u] * x0[u];
+            if (j + u < len1 && ((m1 >> (j + u)) & 1u)) acc1 = acc1 + v1[u] * x1[u];
+        }
+    }
+    y0 = acc0;
+    y1 = acc1;
+}
+
+// tile over SellD slices two at a time; a slice without a table (ooff step
+// 0: its rows do not ascend, or their offsets do not fit 32 slots) reads the
+// int32 slot columns instead.
+template <typename T, class XOp>
+__device__ __forceinline__ void tile_sell_d(const int* __restrict__ soff,
+                                            const int* __restrict__ scol,
+                                            const T* __restrict__ sval,
+                                            const int* __restrict__ ooff,
+                                            const int* __restrict__ doffs,
+                                            const unsigned* __restrict__ dmask,
+                                            const T* __restrict__ dval, int N, int row0,
+                                            XOp&& xop, T* __restrict__ sh_y) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nslices = (N + kSlice - 1) / kSlice;
+    const int s0 = row0 / kSlice + w * 4;
+#pragma unroll
+    for (int c = 0; c < 4; c += 2) {
+        const int s = s0 + c;
+        int u0 = 0, u1 = 0, o0 = 0, o1 = 0;
+        if (s < nslices) {
+            o0 = ooff[s];
+            u0 = ooff[s + 1] - o0;
+        }
+        if (s + 1 < nslices) {
+            o1 = ooff[s + 1];
+            u1 = ooff[s + 2] - o1;
+        }
+        T y0 = T(0), y1 = T(0);
+        const int r0 = s * kSlice + lane, r1 = (s + 1) * kSlice + lane;
+        if (u0 > 0 && u1 > 0) {
+            const unsigned m0 = r0 < N ? dmask[r0] : 0u;
+            const unsigned m1 = r1 < N ? dmask[r1] : 0u;
+            sell_row2_d<T>(doffs + o0, m0, o0 * kSlice, u0, r0, doffs + o1, m1, o1 * kSlice, u1,
+                           r1, dval, lane, xop, y0, y1);
+        } else {
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int sp = s + p;
+                if (sp >= nslices) continue;
+                const int up = p ? u1 : u0, op = p ? o1 : o0, rp = p ? r1 : r0;
+                T y;
+                if (up > 0) {
+                    const unsigned mp = rp < N ? dmask[rp] : 0u;
+                    y = sell_row_d<T>(doffs + op, mp, dval, op * kSlice, up, lane, rp, xop);
+                } else {
+                    const int a = soff[sp];
+                    y = sell_row<T>(scol, sval, a, (soff[sp + 1] - a) / kSlice, lane, xop);
+                }
+                if (p) y1 = y; else y0 = y;
+            }
+        }
+        sh_y[(w * 4 + c) * kSlice + lane] = y0;
+        sh_y[(w * 4 + c + 1) * kSlice + lane] = y1;
+    }
+    wave_sync_lds();
+}
+
 template <typename T, class XOp>
 __device__ __forceinline__ void tile_sell(const int* __restrict__ soff,
                                           const int* __restrict__ scol,

@@ -49,7 +49,7 @@ struct DevMem {
 // SpMV schedule: how a tile's products are formed.  Every schedule yields
 // bit-identical results (same per-row order), so the choice is pure tuning.
 enum class SpmvKind : int {
-    Sell = 0, Staged2K = 1, Staged4K = 2, Direct = 3, Sell2 = 4, SellC = 5
+    Sell = 0, Staged2K = 1, Staged4K = 2, Direct = 3, Sell2 = 4, SellC = 5, SellD = 6
 };
 
 struct Schedule {
@@ -165,6 +165,13 @@ struct gis_context {
     // compressed slot columns: col = row + sbase[slot row] + sres[slot], int8
     // residuals (-128 marks padding); sflag[slice] = 1 where every residual fits
     gis::DevMem sres, sbase, sflag;
+    // column-offset tables (SellD): per slice the sorted column offsets of its
+    // rows (doffs from ooff[s]; ooff[s+1] == ooff[s]: no table, int32 columns),
+    // a bit mask per row of the offsets it holds, the slot -> value code map
+    // and the slot values (64 slots per table entry)
+    gis::DevMem ooff, doffs, dmask, dperm;
+    long long n_dslots = 0;
+    gis::DevMem dval, dvalT, dval32, dval32T;
     gis::DevMem dinv;                           // per-row helper (1/a_ii or sqrt(1/a_ii))
     gis::AmgData* amg = nullptr;                // GAMG hierarchy (built on first use)
 

@@ -20,6 +20,10 @@ struct Mat {
     const signed char* sres;      // compressed slot columns (SellC)
     const int* sbase;
     const unsigned char* sflag;
+    const int* ooff;              // column-offset tables (SellD)
+    const int* doffs;
+    const unsigned* dmask;
+    const T* dval;
 };
 
 // ---------------------------------------------------------------------------
@@ -43,6 +47,18 @@ struct SellC {
     __device__ static void run(const Mat<T>& m, int N, int row0, int, XOp&& xop, Shared<T>&,
                                T* y) {
         tile_sell_c<T>(m.soff, m.scol, m.sres, m.sbase, m.sflag, m.sval, N, row0, xop, y);
+    }
+};
+
+// Sliced rows with per-slice column-offset tables and row masks, two slices at
+// a time (no column index per slot).
+struct SellD {
+    template <typename T>
+    struct Shared {};
+    template <typename T, class XOp>
+    __device__ static void run(const Mat<T>& m, int N, int row0, int, XOp&& xop, Shared<T>&,
+                               T* y) {
+        tile_sell_d<T>(m.soff, m.scol, m.sval, m.ooff, m.doffs, m.dmask, m.dval, N, row0, xop, y);
     }
 };
 
@@ -98,6 +114,10 @@ struct Direct {
         } break;                                      \
         case SpmvKind::SellC: {                       \
             using POLICY = SellC;                     \
+            __VA_ARGS__;                              \
+        } break;                                      \
+        case SpmvKind::SellD: {                       \
+            using POLICY = SellD;                     \
             __VA_ARGS__;                              \
         } break;                                      \
         default: {                                    \
@@ -290,28 +310,36 @@ inline Mat<double> mat64(const gis_context* h) {
     return Mat<double>{h->rowptr.as<int>(), h->col.as<int>(), h->val.as<double>(),
                        h->soff.as<int>(), h->scol.as<int>(), h->sval.as<double>(),
                        h->sres.as<signed char>(), h->sbase.as<int>(),
-                       h->sflag.as<unsigned char>()};
+                       h->sflag.as<unsigned char>(),
+                       h->ooff.as<int>(), h->doffs.as<int>(), h->dmask.as<unsigned>(),
+                       h->dval.as<double>()};
 }
 
 inline Mat<double> mat64T(const gis_context* h) {
     return Mat<double>{h->rowptr.as<int>(), h->col.as<int>(), h->valT.as<double>(),
                        h->soff.as<int>(), h->scol.as<int>(), h->svalT.as<double>(),
                        h->sres.as<signed char>(), h->sbase.as<int>(),
-                       h->sflag.as<unsigned char>()};
+                       h->sflag.as<unsigned char>(),
+                       h->ooff.as<int>(), h->doffs.as<int>(), h->dmask.as<unsigned>(),
+                       h->dvalT.as<double>()};
 }
 
 inline Mat<float> mat32T(const gis_context* h) {
     return Mat<float>{h->rowptr.as<int>(), h->col.as<int>(), h->val32T.as<float>(),
                       h->soff.as<int>(), h->scol.as<int>(), h->sval32T.as<float>(),
                       h->sres.as<signed char>(), h->sbase.as<int>(),
-                      h->sflag.as<unsigned char>()};
+                      h->sflag.as<unsigned char>(),
+                      h->ooff.as<int>(), h->doffs.as<int>(), h->dmask.as<unsigned>(),
+                      h->dval32T.as<float>()};
 }
 
 inline Mat<float> mat32(const gis_context* h) {
     return Mat<float>{h->rowptr.as<int>(), h->col.as<int>(), h->val32.as<float>(),
                       h->soff.as<int>(), h->scol.as<int>(), h->sval32.as<float>(),
                       h->sres.as<signed char>(), h->sbase.as<int>(),
-                      h->sflag.as<unsigned char>()};
+                      h->sflag.as<unsigned char>(),
+                      h->ooff.as<int>(), h->doffs.as<int>(), h->dmask.as<unsigned>(),
+                      h->dval32.as<float>()};
 }
 
 // Iterations the host may launch: the cap, or min_steps when that is larger

@@ -162,6 +162,80 @@ __global__ __launch_bounds__(kBlk) void k_sort_lower(int N, const int* __restric
     for (int i = 0; i < n; ++i) col[a + i] = la[perm[a + i]];
 }
 
+// SellD tables, one wave per 64-row slice.  A slice gets a table when every
+// row's columns strictly ascend, no row holds more than 32 entries, and the
+// union of the rows' column offsets (col - row) has at most 32 entries and
+// pads the slice by at most a quarter.  ucount[s] = table size (0: none).
+__global__ __launch_bounds__(kBlk) void k_selld_union(int N, int S,
+                                                      const int* __restrict__ rowptr,
+                                                      const int* __restrict__ col, int* ucount,
+                                                      int* tmp_offs, unsigned* dmask) {
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * (kBlk / 64) + (threadIdx.x >> 6);
+    if (s >= S) return;   // whole waves
+    const int row = s * 64 + lane;
+    const bool valid = row < N;
+    const int a = valid ? rowptr[row] : 0;
+    const int len = valid ? rowptr[row + 1] - a : 0;
+    bool ok = len <= 32;
+    for (int j = 1; j < len && ok; ++j)
+        if (col[a + j] <= col[a + j - 1]) ok = false;
+    int bad = ok ? 0 : 1, total = len;
+    for (int o = 32; o > 0; o >>= 1) {
+        bad += __shfl_xor(bad, o, 64);
+        total += __shfl_xor(total, o, 64);
+    }
+    bool fit = bad == 0;
+    unsigned mask = 0u;
+    int U = 0, p = 0;
+    while (fit) {
+        const int head = p < len ? col[a + p] - row : INT_MAX;
+        int m = head;
+        for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o, 64));
+        if (m == INT_MAX) break;
+        if (U == 32) {
+            fit = false;
+            break;
+        }
+        if (head == m) {
+            mask |= 1u << U;
+            ++p;
+        }
+        if (lane == 0) tmp_offs[(size_t)s * 32 + U] = m;
+        ++U;
+    }
+    if (fit && U * 64 * 4 > 5 * total) fit = false;
+    if (valid) dmask[row] = fit ? mask : 0u;
+    if (lane == 0) ucount[s] = fit ? U : 0;
+}
+
+// offsets compacted to doffs[ooff[s] ..]; slot (table entry k, lane) -> value
+// code of the row's next entry when bit k is set, -1 for a padding slot
+__global__ __launch_bounds__(kBlk) void k_selld_fill(int N, int S,
+                                                     const int* __restrict__ rowptr,
+                                                     const int* __restrict__ perm,
+                                                     const int* __restrict__ ooff,
+                                                     const int* __restrict__ tmp_offs,
+                                                     const unsigned* __restrict__ dmask,
+                                                     int* doffs, int* dperm) {
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * (kBlk / 64) + (threadIdx.x >> 6);
+    if (s >= S) return;
+    const int o = ooff[s];
+    const int U = ooff[s + 1] - o;
+    if (U == 0) return;
+    if (lane < U) doffs[o + lane] = tmp_offs[(size_t)s * 32 + lane];
+    const int row = s * 64 + lane;
+    const unsigned m = row < N ? dmask[row] : 0u;
+    const int a = row < N ? rowptr[row] : 0;
+    int j = 0;
+    for (int k = 0; k < U; ++k) {
+        int code = -1;
+        if ((m >> k) & 1u) code = perm[a + j++];
+        dperm[((size_t)o + k) * 64 + lane] = code;
+    }
+}
+
 // coupling entries at the end of their row, then ordered by entry index
 __global__ __launch_bounds__(kBlk) void k_fill_cpl(int N, int M, int C,
                                                    const int* __restrict__ cr,
@@ -420,6 +494,33 @@ int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua
                        h->soff.as<int>(), h->scol.as<int>(), h->sres.as<signed char>(),
                        h->sbase.as<int>(), h->sflag.as<unsigned char>());
     GIS_HIP(h, hipGetLastError());
+    // column-offset tables (SellD)
+    {
+        DevMem ucount, tmp_offs;
+        GIS_HIP(h, ucount.ensure(sizeof(int) * ((size_t)S + 1)));
+        GIS_HIP(h, tmp_offs.ensure(sizeof(int) * 32 * (size_t)S));
+        GIS_HIP(h, h->ooff.ensure(sizeof(int) * ((size_t)S + 1)));
+        GIS_HIP(h, h->dmask.ensure(sizeof(unsigned) * (size_t)N));
+        GIS_HIP(h, hipMemsetAsync(ucount.p, 0, sizeof(int) * ((size_t)S + 1), h->stream));
+        const dim3 gs((unsigned)((S + kBlk / 64 - 1) / (kBlk / 64)));
+        hipLaunchKernelGGL(k_selld_union, gs, dim3(kBlk), 0, h->stream, N, S,
+                           h->rowptr.as<int>(), h->col.as<int>(), ucount.as<int>(),
+                           tmp_offs.as<int>(), h->dmask.as<unsigned>());
+        GIS_HIP(h, hipGetLastError());
+        GIS_TRY(scan_exclusive(h, ucount.as<int>(), h->ooff.as<int>(), S + 1, tmp, scan_tmp));
+        int n_tab = 0;
+        GIS_HIP(h, hipMemcpyAsync(&n_tab, h->ooff.as<int>() + S, sizeof(int),
+                                  hipMemcpyDeviceToHost, h->stream));
+        GIS_HIP(h, hipStreamSynchronize(h->stream));
+        h->n_dslots = 64LL * n_tab;
+        GIS_HIP(h, h->doffs.ensure(sizeof(int) * std::max(n_tab, 1)));
+        GIS_HIP(h, h->dperm.ensure(sizeof(int) * (size_t)std::max<long long>(h->n_dslots, 1)));
+        hipLaunchKernelGGL(k_selld_fill, gs, dim3(kBlk), 0, h->stream, N, S, h->rowptr.as<int>(),
+                           h->perm.as<int>(), h->ooff.as<int>(), tmp_offs.as<int>(),
+                           h->dmask.as<unsigned>(), h->doffs.as<int>(), h->dperm.as<int>());
+        GIS_HIP(h, hipGetLastError());
+        GIS_HIP(h, hipStreamSynchronize(h->stream));   // the temporaries are freed next
+    }
     h->n_slots = n_slots;
     h->N = N;
     h->M = M;
@@ -431,7 +532,8 @@ int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua
 // CSR values and slot values of one matrix: A, or A^T (lower and upper swapped,
 // each coupling entry taking the value of its mirrored entry)
 template <typename T>
-static int gather_both(gis_context* h, bool transpose, DevMem& csr, DevMem& sell) {
+static int gather_both(gis_context* h, bool transpose, DevMem& csr, DevMem& sell,
+                       DevMem& dsell) {
     const double* lower = (transpose ? h->upper : h->lower).as<double>();
     const double* upper = (transpose ? h->lower : h->upper).as<double>();
     const double* cpl = h->n_cpl ? h->cpl_val.as<double>() : nullptr;
@@ -446,12 +548,18 @@ static int gather_both(gis_context* h, bool transpose, DevMem& csr, DevMem& sell
                        h->N, h->M, h->sperm.as<int>(), h->diag.as<double>(), lower, upper, cpl,
                        map, sell.as<T>());
     GIS_HIP(h, hipGetLastError());
+    GIS_HIP(h, dsell.ensure(sizeof(T) * (size_t)std::max<long long>(h->n_dslots, 1)));
+    if (h->n_dslots > 0)
+        hipLaunchKernelGGL(k_gather<T>, grid_for(h->n_dslots), dim3(kBlk), 0, h->stream,
+                           h->n_dslots, h->N, h->M, h->dperm.as<int>(), h->diag.as<double>(),
+                           lower, upper, cpl, map, dsell.as<T>());
+    GIS_HIP(h, hipGetLastError());
     return kOK;
 }
 
 int gather_values(gis_context* h) {
     if (!h->has_pattern) return fail(h, kErrState, "pattern not set");
-    GIS_TRY(gather_both<double>(h, false, h->val, h->sval));
+    GIS_TRY(gather_both<double>(h, false, h->val, h->sval, h->dval));
     if (h->amg) h->amg->values_ok = false;
     h->has_values = true;
     h->has_val32 = false;
@@ -463,7 +571,7 @@ int gather_values(gis_context* h) {
 int ensure_val32(gis_context* h) {
     if (!h->has_values) return fail(h, kErrState, "values not set");
     if (h->has_val32) return kOK;
-    GIS_TRY(gather_both<float>(h, false, h->val32, h->sval32));
+    GIS_TRY(gather_both<float>(h, false, h->val32, h->sval32, h->dval32));
     h->has_val32 = true;
     return kOK;
 }
@@ -473,7 +581,7 @@ int ensure_val32(gis_context* h) {
 int ensure_val32T(gis_context* h) {
     if (!h->has_values) return fail(h, kErrState, "values not set");
     if (h->has_val32T) return kOK;
-    GIS_TRY(gather_both<float>(h, true, h->val32T, h->sval32T));
+    GIS_TRY(gather_both<float>(h, true, h->val32T, h->sval32T, h->dval32T));
     h->has_val32T = true;
     return kOK;
 }
@@ -481,7 +589,7 @@ int ensure_val32T(gis_context* h) {
 int ensure_valT(gis_context* h) {
     if (!h->has_values) return fail(h, kErrState, "values not set");
     if (h->has_valT) return kOK;
-    GIS_TRY(gather_both<double>(h, true, h->valT, h->svalT));
+    GIS_TRY(gather_both<double>(h, true, h->valT, h->svalT, h->dvalT));
     h->has_valT = true;
     return kOK;
 }

@@ -76,7 +76,8 @@ constexpr Candidate kCandidates[] = {
     {2, 0, 1}, {2, 1, 1}, {2, 0, 2}, {2, 1, 2}, {2, 0, 4}, {2, 1, 4},
     {3, 0, 1}, {3, 1, 1}, {3, 0, 2}, {3, 1, 2}, {3, 0, 4}, {3, 1, 4},
     {4, 0, 1}, {4, 1, 1}, {4, 0, 2}, {4, 1, 2}, {4, 0, 4}, {4, 1, 4},
-    {5, 0, 1}, {5, 1, 1}, {5, 0, 2}, {5, 1, 2}, {5, 0, 4}, {5, 1, 4}};
+    {5, 0, 1}, {5, 1, 1}, {5, 0, 2}, {5, 1, 2}, {5, 0, 4}, {5, 1, 4},
+    {6, 0, 1}, {6, 1, 1}, {6, 0, 2}, {6, 1, 2}, {6, 0, 4}, {6, 1, 4}};
 
 int elapsed(gis_context* h, hipEvent_t a, hipEvent_t b, float* ms) {
     GIS_HIP(h, hipEventSynchronize(b));

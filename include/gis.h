@@ -123,7 +123,12 @@ typedef struct gis_schedule {
                             4 = sliced rows, two slices per lane at a time;
                             5 = sliced rows with compressed columns (an int32 base
                             per 64 slots plus one signed byte per slot; slices
-                            whose offsets do not fit read the int32 columns) */
+                            whose offsets do not fit read the int32 columns);
+                            6 = sliced rows with a column-offset table per slice
+                            and a bit mask per row (no column index per slot;
+                            slices whose rows do not ascend or whose offsets
+                            exceed 32 entries read the int32 columns), two
+                            slices per lane at a time */
     int spmv32;          /* FP32 SpMV (mixed-precision inner loop), same ids */
     int block_iters;     /* iterations enqueued between host checks of the stop flag */
     int fp64_direction;  /* CG direction p = r + beta p: 0 formed inside the SpMV kernel's

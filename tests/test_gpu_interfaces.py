@@ -40,7 +40,7 @@ def test_device_csr_and_transpose_match_the_oracle(gpu, n):
     np.testing.assert_array_equal(val, T.val)
 
 
-@pytest.mark.parametrize("schedule", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3, 4, 5, 6])
 def test_spmv_schedules_with_couplings(gpu, schedule):
     torch = _torch()
     s = _case(24, 2, False)

@@ -51,7 +51,7 @@ def test_method_bit_exact_to_convergence(gpu, method):
 
 
 @pytest.mark.parametrize("method", ["BiCG", "MRJacobi", "Richardson"])
-@pytest.mark.parametrize("schedule", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3, 4, 5, 6])
 def test_schedules_are_bit_identical(gpu, schedule, method):
     s = _case(32, 6, method == "Richardson", 0.05)
     gpu.set_system(s)
@@ -63,7 +63,7 @@ def test_schedules_are_bit_identical(gpu, schedule, method):
 # two reduction groups and a partial last workgroup for 2 and 4 tiles each.
 @pytest.mark.parametrize("coarsen", [1, 2, 4])
 @pytest.mark.parametrize("direction", [0, 1])
-@pytest.mark.parametrize("schedule", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("n", [8, 48])
 def test_cg_schedules_are_bit_identical(gpu, n, schedule, direction, coarsen):
     s = _case(n, 6, True, 0.05)
@@ -74,7 +74,7 @@ def test_cg_schedules_are_bit_identical(gpu, n, schedule, direction, coarsen):
 
 # BiCG: SpMV schedule x direction form (fused on-the-fly or a streamed pass)
 @pytest.mark.parametrize("direction", [0, 1])
-@pytest.mark.parametrize("schedule", [0, 3, 4, 5])
+@pytest.mark.parametrize("schedule", [0, 3, 4, 5, 6])
 @pytest.mark.parametrize("n", [8, 48])
 def test_bicg_direction_forms_are_bit_identical(gpu, n, schedule, direction):
     s = _case(n, 7, False, 0.1)

@@ -66,3 +66,20 @@ def test_mixed_128_reaches_fp64_target(gpu, mode):
     print(f"[mixed128] mode={mode} inner_iters={it} updates={len(hist) - 1} "
           f"rel_res={true_rel:.3e} err_vs_fp64={err:.3e}", flush=True)
     assert err < 1e-4
+
+
+@pytest.mark.parametrize("spmv", [0, 4, 5, 6])
+@pytest.mark.parametrize("direction,coarsen", [(0, 1), (1, 2)])
+def test_mixed_spmv_schedules_bit_exact(gpu, spmv, direction, coarsen):
+    s = _case(40, 6, 1e-4)
+    gpu.set_system(s)
+    gpu.set_schedule(spmv64=spmv, spmv32=spmv, fp32_direction=direction, coarsen32=coarsen)
+    try:
+        A = O.ldu2csr(s)
+        rx, rh, rit, rst = O.mixed_cg(A, s.b, s.x0, 1e-10, 1e-2, 5000, 0)
+        x, hist, it, rel, st = gpu.solve_mixed(s.b, s.x0, 1e-10, 1e-2, 5000, 0)
+        assert (st, it) == (rst, rit)
+        np.testing.assert_array_equal(hist, rh)
+        np.testing.assert_array_equal(x, rx)
+    finally:
+        gpu.set_schedule(spmv64=0, spmv32=0)

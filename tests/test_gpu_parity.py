@@ -89,7 +89,7 @@ def _mixed_locality_system(seed):
                           np.concatenate([a.b, r.b]), np.zeros(n0 + r.n_cells))
 
 
-@pytest.mark.parametrize("schedule", [0, 4, 5])
+@pytest.mark.parametrize("schedule", [0, 4, 5, 6])
 def test_spmv_schedules_mixed_locality(gpu, schedule):
     torch = _torch()
     s = _mixed_locality_system(3)

@@ -73,7 +73,7 @@ def test_pcg_gamg_bit_exact(gpu, pc, n, seed):
     assert ref.hist[-1] <= 1e-10 * np.linalg.norm(s.b)
 
 
-@pytest.mark.parametrize("schedule", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("coarsen", [1, 2, 4])
 def test_pcg_gamg_schedules(gpu, schedule, coarsen):
     s = _case(20, 5, shift=1e-3)

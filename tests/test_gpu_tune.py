@@ -18,8 +18,8 @@ def test_autotune_applies_caches_and_keeps_parity(gpu):
     gpu.set_system(s)
     sched, cached, ms = gpu.autotune(iterations=12)
     assert not cached and ms[0] > 0 and ms[1] > 0
-    assert sched["spmv64"] in range(6) and sched["fp64_direction"] in (0, 1)
-    assert sched["spmv32"] in range(6) and sched["fp32_direction"] in (0, 1)
+    assert sched["spmv64"] in range(7) and sched["fp64_direction"] in (0, 1)
+    assert sched["spmv32"] in range(7) and sched["fp32_direction"] in (0, 1)
     assert gpu.get_schedule() == sched
     # a second handle on the same mesh takes the cached choice without timing
     with Gis(0) as other:
