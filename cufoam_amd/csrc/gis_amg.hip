@@ -1156,8 +1156,6 @@ int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const doubl
                                    h->st);
                 hipLaunchKernelGGL((k_cg1s<P, CC>), gc, dim3(kWG), 0, h->stream, m, N, pb0, pb1,
                                    q, h->st, h->red);
-                hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(kFoldThreads), 0, h->stream,
-                                   h->st, h->red);
                 if (pc == 0) {
                     hipLaunchKernelGGL((k_pcg2<CC, 0>), gc, dim3(kWG), 0, h->stream, N, r, q,
                                        dinv, z, sm32, r32, z32, h->st, h->red);

@@ -284,71 +284,6 @@ struct TileReduce {
     }
 };
 
-// The level-0 partials of C tiles without the cross-workgroup handshake: each
-// tile's subtree (block_tree) is written with a plain store, and a fold kernel
-// launched next (fold_partials) forms the upper levels.  Same partials, same
-// tree, same totals as finish_tiles, without the per-workgroup atomic round
-// trip that keeps every workgroup alive until its ticket returns.
-template <int NV, int C>
-__device__ __forceinline__ void write_partials(const RedWs& red, int tile0, int nt,
-                                               double (&v)[C][NV], RedShared& sh) {
-    static_assert(C * NV <= kMaxNV, "RedShared holds kMaxNV values per wave");
-    double flat[C * NV];
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-#pragma unroll
-        for (int i = 0; i < NV; ++i) flat[c * NV + i] = v[c][i];
-    block_tree<C * NV>(flat, sh.w);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int c = 0; c < C; ++c)
-            if (c < nt) {
-#pragma unroll
-                for (int i = 0; i < NV; ++i)
-                    red.part[0][(size_t)(tile0 + c) * kMaxNV + i] = flat[c * NV + i];
-            }
-    }
-}
-
-// Upper levels of the reduction in one workgroup (kFoldThreads threads), in
-// the kernel after the one that wrote the level-0 partials: level by level each
-// wave folds aligned groups of kGroup partials with wave_tree, exactly the
-// groups fold_upper folds; thread 0 calls fin(totals).
-constexpr int kFoldThreads = 1024;
-
-template <int NV, class Fin>
-__device__ __forceinline__ void fold_partials(const RedWs& red, Fin&& fin) {
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
-    for (int L = 0; L < kMaxLevels; ++L) {
-        const int n = red.count[L];
-        if (n == 1) {
-            if (threadIdx.x == 0) {
-                double t[NV];
-#pragma unroll
-                for (int i = 0; i < NV; ++i) t[i] = red.part[L][i];
-                fin(t);
-            }
-            return;
-        }
-        const int groups = (n + kGroup - 1) / kGroup;
-        for (int g = w; g < groups; g += nw) {
-            double v[NV];
-            const int src = g * kGroup + lane;
-#pragma unroll
-            for (int i = 0; i < NV; ++i)
-                v[i] = src < n ? red.part[L][(size_t)src * kMaxNV + i] : 0.0;
-            wave_tree<NV>(v);
-            if (lane == 0) {
-#pragma unroll
-                for (int i = 0; i < NV; ++i) red.part[L + 1][(size_t)g * kMaxNV + i] = v[i];
-            }
-        }
-        __syncthreads();
-    }
-}
-
 // One tile per workgroup.
 template <int NV, class Fin>
 __device__ __forceinline__ void finish_tile(const RedWs& red, int tile, double (&v)[NV],

@@ -226,8 +226,7 @@ __global__ __launch_bounds__(kWG) void k_setup(Mat<double> m, int N,
 // ---------------------------------------------------------------------------
 // CG pieces shared by CG (gis_solvers.hip) and PCG (gis_amg.hip)
 // ---------------------------------------------------------------------------
-// Split schedule, pass 2: q = A p_k with one gathered vector, (p_k, q) -> the
-// level-0 partials; k_fold_alpha then forms alpha_k.
+// Split schedule, pass 2: q = A p_k with one gathered vector, (p_k, q) -> alpha_k.
 template <class P, int C>
 __global__ __launch_bounds__(kWG) void k_cg1s(Mat<double> m, int N,
                                               const double* __restrict__ pb0,
@@ -256,19 +255,15 @@ __global__ __launch_bounds__(kWG) void k_cg1s(Mat<double> m, int N,
             if (t.i0 + kk < N) leaf[c][0] = fma(pv[kk], qv[c][kk], leaf[c][0]);
         }
     }
+    TileReduce<1, C> tr;
+    tr.begin(red, sup.tile0, sup.nt, leaf, shr);
 #pragma unroll
     for (int c = 0; c < C; ++c)
         if (c < sup.nt) store4(q, TileRows(N, sup.tile0 + c).i0, N, qv[c]);
-    write_partials<1, C>(red, sup.tile0, sup.nt, leaf, shr);
+    tr.end(red, [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
 }
 
 namespace {
-
-// after k_cg1s: alpha = rho / (p, q)
-__global__ __launch_bounds__(kFoldThreads) void k_fold_alpha(KState* st, RedWs red) {
-    if (st->done) return;
-    fold_partials<1>(red, [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
-}
 
 // x += alpha_k p_k for the last iteration (the update cg1 defers)
 __global__ __launch_bounds__(kWG) void k_cg_flush(int N, double* __restrict__ x,

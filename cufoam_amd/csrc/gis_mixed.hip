@@ -209,16 +209,12 @@ __global__ __launch_bounds__(kWG) void k_mx1s(Mat<float> m, int N, const float* 
             if (t.i0 + kk < N) leaf[c][0] = fma((double)pv[kk], (double)qv[c][kk], leaf[c][0]);
         }
     }
+    TileReduce<1, C> tr;
+    tr.begin(red, sup.tile0, sup.nt, leaf, shr);
 #pragma unroll
     for (int c = 0; c < C; ++c)
         if (c < sup.nt) store4(q, TileRows(N, sup.tile0 + c).i0, N, qv[c]);
-    write_partials<1, C>(red, sup.tile0, sup.nt, leaf, shr);
-}
-
-// after k_mx1s: alpha_k = rho / (p_k, q)
-__global__ __launch_bounds__(kFoldThreads) void k_mx1s_fold(KState* st, RedWs red) {
-    if (st->done || (st->flag & F_UPD)) return;
-    fold_partials<1>(red, [&](const double (&v)[1]) {
+    tr.end(red, [&](const double (&v)[1]) {
         st->alpha = st->rho0 / v[0];
         st->flag = st->flag & ~(F_RESTART | F_FOLDED);
     });
@@ -248,16 +244,12 @@ __global__ __launch_bounds__(kWG) void k_mx2(int N, float* __restrict__ r,
             if (t.i0 + kk < N) leaf[c][0] = fma((double)rv[c][kk], (double)rv[c][kk], leaf[c][0]);
         }
     }
+    TileReduce<1, C> tr;
+    tr.begin(red, sup.tile0, sup.nt, leaf, shr);
 #pragma unroll
     for (int c = 0; c < C; ++c)
         if (c < sup.nt) store4(r, TileRows(N, sup.tile0 + c).i0, N, rv[c]);
-    write_partials<1, C>(red, sup.tile0, sup.nt, leaf, shr);
-}
-
-// after k_mx2: the iteration count, rho, the FP64-update trigger, beta, the cap
-__global__ __launch_bounds__(kFoldThreads) void k_mx2_fold(KState* st, RedWs red) {
-    if (st->done || (st->flag & F_UPD)) return;
-    fold_partials<1>(red, [&](const double (&v)[1]) {
+    tr.end(red, [&](const double (&v)[1]) {
         st->k = st->k + 1;
         st->rho1 = st->rho0;
         st->rho0 = v[0];
@@ -421,21 +413,16 @@ int device_solve_mixed(gis_context* h, const MixedArgs& a, double* x, const doub
                         GIS_TRY(timer_begin(h, 0));
                         hipLaunchKernelGGL((k_mx1s<P32, CC>), gc, dim3(kWG), 0, h->stream, m32,
                                            N, pb0, pb1, q, h->st, h->red);
-                        GIS_TRY(timer_end(h));
-                        hipLaunchKernelGGL(k_mx1s_fold, dim3(1), dim3(kFoldThreads), 0,
-                                           h->stream, h->st, h->red);
                     } else {
                         GIS_TRY(timer_begin(h, 0));
                         hipLaunchKernelGGL((k_mx1<P32, CC>), gc, dim3(kWG), 0, h->stream, m32, N,
                                            r32, pb0, pb1, xc, q, h->st, h->red);
-                        GIS_TRY(timer_end(h));
                     }
+                    GIS_TRY(timer_end(h));
                     GIS_TRY(timer_begin(h, 1));
                     hipLaunchKernelGGL((k_mx2<CC>), gc, dim3(kWG), 0, h->stream, N, r32, q,
                                        h->st, h->red);
                     GIS_TRY(timer_end(h));
-                    hipLaunchKernelGGL(k_mx2_fold, dim3(1), dim3(kFoldThreads), 0, h->stream,
-                                       h->st, h->red);
                     GIS_TRY(timer_begin(h, 2));
                     hipLaunchKernelGGL((k_mxu<P64>), gu, dim3(kWG), 0, h->stream, m64, N, b, x,
                                        xb1, pb0, pb1, xc, r32, h->st, h->red);

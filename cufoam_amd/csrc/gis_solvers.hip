@@ -216,16 +216,12 @@ __global__ __launch_bounds__(kWG) void k_cg2(int N, double* __restrict__ r,
             if (t.i0 + kk < N) leaf[c][0] = fma(rv[c][kk], rv[c][kk], leaf[c][0]);
         }
     }
+    TileReduce<1, C> tr;
+    tr.begin(red, sup.tile0, sup.nt, leaf, shr);
 #pragma unroll
     for (int c = 0; c < C; ++c)
         if (c < sup.nt) store4(r, TileRows(N, sup.tile0 + c).i0, N, rv[c]);
-    write_partials<1, C>(red, sup.tile0, sup.nt, leaf, shr);
-}
-
-// after k_cg2: rho = (r, r), norm, history, beta, stopping rule
-__global__ __launch_bounds__(kFoldThreads) void k_cg2_fold(KState* st, RedWs red) {
-    if (st->done) return;
-    fold_partials<1>(red, [&](const double (&v)[1]) {
+    tr.end(red, [&](const double (&v)[1]) {
         st->rho1 = st->rho0;
         st->rho0 = v[0];
         st->beta = st->rho0 / st->rho1;
@@ -457,21 +453,16 @@ int solve_cg(gis_context* h, const SolveArgs& a, double* x, const double* b) {
                     GIS_TRY(timer_begin(h, 0));
                     hipLaunchKernelGGL((k_cg1s<P, CC>), gc, dim3(kWG), 0, h->stream, m, N, pb0,
                                        pb1, q, h->st, h->red);
-                    GIS_TRY(timer_end(h));
-                    hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(kFoldThreads), 0, h->stream,
-                                       h->st, h->red);
                 } else {
                     GIS_TRY(timer_begin(h, 0));
                     hipLaunchKernelGGL((k_cg1<P, CC>), gc, dim3(kWG), 0, h->stream, m, N, r, pb0,
                                        pb1, x, q, h->st, h->red);
-                    GIS_TRY(timer_end(h));
                 }
+                GIS_TRY(timer_end(h));
                 GIS_TRY(timer_begin(h, 1));
                 hipLaunchKernelGGL((k_cg2<CC>), gc, dim3(kWG), 0, h->stream, N, r, q, h->st,
                                    h->red);
                 GIS_TRY(timer_end(h));
-                hipLaunchKernelGGL(k_cg2_fold, dim3(1), dim3(kFoldThreads), 0, h->stream, h->st,
-                                   h->red);
                 return kOK;
             }));
         });
