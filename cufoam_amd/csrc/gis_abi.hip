@@ -120,6 +120,7 @@ GIS_API const char* gis_status_string(int s) {
         case GIS_BREAKDOWN: return "BiCG breakdown";
         case GIS_NOT_CONVERGED: return "iteration cap reached";
         case GIS_NAN: return "residual is NaN";
+        case GIS_ERR_SYNC: return "a device reduction did not complete";
         default: return "unknown status";
     }
 }

@@ -692,7 +692,7 @@ __global__ __launch_bounds__(kWG) void k_pcg2(int N, double* __restrict__ r,
             }
         }
     }
-    TileReduce<NV, C> tr;
+    SlotReduce<NV, C> tr;
     tr.begin(red, sup.tile0, sup.nt, leaf, shr);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
@@ -712,7 +712,7 @@ __global__ __launch_bounds__(kWG) void k_pcg2(int N, double* __restrict__ r,
             store4(z32, i0, N, gv);
         }
     }
-    tr.end(red, [&](const double (&v)[NV]) {
+    tr.end(red, st, shr, [&](const double (&v)[NV]) {
         const double norm = sqrt(v[0]);
         record_norm(st, norm);
         if (PC < 2) {

@@ -27,6 +27,9 @@ constexpr int kMaxNV = 16;            // reductions carried by one kernel (8 RHS
 // Cross-workgroup reduction workspace (device pointers, passed by value).
 struct RedWs {
     double* part[kMaxLevels];         // level-L partials, count[L] * kMaxNV doubles
+    double* slot;                     // published level-0 partials, count[0] * kMaxNV
+                                      // doubles; every entry holds kSlotEmpty between launches
+    double* slot1;                    // published group sums (level 1), count[1] * kMaxNV
     unsigned* ticket[kMaxLevels];     // one arrival counter per group of level L
     int count[kMaxLevels];            // number of partials at level L
     int nlevels;                      // levels until count == 1
@@ -131,6 +134,7 @@ __device__ __forceinline__ void block_tree(double (&v)[NV], double (*sh)[kMaxNV]
 // LDS a kernel hands to finish_tile().
 struct RedShared {
     double w[4][kMaxNV];
+    double top[2 * kGroup];   // SlotReduce: the collecting workgroup's upper levels
     int last;
 };
 
@@ -281,6 +285,241 @@ struct TileReduce {
         last = __shfl(last, 0, 64);
         if (!last) return;
         fold_upper<NV>(red, tile0 / kGroup, fin);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Reduction without a per-workgroup round trip.
+//
+// A workgroup holding a ticket cannot retire until the ticket's atomic has
+// come back, and the last arriver's fold is a chain of such round trips at the
+// very end of the kernel.  Here a workgroup only publishes its tiles' subtree
+// sums, with plain agent-scope stores, into slots that hold the signalling-NaN
+// pattern kSlotEmpty until written, and retires.  No arithmetic result is a
+// signalling NaN (IEEE operations quiet them), so a slot that no longer holds
+// the pattern holds its final value.
+//   * The workgroup holding the last tile of an aligned group of kGroup tiles
+//     collects that group's slots (one wave, one round trip), folds them with
+//     wave_tree and publishes the group sum into a level-1 slot.
+//   * The workgroup holding the last tile overall does the same for its own
+//     group, then collects the level-1 slots and folds the levels above in
+//     registers and LDS, and calls fin(totals).
+// Every fold is over the aligned groups of kGroup that fold_upper uses: same
+// partials, same tree, same bits as finish_tiles.  Collectors restore the
+// pattern in every slot they read.  A collector waits only for workgroups that
+// never wait themselves, and every wait is bounded: after kSlotSpins polls
+// without the data it records kErrSync and stops the solve instead of hanging.
+// ---------------------------------------------------------------------------
+constexpr unsigned long long kSlotEmpty = 0x7FF5A5A5A5A5A5A5ull;   // a signalling NaN
+constexpr int kSlotSpins = 1 << 20;
+constexpr int kErrSync = 13;
+
+__device__ __forceinline__ bool slot_empty(double v) {
+    return static_cast<unsigned long long>(__double_as_longlong(v)) == kSlotEmpty;
+}
+
+__device__ __forceinline__ void slot_clear(double* p) {
+    st_agent(p, __longlong_as_double(static_cast<long long>(kSlotEmpty)));
+}
+
+// The NV values of entry e (stride NV) of a slot array, once every lane of the
+// wave sees its entry written; entries e >= n read as 0.  Each read slot is
+// restored to the pattern.  False when the bounded wait ran out.
+template <int NV>
+__device__ __forceinline__ bool poll_entry(double* base, int e, int n, double (&x)[NV]) {
+    for (int spins = 0;; ++spins) {
+        int miss = 0;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            x[i] = e < n ? ld_agent(base + (size_t)e * NV + i) : 0.0;
+            miss |= slot_empty(x[i]) ? 1 : 0;
+        }
+        if (!__any(miss)) break;
+        if (spins >= kSlotSpins) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if (e < n) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) slot_clear(base + (size_t)e * NV + i);
+    }
+    return true;
+}
+
+// TileReduce's interface over published slots:
+//   begin(): the tiles' subtrees are formed and thread 0 publishes them; the
+//            workgroup is then free to issue its output stores and retire;
+//   end():   group and top collectors as described above; fin on thread 0 of
+//            the top collector.
+// With a single tile the totals go to fin at once, as in finish_tiles.
+template <int NV, int C>
+struct SlotReduce {
+    static_assert(NV <= 2, "RedShared::top holds two values per level-2 entry");
+    double flat[C * NV];
+    int last_tile;
+
+    __device__ void begin(const RedWs& red, int tile0, int nt, double (&v)[C][NV],
+                          RedShared& sh) {
+        static_assert(C * NV <= kMaxNV, "RedShared holds kMaxNV values per wave");
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) flat[c * NV + i] = v[c][i];
+        block_tree<C * NV>(flat, sh.w);
+        last_tile = nt > 0 ? tile0 + nt - 1 : -1;
+        if (red.count[0] != 1 && threadIdx.x == 0) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                if (c < nt) {
+#pragma unroll
+                    for (int i = 0; i < NV; ++i)
+                        st_agent(red.slot + (size_t)(tile0 + c) * NV + i, flat[c * NV + i]);
+                }
+            }
+        }
+    }
+
+    template <class Fin>
+    __device__ void end(const RedWs& red, KState* st, RedShared& sh, Fin&& fin) {
+        const int n0 = red.count[0];
+        if (n0 == 1) {
+            if (threadIdx.x == 0) {
+                double t[NV];
+#pragma unroll
+                for (int i = 0; i < NV; ++i) t[i] = flat[i];
+                fin(t);
+            }
+            return;
+        }
+        const int t = last_tile;
+        if (t < 0 || ((t + 1) % kGroup != 0 && t != n0 - 1)) return;
+        const bool top = (t == n0 - 1);
+        if (!top && threadIdx.x >= 64) return;
+        const int lane = threadIdx.x & 63;
+        const int g = t / kGroup;
+        if (threadIdx.x < 64) {
+            double v[NV];
+            const bool ok = poll_entry<NV>(red.slot, g * kGroup + lane, n0, v);
+            if (ok) wave_tree<NV>(v);
+            if (!top) {
+                if (ok && lane == 0) {
+#pragma unroll
+                    for (int i = 0; i < NV; ++i) st_agent(red.slot1 + (size_t)g * NV + i, v[i]);
+                }
+                return;
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) sh.w[0][i] = v[i];
+                sh.last = ok ? 1 : 0;
+            }
+        }
+        __syncthreads();
+        top_collect(red, st, sh, fin);
+    }
+
+    // the top collector, all threads: level-1 entries (its own group's sum
+    // from sh.w[0], the others from the level-1 slots), then the levels above
+    template <class Fin>
+    __device__ void top_collect(const RedWs& red, KState* st, RedShared& sh, Fin&& fin) {
+        const int lane = threadIdx.x & 63;
+        const int w = threadIdx.x >> 6;
+        constexpr int kWaves = kWG / 64;
+        const int n1 = red.count[1];
+        const int n2 = red.count[2];
+        double own[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) own[i] = sh.w[0][i];
+        if (!sh.last) {
+            if (threadIdx.x == 0) {
+                st->status = kErrSync;
+                st->done = 1;
+            }
+            return;
+        }
+        if (n1 == 1) {
+            if (threadIdx.x == 0) fin(own);
+            return;
+        }
+        bool ok = true;
+        for (int h = w; h < n2; h += kWaves) {
+            const int e = h * kGroup + lane;
+            double x[NV];
+            ok = poll_entry<NV>(red.slot1, e, n1 - 1, x) && ok;
+            if (e == n1 - 1) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) x[i] = own[i];
+            }
+            wave_tree<NV>(x);
+            if (n2 == 1) {
+                if (ok && threadIdx.x == 0) fin(x);
+            } else if (lane == 0) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) {
+                    if (n2 <= kGroup)
+                        sh.top[h * NV + i] = x[i];
+                    else
+                        st_agent(&red.part[2][(size_t)h * kMaxNV + i], x[i]);
+                }
+            }
+        }
+        if (n2 == 1) {
+            if (!ok && threadIdx.x == 0) {
+                st->status = kErrSync;
+                st->done = 1;
+            }
+            return;
+        }
+        __syncthreads();
+        if (!ok) sh.last = 0;   // only 0 is written here
+        order_release();
+        __syncthreads();
+        if (!sh.last) {
+            if (threadIdx.x == 0) {
+                st->status = kErrSync;
+                st->done = 1;
+            }
+            return;
+        }
+        if (n2 <= kGroup) {
+            if (threadIdx.x < 64) {
+                double x[NV];
+#pragma unroll
+                for (int i = 0; i < NV; ++i) x[i] = lane < n2 ? sh.top[lane * NV + i] : 0.0;
+                wave_tree<NV>(x);
+                if (threadIdx.x == 0) fin(x);
+            }
+            return;
+        }
+        // more than kGroup^3 tiles: the levels above 2 through the global partials
+        for (int L = 2; L < kMaxLevels; ++L) {
+            const int n = red.count[L];
+            if (n == 1) {
+                if (threadIdx.x == 0) {
+                    double x[NV];
+#pragma unroll
+                    for (int i = 0; i < NV; ++i) x[i] = ld_agent(&red.part[L][i]);
+                    fin(x);
+                }
+                return;
+            }
+            order_acquire();
+            const int groups = (n + kGroup - 1) / kGroup;
+            for (int q = w; q < groups; q += kWaves) {
+                double x[NV];
+                const int src = q * kGroup + lane;
+#pragma unroll
+                for (int i = 0; i < NV; ++i)
+                    x[i] = src < n ? ld_agent(&red.part[L][(size_t)src * kMaxNV + i]) : 0.0;
+                wave_tree<NV>(x);
+                if (lane == 0) {
+#pragma unroll
+                    for (int i = 0; i < NV; ++i)
+                        st_agent(&red.part[L + 1][(size_t)q * kMaxNV + i], x[i]);
+                }
+            }
+            order_release();
+            __syncthreads();
+        }
     }
 };
 
@@ -572,24 +811,37 @@ __device__ __forceinline__ void tile_sell_c(const int* __restrict__ soff,
 // padding slot otherwise.  A row's entries ascend in column, so the set bits
 // visit them in storage order: same products, same order, same sum as
 // sell_row, with no column index per slot.  Values: slot-major from 64 * ooff[s].
+//
+// The loop body has no per-slot branches: slot indices are clamped into the
+// slice (min(k, U-1)), and a lane whose bit is clear gathers its own row
+// (`safe`, a valid row) and leaves the sum untouched through a select.  Every
+// load of a batch is then issued before the first wait.
 template <typename T, class XOp>
 __device__ __forceinline__ T sell_row_d(const int* __restrict__ offs, unsigned mask,
                                         const T* __restrict__ dval, int a, int len, int lane,
-                                        int row, XOp&& xop) {
+                                        int row, int safe, XOp&& xop) {
     T acc = T(0);
     for (int j = 0; j < len; j += 8) {
         T v[8], xv[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            v[u] = T(0);
-            if (j + u < len) v[u] = __builtin_nontemporal_load(dval + a + (j + u) * kSlice + lane);
+            const int k = min(j + u, len - 1);
+            v[u] = __builtin_nontemporal_load(dval + a + k * kSlice + lane);
+        }
+        int o[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) o[u] = offs[min(j + u, len - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const bool on = (mask >> (j + u)) & 1u;
+            xv[u] = xop(on ? row + o[u] : safe);
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            xv[u] = (j + u < len && ((mask >> (j + u)) & 1u)) ? xop(row + offs[j + u]) : T(0);
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (j + u < len && ((mask >> (j + u)) & 1u)) acc = acc + v[u] * xv[u];
+        for (int u = 0; u < 8; ++u) {
+            const bool on = (mask >> (j + u)) & 1u;
+            const T t = acc + v[u] * xv[u];
+            acc = on ? t : acc;
+        }
     }
     return acc;
 }
@@ -597,8 +849,9 @@ __device__ __forceinline__ T sell_row_d(const int* __restrict__ offs, unsigned m
 // Two offset-table slices per lane at a time (their loads in flight together).
 template <typename T, class XOp>
 __device__ __forceinline__ void sell_row2_d(const int* __restrict__ offs0, unsigned m0, int a0,
-                                            int len0, int row0, const int* __restrict__ offs1,
-                                            unsigned m1, int a1, int len1, int row1,
+                                            int len0, int row0, int safe0,
+                                            const int* __restrict__ offs1, unsigned m1, int a1,
+                                            int len1, int row1, int safe1,
                                             const T* __restrict__ dval, int lane, XOp&& xop,
                                             T& y0, T& y1) {
     T acc0 = T(0), acc1 = T(0);
@@ -607,22 +860,29 @@ __device__ __forceinline__ void sell_row2_d(const int* __restrict__ offs0, unsig
         T v0[8], v1[8], x0[8], x1[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            v0[u] = T(0);
-            v1[u] = T(0);
-            if (j + u < len0)
-                v0[u] = __builtin_nontemporal_load(dval + a0 + (j + u) * kSlice + lane);
-            if (j + u < len1)
-                v1[u] = __builtin_nontemporal_load(dval + a1 + (j + u) * kSlice + lane);
+            const int k0 = min(j + u, len0 - 1), k1 = min(j + u, len1 - 1);
+            v0[u] = __builtin_nontemporal_load(dval + a0 + k0 * kSlice + lane);
+            v1[u] = __builtin_nontemporal_load(dval + a1 + k1 * kSlice + lane);
+        }
+        int o0[8], o1[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            o0[u] = offs0[min(j + u, len0 - 1)];
+            o1[u] = offs1[min(j + u, len1 - 1)];
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            x0[u] = (j + u < len0 && ((m0 >> (j + u)) & 1u)) ? xop(row0 + offs0[j + u]) : T(0);
-            x1[u] = (j + u < len1 && ((m1 >> (j + u)) & 1u)) ? xop(row1 + offs1[j + u]) : T(0);
+            const bool on0 = (m0 >> (j + u)) & 1u, on1 = (m1 >> (j + u)) & 1u;
+            x0[u] = xop(on0 ? row0 + o0[u] : safe0);
+            x1[u] = xop(on1 ? row1 + o1[u] : safe1);
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            if (j + u < len0 && ((m0 >> (j + u)) & 1u)) acc0 = acc0 + v0[u] * x0[u];
-            if (j + u < len1 && ((m1 >> (j + u)) & 1u)) acc1 = acc1 + v1[u] * x1[u];
+            const bool on0 = (m0 >> (j + u)) & 1u, on1 = (m1 >> (j + u)) & 1u;
+            const T t0 = acc0 + v0[u] * x0[u];
+            const T t1 = acc1 + v1[u] * x1[u];
+            acc0 = on0 ? t0 : acc0;
+            acc1 = on1 ? t1 : acc1;
         }
     }
     y0 = acc0;
@@ -659,11 +919,13 @@ __device__ __forceinline__ void tile_sell_d(const int* __restrict__ soff,
         }
         T y0 = T(0), y1 = T(0);
         const int r0 = s * kSlice + lane, r1 = (s + 1) * kSlice + lane;
+        // rows past N have an empty mask; their (unused) gathers read row N-1
+        const int q0 = min(r0, N - 1), q1 = min(r1, N - 1);
         if (u0 > 0 && u1 > 0) {
             const unsigned m0 = r0 < N ? dmask[r0] : 0u;
             const unsigned m1 = r1 < N ? dmask[r1] : 0u;
-            sell_row2_d<T>(doffs + o0, m0, o0 * kSlice, u0, r0, doffs + o1, m1, o1 * kSlice, u1,
-                           r1, dval, lane, xop, y0, y1);
+            sell_row2_d<T>(doffs + o0, m0, o0 * kSlice, u0, r0, q0, doffs + o1, m1, o1 * kSlice,
+                           u1, r1, q1, dval, lane, xop, y0, y1);
         } else {
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
@@ -673,7 +935,8 @@ __device__ __forceinline__ void tile_sell_d(const int* __restrict__ soff,
                 T y;
                 if (up > 0) {
                     const unsigned mp = rp < N ? dmask[rp] : 0u;
-                    y = sell_row_d<T>(doffs + op, mp, dval, op * kSlice, up, lane, rp, xop);
+                    y = sell_row_d<T>(doffs + op, mp, dval, op * kSlice, up, lane, rp,
+                                      p ? q1 : q0, xop);
                 } else {
                     const int a = soff[sp];
                     y = sell_row<T>(scol, sval, a, (soff[sp + 1] - a) / kSlice, lane, xop);

@@ -255,12 +255,12 @@ __global__ __launch_bounds__(kWG) void k_cg1s(Mat<double> m, int N,
             if (t.i0 + kk < N) leaf[c][0] = fma(pv[kk], qv[c][kk], leaf[c][0]);
         }
     }
-    TileReduce<1, C> tr;
+    SlotReduce<1, C> tr;
     tr.begin(red, sup.tile0, sup.nt, leaf, shr);
 #pragma unroll
     for (int c = 0; c < C; ++c)
         if (c < sup.nt) store4(q, TileRows(N, sup.tile0 + c).i0, N, qv[c]);
-    tr.end(red, [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
+    tr.end(red, st, shr, [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
 }
 
 namespace {

@@ -390,10 +390,10 @@ __global__ __launch_bounds__(kWG) void k_bicg1s(Mat<double> m, int N,
         qv[kk] = sh_y[threadIdx.x * kLeaf + kk];
         if (t.i0 + kk < N) leaf[0][0] = fma(psv[kk], qv[kk], leaf[0][0]);
     }
-    TileReduce<1, 1> tr;
+    SlotReduce<1, 1> tr;
     tr.begin(red, t.tile, 1, leaf, shr);
     store4(q, t.i0, N, qv);
-    tr.end(red, [&](const double (&v)[1]) {
+    tr.end(red, st, shr, [&](const double (&v)[1]) {
         const double xi = st->rho0 / v[0];
         st->alpha = xi;
         if (xi > -st->int_bound && xi < st->int_bound) {

@@ -209,12 +209,12 @@ __global__ __launch_bounds__(kWG) void k_mx1s(Mat<float> m, int N, const float* 
             if (t.i0 + kk < N) leaf[c][0] = fma((double)pv[kk], (double)qv[c][kk], leaf[c][0]);
         }
     }
-    TileReduce<1, C> tr;
+    SlotReduce<1, C> tr;
     tr.begin(red, sup.tile0, sup.nt, leaf, shr);
 #pragma unroll
     for (int c = 0; c < C; ++c)
         if (c < sup.nt) store4(q, TileRows(N, sup.tile0 + c).i0, N, qv[c]);
-    tr.end(red, [&](const double (&v)[1]) {
+    tr.end(red, st, shr, [&](const double (&v)[1]) {
         st->alpha = st->rho0 / v[0];
         st->flag = st->flag & ~(F_RESTART | F_FOLDED);
     });
@@ -244,12 +244,12 @@ __global__ __launch_bounds__(kWG) void k_mx2(int N, float* __restrict__ r,
             if (t.i0 + kk < N) leaf[c][0] = fma((double)rv[c][kk], (double)rv[c][kk], leaf[c][0]);
         }
     }
-    TileReduce<1, C> tr;
+    SlotReduce<1, C> tr;
     tr.begin(red, sup.tile0, sup.nt, leaf, shr);
 #pragma unroll
     for (int c = 0; c < C; ++c)
         if (c < sup.nt) store4(r, TileRows(N, sup.tile0 + c).i0, N, rv[c]);
-    tr.end(red, [&](const double (&v)[1]) {
+    tr.end(red, st, shr, [&](const double (&v)[1]) {
         st->k = st->k + 1;
         st->rho1 = st->rho0;
         st->rho0 = v[0];

@@ -216,12 +216,12 @@ __global__ __launch_bounds__(kWG) void k_cg2(int N, double* __restrict__ r,
             if (t.i0 + kk < N) leaf[c][0] = fma(rv[c][kk], rv[c][kk], leaf[c][0]);
         }
     }
-    TileReduce<1, C> tr;
+    SlotReduce<1, C> tr;
     tr.begin(red, sup.tile0, sup.nt, leaf, shr);
 #pragma unroll
     for (int c = 0; c < C; ++c)
         if (c < sup.nt) store4(r, TileRows(N, sup.tile0 + c).i0, N, rv[c]);
-    tr.end(red, [&](const double (&v)[1]) {
+    tr.end(red, st, shr, [&](const double (&v)[1]) {
         st->rho1 = st->rho0;
         st->rho0 = v[0];
         st->beta = st->rho0 / st->rho1;
@@ -234,9 +234,16 @@ __global__ __launch_bounds__(kWG) void k_cg2(int N, double* __restrict__ r,
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// every published-partial slot to the empty pattern (SlotReduce)
+__global__ void k_slot_fill(double* slot, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) slot[i] = __longlong_as_double(static_cast<long long>(kSlotEmpty));
+}
+
 // The level counts must describe exactly `rows` (the group-final logic waits
-// for count[L] arrivals); memory only grows.  Tickets are re-zeroed whenever
-// the shape changes.
+// for count[L] arrivals); memory only grows.  Tickets are re-zeroed and the
+// slots refilled whenever the shape changes.
+
 int ensure_reduction(gis_context* h, int rows) {
     if (h->red_rows == rows && h->red_mem.p) return kOK;
     RedWs r{};
@@ -253,13 +260,21 @@ int ensure_reduction(gis_context* h, int rows) {
     if (L == kMaxLevels) return fail(h, kErrShape, "reduction depth exceeded");
     r.nlevels = L + 1;
     for (int j = L + 1; j < kMaxLevels; ++j) r.count[j] = 1;
-    GIS_HIP(h, h->red_mem.ensure(dbl * sizeof(double) + ints * sizeof(unsigned) + 256));
+    // published-partial slots: level 0, then level 1, contiguous
+    const size_t nslot = ((size_t)r.count[0] + (size_t)r.count[1]) * kMaxNV;
+    GIS_HIP(h, h->red_mem.ensure((dbl + nslot) * sizeof(double) + ints * sizeof(unsigned) + 256));
     char* base = h->red_mem.as<char>();
     size_t off = 0;
     for (int j = 0; j <= L; ++j) {
         r.part[j] = reinterpret_cast<double*>(base + off);
         off += (size_t)r.count[j] * kMaxNV * sizeof(double);
     }
+    r.slot = reinterpret_cast<double*>(base + off);
+    r.slot1 = r.slot + (size_t)r.count[0] * kMaxNV;
+    off += nslot * sizeof(double);
+    hipLaunchKernelGGL(k_slot_fill, dim3((unsigned)((nslot + kWG - 1) / kWG)), dim3(kWG), 0,
+                       h->stream, r.slot, (long long)nslot);
+    GIS_HIP(h, hipGetLastError());
     unsigned* tk = reinterpret_cast<unsigned*>(base + off);
     GIS_HIP(h, hipMemsetAsync(tk, 0, ints * sizeof(unsigned), h->stream));
     for (int j = 0; j <= L; ++j) {
@@ -293,6 +308,9 @@ int read_state(gis_context* h) {
     GIS_HIP(h, hipMemcpyAsync(h->st_host, h->st, sizeof(KState), hipMemcpyDeviceToHost,
                               h->stream));
     GIS_HIP(h, hipStreamSynchronize(h->stream));
+    // a reduction whose wait ran out may have left slots written: rebuild the
+    // workspace (and its empty pattern) before the next launch
+    if (h->st_host->status == kErrSync) h->red_rows = -1;
     return timer_collect(h);
 }
 

@@ -18,6 +18,7 @@ from . import build as _build
 
 OK, ERR_ARG, ERR_HIP, ERR_SHAPE, ERR_STATE, ERR_NOMEM = 0, 1, 2, 3, 4, 5
 BREAKDOWN, NOT_CONVERGED, NAN = 10, 11, 12
+ERR_SYNC = 13   # a device reduction did not complete: an error, raised
 
 METHODS = {"CG": 0, "BiCG": 1, "Jacobi": 2, "MR": 3, "MRJacobi": 4, "JMR": 5, "MRS": 6,
            "Richardson": 7}
@@ -211,7 +212,7 @@ class Gis:
         self.close()
 
     def _check(self, st: int, what: str) -> int:
-        if st != OK and st < BREAKDOWN:
+        if st != OK and (st < BREAKDOWN or st == ERR_SYNC):
             msg = self._L.gis_last_error(self._h)
             raise GisError(st, f"{what}: {msg.decode() if msg else ''}")
         return st

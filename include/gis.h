@@ -41,6 +41,7 @@ extern "C" {
 #define GIS_BREAKDOWN 10       /* BiCG |xi| < int_bound (SolverSingleGPU.h:629)   */
 #define GIS_NOT_CONVERGED 11   /* iteration cap reached above the threshold       */
 #define GIS_NAN 12             /* residual norm became NaN                         */
+#define GIS_ERR_SYNC 13        /* a cross-workgroup reduction did not complete     */
 
 /* ---- solver methods (gis/gis.h:13-87; wrappers cuFoam<Name>) ---- */
 #define GIS_CG 0
