@@ -344,7 +344,7 @@ __global__ __launch_bounds__(kWG) void k_vc_sweep(Mat<T> m, int N, const T* __re
     }
     store4(zn, t.i0, N, zv);
     if constexpr (RED) {
-        finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+        finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) {
             st->rho1 = st->rho0;
             st->rho0 = v[0];
             st->beta = st->rho0 / st->rho1;
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(kWG) void k_vc_sweep_out(Mat<float> m, int N,
         if (t.i0 + k < N) leaf[0] = fma(rv[k], ov[k], leaf[0]);
     }
     store4(out, t.i0, N, ov);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) {
         st->rho1 = st->rho0;
         st->rho0 = v[0];
         st->beta = (st->rho0 - st->aux[kGammaSlot]) / st->rho1;
@@ -568,7 +568,7 @@ __global__ __launch_bounds__(kWG) void k_pcg_rho(int N, const double* __restrict
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         if (t.i0 + k < N) leaf[0] = fma(rv[k], zv[k], leaf[0]);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) {
         st->rho1 = st->rho0;
         st->rho0 = v[0];
         st->beta = st->rho0 / st->rho1;
@@ -594,7 +594,7 @@ __global__ __launch_bounds__(kWG) void k_pcg_rho32(int N, const double* __restri
         if (t.i0 + k < N) leaf[0] = fma(rv[k], ov[k], leaf[0]);
     }
     store4(out, t.i0, N, ov);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) {
         st->rho1 = st->rho0;
         st->rho0 = v[0];
         st->beta = (st->rho0 - st->aux[kGammaSlot]) / st->rho1;
@@ -619,7 +619,7 @@ __global__ __launch_bounds__(kWG) void k_pcg_diag0(int N, const double* __restri
         if (t.i0 + k < N) leaf[0] = fma(rv[k], zv[k], leaf[0]);
     }
     store4(z, t.i0, N, zv);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) { st->rho0 = v[0]; });
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) { st->rho0 = v[0]; });
 }
 
 // p = z (k = 1) or fma(beta, p_prev, z); x = fma(alpha_prev, p_prev, x) (deferred)

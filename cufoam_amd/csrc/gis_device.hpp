@@ -523,6 +523,20 @@ struct SlotReduce {
     }
 };
 
+// finish_tile over published slots (SlotReduce, one tile per workgroup): the
+// same totals, passed to fin by one workgroup.  Every workgroup of the launch
+// must call it.
+template <int NV, class Fin>
+__device__ __forceinline__ void finish_tile_slots(const RedWs& red, KState* st, int tile,
+                                                  double (&v)[NV], RedShared& sh, Fin&& fin) {
+    double vv[1][NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) vv[0][i] = v[i];
+    SlotReduce<NV, 1> tr;
+    tr.begin(red, tile, 1, vv, sh);
+    tr.end(red, st, sh, fin);
+}
+
 // One tile per workgroup.
 template <int NV, class Fin>
 __device__ __forceinline__ void finish_tile(const RedWs& red, int tile, double (&v)[NV],

@@ -209,7 +209,7 @@ __global__ __launch_bounds__(kWG) void k_setup(Mat<double> m, int N,
     }
     store4(r, t.i0, N, rv);
     if (r2) store4(r2, t.i0, N, rv);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) {
         const double norm = sqrt(v[0]);
         st->rho0 = v[0];
         st->rho1 = 0.0;

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kWG) void k_mr1(Mat<double> m, int N, const double*
         }
     }
     store4(w, t.i0, N, wv);
-    finish_tile<2>(red, t.tile, leaf, shr,
+    finish_tile_slots<2>(red, st, t.tile, leaf, shr,
                    [&](const double (&v)[2]) { st->alpha = v[0] / v[1]; });
 }
 
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kWG) void k_mr2(int N, double* __restrict__ r,
     }
     store4(x, t.i0, N, xv);
     store4(r, t.i0, N, rv);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) {
         const double norm = sqrt(v[0]);
         record_norm(st, norm);
         if (stop_after(st, !(norm > st->error))) st->done = 1;
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(kWG) void k_jacobi(Mat<double> m, int N, double* __
     }
     store4(x, t.i0, N, xv);
     store4(r_new, t.i0, N, rv);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) {
         const double norm = sqrt(v[0]);
         record_norm(st, norm);
         if (stop_after(st, !(norm > st->error))) st->done = 1;
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(kWG) void k_richardson(Mat<double> m, int N,
     }
     store4(x_new, t.i0, N, xv);
     store4(r_new, t.i0, N, rv);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) {
         const double norm = sqrt(v[0]);
         record_norm(st, norm);
         if (stop_after(st, !(norm > st->error))) st->done = 1;
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kWG) void k_power(Mat<double> m, int N,
         if (t.i0 + k < N) leaf[0] = fma(yv[k], yv[k], leaf[0]);
     }
     store4(t_out, t.i0, N, yv);
-    finish_tile<1>(red, t.tile, leaf, shr,
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr,
                    [&](const double (&v)[1]) { st->aux[2] = 1 / sqrt(v[0]); });
 }
 
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(kWG) void k_power_final(Mat<double> m, int N,
         const int i = t.i0 + k;
         if (i < N) leaf[0] = fma(v(i), sh_y[threadIdx.x * kLeaf + k], leaf[0]);
     }
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&s)[1]) {
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&s)[1]) {
         const double lmax = s[0];
         const double m_abs = st->aux[4];   // written as bits by k_diag_absmin
         double lmin = DBL_MAX;
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(kWG) void k_bicg1(Mat<double> m, int N, const doubl
     store4(p_new, t.i0, N, pn);
     store4(ps, t.i0, N, psv);
     store4(q, t.i0, N, qv);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) {
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) {
         const double xi = st->rho0 / v[0];
         st->alpha = xi;
         if (xi > -st->int_bound && xi < st->int_bound) {
@@ -440,7 +440,7 @@ __global__ __launch_bounds__(kWG) void k_bicg2(Mat<double> mt, int N, double* __
     store4(x, t.i0, N, xv);
     store4(r, t.i0, N, rv);
     store4(rs, t.i0, N, rsv);
-    finish_tile<2>(red, t.tile, leaf, shr, [&](const double (&v)[2]) {
+    finish_tile_slots<2>(red, st, t.tile, leaf, shr, [&](const double (&v)[2]) {
         const double norm = sqrt(v[0]);
         record_norm(st, norm);
         if (stop_after(st, norm < st->error)) {

@@ -141,7 +141,9 @@ __global__ __launch_bounds__(kWG) void k_mx1(Mat<float> m, int N, const float* _
         store4(q, t.i0, N, qv);
         store4(xc, t.i0, N, xv);
     }
-    finish_tiles<1, C>(red, sup.tile0, sup.nt, leaf, shr, [&](const double (&v)[1]) {
+    SlotReduce<1, C> tr;
+    tr.begin(red, sup.tile0, sup.nt, leaf, shr);
+    tr.end(red, st, shr, [&](const double (&v)[1]) {
         st->alpha = st->rho0 / v[0];
         st->flag = st->flag & ~(F_RESTART | F_FOLDED);
     });
@@ -309,7 +311,7 @@ __global__ __launch_bounds__(kWG) void k_mxu(Mat<double> m, int N, const double*
         }
         store4(x_new, t.i0, N, xv);
         store4(r32, t.i0, N, rf);
-        finish_tile<2>(red, t.tile, leaf, shr, [&](const double (&v)[2]) {
+        finish_tile_slots<2>(red, st, t.tile, leaf, shr, [&](const double (&v)[2]) {
             const double rnorm = sqrt(v[0]);
             st->norm = rnorm;
             push_hist(st, rnorm);

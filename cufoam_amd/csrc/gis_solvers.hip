@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kWG) void k_dot(int N, const double* __restrict__ a
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         if (t.i0 + k < N) leaf[0] = fma(av[k], bv[k], leaf[0]);
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) { st->aux[slot] = v[0]; });
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) { st->aux[slot] = v[0]; });
 }
 
 // aux[0] = sum(x) / N   (ReductionKernel kernels.h:100-145 + gis.cu:108)
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kWG) void k_mean(int N, const double* __restrict__ 
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         if (t.i0 + k < N) leaf[0] = leaf[0] + xv[k];
-    finish_tile<1>(red, t.tile, leaf, shr, [&](const double (&v)[1]) { st->aux[0] = v[0] / N; });
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) { st->aux[0] = v[0] / N; });
 }
 
 // NormFactor (gis/gis.cu:106-123):
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kWG) void k_norm_factor(Mat<double> m, int N,
             leaf[1] = fma(d2, d2, leaf[1]);
         }
     }
-    finish_tile<2>(red, t.tile, leaf, shr, [&](const double (&v)[2]) {
+    finish_tile_slots<2>(red, st, t.tile, leaf, shr, [&](const double (&v)[2]) {
         st->aux[1] = sqrt(v[0]) + sqrt(v[1]) + 1e-20;
     });
 }
@@ -152,8 +152,9 @@ __global__ __launch_bounds__(kWG) void k_cg1(Mat<double> m, int N, const double*
         store4(q, t.i0, N, qv);
         if (!first) store4(x, t.i0, N, xv);
     }
-    finish_tiles<1, C>(red, sup.tile0, sup.nt, leaf, shr,
-                       [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
+    SlotReduce<1, C> tr;
+    tr.begin(red, sup.tile0, sup.nt, leaf, shr);
+    tr.end(red, st, shr, [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
 }
 
 // Split schedule, pass 1: p_k = r + beta p_{k-1} and the deferred
