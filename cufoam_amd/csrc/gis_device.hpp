@@ -1027,6 +1027,20 @@ __device__ __forceinline__ void store4(double* __restrict__ p, int i0, int n, co
     }
 }
 
+// Streaming (non-temporal) stores for the output of a kernel that gathers x:
+// the fresh output lines then do not evict the gathered lines from the L2.
+template <typename T>
+__device__ __forceinline__ void store4nt(T* __restrict__ p, int i0, int n, const T (&o)[4]) {
+    if (i0 + 3 < n) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) __builtin_nontemporal_store(o[k], p + i0 + k);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i0 + k < n) p[i0 + k] = o[k];
+    }
+}
+
 __device__ __forceinline__ void load4(const float* __restrict__ p, int i0, int n, float (&o)[4]) {
     if (i0 + 3 < n) {
         const float4 a = *reinterpret_cast<const float4*>(p + i0);

@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kWG) void k_cg1s(Mat<double> m, int N,
     tr.begin(red, sup.tile0, sup.nt, leaf, shr);
 #pragma unroll
     for (int c = 0; c < C; ++c)
-        if (c < sup.nt) store4(q, TileRows(N, sup.tile0 + c).i0, N, qv[c]);
+        if (c < sup.nt) store4nt(q, TileRows(N, sup.tile0 + c).i0, N, qv[c]);
     tr.end(red, st, shr, [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
 }
 

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kWG) void k_mx1(Mat<float> m, int N, const float* _
             if (t.i0 + kk < N) leaf[c][0] = fma((double)pn[kk], (double)qv[kk], leaf[c][0]);
         }
         store4(p_new, t.i0, N, pn);
-        store4(q, t.i0, N, qv);
+        store4nt(q, t.i0, N, qv);
         store4(xc, t.i0, N, xv);
     }
     SlotReduce<1, C> tr;
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kWG) void k_mx1s(Mat<float> m, int N, const float* 
     tr.begin(red, sup.tile0, sup.nt, leaf, shr);
 #pragma unroll
     for (int c = 0; c < C; ++c)
-        if (c < sup.nt) store4(q, TileRows(N, sup.tile0 + c).i0, N, qv[c]);
+        if (c < sup.nt) store4nt(q, TileRows(N, sup.tile0 + c).i0, N, qv[c]);
     tr.end(red, st, shr, [&](const double (&v)[1]) {
         st->alpha = st->rho0 / v[0];
         st->flag = st->flag & ~(F_RESTART | F_FOLDED);

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kWG) void k_spmv(Mat<T> m, int N, const T* __restri
     T o[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) o[k] = sh_y[threadIdx.x * kLeaf + k];
-    store4(y, t.i0, N, o);
+    store4nt(y, t.i0, N, o);
 }
 
 // ---------------------------------------------------------------------------
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kWG) void k_cg1(Mat<double> m, int N, const double*
             if (t.i0 + kk < N) leaf[c][0] = fma(pn[kk], qv[kk], leaf[c][0]);
         }
         store4(p_new, t.i0, N, pn);
-        store4(q, t.i0, N, qv);
+        store4nt(q, t.i0, N, qv);
         if (!first) store4(x, t.i0, N, xv);
     }
     SlotReduce<1, C> tr;
