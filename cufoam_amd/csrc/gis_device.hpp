@@ -1029,11 +1029,31 @@ __device__ __forceinline__ void store4(double* __restrict__ p, int i0, int n, co
 
 // Streaming (non-temporal) stores for the output of a kernel that gathers x:
 // the fresh output lines then do not evict the gathered lines from the L2.
-template <typename T>
-__device__ __forceinline__ void store4nt(T* __restrict__ p, int i0, int n, const T (&o)[4]) {
+// Whole rows go out as 16-byte vector stores, as store4 does.
+typedef double gis_d2 __attribute__((ext_vector_type(2)));
+typedef float gis_f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void store4nt(double* __restrict__ p, int i0, int n,
+                                         const double (&o)[4]) {
     if (i0 + 3 < n) {
+        gis_d2 a, b;
+        a.x = o[0]; a.y = o[1];
+        b.x = o[2]; b.y = o[3];
+        __builtin_nontemporal_store(a, reinterpret_cast<gis_d2*>(p + i0));
+        __builtin_nontemporal_store(b, reinterpret_cast<gis_d2*>(p + i0 + 2));
+    } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) __builtin_nontemporal_store(o[k], p + i0 + k);
+        for (int k = 0; k < 4; ++k)
+            if (i0 + k < n) p[i0 + k] = o[k];
+    }
+}
+
+__device__ __forceinline__ void store4nt(float* __restrict__ p, int i0, int n,
+                                         const float (&o)[4]) {
+    if (i0 + 3 < n) {
+        gis_f4 a;
+        a.x = o[0]; a.y = o[1]; a.z = o[2]; a.w = o[3];
+        __builtin_nontemporal_store(a, reinterpret_cast<gis_f4*>(p + i0));
     } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k)

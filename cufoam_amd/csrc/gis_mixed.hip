@@ -271,7 +271,10 @@ __global__ __launch_bounds__(kWG) void k_mx2(int N, float* __restrict__ r,
     });
 }
 
-// FP64 update; grid-strided over tiles so that the (usual) no-op launch is cheap
+// FP64 update; grid-strided over tiles so that the (usual) no-op launch is cheap.
+// The grid is min(tiles, 1024): either one tile per workgroup, or a multiple
+// of 64 workgroups, so that only workgroups with index 63 mod 64 ever collect a
+// group of slots and every other workgroup runs through without waiting.
 template <class P>
 __global__ __launch_bounds__(kWG) void k_mxu(Mat<double> m, int N, const double* __restrict__ b,
                                              double* __restrict__ xb0, double* __restrict__ xb1,
