@@ -342,7 +342,7 @@ __global__ __launch_bounds__(kWG) void k_vc_sweep(Mat<T> m, int N, const T* __re
         zv[k] = fmaT(sv[k], rv[k] - sh_y[threadIdx.x * kLeaf + k], zv[k]);
         if (RED && t.i0 + k < N) leaf[0] = fma((double)rv[k], (double)zv[k], leaf[0]);
     }
-    store4(zn, t.i0, N, zv);
+    store4nt(zn, t.i0, N, zv);
     if constexpr (RED) {
         finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) {
             st->rho1 = st->rho0;
@@ -380,7 +380,7 @@ __global__ __launch_bounds__(kWG) void k_vc_sweep_out(Mat<float> m, int N,
         ov[k] = (double)zv[k];
         if (t.i0 + k < N) leaf[0] = fma(rv[k], ov[k], leaf[0]);
     }
-    store4(out, t.i0, N, ov);
+    store4nt(out, t.i0, N, ov);
     finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) {
         st->rho1 = st->rho0;
         st->rho0 = v[0];
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(kWG) void k_vc_resid(Mat<T> m, int N, const T* __re
     load4(r, t.i0, N, rv);
 #pragma unroll
     for (int k = 0; k < 4; ++k) tv[k] = rv[k] - sh_y[threadIdx.x * kLeaf + k];
-    store4(tr, t.i0, N, tv);
+    store4nt(tr, t.i0, N, tv);
 }
 
 // rc[C] = sum of t over C's cells, ascending, and the next level's first
@@ -593,7 +593,7 @@ __global__ __launch_bounds__(kWG) void k_pcg_rho32(int N, const double* __restri
         ov[k] = (double)zv[k];
         if (t.i0 + k < N) leaf[0] = fma(rv[k], ov[k], leaf[0]);
     }
-    store4(out, t.i0, N, ov);
+    store4nt(out, t.i0, N, ov);
     finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) {
         st->rho1 = st->rho0;
         st->rho0 = v[0];

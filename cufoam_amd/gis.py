@@ -306,6 +306,9 @@ class Gis:
         self._check(f(self._h, _ptr(x), _ptr(y)), "spmv_async")
 
     def dot(self, a, b) -> float:
+        """(a, b) of two device vectors (torch CUDA tensors, float64)."""
+        if not (_is_dev(a) and _is_dev(b)):
+            raise GisError(ERR_ARG, "dot: a and b must be device tensors")
         out = C.c_double(0.0)
         self._check(self._L.gis_dot_device(self._h, _ptr(a), _ptr(b), C.byref(out)), "dot")
         return out.value
