@@ -7,8 +7,10 @@ A step is one complete solve from x0 = 0 of the 256^3 7-point FV Laplacian
 matrix and vectors already resident in HBM.  `value` is the mixed-precision
 solve time (FP32 Krylov loop, FP64 residual/correction/norms); the plain FP64
 CG time of the same solve is reported beside it.  The roofline entry is for
-the dominant kernel of the mixed solve (the fused FP32 SpMV + direction +
-correction kernel), timed live with HIP events on its launch stream.
+the dominant kernel of the mixed solve (the FP32 SpMV + (p, q) kernel of the
+tuned split schedule, or the fused SpMV + direction + correction kernel), timed
+live with HIP events on its launch stream; `achieved` counts CSR algorithmic
+bytes, and `moved_GBps` the HBM bytes the kernel was measured to move.
 """
 from __future__ import annotations
 
@@ -519,6 +521,12 @@ def main():
                            "k_mx1 (FP32 SpMV + direction + deferred correction)"),
                 "bytes_per_launch": mx1_bytes(N, nnz, split32),
                 "avg_launch_us": round(mx1_avg_s * 1e6, 2), "launches": mx1_n}
+    if traffic:
+        # `achieved` counts the CSR algorithmic bytes (an int32 column per
+        # nonzero); the sliced layout with per-slice column-offset tables moves
+        # fewer, so the rate of the bytes actually moved is given beside it
+        roofline["moved_GBps"] = round(traffic / mx1_avg_s / 1e9, 1)
+        roofline["moved_frac"] = round(traffic / mx1_avg_s / 1e9 / PEAK_HBM_GBS, 4)
 
     literal = None
     if args.literal:

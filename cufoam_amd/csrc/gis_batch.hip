@@ -59,7 +59,7 @@ __global__ __launch_bounds__(kWG) void k_bsetup(Mat<double> m, int N, const doub
         if (t.i0 + k < N) leaf[0] = fma(rv[k], rv[k], leaf[0]);
     }
     store4(r, t.i0, N, rv);
-    finish_tile<1>(bv.red[s.sys], s.local, leaf, shr, [&](const double (&v)[1]) {
+    finish_tile_slots<1>(bv.red[s.sys], st, s.local, leaf, shr, [&](const double (&v)[1]) {
         const double norm = sqrt(v[0]);
         st->rho0 = v[0];
         st->rho1 = 0.0;
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kWG) void k_bcg1(Mat<double> m, int N, const double
     store4(p_new, t.i0, N, pn);
     store4(q, t.i0, N, qv);
     if (!first) store4(x, t.i0, N, xv);
-    finish_tile<1>(bv.red[s.sys], s.local, leaf, shr,
+    finish_tile_slots<1>(bv.red[s.sys], st, s.local, leaf, shr,
                    [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
 }
 
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kWG) void k_bcg2(int N, double* __restrict__ r,
         if (t.i0 + k < N) leaf[0] = fma(rv[k], rv[k], leaf[0]);
     }
     store4(r, t.i0, N, rv);
-    finish_tile<1>(bv.red[s.sys], s.local, leaf, shr, [&](const double (&v)[1]) {
+    finish_tile_slots<1>(bv.red[s.sys], st, s.local, leaf, shr, [&](const double (&v)[1]) {
         st->rho1 = st->rho0;
         st->rho0 = v[0];
         st->beta = st->rho0 / st->rho1;
@@ -276,6 +276,19 @@ int batch_set_pattern(gis_context* h, int S, const int* n_cells, const int* n_fa
             reds[s].ticket[j] = nullptr;
         }
     }
+    // published-partial slots per system (level 0, then level 1)
+    size_t nslot = 0;
+    for (int s = 0; s < S; ++s) nslot += ((size_t)reds[s].count[0] + reds[s].count[1]) * kMaxNV;
+    GIS_HIP(h, B.slot_mem.ensure(sizeof(double) * nslot));
+    double* sp = B.slot_mem.as<double>();
+    for (int s = 0; s < S; ++s) {
+        reds[s].slot = sp;
+        reds[s].slot1 = sp + (size_t)reds[s].count[0] * kMaxNV;
+        sp += ((size_t)reds[s].count[0] + reds[s].count[1]) * kMaxNV;
+    }
+    B.slot_n = nslot;
+    B.slots_dirty = false;
+    GIS_TRY(fill_slots(h, B.slot_mem.as<double>(), nslot));
     GIS_HIP(h, B.meta.ensure(sizeof(RedWs) * S + sizeof(int) * ((size_t)T + 3 * (S + 1)) + 64));
     char* mb = B.meta.as<char>();
     B.d_red = reinterpret_cast<RedWs*>(mb);
@@ -396,6 +409,10 @@ int batch_solve(gis_context* h, double* x_packed, const double* b_packed, bool d
     }
     GIS_HIP(h, hipMemcpyAsync(B.st.p, B.st_host, sizeof(KState) * S, hipMemcpyHostToDevice,
                               h->stream));
+    if (B.slots_dirty) {
+        GIS_TRY(fill_slots(h, B.slot_mem.as<double>(), B.slot_n));
+        B.slots_dirty = false;
+    }
     const BatchView bv{B.d_tile_sys, B.d_tile0, B.st.as<KState>(), B.d_red};
     const Mat<double> m = mat64(h);
     const dim3 g = tiles_grid(N);
@@ -452,6 +469,7 @@ int batch_solve(gis_context* h, double* x_packed, const double* b_packed, bool d
     for (int s = 0; s < S; ++s) {
         const KState& st = B.st_host[s];
         int sk = st.status;
+        if (sk == kErrSync) B.slots_dirty = true;
         if (sk == kOK && !(st.norm == st.norm)) sk = kNaN;
         if (iters) iters[s] = st.k;
         if (final_res) final_res[s] = st.norm;

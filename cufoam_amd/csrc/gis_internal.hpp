@@ -88,7 +88,9 @@ struct BatchInfo {
     bool ready = false;
     int S = 0;
     std::vector<int> n, m, pad_off, pack_off, face_off, tile0;
-    DevMem red_mem, meta, st;
+    DevMem red_mem, meta, st, slot_mem;
+    size_t slot_n = 0;                   // doubles in slot_mem (every one holds kSlotEmpty)
+    bool slots_dirty = false;            // a reduction timed out: refill before the next solve
     RedWs* d_red = nullptr;
     int* d_tile_sys = nullptr;
     int* d_tile0 = nullptr;
@@ -236,6 +238,7 @@ void timer_release(gis_context* h);
 
 // reductions / state
 int ensure_reduction(gis_context* h, int rows);
+int fill_slots(gis_context* h, double* slot, size_t n);   // every slot to the empty pattern
 int ensure_state(gis_context* h);
 int read_state(gis_context* h);
 

@@ -241,6 +241,14 @@ __global__ void k_slot_fill(double* slot, long long n) {
     if (i < n) slot[i] = __longlong_as_double(static_cast<long long>(kSlotEmpty));
 }
 
+int fill_slots(gis_context* h, double* slot, size_t n) {
+    if (n == 0) return kOK;
+    hipLaunchKernelGGL(k_slot_fill, dim3((unsigned)((n + kWG - 1) / kWG)), dim3(kWG), 0,
+                       h->stream, slot, (long long)n);
+    GIS_HIP(h, hipGetLastError());
+    return kOK;
+}
+
 // The level counts must describe exactly `rows` (the group-final logic waits
 // for count[L] arrivals); memory only grows.  Tickets are re-zeroed and the
 // slots refilled whenever the shape changes.
@@ -273,9 +281,7 @@ int ensure_reduction(gis_context* h, int rows) {
     r.slot = reinterpret_cast<double*>(base + off);
     r.slot1 = r.slot + (size_t)r.count[0] * kMaxNV;
     off += nslot * sizeof(double);
-    hipLaunchKernelGGL(k_slot_fill, dim3((unsigned)((nslot + kWG - 1) / kWG)), dim3(kWG), 0,
-                       h->stream, r.slot, (long long)nslot);
-    GIS_HIP(h, hipGetLastError());
+    GIS_TRY(fill_slots(h, r.slot, nslot));
     unsigned* tk = reinterpret_cast<unsigned*>(base + off);
     GIS_HIP(h, hipMemsetAsync(tk, 0, ints * sizeof(unsigned), h->stream));
     for (int j = 0; j <= L; ++j) {
