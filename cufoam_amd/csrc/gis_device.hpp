@@ -134,7 +134,7 @@ __device__ __forceinline__ void block_tree(double (&v)[NV], double (*sh)[kMaxNV]
 // LDS a kernel hands to finish_tile().
 struct RedShared {
     double w[4][kMaxNV];
-    double top[2 * kGroup];   // SlotReduce: the collecting workgroup's upper levels
+    double top[2 * kGroup];   // SlotReduce: the collecting workgroup's level-2 sums
     int last;
 };
 
@@ -345,6 +345,14 @@ __device__ __forceinline__ bool poll_entry(double* base, int e, int n, double (&
     return true;
 }
 
+// A reduction whose bounded wait ran out: stop the solve(s) it serves.
+__device__ __forceinline__ void slot_fail(KState* st, int nst) {
+    for (int i = 0; i < nst; ++i) {
+        st[i].status = kErrSync;
+        st[i].done = 1;
+    }
+}
+
 // TileReduce's interface over published slots:
 //   begin(): the tiles' subtrees are formed and thread 0 publishes them; the
 //            workgroup is then free to issue its output stores and retire;
@@ -353,7 +361,9 @@ __device__ __forceinline__ bool poll_entry(double* base, int e, int n, double (&
 // With a single tile the totals go to fin at once, as in finish_tiles.
 template <int NV, int C>
 struct SlotReduce {
-    static_assert(NV <= 2, "RedShared::top holds two values per level-2 entry");
+    static_assert(NV <= kMaxNV, "at most kMaxNV totals");
+    // level-2 sums fold in LDS when they fit RedShared::top
+    static constexpr int kTopCap = 2 * kGroup;
     double flat[C * NV];
     int last_tile;
 
@@ -378,8 +388,9 @@ struct SlotReduce {
         }
     }
 
+    // st[0 .. nst) are marked failed if a wait runs out
     template <class Fin>
-    __device__ void end(const RedWs& red, KState* st, RedShared& sh, Fin&& fin) {
+    __device__ void end(const RedWs& red, KState* st, RedShared& sh, Fin&& fin, int nst = 1) {
         const int n0 = red.count[0];
         if (n0 == 1) {
             if (threadIdx.x == 0) {
@@ -414,13 +425,14 @@ struct SlotReduce {
             }
         }
         __syncthreads();
-        top_collect(red, st, sh, fin);
+        top_collect(red, st, nst, sh, fin);
     }
 
     // the top collector, all threads: level-1 entries (its own group's sum
     // from sh.w[0], the others from the level-1 slots), then the levels above
     template <class Fin>
-    __device__ void top_collect(const RedWs& red, KState* st, RedShared& sh, Fin&& fin) {
+    __device__ void top_collect(const RedWs& red, KState* st, int nst, RedShared& sh,
+                                Fin&& fin) {
         const int lane = threadIdx.x & 63;
         const int w = threadIdx.x >> 6;
         constexpr int kWaves = kWG / 64;
@@ -430,10 +442,7 @@ struct SlotReduce {
 #pragma unroll
         for (int i = 0; i < NV; ++i) own[i] = sh.w[0][i];
         if (!sh.last) {
-            if (threadIdx.x == 0) {
-                st->status = kErrSync;
-                st->done = 1;
-            }
+            if (threadIdx.x == 0) slot_fail(st, nst);
             return;
         }
         if (n1 == 1) {
@@ -455,7 +464,7 @@ struct SlotReduce {
             } else if (lane == 0) {
 #pragma unroll
                 for (int i = 0; i < NV; ++i) {
-                    if (n2 <= kGroup)
+                    if (n2 <= kGroup && n2 * NV <= kTopCap)
                         sh.top[h * NV + i] = x[i];
                     else
                         st_agent(&red.part[2][(size_t)h * kMaxNV + i], x[i]);
@@ -463,10 +472,7 @@ struct SlotReduce {
             }
         }
         if (n2 == 1) {
-            if (!ok && threadIdx.x == 0) {
-                st->status = kErrSync;
-                st->done = 1;
-            }
+            if (!ok && threadIdx.x == 0) slot_fail(st, nst);
             return;
         }
         __syncthreads();
@@ -474,13 +480,10 @@ struct SlotReduce {
         order_release();
         __syncthreads();
         if (!sh.last) {
-            if (threadIdx.x == 0) {
-                st->status = kErrSync;
-                st->done = 1;
-            }
+            if (threadIdx.x == 0) slot_fail(st, nst);
             return;
         }
-        if (n2 <= kGroup) {
+        if (n2 <= kGroup && n2 * NV <= kTopCap) {   // the next level is the total
             if (threadIdx.x < 64) {
                 double x[NV];
 #pragma unroll
@@ -490,7 +493,8 @@ struct SlotReduce {
             }
             return;
         }
-        // more than kGroup^3 tiles: the levels above 2 through the global partials
+        // level-2 sums that do not fit RedShared::top (many totals, or more than
+        // kGroup^3 tiles): the levels above 2 through the global partials
         for (int L = 2; L < kMaxLevels; ++L) {
             const int n = red.count[L];
             if (n == 1) {
@@ -528,13 +532,14 @@ struct SlotReduce {
 // must call it.
 template <int NV, class Fin>
 __device__ __forceinline__ void finish_tile_slots(const RedWs& red, KState* st, int tile,
-                                                  double (&v)[NV], RedShared& sh, Fin&& fin) {
+                                                  double (&v)[NV], RedShared& sh, Fin&& fin,
+                                                  int nst = 1) {
     double vv[1][NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) vv[0][i] = v[i];
     SlotReduce<NV, 1> tr;
     tr.begin(red, tile, 1, vv, sh);
-    tr.end(red, st, sh, fin);
+    tr.end(red, st, sh, fin, nst);
 }
 
 // One tile per workgroup.

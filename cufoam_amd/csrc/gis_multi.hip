@@ -121,7 +121,7 @@ __global__ __launch_bounds__(kWG) void k_msetup(Mat<T> m, int N, const T* __rest
         store4(R + k * n, t.i0, N, rv);
         if (R2) store4(R2 + k * n, t.i0, N, rv);
     }
-    finish_tile<NR>(red, t.tile, leaf, shr, [&](const double (&v)[NR]) {
+    finish_tile_slots<NR>(red, st, t.tile, leaf, shr, [&](const double (&v)[NR]) {
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             if (!(act & (1u << k))) {
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kWG) void k_msetup(Mat<T> m, int N, const T* __rest
             if (st[k].hist_cap > 0) st[k].hist[0] = norm;
             st[k].done = (done_rule == 0) ? !(norm > st[k].error) : 0;
         }
-    });
+    }, NR);
 }
 
 // ---- directions (both methods), streamed ----------------------------------------
@@ -228,11 +228,11 @@ __global__ __launch_bounds__(kWG) void k_mcg1(Mat<T> m, int N, const T* __restri
             if (t.i0 + q < N) leaf[k] = fma((double)pv[q], (double)yl[q], leaf[k]);
         store4(Q + k * n, t.i0, N, yl);
     }
-    finish_tile<NR>(red, t.tile, leaf, shr, [&](const double (&v)[NR]) {
+    finish_tile_slots<NR>(red, st, t.tile, leaf, shr, [&](const double (&v)[NR]) {
 #pragma unroll
         for (int k = 0; k < NR; ++k)
             if (act & (1u << k)) st[k].alpha = st[k].rho0 / v[k];
-    });
+    }, NR);
 }
 
 template <int NR, typename T>
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kWG) void k_mcg2(int N, T* __restrict__ R, const T*
         }
         store4(R + k * n, t.i0, N, rv);
     }
-    finish_tile<NR>(red, t.tile, leaf, shr, [&](const double (&v)[NR]) {
+    finish_tile_slots<NR>(red, st, t.tile, leaf, shr, [&](const double (&v)[NR]) {
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             if (!(act & (1u << k))) continue;
@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kWG) void k_mcg2(int N, T* __restrict__ R, const T*
             record_norm(s, norm);
             if (!(norm > s->error) || s->k >= s->max_steps) s->done = 1;
         }
-    });
+    }, NR);
 }
 
 template <int NR, typename T>
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(kWG) void k_mbi1(Mat<T> m, int N, const T* __restri
             if (t.i0 + q < N) leaf[k] = fma((double)psv[q], (double)yl[q], leaf[k]);
         store4(Q + k * n, t.i0, N, yl);
     }
-    finish_tile<NR>(red, t.tile, leaf, shr, [&](const double (&v)[NR]) {
+    finish_tile_slots<NR>(red, st, t.tile, leaf, shr, [&](const double (&v)[NR]) {
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             if (!(act & (1u << k))) continue;
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(kWG) void k_mbi1(Mat<T> m, int N, const T* __restri
                 s->done = 1;
             }
         }
-    });
+    }, NR);
 }
 
 // q* = A^T p*_k; x += xi p, r -= xi q, r* -= xi q*, ||r||, (r, r*) -> beta, stop
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(kWG) void k_mbi2(Mat<T> mt, int N, T* __restrict__ 
         store4(R + k * n, t.i0, N, rv);
         store4(RS + k * n, t.i0, N, rsv);
     }
-    finish_tile<2 * NR>(red, t.tile, leaf, shr, [&](const double (&v)[2 * NR]) {
+    finish_tile_slots<2 * NR>(red, st, t.tile, leaf, shr, [&](const double (&v)[2 * NR]) {
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             if (!(act & (1u << k))) continue;
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(kWG) void k_mbi2(Mat<T> mt, int N, T* __restrict__ 
                 s->beta = s->rho0 / s->rho1;
             }
         }
-    });
+    }, NR);
 }
 
 // ---- mixed-precision refinement: the FP64 outer steps ------------------------
@@ -441,7 +441,7 @@ __global__ __launch_bounds__(kWG) void k_mres(Mat<double> m, int N, const double
         }
         store4(R32 + k * n, t.i0, N, rf);
     }
-    finish_tile<2 * NR>(red, t.tile, leaf, shr, [&](const double (&v)[2 * NR]) {
+    finish_tile_slots<2 * NR>(red, ost, t.tile, leaf, shr, [&](const double (&v)[2 * NR]) {
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             if (!(act & (1u << k))) continue;
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(kWG) void k_mres(Mat<double> m, int N, const double
                 s->done = 1;
             }
         }
-    });
+    }, NR);
 }
 
 // X += (double) D for the columns refined in this outer step; counts the step
@@ -580,7 +580,11 @@ int multi_read(gis_context* h, int nrhs, MultiBufs& mb, bool* all_done) {
     GIS_HIP(h, hipStreamSynchronize(h->stream));
     GIS_TRY(timer_collect(h));
     bool d = true;
-    for (int k = 0; k < nrhs; ++k) d = d && mb.host[k].done;
+    for (int k = 0; k < nrhs; ++k) {
+        d = d && mb.host[k].done;
+        // a reduction whose wait ran out: rebuild the workspace (and its slots)
+        if (mb.host[k].status == kErrSync) h->red_rows = -1;
+    }
     *all_done = d;
     return kOK;
 }
@@ -677,8 +681,10 @@ int solve_multi_mixed_nr(gis_context* h, int method, const MultiMixedArgs& a, do
                                   h->stream));
         GIS_HIP(h, hipStreamSynchronize(h->stream));
         unsigned skip = 0;
-        for (int k = 0; k < NR; ++k)
+        for (int k = 0; k < NR; ++k) {
             if (ost_host[k].done) skip |= 1u << k;
+            if (ost_host[k].status == kErrSync) h->red_rows = -1;   // refill the slots
+        }
         if (skip == all) break;
         for (int k = 0; k < NR; ++k) {
             KState s{};

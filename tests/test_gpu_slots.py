@@ -68,3 +68,43 @@ def test_mixed_cg_bit_exact(gpu, big):
         assert (st, it) == (rst, rit) == (11, 6)
         np.testing.assert_array_equal(hist, rh)
         np.testing.assert_array_equal(x, rx)
+
+
+def test_multi_rhs_columns_bit_exact(gpu, big):
+    """3 columns: reductions of 3 and 6 totals through the level-2 LDS fold."""
+    s, A = big
+    gpu.set_system(s)
+    rng = np.random.default_rng(23)
+    B = rng.uniform(-1, 1, (3, s.n_cells))
+    X0 = np.zeros_like(B)
+    X, it, nf, sts = gpu.solve_multi("BiCG", B, X0, error=0.0, max_steps=4)
+    At = O.transpose(A)
+    for k in range(3):
+        ref = O.solve("BiCG", A, B[k], X0[k], 0.0, 4, At=At)
+        assert it[k] == ref.iterations == 4, k
+        assert nf[k] == ref.hist[-1], k
+        np.testing.assert_array_equal(X[k], ref.x)
+
+
+def test_multi_rhs_global_level2_path():
+    """330^3 (35,937,000 rows, 35,069 tiles: tree [35069, 548, 9, 1]) with 8
+    columns: BiCG's update kernel reduces 16 totals, whose 9 level-2 sums do
+    not fit the LDS fold, so the top collector takes the global-partials path.
+    Each column must equal its single-RHS solve, whose reductions use the LDS
+    path with the same tree, bit for bit."""
+    import torch
+    from cufoam_amd.gis import Gis
+    print("[case] laplacian 330^3 seed=24, 8 columns", flush=True)
+    s = mesh.laplacian(330, seed=24, shift=1e-3)
+    tiles = -(-s.n_cells // 1024)
+    assert (-(-tiles // 64) + 63) // 64 == 9
+    with Gis(0) as g:
+        g.set_system(s)
+        B = torch.from_numpy(np.random.default_rng(25).uniform(-1, 1, (8, s.n_cells))).cuda()
+        X, it, nf, sts = g.solve_multi("BiCG", B, torch.zeros_like(B), error=0.0, max_steps=3)
+        for k in (0, 5, 7):
+            x1, h1, st1 = g.solve("BiCG", B[k].contiguous(), torch.zeros_like(B[k]), error=0.0,
+                                  max_steps=3)
+            assert it[k] == len(h1) - 1 == 3, k
+            assert nf[k] == h1[-1], k
+            assert torch.equal(X[k], x1), k
