@@ -241,41 +241,31 @@ int batch_set_pattern(gis_context* h, int S, const int* n_cells, const int* n_fa
     std::vector<int> tile_sys(T);
     for (int s = 0; s < S; ++s)
         for (int t = B.tile0[s]; t < B.tile0[s + 1]; ++t) tile_sys[t] = s;
-    size_t dbl = 0, ints = 0;
     std::vector<RedWs> reds(S);
+    size_t dbl = 0;
     for (int s = 0; s < S; ++s) {
         int count = B.tile0[s + 1] - B.tile0[s];
         int L = 0;
         for (; L < kMaxLevels; ++L) {
             reds[s].count[L] = count;
-            dbl += (size_t)count * kMaxNV;
-            ints += (size_t)(count + kGroup - 1) / kGroup;
+            if (L >= 2) dbl += (size_t)count * kMaxNV;
             if (count == 1) break;
             count = (count + kGroup - 1) / kGroup;
         }
         reds[s].nlevels = L + 1;
         for (int j = L + 1; j < kMaxLevels; ++j) reds[s].count[j] = 1;
     }
-    GIS_HIP(h, B.red_mem.ensure(dbl * sizeof(double) + ints * sizeof(unsigned) + 256));
-    char* base = B.red_mem.as<char>();
-    size_t off = 0;
+    // levels 2 and up (the top collector's global path) as plain partials
+    GIS_HIP(h, B.red_mem.ensure(dbl * sizeof(double) + 256));
+    double* pp = B.red_mem.as<double>();
     for (int s = 0; s < S; ++s)
-        for (int j = 0; j < reds[s].nlevels; ++j) {
-            reds[s].part[j] = reinterpret_cast<double*>(base + off);
-            off += (size_t)reds[s].count[j] * kMaxNV * sizeof(double);
-        }
-    unsigned* tk = reinterpret_cast<unsigned*>(base + off);
-    GIS_HIP(h, hipMemsetAsync(tk, 0, ints * sizeof(unsigned), h->stream));
-    for (int s = 0; s < S; ++s) {
-        for (int j = 0; j < reds[s].nlevels; ++j) {
-            reds[s].ticket[j] = tk;
-            tk += (reds[s].count[j] + kGroup - 1) / kGroup;
-        }
-        for (int j = reds[s].nlevels; j < kMaxLevels; ++j) {
+        for (int j = 0; j < kMaxLevels; ++j) {
             reds[s].part[j] = nullptr;
-            reds[s].ticket[j] = nullptr;
+            if (j >= 2 && j < reds[s].nlevels) {
+                reds[s].part[j] = pp;
+                pp += (size_t)reds[s].count[j] * kMaxNV;
+            }
         }
-    }
     // published-partial slots per system (level 0, then level 1)
     size_t nslot = 0;
     for (int s = 0; s < S; ++s) nslot += ((size_t)reds[s].count[0] + reds[s].count[1]) * kMaxNV;

@@ -8,8 +8,8 @@
 //     kLeaf = 4 consecutive elements accumulated from +0.0, then a pairwise
 //     binary tree over the leaves.  A workgroup of kWG threads owns an aligned
 //     tile of kWG*kLeaf rows (one subtree); partials of tiles are folded in
-//     aligned groups of kGroup by the last-arriving workgroup of each group,
-//     so the result never depends on the launch grid or the device.
+//     aligned groups of kGroup, level by level (SlotReduce below), so the
+//     result never depends on the launch grid or the device.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,7 +30,6 @@ struct RedWs {
     double* slot;                     // published level-0 partials, count[0] * kMaxNV
                                       // doubles; every entry holds kSlotEmpty between launches
     double* slot1;                    // published group sums (level 1), count[1] * kMaxNV
-    unsigned* ticket[kMaxLevels];     // one arrival counter per group of level L
     int count[kMaxLevels];            // number of partials at level L
     int nlevels;                      // levels until count == 1
 };
@@ -89,18 +88,6 @@ __device__ __forceinline__ void order_acquire() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-__device__ __forceinline__ unsigned ticket_add(unsigned* t) {
-    return __hip_atomic_fetch_add(t, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ unsigned ticket_add_n(unsigned* t, unsigned n) {
-    return __hip_atomic_fetch_add(t, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void ticket_reset(unsigned* t) {
-    __hip_atomic_store(t, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // ---------------------------------------------------------------------------
 // pairwise trees
 // ---------------------------------------------------------------------------
@@ -131,171 +118,22 @@ __device__ __forceinline__ void block_tree(double (&v)[NV], double (*sh)[kMaxNV]
     }
 }
 
-// LDS a kernel hands to finish_tile().
+// LDS a kernel hands to its reduction (SlotReduce, finish_tile_slots).
 struct RedShared {
     double w[4][kMaxNV];
     double top[2 * kGroup];   // SlotReduce: the collecting workgroup's level-2 sums
     int last;
 };
 
-// Upper levels of the global reduction, run by wave 0 of the workgroup that
-// completed group idx of level 0: the last arriver of each aligned group of
-// kGroup partials folds the group, level by level, and the very last one calls
-// fin(totals) on lane 0.
-template <int NV, class Fin>
-__device__ __forceinline__ void fold_upper(const RedWs& red, int idx, Fin&& fin) {
-    const int lane = threadIdx.x;
-    for (int L = 0; L + 1 < kMaxLevels; ++L) {
-        order_acquire();
-        double w[NV];
-        const int src = idx * kGroup + lane;
-#pragma unroll
-        for (int i = 0; i < NV; ++i)
-            w[i] = (src < red.count[L]) ? ld_agent(&red.part[L][(size_t)src * kMaxNV + i]) : 0.0;
-        wave_tree<NV>(w);
-        if (red.count[L + 1] == 1) {
-            if (lane == 0) fin(w);
-            return;
-        }
-        int last = 0;
-        if (lane == 0) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i)
-                st_agent(&red.part[L + 1][(size_t)idx * kMaxNV + i], w[i]);
-            order_release();
-            const int g = idx / kGroup;
-            const int members = min(kGroup, red.count[L + 1] - g * kGroup);
-            const unsigned prev = ticket_add(&red.ticket[L + 1][g]);
-            last = (prev == (unsigned)(members - 1));
-            if (last) ticket_reset(&red.ticket[L + 1][g]);
-        }
-        last = __shfl(last, 0, 64);
-        if (!last) return;
-        idx = idx / kGroup;
-    }
-}
-
-// Fold the leaf values of C consecutive tiles (tile0 .. tile0+nt-1, all in one
-// aligned group of kGroup, nt <= C) into the global reduction: each tile's
-// subtree is formed here and stored as its own level-0 partial, and one ticket
-// of weight nt is taken for the workgroup.  Because the partials and the tree
-// above them do not depend on C, every C gives bit-identical totals.  Must be
-// reached by every thread of the workgroup exactly once per launch.
-template <int NV, int C, class Fin>
-__device__ __forceinline__ void finish_tiles(const RedWs& red, int tile0, int nt,
-                                             double (&v)[C][NV], RedShared& sh, Fin&& fin) {
-    static_assert(C * NV <= kMaxNV, "RedShared holds kMaxNV values per wave");
-    double flat[C * NV];
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-#pragma unroll
-        for (int i = 0; i < NV; ++i) flat[c * NV + i] = v[c][i];
-    block_tree<C * NV>(flat, sh.w);
-    if (red.count[0] == 1) {
-        if (threadIdx.x == 0) {
-            double t[NV];
-#pragma unroll
-            for (int i = 0; i < NV; ++i) t[i] = flat[i];
-            fin(t);
-        }
-        return;
-    }
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-            if (c < nt) {
-#pragma unroll
-                for (int i = 0; i < NV; ++i)
-                    st_agent(&red.part[0][(size_t)(tile0 + c) * kMaxNV + i], flat[c * NV + i]);
-            }
-        }
-        order_release();
-        const int g = tile0 / kGroup;
-        const int members = min(kGroup, red.count[0] - g * kGroup);
-        const unsigned prev = ticket_add_n(&red.ticket[0][g], (unsigned)nt);
-        const int last = (prev + (unsigned)nt == (unsigned)members);
-        if (last) ticket_reset(&red.ticket[0][g]);
-        sh.last = last;
-    }
-    __syncthreads();
-    if (!sh.last || threadIdx.x >= 64) return;
-    fold_upper<NV>(red, tile0 / kGroup, fin);
-}
-
-// finish_tiles in two phases, so that a kernel's output stores overlap the
-// ticket's round trip instead of delaying it:
-//   begin(): the tiles' subtrees are formed and published as level-0
-//            partials and thread 0 takes the workgroup's ticket (the release
-//            wait covers only what was issued before: loads and partials);
-//   ...      the kernel issues its output stores;
-//   end():   waves 1..3 are done; wave 0 learns whether it arrived last in its
-//            group and, if so, folds the upper levels and calls fin.
-// Partials, tree and totals are those of finish_tiles.
-template <int NV, int C>
-struct TileReduce {
-    double flat[C * NV];
-    unsigned prev;
-    int tile0, nt;
-
-    __device__ void begin(const RedWs& red, int tile0_, int nt_, double (&v)[C][NV],
-                          RedShared& sh) {
-        static_assert(C * NV <= kMaxNV, "RedShared holds kMaxNV values per wave");
-        tile0 = tile0_;
-        nt = nt_;
-#pragma unroll
-        for (int c = 0; c < C; ++c)
-#pragma unroll
-            for (int i = 0; i < NV; ++i) flat[c * NV + i] = v[c][i];
-        block_tree<C * NV>(flat, sh.w);
-        prev = 0;
-        if (red.count[0] != 1 && threadIdx.x == 0) {
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                if (c < nt) {
-#pragma unroll
-                    for (int i = 0; i < NV; ++i)
-                        st_agent(&red.part[0][(size_t)(tile0 + c) * kMaxNV + i],
-                                 flat[c * NV + i]);
-                }
-            }
-            order_release();
-            prev = ticket_add_n(&red.ticket[0][tile0 / kGroup], (unsigned)nt);
-        }
-    }
-
-    template <class Fin>
-    __device__ void end(const RedWs& red, Fin&& fin) {
-        if (threadIdx.x >= 64) return;
-        if (red.count[0] == 1) {
-            if (threadIdx.x == 0) {
-                double t[NV];
-#pragma unroll
-                for (int i = 0; i < NV; ++i) t[i] = flat[i];
-                fin(t);
-            }
-            return;
-        }
-        int last = 0;
-        if (threadIdx.x == 0) {
-            const int g = tile0 / kGroup;
-            const int members = min(kGroup, red.count[0] - g * kGroup);
-            last = (prev + (unsigned)nt == (unsigned)members);
-            if (last) ticket_reset(&red.ticket[0][g]);
-        }
-        last = __shfl(last, 0, 64);
-        if (!last) return;
-        fold_upper<NV>(red, tile0 / kGroup, fin);
-    }
-};
-
 // ---------------------------------------------------------------------------
 // Reduction without a per-workgroup round trip.
 //
-// A workgroup holding a ticket cannot retire until the ticket's atomic has
-// come back, and the last arriver's fold is a chain of such round trips at the
-// very end of the kernel.  Here a workgroup only publishes its tiles' subtree
-// sums, with plain agent-scope stores, into slots that hold the signalling-NaN
-// pattern kSlotEmpty until written, and retires.  No arithmetic result is a
+// Had each workgroup taken an arrival ticket (an atomic counter per group, the
+// last arriver folding the group), it could not retire until the ticket's
+// atomic came back, and the last arriver's fold would be a chain of such round
+// trips at the very end of the kernel.  Here a workgroup only publishes its
+// tiles' subtree sums, with plain agent-scope stores, into slots that hold the
+// signalling-NaN pattern kSlotEmpty until written, and retires.  No arithmetic result is a
 // signalling NaN (IEEE operations quiet them), so a slot that no longer holds
 // the pattern holds its final value.
 //   * The workgroup holding the last tile of an aligned group of kGroup tiles
@@ -304,8 +142,9 @@ struct TileReduce {
 //   * The workgroup holding the last tile overall does the same for its own
 //     group, then collects the level-1 slots and folds the levels above in
 //     registers and LDS, and calls fin(totals).
-// Every fold is over the aligned groups of kGroup that fold_upper uses: same
-// partials, same tree, same bits as finish_tiles.  Collectors restore the
+// Every fold is a wave_tree over an aligned group of kGroup entries, zero-
+// padded: the tree the oracle's reduction fixes, whatever the grid or the
+// number of tiles per workgroup.  Collectors restore the
 // pattern in every slot they read.  A collector waits only for workgroups that
 // never wait themselves, and every wait is bounded: after kSlotSpins polls
 // without the data it records kErrSync and stops the solve instead of hanging.
@@ -353,12 +192,12 @@ __device__ __forceinline__ void slot_fail(KState* st, int nst) {
     }
 }
 
-// TileReduce's interface over published slots:
+// The reduction of a kernel whose workgroups cover C consecutive tiles each:
 //   begin(): the tiles' subtrees are formed and thread 0 publishes them; the
 //            workgroup is then free to issue its output stores and retire;
 //   end():   group and top collectors as described above; fin on thread 0 of
 //            the top collector.
-// With a single tile the totals go to fin at once, as in finish_tiles.
+// With a single tile the totals go to fin at once.
 template <int NV, int C>
 struct SlotReduce {
     static_assert(NV <= kMaxNV, "at most kMaxNV totals");
@@ -542,15 +381,6 @@ __device__ __forceinline__ void finish_tile_slots(const RedWs& red, KState* st, 
     tr.end(red, st, sh, fin, nst);
 }
 
-// One tile per workgroup.
-template <int NV, class Fin>
-__device__ __forceinline__ void finish_tile(const RedWs& red, int tile, double (&v)[NV],
-                                            RedShared& sh, Fin&& fin) {
-    double vv[1][NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) vv[0][i] = v[i];
-    finish_tiles<NV, 1>(red, tile, 1, vv, sh, fin);
-}
 
 // ---------------------------------------------------------------------------
 // CSR tile SpMV: y[row0 .. row0+nrows) = sum_j a_j * X(col_j), written to

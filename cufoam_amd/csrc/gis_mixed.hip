@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kWG) void k_mx_setup(Mat<double> m, int N,
         }
     }
     store4(r32, t.i0, N, rf);
-    finish_tile<3>(red, t.tile, leaf, shr, [&](const double (&v)[3]) {
+    finish_tile_slots<3>(red, st, t.tile, leaf, shr, [&](const double (&v)[3]) {
         const double bnorm = sqrt(v[0]);
         const double rnorm = sqrt(v[1]);
         st->aux[3] = bnorm;

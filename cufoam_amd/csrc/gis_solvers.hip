@@ -249,49 +249,41 @@ int fill_slots(gis_context* h, double* slot, size_t n) {
     return kOK;
 }
 
-// The level counts must describe exactly `rows` (the group-final logic waits
-// for count[L] arrivals); memory only grows.  Tickets are re-zeroed and the
-// slots refilled whenever the shape changes.
+// The level counts must describe exactly `rows` (the collectors wait for
+// count[L] entries); memory only grows.  The slots are refilled whenever the
+// shape changes.
 
 int ensure_reduction(gis_context* h, int rows) {
     if (h->red_rows == rows && h->red_mem.p) return kOK;
     RedWs r{};
     int count = tiles_of(rows);
-    size_t dbl = 0, ints = 0;
     int L = 0;
     for (; L < kMaxLevels; ++L) {
         r.count[L] = count;
-        dbl += (size_t)count * kMaxNV;
-        ints += (size_t)(count + kGroup - 1) / kGroup;
         if (count == 1) break;
         count = (count + kGroup - 1) / kGroup;
     }
     if (L == kMaxLevels) return fail(h, kErrShape, "reduction depth exceeded");
     r.nlevels = L + 1;
     for (int j = L + 1; j < kMaxLevels; ++j) r.count[j] = 1;
-    // published-partial slots: level 0, then level 1, contiguous
+    // level-0 and level-1 partials travel through slots; levels 2 and up (the
+    // top collector's global path) through plain partial arrays
+    size_t dbl = 0;
+    for (int j = 2; j <= L; ++j) dbl += (size_t)r.count[j] * kMaxNV;
     const size_t nslot = ((size_t)r.count[0] + (size_t)r.count[1]) * kMaxNV;
-    GIS_HIP(h, h->red_mem.ensure((dbl + nslot) * sizeof(double) + ints * sizeof(unsigned) + 256));
-    char* base = h->red_mem.as<char>();
-    size_t off = 0;
-    for (int j = 0; j <= L; ++j) {
-        r.part[j] = reinterpret_cast<double*>(base + off);
-        off += (size_t)r.count[j] * kMaxNV * sizeof(double);
-    }
-    r.slot = reinterpret_cast<double*>(base + off);
-    r.slot1 = r.slot + (size_t)r.count[0] * kMaxNV;
-    off += nslot * sizeof(double);
-    GIS_TRY(fill_slots(h, r.slot, nslot));
-    unsigned* tk = reinterpret_cast<unsigned*>(base + off);
-    GIS_HIP(h, hipMemsetAsync(tk, 0, ints * sizeof(unsigned), h->stream));
-    for (int j = 0; j <= L; ++j) {
-        r.ticket[j] = tk;
-        tk += (r.count[j] + kGroup - 1) / kGroup;
-    }
-    for (int j = L + 1; j < kMaxLevels; ++j) {
+    GIS_HIP(h, h->red_mem.ensure((dbl + nslot) * sizeof(double) + 256));
+    double* base = h->red_mem.as<double>();
+    r.slot = base;
+    r.slot1 = base + (size_t)r.count[0] * kMaxNV;
+    double* pp = base + nslot;
+    for (int j = 0; j < kMaxLevels; ++j) {
         r.part[j] = nullptr;
-        r.ticket[j] = nullptr;
+        if (j >= 2 && j <= L) {
+            r.part[j] = pp;
+            pp += (size_t)r.count[j] * kMaxNV;
+        }
     }
+    GIS_TRY(fill_slots(h, r.slot, nslot));
     h->red = r;
     h->red_rows = rows;
     return kOK;
