@@ -554,6 +554,19 @@ GIS_API int gis_solve_pcg_device(gis_handle h, int precond, double* d_x, const d
     return pcg_impl(h, precond, d_x, d_b, params, hist, hist_cap, hist_len, true);
 }
 
+GIS_API int gis_set_slot_fault(gis_handle h, int polls) {
+    if (!h) return GIS_ERR_ARG;
+    if (polls < 0) return fail(h, GIS_ERR_ARG, "polls must be >= 0");
+    h->slot_fault = polls;
+    if (h->st) {   // the live state too, for dot and NormFactor between solves
+        h->st_host->slot_fault = polls;
+        GIS_HIP(h, hipMemcpyAsync(&h->st->slot_fault, &h->st_host->slot_fault, sizeof(int),
+                                  hipMemcpyHostToDevice, h->stream));
+        GIS_HIP(h, hipStreamSynchronize(h->stream));
+    }
+    return GIS_OK;
+}
+
 GIS_API int gis_set_gamg(gis_handle h, double omega, int nu, double cscale) {
     if (!h) return GIS_ERR_ARG;
     if (!(omega > 0.0 && omega < 2.0) || nu < 1 || nu > 8 || !(cscale > 0.0 && cscale < 2.0))

@@ -395,6 +395,7 @@ int batch_solve(gis_context* h, double* x_packed, const double* b_packed, bool d
         st.max_steps = ms;
         st.hist = nullptr;
         st.hist_cap = 0;
+        st.slot_fault = h->slot_fault;
         B.st_host[s] = st;
     }
     GIS_HIP(h, hipMemcpyAsync(B.st.p, B.st_host, sizeof(KState) * S, hipMemcpyHostToDevice,

@@ -52,6 +52,9 @@ struct KState {
     int nhist;                        // entries pushed by solvers with sparse histories
     double rel_tol;                   // > 0: error := max(error, rel_tol * norm0)
     long long min_steps;              // no stop on convergence before this many steps
+    int slot_fault;                   // failure-path tests only: > 0 makes reduction
+                                      // collectors see no slot written and give up
+                                      // after that many polls
 };
 
 // OpenFOAM's loop rule (lduMatrix::solver): iterate while (k < maxIter and not
@@ -165,16 +168,18 @@ __device__ __forceinline__ void slot_clear(double* p) {
 // wave sees its entry written; entries e >= n read as 0.  Each read slot is
 // restored to the pattern.  False when the bounded wait ran out.
 template <int NV>
-__device__ __forceinline__ bool poll_entry(double* base, int e, int n, double (&x)[NV]) {
+__device__ __forceinline__ bool poll_entry(double* base, int e, int n, double (&x)[NV],
+                                           int fault) {
+    const int limit = fault > 0 ? fault : kSlotSpins;
     for (int spins = 0;; ++spins) {
-        int miss = 0;
+        int miss = fault > 0 ? 1 : 0;
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
             x[i] = e < n ? ld_agent(base + (size_t)e * NV + i) : 0.0;
             miss |= slot_empty(x[i]) ? 1 : 0;
         }
         if (!__any(miss)) break;
-        if (spins >= kSlotSpins) return false;
+        if (spins >= limit) return false;
         __builtin_amdgcn_s_sleep(1);
     }
     if (e < n) {
@@ -248,7 +253,7 @@ struct SlotReduce {
         const int g = t / kGroup;
         if (threadIdx.x < 64) {
             double v[NV];
-            const bool ok = poll_entry<NV>(red.slot, g * kGroup + lane, n0, v);
+            const bool ok = poll_entry<NV>(red.slot, g * kGroup + lane, n0, v, st->slot_fault);
             if (ok) wave_tree<NV>(v);
             if (!top) {
                 if (ok && lane == 0) {
@@ -292,7 +297,7 @@ struct SlotReduce {
         for (int h = w; h < n2; h += kWaves) {
             const int e = h * kGroup + lane;
             double x[NV];
-            ok = poll_entry<NV>(red.slot1, e, n1 - 1, x) && ok;
+            ok = poll_entry<NV>(red.slot1, e, n1 - 1, x, st->slot_fault) && ok;
             if (e == n1 - 1) {
 #pragma unroll
                 for (int i = 0; i < NV; ++i) x[i] = own[i];

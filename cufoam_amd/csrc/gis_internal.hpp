@@ -183,6 +183,7 @@ struct gis_context {
     gis::DevMem red_mem;
     gis::RedWs red{};
     int red_rows = -1;
+    int slot_fault = 0;                         // gis_set_slot_fault (failure-path tests)
     gis::DevMem state_mem;
     gis::KState* st = nullptr;                  // device state
     gis::KState* st_host = nullptr;             // pinned mirror

@@ -564,6 +564,7 @@ int multi_prepare(gis_context* h, int nrhs, const SolveArgs& a, size_t vec_bytes
         KState s{};
         s.error = a.error;
         s.max_steps = a.max_steps <= 0 ? (long long)INT_MAX : a.max_steps;
+        s.slot_fault = h->slot_fault;
         s.int_bound = a.int_bound;
         s.hist = a.hist_cap > 0 ? h->hist.as<double>() + (size_t)k * cap : nullptr;
         s.hist_cap = a.hist_cap > 0 ? cap : 0;
@@ -692,6 +693,7 @@ int solve_multi_mixed_nr(gis_context* h, int method, const MultiMixedArgs& a, do
             // target (the last correction need not overshoot it)
             s.error = std::max(a.inner_rel_tol * ost_host[k].norm, 0.5 * ost_host[k].error);
             s.max_steps = a.max_inner <= 0 ? (long long)INT_MAX : a.max_inner;
+            s.slot_fault = h->slot_fault;
             s.int_bound = a.int_bound;
             mb.host[k] = s;
         }
@@ -766,6 +768,7 @@ int device_solve_multi_mixed(gis_context* h, int method, int nrhs, const MultiMi
     KState* ost_host = mb.host + kMaxRhs;
     for (int k = 0; k < nrhs; ++k) {
         KState s{};
+        s.slot_fault = h->slot_fault;
         s.max_steps = a.max_outer <= 0 ? (long long)INT_MAX : a.max_outer;
         s.hist = h->hist.as<double>() + (size_t)k * cap;
         s.hist_cap = cap;

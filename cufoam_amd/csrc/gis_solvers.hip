@@ -435,6 +435,7 @@ int prepare_solve(gis_context* h, const SolveArgs& a, int nvec) {
     s.int_bound = a.int_bound;
     s.rel_tol = a.rel_tol;
     s.min_steps = a.min_steps;
+    s.slot_fault = h->slot_fault;
     s.hist = h->hist.as<double>();
     s.hist_cap = cap;
     s.aux[4] = INFINITY;   // running min for FindDiagAbsMin

@@ -92,6 +92,7 @@ SIGNATURES = {
     "gis_solve_pcg_device": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
     "gis_foam_solve_pcg": (_i, [_vp, _i, _vp, _vp, _d, _d, _ll, _ll, C.POINTER(_Perf)]),
     "gis_set_gamg": (_i, [_vp, _d, _i, _d]),
+    "gis_set_slot_fault": (_i, [_vp, _i]),
     "gis_gamg_levels": (_i, [_vp, _ip]),
     "gis_gamg_level": (_i, [_vp, _i, _ip, _ip, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gis_solve_multi": (_i, [_vp, _i, _i, _vp, _vp, C.POINTER(_Params), _i, _vp, _vp, _vp]),
@@ -399,6 +400,11 @@ class Gis:
         self._check(st, f"solve_pcg({precond})")
         self.last_iterations = max(hl.value - 1, 0)
         return x, hist[:min(hl.value, hist_cap)].copy(), st
+
+    def set_slot_fault(self, polls: int = 0):
+        """Failure-path tests only: polls > 0 makes every device reduction give up
+        after that many polls (solves then raise GisError 13); 0 restores."""
+        self._check(self._L.gis_set_slot_fault(self._h, int(polls)), "set_slot_fault")
 
     def set_gamg(self, omega: float = 0.8, nu: int = 1, cscale: float = 1.8):
         self._check(self._L.gis_set_gamg(self._h, omega, nu, cscale), "set_gamg")

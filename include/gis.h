@@ -242,6 +242,14 @@ GIS_API int gis_foam_solve_pcg(gis_handle h, int precond, double *psi, const dou
                                long long max_iter, gis_performance *perf);
 /* GAMG settings (defaults omega 0.8, nu 1, cscale 1.8). */
 GIS_API int gis_set_gamg(gis_handle h, double omega, int nu, double cscale);
+
+/* ---- failure-path tests only (no counterpart in the reference): with
+ * polls > 0 every workgroup that collects reduction partials on the device
+ * treats them as never written and gives up after `polls` polls, so every
+ * solve (and dot / NormFactor) of the handle on more than one 1024-row tile
+ * stops with GIS_ERR_SYNC; polls = 0 restores normal operation.  The kernels
+ * still exit normally.  Applies to solves started after the call. ---- */
+GIS_API int gis_set_slot_fault(gis_handle h, int polls);
 /* The hierarchy (built if needed): level count, then one level's sizes,
  * aggregation map (cell -> next-level cell; levels below the last) and LDU
  * matrix (any output pointer may be NULL). */

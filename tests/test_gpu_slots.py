@@ -108,3 +108,54 @@ def test_multi_rhs_global_level2_path():
             assert it[k] == len(h1) - 1 == 3, k
             assert nf[k] == h1[-1], k
             assert torch.equal(X[k], x1), k
+
+
+def test_sync_failure_is_reported_and_recovers(gpu):
+    """The collectors' bounded wait, forced from the host (gis_set_slot_fault:
+    every collector sees its slots unwritten and gives up after 4 polls; the
+    kernels still exit normally): single, mixed, multi-RHS and batched solves
+    stop with GIS_ERR_SYNC, and after the hook is cleared the refilled slots give
+    bit-exact results again."""
+    from cufoam_amd.gis import GisError
+    s = mesh.laplacian(48, seed=26, shift=1e-2)   # 110,592 rows: 108 tiles, 2 groups
+    A = O.ldu2csr(s)
+    gpu.set_system(s)
+    B = np.random.default_rng(27).uniform(-1, 1, (2, s.n_cells))
+    small = [mesh.laplacian(12, seed=28, shift=1e-3), mesh.laplacian(3, seed=29, shift=1e-3)]
+    gpu.set_slot_fault(4)
+    try:
+        for call in (lambda: gpu.solve("CG", s.b, s.x0, error=0.0, max_steps=5),
+                     lambda: gpu.solve_mixed(s.b, s.x0, 1e-10, 1e-2, 5, 0),
+                     lambda: gpu.solve_multi("BiCG", B, None, error=0.0, max_steps=5)):
+            with pytest.raises(GisError) as e:
+                call()
+            assert e.value.status == 13
+        gpu.batch_set_systems(small)
+        bb = np.concatenate([t.b for t in small])
+        with pytest.raises(GisError) as e:   # the 2-tile system fails, the 1-tile one cannot
+            gpu.batch_solve(bb, None, 1e-9, 500, use_graph=True)
+        assert e.value.status == 13
+    finally:
+        gpu.set_slot_fault(0)
+    x, it, nf, st = gpu.batch_solve(bb, None, 1e-9, 500, use_graph=True)
+    off = 0
+    for k, t in enumerate(small):
+        ref = O.solve("CG", O.ldu2csr(t), t.b, t.x0, 1e-9, 500)
+        assert it[k] == ref.iterations and nf[k] == ref.hist[-1]
+        np.testing.assert_array_equal(x[off:off + t.n_cells], ref.x)
+        off += t.n_cells
+    gpu.set_system(s)
+    ref = O.solve("CG", A, s.b, s.x0, 0.0, 5)
+    x, hist, st = gpu.solve("CG", s.b, s.x0, error=0.0, max_steps=5)
+    np.testing.assert_array_equal(hist, ref.hist)
+    np.testing.assert_array_equal(x, ref.x)
+    rx, rh, rit, rst = O.mixed_cg(A, s.b, s.x0, 1e-10, 1e-2, 5, 0)
+    x, hist, it, rel, st = gpu.solve_mixed(s.b, s.x0, 1e-10, 1e-2, 5, 0)
+    assert (st, it) == (rst, rit)
+    np.testing.assert_array_equal(x, rx)
+    X, its, nfs, sts = gpu.solve_multi("BiCG", B, None, error=0.0, max_steps=5)
+    At = O.transpose(A)
+    for k in range(2):
+        r1 = O.solve("BiCG", A, B[k], np.zeros(s.n_cells), 0.0, 5, At=At)
+        assert nfs[k] == r1.hist[-1]
+        np.testing.assert_array_equal(X[k], r1.x)
