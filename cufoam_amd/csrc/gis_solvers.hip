@@ -3,8 +3,8 @@
 //
 // Each method keeps its scalars (rho, alpha, ...) in a device KState and
 // advances with a fixed sequence of launches per iteration; reductions finish
-// inside the producing kernel (last-arriving workgroup), so no iteration
-// needs a host round-trip.  The host enqueues iterations in blocks and only
+// inside the producing kernel (its slot collectors), so no iteration needs a
+// host round-trip.  The host enqueues iterations in blocks and only
 // reads the `done` flag between blocks; launches after convergence return at
 // once, which preserves the reference's per-iteration stopping rule and
 // therefore its iteration count.
