@@ -403,7 +403,15 @@ GIS_API int gis_set_timing(gis_handle h, int slot_mask) {
     for (int i = 0; i < kTimerSlots; ++i) {
         h->timer.total_ms[i] = 0.0;
         h->timer.count[i] = 0;
+        h->timer.seen[i] = 0;
     }
+    return GIS_OK;
+}
+
+GIS_API int gis_set_timing_sample(gis_handle h, int period) {
+    if (!h) return GIS_ERR_ARG;
+    if (period < 1) return fail(h, GIS_ERR_ARG, "period must be >= 1");
+    h->timer.period = period;
     return GIS_OK;
 }
 

@@ -81,6 +81,8 @@ struct Timer {
     int used = 0;                           // pairs recorded since the last collect
     double total_ms[kTimerSlots] = {};
     long long count[kTimerSlots] = {};
+    int period = 1;                         // time one launch in `period` of each slot
+    long long seen[kTimerSlots] = {};       // launches of each slot since gis_set_timing
 };
 
 // Batched systems: block-diagonal stack, each system padded to whole tiles.

@@ -93,6 +93,7 @@ SIGNATURES = {
     "gis_foam_solve_pcg": (_i, [_vp, _i, _vp, _vp, _d, _d, _ll, _ll, C.POINTER(_Perf)]),
     "gis_set_gamg": (_i, [_vp, _d, _i, _d]),
     "gis_set_slot_fault": (_i, [_vp, _i]),
+    "gis_set_timing_sample": (_i, [_vp, _i]),
     "gis_gamg_levels": (_i, [_vp, _ip]),
     "gis_gamg_level": (_i, [_vp, _i, _ip, _ip, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gis_solve_multi": (_i, [_vp, _i, _i, _vp, _vp, C.POINTER(_Params), _i, _vp, _vp, _vp]),
@@ -359,11 +360,13 @@ class Gis:
     TIMER_SLOTS = {"iter_spmv": 0, "iter_update": 1, "fp64_update": 2, "spmv": 3,
                    "iter_direction": 4}
 
-    def set_timing(self, *slots):
-        """Record HIP-event durations of the named slots (none: stop timing)."""
+    def set_timing(self, *slots, period: int = 1):
+        """Record HIP-event durations of the named slots (none: stop timing), one
+        launch in `period` of each."""
         mask = 0
         for s in slots:
             mask |= 1 << (self.TIMER_SLOTS[s] if isinstance(s, str) else int(s))
+        self._check(self._L.gis_set_timing_sample(self._h, int(period)), "set_timing")
         self._check(self._L.gis_set_timing(self._h, mask), "set_timing")
 
     def get_timing(self, slot) -> tuple[float, int]:

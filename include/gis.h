@@ -169,6 +169,10 @@ GIS_API int gis_autotune_clear(void);
  * launch adds two event records to the stream. ---- */
 GIS_API int gis_set_timing(gis_handle h, int slot_mask);
 GIS_API int gis_get_timing(gis_handle h, int slot, double *total_ms, long long *count);
+/* Time one launch in `period` (>= 1, default 1) of each recorded slot: each
+ * timed launch adds a few microseconds of launch gap, so a timed solve samples;
+ * count then reports the launches actually timed. */
+GIS_API int gis_set_timing_sample(gis_handle h, int period);
 
 /* ---- NormFactor (gis/gis.cu:106-123) ---- */
 GIS_API int gis_norm_factor(gis_handle h, const double *b, const double *x, double *nf);

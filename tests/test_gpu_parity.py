@@ -167,3 +167,14 @@ def test_foam_cg_matches_oracle_wrapper(gpu, n, seed):
     assert perf.initial_residual == ref.initial_residual
     assert perf.final_residual == ref.final_residual
     assert np.array_equal(psi, ref_x)
+
+
+def test_kernel_timing_samples_one_launch_in_period(gpu):
+    s = mesh.laplacian(24, seed=31, shift=1e-2)
+    gpu.set_system(s)
+    for period, expect in ((1, 40), (4, 10)):
+        gpu.set_timing("iter_spmv", period=period)
+        gpu.solve("CG", s.b, s.x0, error=0.0, max_steps=40)
+        ms, n = gpu.get_timing("iter_spmv")
+        gpu.set_timing()
+        assert n == expect and ms > 0.0
