@@ -405,8 +405,55 @@ __global__ __launch_bounds__(kWG) void k_vc_resid(Mat<T> m, int N, const T* __re
     store4nt(tr, t.i0, N, tv);
 }
 
+// Coarse levels (below the finest): aggregation leaves no slice whose column
+// offsets fit a table, and a level of a few thousand to a few hundred thousand
+// rows fills only part of the chip, so the per-tile kernels above wait on
+// memory latency: each wave walks 4 slices one after another, 8 slots at a
+// time.  Here each wave takes one 64-row slice (one row per lane), with
+// kCoarseBatch slots' loads in flight before the gathers.  Same products in
+// the same order as every other schedule: the results are bit-identical.
+constexpr int kCoarseBatch = 32;
+constexpr int kSlicesPerWG = kWG / kSlice;
+constexpr int kCoarseRows = INT_MAX;   // levels of at most this many rows use them
+
+__device__ __forceinline__ int wave_slice() {
+    return (int)blockIdx.x * kSlicesPerWG + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_vc_sweep_w(Mat<T> m, int N, const T* __restrict__ sm,
+                                                    const T* __restrict__ r,
+                                                    const T* __restrict__ z, T* __restrict__ zn,
+                                                    const KState* st) {
+    if (st->done) return;
+    const int s = wave_slice();
+    if (s * kSlice >= N) return;   // whole waves
+    const int lane = threadIdx.x & 63;
+    const int a = m.soff[s];
+    const T y = sell_row<T, kCoarseBatch>(m.scol, m.sval, a, (m.soff[s + 1] - a) / kSlice, lane,
+                                          [&](int c) { return z[c]; });
+    const int row = s * kSlice + lane;
+    if (row < N) zn[row] = fmaT(sm[row], r[row] - y, z[row]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_vc_resid_w(Mat<T> m, int N, const T* __restrict__ r,
+                                                    const T* __restrict__ z, T* __restrict__ tr,
+                                                    const KState* st) {
+    if (st->done) return;
+    const int s = wave_slice();
+    if (s * kSlice >= N) return;
+    const int lane = threadIdx.x & 63;
+    const int a = m.soff[s];
+    const T y = sell_row<T, kCoarseBatch>(m.scol, m.sval, a, (m.soff[s + 1] - a) / kSlice, lane,
+                                          [&](int c) { return z[c]; });
+    const int row = s * kSlice + lane;
+    if (row < N) tr[row] = r[row] - y;
+}
+
 // rc[C] = sum of t over C's cells, ascending, and the next level's first
-// sweep from zero, zc0 = smc .* rc
+// sweep from zero, zc0 = smc .* rc.  Members are read 4 at a time (their
+// loads in flight together), summed one by one in order.
 template <typename T>
 __global__ __launch_bounds__(kBlk) void k_vc_restrict(int nc, const int* __restrict__ mptr,
                                                       const int* __restrict__ mlist,
@@ -417,8 +464,19 @@ __global__ __launch_bounds__(kBlk) void k_vc_restrict(int nc, const int* __restr
     if (st->done) return;
     const int c = blockIdx.x * kBlk + threadIdx.x;
     if (c >= nc) return;
+    const int p0 = mptr[c], p1 = mptr[c + 1];
     T s = T(0);
-    for (int p = mptr[c]; p < mptr[c + 1]; ++p) s = s + t[mlist[p]];
+    for (int p = p0; p < p1; p += 4) {
+        int q[4];
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = mlist[min(p + u, p1 - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = t[q[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (p + u < p1) s = s + v[u];
+    }
     rc[c] = s;
     zc0[c] = smc[c] * s;
 }
@@ -1039,12 +1097,25 @@ const T* vcycle(gis_context* h, int l, const T* r, double* out, const double* r6
         }
         return dst;
     }
+    // non-reducing sweeps and the residual: per-slice waves below the finest level
+    const bool wave = l > 0 && N <= kCoarseRows;
+    const dim3 gw((unsigned)((N + kSlicesPerWG * kSlice - 1) / (kSlicesPerWG * kSlice)));
+    auto sweep = [&](const T* zin, T* zout) {
+        if (wave)
+            hipLaunchKernelGGL((k_vc_sweep_w<T>), gw, dim3(kWG), 0, h->stream, m, N, sm, r, zin,
+                               zout, h->st);
+        else
+            hipLaunchKernelGGL((k_vc_sweep<P, false, T>), g, dim3(kWG), 0, h->stream, m, N, sm,
+                               r, zin, zout, h->st, h->red);
+    };
     for (int s = 1; s < A.nu; ++s) {
-        hipLaunchKernelGGL((k_vc_sweep<P, false, T>), g, dim3(kWG), 0, h->stream, m, N, sm, r, za,
-                           zb, h->st, h->red);
+        sweep(za, zb);
         std::swap(za, zb);
     }
-    hipLaunchKernelGGL((k_vc_resid<P, T>), g, dim3(kWG), 0, h->stream, m, N, r, za, t, h->st);
+    if (wave)
+        hipLaunchKernelGGL((k_vc_resid_w<T>), gw, dim3(kWG), 0, h->stream, m, N, r, za, t, h->st);
+    else
+        hipLaunchKernelGGL((k_vc_resid<P, T>), g, dim3(kWG), 0, h->stream, m, N, r, za, t, h->st);
     AmgLevel& C = A.lv[l + 1];
     T* zc0 = LevelBufs<T>::vecs(C);
     T* rc = zc0 + 3 * (size_t)C.n;
@@ -1076,8 +1147,7 @@ const T* vcycle(gis_context* h, int l, const T* r, double* out, const double* r6
                                    r64, za, out, h->st, h->red);
             return za;
         }
-        hipLaunchKernelGGL((k_vc_sweep<P, false, T>), g, dim3(kWG), 0, h->stream, m, N, sm, r, za,
-                           zb, h->st, h->red);
+        sweep(za, zb);
         std::swap(za, zb);
     }
     return za;

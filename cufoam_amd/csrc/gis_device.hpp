@@ -474,15 +474,16 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename T, class XOp>
+// B = slots per batch (all of a batch's loads are issued before its gathers).
+template <typename T, int B = 8, class XOp>
 __device__ __forceinline__ T sell_row(const int* __restrict__ scol, const T* __restrict__ sval,
                                       int a, int len, int lane, XOp&& xop) {
     T acc = T(0);
-    for (int j = 0; j < len; j += 8) {
-        int cc[8];
-        T v[8];
+    for (int j = 0; j < len; j += B) {
+        int cc[B];
+        T v[B];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < B; ++u) {
             const int idx = a + (j + u) * kSlice + lane;
             if (j + u < len) {
                 cc[u] = __builtin_nontemporal_load(scol + idx);
@@ -492,11 +493,11 @@ __device__ __forceinline__ T sell_row(const int* __restrict__ scol, const T* __r
                 v[u] = T(0);
             }
         }
-        T xv[8];
+        T xv[B];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) xv[u] = cc[u] >= 0 ? xop(cc[u]) : T(0);
+        for (int u = 0; u < B; ++u) xv[u] = cc[u] >= 0 ? xop(cc[u]) : T(0);
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < B; ++u)
             if (cc[u] >= 0) acc = acc + v[u] * xv[u];
     }
     return acc;
