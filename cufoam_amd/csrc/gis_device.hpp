@@ -747,6 +747,21 @@ __device__ __forceinline__ void sell_row2_d(const int* __restrict__ offs0, unsig
 // tile over SellD slices two at a time; a slice without a table (ooff step
 // 0: its rows do not ascend, or their offsets do not fit 32 slots) reads the
 // int32 slot columns instead.
+// Row mask of a table slice: one byte per row where the table has at most 8
+// entries (3 bytes a row less to stream), the 32-bit mask otherwise.  U is
+// uniform over the wave, so only one of the two arrays is read.
+__device__ __forceinline__ unsigned row_mask(const unsigned* __restrict__ dmask,
+                                             const unsigned char* __restrict__ dmask8, int U,
+                                             int row, int N) {
+    unsigned m = 0u;
+    if (U <= 8) {
+        if (row < N) m = dmask8[row];
+    } else {
+        if (row < N) m = dmask[row];
+    }
+    return m;
+}
+
 template <typename T, class XOp>
 __device__ __forceinline__ void tile_sell_d(const int* __restrict__ soff,
                                             const int* __restrict__ scol,
@@ -754,6 +769,7 @@ __device__ __forceinline__ void tile_sell_d(const int* __restrict__ soff,
                                             const int* __restrict__ ooff,
                                             const int* __restrict__ doffs,
                                             const unsigned* __restrict__ dmask,
+                                            const unsigned char* __restrict__ dmask8,
                                             const T* __restrict__ dval, int N, int row0,
                                             XOp&& xop, T* __restrict__ sh_y) {
     const int lane = threadIdx.x & 63;
@@ -777,8 +793,8 @@ __device__ __forceinline__ void tile_sell_d(const int* __restrict__ soff,
         // rows past N have an empty mask; their (unused) gathers read row N-1
         const int q0 = min(r0, N - 1), q1 = min(r1, N - 1);
         if (u0 > 0 && u1 > 0) {
-            const unsigned m0 = r0 < N ? dmask[r0] : 0u;
-            const unsigned m1 = r1 < N ? dmask[r1] : 0u;
+            const unsigned m0 = row_mask(dmask, dmask8, u0, r0, N);
+            const unsigned m1 = row_mask(dmask, dmask8, u1, r1, N);
             sell_row2_d<T>(doffs + o0, m0, o0 * kSlice, u0, r0, q0, doffs + o1, m1, o1 * kSlice,
                            u1, r1, q1, dval, lane, xop, y0, y1);
         } else {
@@ -789,7 +805,7 @@ __device__ __forceinline__ void tile_sell_d(const int* __restrict__ soff,
                 const int up = p ? u1 : u0, op = p ? o1 : o0, rp = p ? r1 : r0;
                 T y;
                 if (up > 0) {
-                    const unsigned mp = rp < N ? dmask[rp] : 0u;
+                    const unsigned mp = row_mask(dmask, dmask8, up, rp, N);
                     y = sell_row_d<T>(doffs + op, mp, dval, op * kSlice, up, lane, rp,
                                       p ? q1 : q0, xop);
                 } else {

@@ -173,7 +173,7 @@ struct gis_context {
     // rows (doffs from ooff[s]; ooff[s+1] == ooff[s]: no table, int32 columns),
     // a bit mask per row of the offsets it holds, the slot -> value code map
     // and the slot values (64 slots per table entry)
-    gis::DevMem ooff, doffs, dmask, dperm;
+    gis::DevMem ooff, doffs, dmask, dmask8, dperm;
     long long n_dslots = 0;
     gis::DevMem dval, dvalT, dval32, dval32T;
     gis::DevMem dinv;                           // per-row helper (1/a_ii or sqrt(1/a_ii))

@@ -24,6 +24,7 @@ struct Mat {
     const int* doffs;
     const unsigned* dmask;
     const T* dval;
+    const unsigned char* dmask8;  // row masks of slices with at most 8 table entries
 };
 
 // ---------------------------------------------------------------------------
@@ -58,7 +59,8 @@ struct SellD {
     template <typename T, class XOp>
     __device__ static void run(const Mat<T>& m, int N, int row0, int, XOp&& xop, Shared<T>&,
                                T* y) {
-        tile_sell_d<T>(m.soff, m.scol, m.sval, m.ooff, m.doffs, m.dmask, m.dval, N, row0, xop, y);
+        tile_sell_d<T>(m.soff, m.scol, m.sval, m.ooff, m.doffs, m.dmask, m.dmask8, m.dval, N, row0,
+                       xop, y);
     }
 };
 
@@ -312,7 +314,7 @@ inline Mat<double> mat64(const gis_context* h) {
                        h->sres.as<signed char>(), h->sbase.as<int>(),
                        h->sflag.as<unsigned char>(),
                        h->ooff.as<int>(), h->doffs.as<int>(), h->dmask.as<unsigned>(),
-                       h->dval.as<double>()};
+                       h->dval.as<double>(), h->dmask8.as<unsigned char>()};
 }
 
 inline Mat<double> mat64T(const gis_context* h) {
@@ -321,7 +323,7 @@ inline Mat<double> mat64T(const gis_context* h) {
                        h->sres.as<signed char>(), h->sbase.as<int>(),
                        h->sflag.as<unsigned char>(),
                        h->ooff.as<int>(), h->doffs.as<int>(), h->dmask.as<unsigned>(),
-                       h->dvalT.as<double>()};
+                       h->dvalT.as<double>(), h->dmask8.as<unsigned char>()};
 }
 
 inline Mat<float> mat32T(const gis_context* h) {
@@ -330,7 +332,7 @@ inline Mat<float> mat32T(const gis_context* h) {
                       h->sres.as<signed char>(), h->sbase.as<int>(),
                       h->sflag.as<unsigned char>(),
                       h->ooff.as<int>(), h->doffs.as<int>(), h->dmask.as<unsigned>(),
-                      h->dval32T.as<float>()};
+                      h->dval32T.as<float>(), h->dmask8.as<unsigned char>()};
 }
 
 inline Mat<float> mat32(const gis_context* h) {
@@ -339,7 +341,7 @@ inline Mat<float> mat32(const gis_context* h) {
                       h->sres.as<signed char>(), h->sbase.as<int>(),
                       h->sflag.as<unsigned char>(),
                       h->ooff.as<int>(), h->doffs.as<int>(), h->dmask.as<unsigned>(),
-                      h->dval32.as<float>()};
+                      h->dval32.as<float>(), h->dmask8.as<unsigned char>()};
 }
 
 // Iterations the host may launch: the cap, or min_steps when that is larger

@@ -169,7 +169,8 @@ __global__ __launch_bounds__(kBlk) void k_sort_lower(int N, const int* __restric
 __global__ __launch_bounds__(kBlk) void k_selld_union(int N, int S,
                                                       const int* __restrict__ rowptr,
                                                       const int* __restrict__ col, int* ucount,
-                                                      int* tmp_offs, unsigned* dmask) {
+                                                      int* tmp_offs, unsigned* dmask,
+                                                      unsigned char* dmask8) {
     const int lane = threadIdx.x & 63;
     const int s = blockIdx.x * (kBlk / 64) + (threadIdx.x >> 6);
     if (s >= S) return;   // whole waves
@@ -205,7 +206,10 @@ __global__ __launch_bounds__(kBlk) void k_selld_union(int N, int S,
         ++U;
     }
     if (fit && U * 64 * 4 > 5 * total) fit = false;
-    if (valid) dmask[row] = fit ? mask : 0u;
+    if (valid) {
+        dmask[row] = fit ? mask : 0u;
+        dmask8[row] = (unsigned char)(fit && U <= 8 ? mask : 0u);
+    }
     if (lane == 0) ucount[s] = fit ? U : 0;
 }
 
@@ -501,11 +505,13 @@ int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua
         GIS_HIP(h, tmp_offs.ensure(sizeof(int) * 32 * (size_t)S));
         GIS_HIP(h, h->ooff.ensure(sizeof(int) * ((size_t)S + 1)));
         GIS_HIP(h, h->dmask.ensure(sizeof(unsigned) * (size_t)N));
+        GIS_HIP(h, h->dmask8.ensure((size_t)N));
         GIS_HIP(h, hipMemsetAsync(ucount.p, 0, sizeof(int) * ((size_t)S + 1), h->stream));
         const dim3 gs((unsigned)((S + kBlk / 64 - 1) / (kBlk / 64)));
         hipLaunchKernelGGL(k_selld_union, gs, dim3(kBlk), 0, h->stream, N, S,
                            h->rowptr.as<int>(), h->col.as<int>(), ucount.as<int>(),
-                           tmp_offs.as<int>(), h->dmask.as<unsigned>());
+                           tmp_offs.as<int>(), h->dmask.as<unsigned>(),
+                           h->dmask8.as<unsigned char>());
         GIS_HIP(h, hipGetLastError());
         GIS_TRY(scan_exclusive(h, ucount.as<int>(), h->ooff.as<int>(), S + 1, tmp, scan_tmp));
         int n_tab = 0;
