@@ -414,7 +414,6 @@ __global__ __launch_bounds__(kWG) void k_vc_resid(Mat<T> m, int N, const T* __re
 // the same order as every other schedule: the results are bit-identical.
 constexpr int kCoarseBatch = 32;
 constexpr int kSlicesPerWG = kWG / kSlice;
-constexpr int kCoarseRows = INT_MAX;   // levels of at most this many rows use them
 
 __device__ __forceinline__ int wave_slice() {
     return (int)blockIdx.x * kSlicesPerWG + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1098,7 +1097,7 @@ const T* vcycle(gis_context* h, int l, const T* r, double* out, const double* r6
         return dst;
     }
     // non-reducing sweeps and the residual: per-slice waves below the finest level
-    const bool wave = l > 0 && N <= kCoarseRows;
+    const bool wave = l > 0;
     const dim3 gw((unsigned)((N + kSlicesPerWG * kSlice - 1) / (kSlicesPerWG * kSlice)));
     auto sweep = [&](const T* zin, T* zout) {
         if (wave)
