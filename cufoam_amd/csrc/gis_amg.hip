@@ -490,9 +490,23 @@ __global__ __launch_bounds__(kWG) void k_vc_prolong(int N, T cscale, const int* 
     StreamRows t(N);
     T zv[4];
     load4(z, t.i0, N, zv);
+    int a[4];
+    if (t.i0 + 3 < N) {   // the row's 4 aggregates in one 16-byte load
+        const int4 q = *reinterpret_cast<const int4*>(agg + t.i0);
+        a[0] = q.x;
+        a[1] = q.y;
+        a[2] = q.z;
+        a[3] = q.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a[k] = t.i0 + k < N ? agg[t.i0 + k] : 0;
+    }
+    T c[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c[k] = zc[a[k]];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        if (t.i0 + k < N) zv[k] = fmaT(cscale, zc[agg[t.i0 + k]], zv[k]);
+        if (t.i0 + k < N) zv[k] = fmaT(cscale, c[k], zv[k]);
     store4(z, t.i0, N, zv);
 }
 
