@@ -532,10 +532,27 @@ struct TailLevel {
 constexpr int kTail = 256;        // one workgroup runs a level in ~10 us per step: only tiny levels
 constexpr int kTailThreads = 1024;
 
+// Row i of a tail level: a batch's column and value loads, then its gathers,
+// are in flight together; the products are summed one by one in storage order.
 template <typename T>
 __device__ __forceinline__ T csr_row(const TailLevel<T>& L, int i, const T* x) {
+    const int a = L.rowptr[i], b = L.rowptr[i + 1];
     T s = T(0);
-    for (int j = L.rowptr[i]; j < L.rowptr[i + 1]; ++j) s = s + L.val[j] * x[L.col[j]];
+    for (int j = a; j < b; j += 8) {
+        int c[8];
+        T v[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = min(j + u, b - 1);
+            c[u] = L.col[q];
+            v[u] = L.val[q];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (j + u < b) s = s + v[u] * xv[u];
+    }
     return s;
 }
 
@@ -566,8 +583,16 @@ __global__ __launch_bounds__(kTailThreads) void k_vc_tail(const TailLevel<T>* __
         __syncthreads();
         const TailLevel<T>& C = lv[l + 1];
         for (int c = tid; c < C.n; c += nt) {
+            const int p0 = L.mptr[c], p1 = L.mptr[c + 1];
             T s = T(0);
-            for (int p = L.mptr[c]; p < L.mptr[c + 1]; ++p) s = s + L.t[L.mlist[p]];
+            for (int p = p0; p < p1; p += 4) {
+                T v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = L.t[L.mlist[min(p + u, p1 - 1)]];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (p + u < p1) s = s + v[u];
+            }
             C.r[c] = s;
             C.z[c] = C.sm[c] * s;
         }
