@@ -405,13 +405,15 @@ __global__ __launch_bounds__(kWG) void k_vc_resid(Mat<T> m, int N, const T* __re
     store4nt(tr, t.i0, N, tv);
 }
 
-// Coarse levels (below the finest): aggregation leaves no slice whose column
-// offsets fit a table, and a level of a few thousand to a few hundred thousand
-// rows fills only part of the chip, so the per-tile kernels above wait on
-// memory latency: each wave walks 4 slices one after another, 8 slots at a
-// time.  Here each wave takes one 64-row slice (one row per lane), with
-// kCoarseBatch slots' loads in flight before the gathers.  Same products in
-// the same order as every other schedule: the results are bit-identical.
+// Coarse levels (below the finest): aggregation scatters a row's columns over
+// thousands of rows, so offset tables rarely fit and these kernels read the
+// int32-column slices, which every slice has.  A level of a few thousand to a
+// few hundred thousand rows fills only part of the chip, so the per-tile
+// kernels above wait on memory latency: each wave walks 4 slices one after
+// another, 8 slots at a time.  Here each wave takes one 64-row slice (one row
+// per lane), with kCoarseBatch slots' loads in flight before the gathers.
+// Same products in the same order as every other schedule: the results are
+// bit-identical.
 constexpr int kCoarseBatch = 32;
 constexpr int kSlicesPerWG = kWG / kSlice;
 
