@@ -23,17 +23,6 @@ int hip_fail(gis_context* h, hipError_t e, const char* what) {
 
 namespace {
 
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        if (prev != dev) (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard() {
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-};
-
 int upload(gis_context* h, DevMem& dst, const void* src, size_t bytes) {
     GIS_HIP(h, dst.ensure(bytes));
     if (bytes) GIS_HIP(h, hipMemcpyAsync(dst.p, src, bytes, hipMemcpyHostToDevice, h->stream));
@@ -141,6 +130,7 @@ GIS_API int gis_create(gis_handle* out, int device) {
     }
     h->own_stream = true;
     if (ensure_state(h) != kOK) {
+        if (h->st_host) (void)hipHostFree(h->st_host);
         (void)hipStreamDestroy(h->stream);
         delete h;
         return GIS_ERR_HIP;

@@ -216,6 +216,18 @@ enum : int {
     kNaN = 12,
 };
 
+// makes `dev` the current device for the scope
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
 int fail(gis_context* h, int code, const std::string& msg);
 int hip_fail(gis_context* h, hipError_t e, const char* what);
 
@@ -240,7 +252,11 @@ int timer_collect(gis_context* h);            // after a host sync: accumulate t
 void timer_release(gis_context* h);
 
 // reductions / state
+// workspace layout for `rows` rows at base (nullptr: sizes only); false when
+// the tree would need more than kMaxLevels levels
+bool reduction_layout(int rows, double* base, RedWs* out, size_t* n_doubles, size_t* n_slots);
 int ensure_reduction(gis_context* h, int rows);
+int reset_state(gis_context* h, KState* st);   // fresh state for dot / NormFactor
 int fill_slots(gis_context* h, double* slot, size_t n);   // every slot to the empty pattern
 int ensure_state(gis_context* h);
 int read_state(gis_context* h);
@@ -258,6 +274,9 @@ int launch_spmv_t(gis_context* h, const double* x, double* y);
 int launch_spmv32(gis_context* h, const float* x, float* y);
 int device_dot(gis_context* h, const double* a, const double* b, double* out);
 int device_norm_factor(gis_context* h, const double* d_b, const double* d_x, double* nf);
+// the two NormFactor launches on a given workspace and state (results in st->aux[1])
+int launch_norm_factor(gis_context* h, const RedWs& red, KState* st, const double* d_b,
+                       const double* d_x);
 
 struct SolveArgs {
     int method;

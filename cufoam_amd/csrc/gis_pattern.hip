@@ -300,8 +300,11 @@ __global__ __launch_bounds__(kBlk) void k_gather(long long nnz, int N, int M,
 }
 
 // sliced layout: slot count of each 64-row slice = 64 * (longest row)
+// Slots of each 64-row slice (64 x its longest row) and their 64-bit total: a
+// skewed pattern (a few long rows spread over many slices) can pad past 2^31
+// slots although nnz < 2^31; the host rejects it before the int32 scan.
 __global__ __launch_bounds__(kBlk) void k_slice_len(int N, int S, const int* __restrict__ rowptr,
-                                                    int* slots) {
+                                                    int* slots, unsigned long long* total) {
     const int s = blockIdx.x * kBlk + threadIdx.x;
     if (s > S) return;
     if (s == S) {
@@ -311,7 +314,9 @@ __global__ __launch_bounds__(kBlk) void k_slice_len(int N, int S, const int* __r
     int len = 0;
     const int r1 = min(N, (s + 1) * kSlice);
     for (int r = s * kSlice; r < r1; ++r) len = max(len, rowptr[r + 1] - rowptr[r]);
-    slots[s] = len * kSlice;
+    const long long n = (long long)len * kSlice;
+    slots[s] = (int)min(n, (long long)INT_MAX);
+    atomicAdd(total, (unsigned long long)n);
 }
 
 __global__ __launch_bounds__(kBlk) void k_sell_fill(int N, const int* __restrict__ rowptr,
@@ -475,9 +480,21 @@ int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua
     const int S = (N + kSlice - 1) / kSlice;
     GIS_HIP(h, h->soff.ensure(sizeof(int) * ((size_t)S + 1)));
     int* slots = rowlen;   // reuse: S + 1 <= N + 1
+    // own_cnt and first_own (adjacent) are free once the rows are filled: the
+    // first 8 bytes hold the total
+    unsigned long long* slot_total = reinterpret_cast<unsigned long long*>(own_cnt);
+    GIS_HIP(h, hipMemsetAsync(slot_total, 0, sizeof(unsigned long long), h->stream));
     hipLaunchKernelGGL(k_slice_len, grid_for((long long)S + 1), dim3(kBlk), 0, h->stream, N, S,
-                       h->rowptr.as<int>(), slots);
+                       h->rowptr.as<int>(), slots, slot_total);
     GIS_HIP(h, hipGetLastError());
+    unsigned long long total_slots = 0;
+    GIS_HIP(h, hipMemcpyAsync(&total_slots, slot_total, sizeof(total_slots),
+                              hipMemcpyDeviceToHost, h->stream));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    if (total_slots >= (1ULL << 31))
+        return fail(h, kErrShape,
+                    "sliced layout needs 2^31 or more slots (64 x the longest row of each "
+                    "64-row slice): pattern too skewed");
     GIS_TRY(scan_exclusive(h, slots, h->soff.as<int>(), S + 1, tmp, scan_tmp));
     int n_slots = 0;
     GIS_HIP(h, hipMemcpyAsync(&n_slots, h->soff.as<int>() + S, sizeof(int),

@@ -250,11 +250,10 @@ int fill_slots(gis_context* h, double* slot, size_t n) {
 }
 
 // The level counts must describe exactly `rows` (the collectors wait for
-// count[L] entries); memory only grows.  The slots are refilled whenever the
-// shape changes.
-
-int ensure_reduction(gis_context* h, int rows) {
-    if (h->red_rows == rows && h->red_mem.p) return kOK;
+// count[L] entries).  Level-0 and level-1 partials travel through slots;
+// levels 2 and up (the top collector's global path) through plain partial
+// arrays.  With base == nullptr only the size is computed.
+bool reduction_layout(int rows, double* base, RedWs* out, size_t* n_doubles, size_t* n_slots) {
     RedWs r{};
     int count = tiles_of(rows);
     int L = 0;
@@ -263,16 +262,15 @@ int ensure_reduction(gis_context* h, int rows) {
         if (count == 1) break;
         count = (count + kGroup - 1) / kGroup;
     }
-    if (L == kMaxLevels) return fail(h, kErrShape, "reduction depth exceeded");
+    if (L == kMaxLevels) return false;
     r.nlevels = L + 1;
     for (int j = L + 1; j < kMaxLevels; ++j) r.count[j] = 1;
-    // level-0 and level-1 partials travel through slots; levels 2 and up (the
-    // top collector's global path) through plain partial arrays
     size_t dbl = 0;
     for (int j = 2; j <= L; ++j) dbl += (size_t)r.count[j] * kMaxNV;
     const size_t nslot = ((size_t)r.count[0] + (size_t)r.count[1]) * kMaxNV;
-    GIS_HIP(h, h->red_mem.ensure((dbl + nslot) * sizeof(double) + 256));
-    double* base = h->red_mem.as<double>();
+    *n_doubles = dbl + nslot;
+    *n_slots = nslot;
+    if (!base) return true;
     r.slot = base;
     r.slot1 = base + (size_t)r.count[0] * kMaxNV;
     double* pp = base + nslot;
@@ -283,9 +281,38 @@ int ensure_reduction(gis_context* h, int rows) {
             pp += (size_t)r.count[j] * kMaxNV;
         }
     }
+    *out = r;
+    return true;
+}
+
+// Memory only grows; the slots are refilled whenever the shape changes.
+int ensure_reduction(gis_context* h, int rows) {
+    if (h->red_rows == rows && h->red_mem.p) return kOK;
+    RedWs r{};
+    size_t ndbl = 0, nslot = 0;
+    if (!reduction_layout(rows, nullptr, &r, &ndbl, &nslot))
+        return fail(h, kErrShape, "reduction depth exceeded");
+    GIS_HIP(h, h->red_mem.ensure(ndbl * sizeof(double) + 256));
+    reduction_layout(rows, h->red_mem.as<double>(), &r, &ndbl, &nslot);
     GIS_TRY(fill_slots(h, r.slot, nslot));
     h->red = r;
     h->red_rows = rows;
+    return kOK;
+}
+
+// A fresh state for the one-shot reductions (dot, NormFactor): status GIS_OK,
+// the handle's failure-path poll limit.
+__global__ void k_state_reset(KState* st, int slot_fault) {
+    if (threadIdx.x == 0) {
+        KState s{};
+        s.slot_fault = slot_fault;
+        *st = s;
+    }
+}
+
+int reset_state(gis_context* h, KState* st) {
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, h->stream, st, h->slot_fault);
+    GIS_HIP(h, hipGetLastError());
     return kOK;
 }
 
@@ -398,11 +425,28 @@ int launch_spmv32(gis_context* h, const float* x, float* y) {
 int device_dot(gis_context* h, const double* a, const double* b, double* out) {
     GIS_TRY(ensure_reduction(h, h->N));
     GIS_TRY(ensure_state(h));
+    GIS_TRY(reset_state(h, h->st));
     hipLaunchKernelGGL(k_dot, tiles_grid(h->N), dim3(kWG), 0, h->stream, h->N, a, b, h->st,
                        h->red, 7);
     GIS_HIP(h, hipGetLastError());
     GIS_TRY(read_state(h));
+    if (h->st_host->status == kErrSync) return fail(h, kErrSync, "dot: reduction did not complete");
     *out = h->st_host->aux[7];
+    return kOK;
+}
+
+// x_bar into st->aux[0], then NormFactor into st->aux[1]; the workspace and
+// the state are the caller's (the handle's, or the allocation-free scratch).
+int launch_norm_factor(gis_context* h, const RedWs& red, KState* st, const double* d_b,
+                       const double* d_x) {
+    GIS_TRY(reset_state(h, st));
+    hipLaunchKernelGGL(k_mean, tiles_grid(h->N), dim3(kWG), 0, h->stream, h->N, d_x, st, red);
+    GIS_HIP(h, hipGetLastError());
+    const Mat<double> m = mat64(h);
+    GIS_SPMV_DISPATCH(h->sched.spmv64, P,
+                      hipLaunchKernelGGL((k_norm_factor<P>), tiles_grid(h->N), dim3(kWG), 0,
+                                         h->stream, m, h->N, d_b, d_x, st, red));
+    GIS_HIP(h, hipGetLastError());
     return kOK;
 }
 
@@ -410,15 +454,10 @@ int device_norm_factor(gis_context* h, const double* d_b, const double* d_x, dou
     if (!h->has_values) return fail(h, kErrState, "values not set");
     GIS_TRY(ensure_reduction(h, h->N));
     GIS_TRY(ensure_state(h));
-    hipLaunchKernelGGL(k_mean, tiles_grid(h->N), dim3(kWG), 0, h->stream, h->N, d_x, h->st,
-                       h->red);
-    GIS_HIP(h, hipGetLastError());
-    const Mat<double> m = mat64(h);
-    GIS_SPMV_DISPATCH(h->sched.spmv64, P,
-                      hipLaunchKernelGGL((k_norm_factor<P>), tiles_grid(h->N), dim3(kWG), 0,
-                                         h->stream, m, h->N, d_b, d_x, h->st, h->red));
-    GIS_HIP(h, hipGetLastError());
+    GIS_TRY(launch_norm_factor(h, h->red, h->st, d_b, d_x));
     GIS_TRY(read_state(h));
+    if (h->st_host->status == kErrSync)
+        return fail(h, kErrSync, "NormFactor: reduction did not complete");
     *nf = h->st_host->aux[1];
     return kOK;
 }

@@ -131,8 +131,9 @@ class cuFoamSolver:
         self._pattern_key = None
 
     def _upload(self):
-        key = (id(self.matrix.lowerAddr), self.matrix.size(),
-               None if self._cpl is None else id(self._cpl[0]))
+        # a reused address with another mesh behind it differs in its sizes
+        key = (id(self.matrix.lowerAddr), self.matrix.size(), len(self.matrix.lowerAddr),
+               len(self.matrix.upperAddr), None if self._cpl is None else id(self._cpl[0]))
         if key != self._pattern_key:
             self._gis.set_pattern(self.matrix.size(), self.matrix.lowerAddr,
                                   self.matrix.upperAddr)
@@ -148,6 +149,8 @@ class cuFoamSolver:
     def solve(self, psi: np.ndarray, source: np.ndarray, cmpt: int = 0) -> Performance:
         if psi.dtype != np.float64 or not psi.flags.c_contiguous:
             raise FatalError("psi must be a contiguous float64 field")
+        if psi.size != self.matrix.size() or np.size(source) != self.matrix.size():
+            raise FatalError(f"psi and source must have {self.matrix.size()} cells")
         self._upload()
         src = np.ascontiguousarray(source, dtype=np.float64)
         if self.typeName == "cuFoamPCG":

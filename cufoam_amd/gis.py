@@ -181,6 +181,7 @@ class Gis:
             raise GisError(st, "gis_create failed (is a GPU visible?)")
         self.device = device
         self.n = 0
+        self.n_faces = 0
         self.nnz = 0
         self.use_torch_stream()
 
@@ -213,6 +214,81 @@ class Gis:
     def __exit__(self, *exc):
         self.close()
 
+    # -- argument checks (every vector handed to the C ABI) ----------------------
+    def _dev_vec(self, a, count: int, what: str, itemsize: int = 8):
+        """A device tensor the kernels may read or write as `count` contiguous
+        float64 (itemsize 8) or float32 (itemsize 4) entries on this handle's GPU."""
+        import torch
+        want = torch.float64 if itemsize == 8 else torch.float32
+        if a.dtype != want:
+            raise GisError(ERR_ARG, f"{what}: needs {want}, got {a.dtype}")
+        if not a.is_contiguous():
+            raise GisError(ERR_ARG, f"{what}: tensor is not contiguous")
+        if a.numel() != count:
+            raise GisError(ERR_ARG, f"{what}: needs {count} entries, got {a.numel()}")
+        if a.device.index is not None and a.device.index != self.device:
+            raise GisError(ERR_ARG, f"{what}: tensor is on {a.device}, handle on cuda:{self.device}")
+        return a
+
+    def _host_vec(self, a, count: int, what: str, copy: bool = False, dtype=np.float64):
+        """A host array of `count` entries as contiguous `dtype` (a fresh copy when
+        `copy`: the C ABI writes into it)."""
+        if _is_dev(a):
+            raise GisError(ERR_ARG, f"{what}: mixes a device tensor with host arrays")
+        h = np.array(a, dtype=dtype, copy=True) if copy else np.ascontiguousarray(a, dtype=dtype)
+        h = np.ascontiguousarray(h)
+        if h.size != count:
+            raise GisError(ERR_ARG, f"{what}: needs {count} entries, got {h.size}")
+        return h
+
+    def _vecs(self, count: int, what: str, **arrays):
+        """Checks a group of vectors that go to one call: all device tensors (checked
+        in place) or all host arrays (converted); names ending in '_out' are copied."""
+        dev = [_is_dev(a) for a in arrays.values() if a is not None]
+        if any(dev) and not all(dev):
+            raise GisError(ERR_ARG, f"{what}: mixes device tensors and host arrays")
+        out = {}
+        for name, a in arrays.items():
+            if a is None:
+                out[name] = None
+            elif _is_dev(a):
+                out[name] = self._dev_vec(a, count, f"{what} {name}")
+            else:
+                out[name] = self._host_vec(a, count, f"{what} {name}", copy=name.endswith("_out"))
+        return out
+
+    def _xb(self, b, x0, what: str, count: int | None = None, shape=None):
+        """(x, b, on_device) for a solve: device tensors are checked and x0 is
+        updated in place (zeros when None); host arrays are converted and x is a
+        fresh copy of x0."""
+        self._need_pattern(what)
+        count = self.n if count is None else count
+        if _is_dev(b):
+            import torch
+            x = x0 if x0 is not None else torch.zeros_like(b)
+            v = self._vecs(count, what, b=b, x=x)
+            return v["x"], v["b"], True
+        if x0 is None:
+            x0 = np.zeros(count)
+        v = self._vecs(count, what, b=b, x_out=x0)
+        x, bb = v["x_out"], v["b"]
+        if shape is not None:
+            x, bb = x.reshape(shape), bb.reshape(shape)
+        return x, bb, False
+
+    def _nrhs(self, B, what: str) -> int:
+        """Columns of a right-hand-side block of shape (nrhs, N)."""
+        self._need_pattern(what)
+        shape = tuple(B.shape) if hasattr(B, "shape") else np.shape(B)
+        if len(shape) != 2 or shape[1] != self.n or not 1 <= shape[0] <= 8:
+            raise GisError(ERR_ARG, f"{what}: needs shape (nrhs, {self.n}) with 1 <= nrhs <= 8, "
+                                    f"got {shape}")
+        return int(shape[0])
+
+    def _need_pattern(self, what: str):
+        if self.n <= 0:
+            raise GisError(ERR_STATE, f"{what}: set_pattern first (vector length unknown)")
+
     def _check(self, st: int, what: str) -> int:
         if st != OK and (st < BREAKDOWN or st == ERR_SYNC):
             msg = self._L.gis_last_error(self._h)
@@ -233,25 +309,43 @@ class Gis:
 
     # -- matrix --------------------------------------------------------------
     def set_pattern(self, n_cells: int, lower_addr, upper_addr):
+        if _is_dev(lower_addr) != _is_dev(upper_addr):
+            raise GisError(ERR_ARG, "set_pattern: mixes device tensors and host arrays")
         if _is_dev(lower_addr):
-            st = self._L.gis_set_pattern_device(self._h, n_cells, int(lower_addr.numel()),
-                                                _ptr(lower_addr), _ptr(upper_addr))
+            import torch
+            m = int(lower_addr.numel())
+            for name, a in (("lower_addr", lower_addr), ("upper_addr", upper_addr)):
+                if a.dtype != torch.int32 or not a.is_contiguous() or a.numel() != m:
+                    raise GisError(ERR_ARG, f"set_pattern: {name} must be contiguous int32 "
+                                            f"of {m} entries")
+            st = self._L.gis_set_pattern_device(self._h, n_cells, m, _ptr(lower_addr),
+                                                _ptr(upper_addr))
         else:
-            la = _host(lower_addr, np.int32)
-            ua = _host(upper_addr, np.int32)
-            st = self._L.gis_set_pattern(self._h, n_cells, la.shape[0], _ptr(la), _ptr(ua))
+            la = _host(lower_addr, np.int32).ravel()
+            ua = _host(upper_addr, np.int32).ravel()
+            if la.shape != ua.shape:
+                raise GisError(ERR_ARG, "set_pattern: lower_addr and upper_addr differ in length")
+            m = la.shape[0]
+            st = self._L.gis_set_pattern(self._h, n_cells, m, _ptr(la), _ptr(ua))
         self._check(st, "set_pattern")
+        self.n_faces = m
+        self.n_cpl = 0
         n = C.c_int(0)
         nnz = C.c_longlong(0)
         self._check(self._L.gis_get_size(self._h, C.byref(n), C.byref(nnz)), "get_size")
         self.n, self.nnz = n.value, nnz.value
 
     def set_values(self, diag, lower, upper):
-        if _is_dev(diag):
-            st = self._L.gis_set_values_device(self._h, _ptr(diag), _ptr(lower), _ptr(upper))
+        self._need_pattern("set_values")
+        d = self._vecs(self.n, "set_values", diag=diag)["diag"]
+        f = self._vecs(self.n_faces, "set_values", lower=lower, upper=upper)
+        if _is_dev(d) != _is_dev(f["lower"]):
+            raise GisError(ERR_ARG, "set_values: mixes device tensors and host arrays")
+        if _is_dev(d):
+            st = self._L.gis_set_values_device(self._h, _ptr(d), _ptr(f["lower"]),
+                                               _ptr(f["upper"]))
         else:
-            d, lo, up = _host(diag, np.float64), _host(lower, np.float64), _host(upper, np.float64)
-            st = self._L.gis_set_values(self._h, _ptr(d), _ptr(lo), _ptr(up))
+            st = self._L.gis_set_values(self._h, _ptr(d), _ptr(f["lower"]), _ptr(f["upper"]))
         self._check(st, "set_values")
 
     def set_interfaces(self, row, col):
@@ -262,20 +356,24 @@ class Gis:
             raise ValueError("row and col must have the same length")
         self._check(self._L.gis_set_interfaces(self._h, r.shape[0], _ptr(r), _ptr(c)),
                     "set_interfaces")
+        self.n_cpl = int(r.shape[0])
         n = C.c_int(0)
         nnz = C.c_longlong(0)
         self._check(self._L.gis_get_size(self._h, C.byref(n), C.byref(nnz)), "get_size")
         self.n, self.nnz = n.value, nnz.value
 
     def set_coupled_values(self, diag, lower, upper, coeffs):
-        if _is_dev(diag):
-            st = self._L.gis_set_coupled_values_device(self._h, _ptr(diag), _ptr(lower),
-                                                       _ptr(upper), _ptr(coeffs))
-        else:
-            d, lo, up = _host(diag, np.float64), _host(lower, np.float64), _host(upper, np.float64)
-            cv = None if coeffs is None else _host(coeffs, np.float64)
-            st = self._L.gis_set_coupled_values(self._h, _ptr(d), _ptr(lo), _ptr(up),
-                                                None if cv is None else _ptr(cv))
+        self._need_pattern("set_coupled_values")
+        d = self._vecs(self.n, "set_coupled_values", diag=diag)["diag"]
+        f = self._vecs(self.n_faces, "set_coupled_values", lower=lower, upper=upper)
+        cv = self._vecs(getattr(self, "n_cpl", 0), "set_coupled_values", coeffs=coeffs)["coeffs"]
+        devs = {_is_dev(a) for a in (d, f["lower"], cv) if a is not None}
+        if len(devs) > 1:
+            raise GisError(ERR_ARG, "set_coupled_values: mixes device tensors and host arrays")
+        fn = (self._L.gis_set_coupled_values_device if _is_dev(d)
+              else self._L.gis_set_coupled_values)
+        st = fn(self._h, _ptr(d), _ptr(f["lower"]), _ptr(f["upper"]),
+                None if cv is None else _ptr(cv))
         self._check(st, "set_coupled_values")
 
     def set_system(self, sys):
@@ -297,32 +395,44 @@ class Gis:
     # -- kernels on device tensors ---------------------------------------------
     def spmv(self, x, y=None, transpose: bool = False):
         import torch
+        self._need_pattern("spmv")
+        if not _is_dev(x) or (y is not None and not _is_dev(y)):
+            raise GisError(ERR_ARG, "spmv: x and y must be device tensors")
+        self._dev_vec(x, self.n, "spmv x")
         if y is None:
             y = torch.empty_like(x)
+        self._dev_vec(y, self.n, "spmv y")
         f = self._L.gis_spmv_transpose_device if transpose else self._L.gis_spmv_device
         self._check(f(self._h, _ptr(x), _ptr(y)), "spmv")
         return y
 
     def spmv_async(self, x, y):
-        f = self._L.gis_spmv32_device_async if x.dtype.itemsize == 4 else self._L.gis_spmv_device_async
+        self._need_pattern("spmv_async")
+        if not (_is_dev(x) and _is_dev(y)):
+            raise GisError(ERR_ARG, "spmv_async: x and y must be device tensors")
+        size = 4 if x.dtype.itemsize == 4 else 8
+        self._dev_vec(x, self.n, "spmv_async x", size)
+        self._dev_vec(y, self.n, "spmv_async y", size)
+        f = self._L.gis_spmv32_device_async if size == 4 else self._L.gis_spmv_device_async
         self._check(f(self._h, _ptr(x), _ptr(y)), "spmv_async")
 
     def dot(self, a, b) -> float:
         """(a, b) of two device vectors (torch CUDA tensors, float64)."""
+        self._need_pattern("dot")
         if not (_is_dev(a) and _is_dev(b)):
             raise GisError(ERR_ARG, "dot: a and b must be device tensors")
+        self._vecs(self.n, "dot", a=a, b=b)
         out = C.c_double(0.0)
         self._check(self._L.gis_dot_device(self._h, _ptr(a), _ptr(b), C.byref(out)), "dot")
         return out.value
 
     def norm_factor(self, b, x) -> float:
+        """NormFactor(A, b, x) (gis/gis.cu:106-123)."""
+        self._need_pattern("norm_factor")
+        v = self._vecs(self.n, "norm_factor", b=b, x=x)
         out = C.c_double(0.0)
-        if _is_dev(b):
-            st = self._L.gis_norm_factor_device(self._h, _ptr(b), _ptr(x), C.byref(out))
-        else:
-            b, x = _host(b, np.float64), _host(x, np.float64)
-            st = self._L.gis_norm_factor(self._h, _ptr(b), _ptr(x), C.byref(out))
-        self._check(st, "norm_factor")
+        f = self._L.gis_norm_factor_device if _is_dev(b) else self._L.gis_norm_factor
+        self._check(f(self._h, _ptr(v["b"]), _ptr(v["x"]), C.byref(out)), "norm_factor")
         return out.value
 
     SCHEDULE_FIELDS = ("spmv64", "spmv32", "block_iters", "fp64_direction", "fp32_direction",
@@ -390,16 +500,9 @@ class Gis:
         hist = np.zeros(max(hist_cap, 1))
         hl = C.c_int(0)
         pc = PRECONDITIONERS[precond]
-        if _is_dev(b):
-            import torch
-            x = x0 if x0 is not None else torch.zeros_like(b)
-            st = self._L.gis_solve_pcg_device(self._h, pc, _ptr(x), _ptr(b), C.byref(p),
-                                              _ptr(hist), hist_cap, C.byref(hl))
-        else:
-            b = _host(b, np.float64)
-            x = np.array(x0 if x0 is not None else np.zeros_like(b), dtype=np.float64, copy=True)
-            st = self._L.gis_solve_pcg(self._h, pc, _ptr(x), _ptr(b), C.byref(p), _ptr(hist),
-                                       hist_cap, C.byref(hl))
+        x, b, dev = self._xb(b, x0, "solve_pcg")
+        f = self._L.gis_solve_pcg_device if dev else self._L.gis_solve_pcg
+        st = f(self._h, pc, _ptr(x), _ptr(b), C.byref(p), _ptr(hist), hist_cap, C.byref(hl))
         self._check(st, f"solve_pcg({precond})")
         self.last_iterations = max(hl.value - 1, 0)
         return x, hist[:min(hl.value, hist_cap)].copy(), st
@@ -451,16 +554,9 @@ class Gis:
         hist = np.zeros(max(hist_cap, 1))
         hl = C.c_int(0)
         m = METHODS[method]
-        if _is_dev(b):
-            import torch
-            x = x0 if x0 is not None else torch.zeros_like(b)
-            st = self._L.gis_solve_device(self._h, m, _ptr(x), _ptr(b), C.byref(p), _ptr(hist),
-                                          hist_cap, C.byref(hl))
-        else:
-            b = _host(b, np.float64)
-            x = np.array(x0 if x0 is not None else np.zeros_like(b), dtype=np.float64, copy=True)
-            st = self._L.gis_solve(self._h, m, _ptr(x), _ptr(b), C.byref(p), _ptr(hist),
-                                   hist_cap, C.byref(hl))
+        x, b, dev = self._xb(b, x0, f"solve({method})")
+        f = self._L.gis_solve_device if dev else self._L.gis_solve
+        st = f(self._h, m, _ptr(x), _ptr(b), C.byref(p), _ptr(hist), hist_cap, C.byref(hl))
         self._check(st, f"solve({method})")
         self.last_iterations = max(hl.value - 1, 0)
         return x, hist[:min(hl.value, hist_cap)].copy(), st
@@ -471,18 +567,10 @@ class Gis:
         relative residual, status).  Device tensors are updated in place."""
         hist = np.zeros(max(hist_cap, 1))
         hl, it, rr = C.c_int(0), C.c_int(0), C.c_double(0.0)
-        if _is_dev(b):
-            import torch
-            x = x0 if x0 is not None else torch.zeros_like(b)
-            st = self._L.gis_solve_mixed_device(self._h, _ptr(x), _ptr(b), rel_tol,
-                                                inner_rel_tol, max_iter, mode, _ptr(hist),
-                                                hist_cap, C.byref(hl), C.byref(it), C.byref(rr))
-        else:
-            b = _host(b, np.float64)
-            x = np.array(x0 if x0 is not None else np.zeros_like(b), dtype=np.float64, copy=True)
-            st = self._L.gis_solve_mixed(self._h, _ptr(x), _ptr(b), rel_tol, inner_rel_tol,
-                                         max_iter, mode, _ptr(hist), hist_cap, C.byref(hl),
-                                         C.byref(it), C.byref(rr))
+        x, b, dev = self._xb(b, x0, "solve_mixed")
+        f = self._L.gis_solve_mixed_device if dev else self._L.gis_solve_mixed
+        st = f(self._h, _ptr(x), _ptr(b), rel_tol, inner_rel_tol, max_iter, mode, _ptr(hist),
+               hist_cap, C.byref(hl), C.byref(it), C.byref(rr))
         self._check(st, "solve_mixed")
         return x, hist[:min(hl.value, hist_cap)].copy(), it.value, rr.value, st
 
@@ -490,16 +578,9 @@ class Gis:
                     int_bound: float = 1e-10, transpose: bool = False):
         """Several right-hand sides (B: nrhs x N) -> (X, iterations, final norms, status)."""
         p = _Params(error, max_steps, 0.0, 30, int_bound)
-        if _is_dev(B):
-            import torch
-            nrhs = int(B.shape[0])
-            X = X0 if X0 is not None else torch.zeros_like(B)
-            f = self._L.gis_solve_multi_device
-        else:
-            B = _host(B, np.float64)
-            nrhs = B.shape[0]
-            X = np.array(X0 if X0 is not None else np.zeros_like(B), dtype=np.float64, copy=True)
-            f = self._L.gis_solve_multi
+        nrhs = self._nrhs(B, "solve_multi")
+        X, B, dev = self._xb(B, X0, f"solve_multi({method})", nrhs * self.n, (nrhs, self.n))
+        f = self._L.gis_solve_multi_device if dev else self._L.gis_solve_multi
         it = np.zeros(nrhs, np.int32)
         nf = np.zeros(nrhs)
         sts = np.zeros(nrhs, np.int32)
@@ -513,16 +594,10 @@ class Gis:
                           max_inner: int = 0, hist_cap: int = 64):
         """FP32/FP64 refinement of several right-hand sides (B: nrhs x N) ->
         (X, outer residual histories, inner iterations, relative residuals, status)."""
-        if _is_dev(B):
-            import torch
-            nrhs = int(B.shape[0])
-            X = X0 if X0 is not None else torch.zeros_like(B)
-            f = self._L.gis_solve_multi_mixed_device
-        else:
-            B = _host(B, np.float64)
-            nrhs = B.shape[0]
-            X = np.array(X0 if X0 is not None else np.zeros_like(B), dtype=np.float64, copy=True)
-            f = self._L.gis_solve_multi_mixed
+        nrhs = self._nrhs(B, "solve_multi_mixed")
+        X, B, dev = self._xb(B, X0, f"solve_multi_mixed({method})", nrhs * self.n,
+                             (nrhs, self.n))
+        f = self._L.gis_solve_multi_mixed_device if dev else self._L.gis_solve_multi_mixed
         hist = np.zeros((nrhs, max(hist_cap, 1)))
         outer = np.zeros(nrhs, np.int32)
         it = np.zeros(nrhs, np.int32)
@@ -539,13 +614,18 @@ class Gis:
         """Gradients of L_k = W[k] . x through x = A^-1 b -> (lambda = dL/db, d_diag,
         d_lower, d_upper, iterations).  numpy in -> numpy out; device tensors in ->
         device tensors out."""
-        m = (self.nnz - self.n) // 2
+        self._need_pattern("adjoint")
+        m = self.n_faces
         p = _Params(error, max_steps, 0.0, 30, 1e-10)
+        if _is_dev(W) != _is_dev(x):
+            raise GisError(ERR_ARG, "adjoint: mixes device tensors and host arrays")
         if _is_dev(W):
             import torch
             if W.dim() == 1:
                 W = W.unsqueeze(0)
             W = W.contiguous()
+            self._nrhs(W, "adjoint")
+            x = self._dev_vec(x.contiguous(), self.n, "adjoint x")
             k, n = W.shape
             lam = torch.zeros_like(W)
             dd = torch.zeros_like(W)
@@ -557,7 +637,8 @@ class Gis:
             W = _host(W, np.float64)
             if W.ndim == 1:
                 W = W[None, :]
-            x = _host(x, np.float64)
+            self._nrhs(W, "adjoint")
+            x = self._host_vec(x, self.n, "adjoint x")
             k, n = W.shape
             lam = np.zeros((k, n))
             dd = np.zeros((k, n))
@@ -581,16 +662,20 @@ class Gis:
         self._check(self._L.gis_batch_set_pattern(self._h, len(systems), _ptr(n), _ptr(m),
                                                   _ptr(la), _ptr(ua)), "batch_set_pattern")
         self.batch_sizes = n
+        self.batch_faces = m
         self.batch_values(np.concatenate([s.diag for s in systems]),
                           np.concatenate([s.lower for s in systems]),
                           np.concatenate([s.upper for s in systems]))
 
     def batch_values(self, diag, lower, upper):
-        if _is_dev(diag):
-            st = self._L.gis_batch_set_values_device(self._h, _ptr(diag), _ptr(lower), _ptr(upper))
-        else:
-            d, lo, up = _host(diag, np.float64), _host(lower, np.float64), _host(upper, np.float64)
-            st = self._L.gis_batch_set_values(self._h, _ptr(d), _ptr(lo), _ptr(up))
+        if getattr(self, "batch_sizes", None) is None:
+            raise GisError(ERR_STATE, "batch_values: batch_set_systems first")
+        d = self._vecs(int(self.batch_sizes.sum()), "batch_values", diag=diag)["diag"]
+        f = self._vecs(int(self.batch_faces.sum()), "batch_values", lower=lower, upper=upper)
+        if _is_dev(d) != _is_dev(f["lower"]):
+            raise GisError(ERR_ARG, "batch_values: mixes device tensors and host arrays")
+        fn = self._L.gis_batch_set_values_device if _is_dev(d) else self._L.gis_batch_set_values
+        st = fn(self._h, _ptr(d), _ptr(f["lower"]), _ptr(f["upper"]))
         self._check(st, "batch_set_values")
 
     def batch_solve(self, b, x0=None, error=None, max_steps: int = 0, use_graph: bool = True):
@@ -601,13 +686,16 @@ class Gis:
         it = np.zeros(S, np.int32)
         nf = np.zeros(S)
         sts = np.zeros(S, np.int32)
+        total = int(self.batch_sizes.sum())
         if _is_dev(b):
             import torch
             x = x0 if x0 is not None else torch.zeros_like(b)
+            v = self._vecs(total, "batch_solve", b=b, x=x)
+            b, x = v["b"], v["x"]
             f = self._L.gis_batch_solve_device
         else:
-            b = _host(b, np.float64)
-            x = np.array(x0 if x0 is not None else np.zeros_like(b), dtype=np.float64, copy=True)
+            v = self._vecs(total, "batch_solve", b=b, x_out=x0 if x0 is not None else np.zeros(total))
+            b, x = v["b"], v["x_out"]
             f = self._L.gis_batch_solve
         st = f(self._h, _ptr(x), _ptr(b), _ptr(err), max_steps, int(bool(use_graph)), _ptr(it),
                _ptr(nf), _ptr(sts))
@@ -623,6 +711,12 @@ class Gis:
         gis_foam_solve_controls; left at None / 0 the call is gis_foam_solve.
         method "PCG" with a preconditioner calls gis_foam_solve_pcg."""
         perf = _Perf()
+        self._need_pattern("foam_solve")
+        if not (isinstance(psi, np.ndarray) and psi.dtype == np.float64
+                and psi.flags.c_contiguous and psi.size == self.n):
+            raise GisError(ERR_ARG, f"foam_solve: psi must be a contiguous float64 numpy array "
+                                    f"of {self.n} entries (updated in place)")
+        source = self._host_vec(source, self.n, "foam_solve source")
         if method == "PCG":
             st = self._L.gis_foam_solve_pcg(self._h, PRECONDITIONERS[preconditioner or "none"],
                                             _ptr(psi), _ptr(source), tolerance,

@@ -60,6 +60,51 @@ def test_ldu2csr_rejects_unsorted_owner(gpu):
         gpu.set_pattern(3, la, ua)
 
 
+def _hub(n):
+    """Cell 0 coupled to every other cell: row 0 holds n entries, the others 2."""
+    return np.zeros(n - 1, np.int32), np.arange(1, n, dtype=np.int32)
+
+
+def test_hub_pattern_bit_exact(gpu):
+    torch = _torch()
+    n = 5000
+    la, ua = _hub(n)
+    rng = np.random.default_rng(11)
+    upper = -rng.uniform(0.5, 1.0, n - 1)
+    lower = upper * rng.uniform(1.0, 1.5, n - 1)
+    diag = np.full(n, 1.0)
+    diag[0] += np.abs(lower).sum()
+    diag[1:] += np.abs(upper) + 0.1
+    s = mesh.LduSystem(n, la, ua, diag, lower, upper, np.ones(n), np.zeros(n), meta={})
+    print("[case] hub pattern n=5000", flush=True)
+    gpu.set_system(s)
+    A = O.ldu2csr(s)
+    rp, col, val = gpu.get_csr()
+    assert np.array_equal(rp, A.rowptr) and np.array_equal(col, A.col)
+    assert np.array_equal(val, A.val)
+    x = rng.uniform(-1, 1, n)
+    for sched in range(7):
+        gpu.set_schedule(spmv64=sched)
+        y = gpu.spmv(torch.as_tensor(x, device="cuda:0")).cpu().numpy()
+        assert np.array_equal(y, O.spmv(A, x)), sched
+    gpu.set_schedule(spmv64=0)
+
+
+def test_sliced_layout_past_2_31_slots_is_rejected():
+    """A hub row of 2^25 entries pads slice 0 to 2^31 slots (nnz is only 1e8):
+    set_pattern reports GIS_ERR_SHAPE instead of overflowing the slot offsets."""
+    from cufoam_amd.gis import Gis, GisError
+    n = 1 << 25
+    la, ua = _hub(n)
+    print("[case] hub pattern n=2^25 (slot overflow)", flush=True)
+    with Gis(0) as g:
+        with pytest.raises(GisError) as e:
+            g.set_pattern(n, la, ua)
+        assert e.value.status == 3
+        g.set_pattern(3, np.array([0, 1], np.int32), np.array([1, 2], np.int32))
+        assert g.n == 3
+
+
 @pytest.mark.parametrize("n,seed", SMALL + [(64, 5)])
 def test_spmv_dot_bit_exact(gpu, n, seed):
     torch = _torch()

@@ -113,8 +113,8 @@ def test_multi_rhs_global_level2_path():
 def test_sync_failure_is_reported_and_recovers(gpu):
     """The collectors' bounded wait, forced from the host (gis_set_slot_fault:
     every collector sees its slots unwritten and gives up after 4 polls; the
-    kernels still exit normally): single, mixed, multi-RHS and batched solves
-    stop with GIS_ERR_SYNC, and after the hook is cleared the refilled slots give
+    kernels still exit normally): dot, NormFactor, single, mixed, multi-RHS and
+    batched solves stop with GIS_ERR_SYNC, and after the hook is cleared the refilled slots give
     bit-exact results again."""
     from cufoam_amd.gis import GisError
     s = mesh.laplacian(48, seed=26, shift=1e-2)   # 110,592 rows: 108 tiles, 2 groups
@@ -122,9 +122,12 @@ def test_sync_failure_is_reported_and_recovers(gpu):
     gpu.set_system(s)
     B = np.random.default_rng(27).uniform(-1, 1, (2, s.n_cells))
     small = [mesh.laplacian(12, seed=28, shift=1e-3), mesh.laplacian(3, seed=29, shift=1e-3)]
+    import torch
+    b_d = torch.as_tensor(s.b, device="cuda:0")
     gpu.set_slot_fault(4)
     try:
-        for call in (lambda: gpu.solve("CG", s.b, s.x0, error=0.0, max_steps=5),
+        for call in (lambda: gpu.dot(b_d, b_d), lambda: gpu.norm_factor(b_d, b_d),
+                     lambda: gpu.solve("CG", s.b, s.x0, error=0.0, max_steps=5),
                      lambda: gpu.solve_mixed(s.b, s.x0, 1e-10, 1e-2, 5, 0),
                      lambda: gpu.solve_multi("BiCG", B, None, error=0.0, max_steps=5)):
             with pytest.raises(GisError) as e:
@@ -145,6 +148,8 @@ def test_sync_failure_is_reported_and_recovers(gpu):
         np.testing.assert_array_equal(x[off:off + t.n_cells], ref.x)
         off += t.n_cells
     gpu.set_system(s)
+    assert gpu.dot(b_d, b_d) == O.dot(s.b, s.b)
+    assert gpu.norm_factor(b_d, b_d) == O.norm_factor(A, s.b, s.b)
     ref = O.solve("CG", A, s.b, s.x0, 0.0, 5)
     x, hist, st = gpu.solve("CG", s.b, s.x0, error=0.0, max_steps=5)
     np.testing.assert_array_equal(hist, ref.hist)
