@@ -129,7 +129,7 @@ GIS_API int gis_create(gis_handle* out, int device) {
         return GIS_ERR_HIP;
     }
     h->own_stream = true;
-    if (ensure_state(h) != kOK) {
+    if (ensure_state(h) != kOK || noalloc_setup(h) != kOK) {
         if (h->st_host) (void)hipHostFree(h->st_host);
         (void)hipStreamDestroy(h->stream);
         delete h;

@@ -277,6 +277,8 @@ int device_norm_factor(gis_context* h, const double* d_b, const double* d_x, dou
 // the two NormFactor launches on a given workspace and state (results in st->aux[1])
 int launch_norm_factor(gis_context* h, const RedWs& red, KState* st, const double* d_b,
                        const double* d_x);
+// allocation-free entry point (gis_noalloc.hip): per-device set-up, run by gis_create
+int noalloc_setup(gis_context* h);
 
 struct SolveArgs {
     int method;

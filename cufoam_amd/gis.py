@@ -84,6 +84,7 @@ SIGNATURES = {
     "gis_get_timing": (_i, [_vp, _i, _dp, _llp]),
     "gis_norm_factor": (_i, [_vp, _vp, _vp, _dp]),
     "gis_norm_factor_device": (_i, [_vp, _vp, _vp, _dp]),
+    "gis_norm_factor_device_noalloc": (_i, [_vp, _vp, _vp, _dp]),
     "gis_solve": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
     "gis_solve_device": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
     "gis_foam_solve": (_i, [_vp, _i, _vp, _vp, _d, _ll, C.POINTER(_Perf)]),
@@ -433,6 +434,18 @@ class Gis:
         out = C.c_double(0.0)
         f = self._L.gis_norm_factor_device if _is_dev(b) else self._L.gis_norm_factor
         self._check(f(self._h, _ptr(v["b"]), _ptr(v["x"]), C.byref(out)), "norm_factor")
+        return out.value
+
+    def norm_factor_noalloc(self, b, x) -> float:
+        """NormFactor of device tensors through the entry point that allocates no
+        device memory (its workspace is the library's fixed per-GPU array)."""
+        self._need_pattern("norm_factor_noalloc")
+        if not (_is_dev(b) and _is_dev(x)):
+            raise GisError(ERR_ARG, "norm_factor_noalloc: b and x must be device tensors")
+        self._vecs(self.n, "norm_factor_noalloc", b=b, x=x)
+        out = C.c_double(0.0)
+        self._check(self._L.gis_norm_factor_device_noalloc(self._h, _ptr(b), _ptr(x),
+                                                           C.byref(out)), "norm_factor_noalloc")
         return out.value
 
     SCHEDULE_FIELDS = ("spmv64", "spmv32", "block_iters", "fp64_direction", "fp32_direction",

@@ -178,6 +178,20 @@ GIS_API int gis_set_timing_sample(gis_handle h, int period);
 GIS_API int gis_norm_factor(gis_handle h, const double *b, const double *x, double *nf);
 GIS_API int gis_norm_factor_device(gis_handle h, const double *d_b, const double *d_x,
                                    double *nf);
+/* gis_norm_factor_device without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under stream
+ * capture, serving loops with a fixed memory budget, harnesses that forbid
+ * hipMalloc).  The same kernels and the same bits as gis_norm_factor_device, but
+ * the reduction workspace and the device state live in a fixed device array
+ * that the library holds on each GPU until the process ends (about 4.3 MB), and
+ * d_b / d_x are read in place instead of staged.  The handle's pattern and
+ * values must be set (those calls allocate).  At most 33,554,432 rows
+ * (GIS_ERR_SHAPE above that).  All calls on one GPU share its array: the
+ * library serializes them with a per-device lock held until the result has been
+ * read back, so they never overlap on the GPU; calls that did overlap there
+ * would race. */
+GIS_API int gis_norm_factor_device_noalloc(gis_handle h, const double *d_b, const double *d_x,
+                                           double *nf);
 
 /* ---- solve A x = b with one method; x holds x0 on entry (host pointers).
  * hist (optional, hist_cap entries) receives the residual norms

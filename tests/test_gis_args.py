@@ -93,6 +93,7 @@ def test_device_only_calls_reject_host_arrays():
     _raises(ERR_ARG, g.dot, b, b)
     _raises(ERR_ARG, g.spmv, b)
     _raises(ERR_ARG, g.spmv_async, b, b)
+    _raises(ERR_ARG, g.norm_factor_noalloc, b, b)
 
 
 def test_device_tensor_checks_with_cpu_stand_ins(monkeypatch):
@@ -109,6 +110,7 @@ def test_device_tensor_checks_with_cpu_stand_ins(monkeypatch):
     _raises(ERR_ARG, g._dev_vec, torch.ones(54, dtype=torch.float64)[::2], 27, "x")
     _raises(ERR_ARG, g.solve, "CG", good, np.zeros(27))          # device b, host x0
     _raises(ERR_ARG, g.solve, "CG", torch.ones(27, dtype=torch.float32))
+    _raises(ERR_ARG, g.norm_factor_noalloc, good, torch.ones(26, dtype=torch.float64))
 
 
 def test_system_of_the_right_size_passes_the_checks():
