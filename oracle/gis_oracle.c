@@ -70,36 +70,7 @@ enum { OR_OK = 0, OR_ERR_ARG = 1, OR_ERR_NOMEM = 5, OR_BREAKDOWN = 10 };
 /* ------------------------------------------------------------------------ */
 /* canonical reduction: 4-element leaves + pairwise tree                     */
 /* ------------------------------------------------------------------------ */
-#define OR_LEAF 4
-
-typedef struct {
-    double stack[64];
-    int64_t count;
-} tree_acc;
-
-static inline void tree_push(tree_acc *t, double v) {
-    int64_t c = t->count;
-    int lvl = 0;
-    while (c & 1) {
-        v = t->stack[lvl] + v;
-        c >>= 1;
-        ++lvl;
-    }
-    t->stack[lvl] = v;
-    t->count++;
-}
-
-static inline double tree_final(const tree_acc *t) {
-    double acc = 0.0;
-    int have = 0;
-    for (int lvl = 0; lvl < 63; ++lvl) {
-        if ((t->count >> lvl) & 1) {
-            acc = have ? t->stack[lvl] + acc : t->stack[lvl];
-            have = 1;
-        }
-    }
-    return have ? acc : 0.0;
-}
+#include "or_tree.h"
 
 OR_API double or_tree_sum(int64_t n, const double *leaf) {
     tree_acc t = {.count = 0};

@@ -93,6 +93,17 @@ def lib():
                                       C.c_void_p, C.c_void_p, _f64p, _f64p, C.c_double,
                                       C.c_double, C.c_int, C.c_int, _f64p, C.c_int,
                                       C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_cg_par.restype = C.c_int
+        L.or_cg_par.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_int,
+                                _f64p, C.c_int, C.POINTER(C.c_int)]
+        L.or_mixed_cg_par.restype = C.c_int
+        L.or_mixed_cg_par.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double,
+                                      C.c_double, C.c_int, C.c_int, _f64p, C.c_int,
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_residual_norm_par.restype = C.c_double
+        L.or_residual_norm_par.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p]
+        L.or_par_threads.restype = C.c_int
+        L.or_par_threads.argtypes = []
         if hasattr(L, "or_mixed_cg"):
             L.or_mixed_cg.restype = C.c_int
             L.or_mixed_cg.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double,
@@ -502,6 +513,53 @@ def mixed_cg(A: Csr, b, x0, rel_tol: float = 1e-10, inner_rel_tol: float = 1e-4,
                            _c(A.val, np.float64), _c(b, np.float64), x, rel_tol, inner_rel_tol,
                            max_iter, mode, hist, hist_cap, C.byref(hl), C.byref(iters))
     return x, hist[:min(hl.value, hist_cap)].copy(), iters.value, st
+
+
+# ---------------------------------------------------------------------------
+# OpenMP restatements for the headline sizes (gis_oracle_par.c): the same bits
+# as solve("CG", ...) and mixed_cg(...)
+# ---------------------------------------------------------------------------
+def par_threads() -> int:
+    """OpenMP threads the parallel restatements use (OMP_NUM_THREADS)."""
+    return lib().or_par_threads()
+
+
+def cg_par(A: Csr, b, x0, error: float, max_steps: int, hist_cap: int | None = None):
+    """CG (SolverSingleGPU.h:366-445) on all host threads -> SolveResult; bit for
+    bit the result of solve("CG", A, b, x0, error, max_steps)."""
+    x = _c(np.array(x0, dtype=np.float64, copy=True), np.float64)
+    if hist_cap is None:
+        hist_cap = (max_steps if max_steps > 0 else 100000) + 1
+    hist = np.zeros(max(hist_cap, 1))
+    hl = C.c_int(0)
+    st = lib().or_cg_par(A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                         _c(A.val, np.float64), _c(b, np.float64), x, error, max_steps, hist,
+                         hist_cap, C.byref(hl))
+    if st != 0:
+        raise RuntimeError(f"or_cg_par failed with status {st}")
+    return SolveResult(x, hist[:min(hl.value, hist_cap)].copy(), hl.value, 0)
+
+
+def mixed_cg_par(A: Csr, b, x0, rel_tol: float = 1e-10, inner_rel_tol: float = 1e-4,
+                 max_iter: int = 100000, mode: int = 0, hist_cap: int = 256):
+    """mixed_cg on all host threads (same bits) -> (x, hist, inner iterations, status)."""
+    x = _c(np.array(x0, dtype=np.float64, copy=True), np.float64)
+    hist = np.zeros(hist_cap)
+    hl = C.c_int(0)
+    iters = C.c_int(0)
+    st = lib().or_mixed_cg_par(A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                               _c(A.val, np.float64), _c(b, np.float64), x, rel_tol,
+                               inner_rel_tol, max_iter, mode, hist, hist_cap, C.byref(hl),
+                               C.byref(iters))
+    return x, hist[:min(hl.value, hist_cap)].copy(), iters.value, st
+
+
+def residual_norm_par(A: Csr, b, x) -> float:
+    """||b - A x||_2 in FP64 with the oracle's row sums and reduction tree."""
+    r = np.empty(A.n)
+    return lib().or_residual_norm_par(A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                                      _c(A.val, np.float64), _c(b, np.float64),
+                                      _c(x, np.float64), r)
 
 
 # ---------------------------------------------------------------------------
