@@ -8,9 +8,12 @@ matrix and vectors already resident in HBM.  `value` is the mixed-precision
 solve time (FP32 Krylov loop, FP64 residual/correction/norms); the plain FP64
 CG time of the same solve is reported beside it.  The roofline entry is for
 the dominant kernel of the mixed solve (the FP32 SpMV + (p, q) kernel of the
-tuned split schedule, or the fused SpMV + direction + correction kernel), timed
-live with HIP events on its launch stream; `achieved` counts CSR algorithmic
-bytes, and `moved_GBps` the HBM bytes the kernel was measured to move.
+split schedule, or the fused SpMV + direction + correction kernel), timed live
+with HIP events on its launch stream.  `achieved` and `frac` count the bytes the
+kernel must move in the layout it runs on (gis_get_layout: the stored matrix of
+the SellD slices, x read once, q written once); `achieved_csr` / `frac_csr` count
+the CSR bytes of the same SpMV (an int32 column index per nonzero), which SellD
+does not store; `traffic` is the HBM bytes the kernel was measured to move.
 """
 from __future__ import annotations
 
@@ -41,48 +44,61 @@ def spmv_bytes(N, nnz, sv, sx):
 
 
 def mx1_bytes(N, nnz, direction):
-    """Algorithmic bytes of the FP32 SpMV-carrying iteration kernel.  Fused form (k_mx1):
-    FP32 matrix (values + columns + row offsets) and six FP32 vector passes (read r,
-    p_{k-1}, xc; write p_k, q, xc).  Split form (k_mx1s): the matrix, p_k read once and
-    q written, i.e. the bytes of one FP32 SpMV."""
+    """CSR algorithmic bytes of the FP32 SpMV-carrying iteration kernel.  Fused form
+    (k_mx1): FP32 matrix (values + columns + row offsets) and six FP32 vector passes
+    (read r, p_{k-1}, xc; write p_k, q, xc).  Split form (k_mx1s): the matrix, p_k read
+    once and q written, i.e. the bytes of one FP32 SpMV."""
     if direction:
         return spmv_bytes(N, nnz, 4, 4)
     return nnz * (4 + 4) + 4 * (N + 1) + 6 * N * 4
 
 
-def cpu_baseline(sys_, iters_fp64: int, sample_iters: int):
-    """FP64 CG of the same system on the host, timed on bounded samples and extrapolated to
-    the iteration count the FP64 solve needs.  The reported value uses every OpenMP thread
-    (CSR matvec and vector loops parallel); one-thread figures are reported beside it:
-    OpenFOAM's PCG with preconditioner none (LDU face loop, normFactor-scaled L1
-    residuals), the oracle's CG (the GPU path's algorithm, bit-identical iterates) and the
-    same CG with the matrix applied by the LDU face loop."""
+def mx1_layout_bytes(g, N, schedule, direction):
+    """The same kernel's bytes in the layout it runs on: one FP32 SpMV of the stored
+    matrix (Gis.spmv_bytes) plus, for the fused form, the four extra FP32 vector
+    passes (r, p_{k-1} and xc read, xc written)."""
+    b = g.spmv_bytes(4, 4, schedule)
+    return b if direction else b + 4 * N * 4
+
+
+def cpu_baseline(sys_, A, x_gpu_fp64, x_gpu_mixed, iters_fp64: int, sample_iters: int):
+    """FP64 CG of the same system on the host, run in full (not extrapolated): the
+    oracle's restatement of SolverSingleGPU.h:366-445 on all OpenMP threads, with the
+    reduction tree of the GPU path, so its iterates are the GPU FP64 CG's bit for bit.
+    Its solution is the FP64 oracle solve the mixed result is measured against.
+    OpenFOAM's PCG (preconditioner none) restated on all threads runs the same number
+    of iterations beside it; one-thread figures are extrapolated from short samples."""
+    import numpy as np
     from oracle import oracle as O
-    A = O.ldu2csr(sys_)
-    threads = O.baseline_threads()
-    omp_iters = 5 * sample_iters
-    t_omp, _ = O.time_cg_csr_omp(A, sys_.b, omp_iters)
-    t_ldu, _ = O.time_cg_ldu(sys_, sample_iters)
+    threads = O.par_threads()
+    bnorm = float(np.linalg.norm(sys_.b))
     t0 = time.perf_counter()
-    O.solve("CG", A, sys_.b, sys_.x0, 0.0, sample_iters, hist_cap=sample_iters + 1)
-    t_orc = time.perf_counter() - t0
-    _, _, _, t_of = O.pcg_openfoam(sys_, 0.0, sample_iters)
-    per = {"omp": t_omp / omp_iters, "ldu": t_ldu / sample_iters, "oracle": t_orc / sample_iters,
-           "openfoam": t_of / sample_iters}
+    ref = O.cg_par(A, sys_.b, sys_.x0, 1e-10 * bnorm, 0, hist_cap=iters_fp64 + 2)
+    t_full = time.perf_counter() - t0
+    xo = ref.x
+    err_m = float(np.linalg.norm(x_gpu_mixed - xo) / np.linalg.norm(xo))
+    _, it_of, res_of, t_of = O.pcg_openfoam_omp(A, sys_.b, 0.0, iters_fp64)
+    t_ldu, _ = O.time_cg_ldu(sys_, sample_iters)
+    _, _, _, t_of1 = O.pcg_openfoam(sys_, 0.0, sample_iters)
     return {
-        "value": per["omp"] * iters_fp64,
+        "value": round(t_full, 3),
         "unit": "s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{omp_iters} FP64 CG iterations of the same 256^3 system on {threads} OpenMP "
-                  f"threads ({t_omp:.1f} s incl. set-up residual), extrapolated to the "
-                  f"{iters_fp64} iterations the FP64 solve needs",
-        "per_iteration_s": per["omp"],
-        "one_thread": {
-            "openfoam_pcg_none_s": per["openfoam"] * iters_fp64,
-            "oracle_cg_s": per["oracle"] * iters_fp64,
-            "ldu_face_loop_cg_s": per["ldu"] * iters_fp64,
-            "sample": f"{sample_iters} iterations each, extrapolated to {iters_fp64}"},
+        "sample": f"the whole FP64 CG solve of the same 256^3 system to ||r|| <= 1e-10 ||b|| "
+                  f"({ref.iterations} iterations) on {threads} OpenMP threads: the oracle's "
+                  f"restatement with the GPU path's reduction tree",
+        "iterations": ref.iterations,
+        "gpu_fp64_bit_exact": bool(np.array_equal(x_gpu_fp64, xo)),
+        "rel_error_mixed_vs_oracle_fp64": err_m,
+        "rel_residual_oracle": O.residual_norm_par(A, sys_.b, xo) / bnorm,
+        "openfoam_pcg_none_all_threads": {
+            "s": round(t_of, 3), "iterations": it_of, "scaled_l1_residual": res_of,
+            "note": "OpenFOAM's PCG loop restated, row-parallel matvec, same iteration count"},
+        "one_thread_extrapolated": {
+            "openfoam_pcg_none_s": round(t_of1 / sample_iters * iters_fp64, 2),
+            "ldu_face_loop_cg_s": round(t_ldu / sample_iters * iters_fp64, 2),
+            "sample": f"{sample_iters} iterations each, scaled to {iters_fp64}"},
     }
 
 
@@ -226,9 +242,13 @@ def dropin(cpu_sample: int):
     """The OpenFOAM drop-in path at 128^3 (cuFoam<Name>::solve through gis_foam_solve:
     host psi/source in and out, error = tolerance * NormFactor, tolerance 1e-8, x0 = 0):
     CG, Jacobi and Richardson on the SPD system (shift 1e-3), BiCG and MRJacobi on the
-    nonsymmetric one (shift 0.1).  Each solve is timed with and without the per-solve
-    coefficient upload (gis_set_values); the oracle's same method on one host thread is
-    timed on a bounded sample of iterations and extrapolated to the GPU's count."""
+    nonsymmetric one (shift 0.1).  The handle runs the schedule gis_set_pattern picks
+    for the pattern (no tuning call, as in the wrappers).  Each solve is timed with and
+    without the per-solve coefficient upload (gis_set_values).  CPU: OpenFOAM's PCG
+    (for cuFoamCG) and PBiCG (for cuFoamBiCG) with preconditioner none, restated on all
+    host threads and run in full to the same tolerance; for the other wrappers, which
+    have no OpenFOAM counterpart, the oracle's same method on one host thread, timed on
+    a bounded sample of iterations and extrapolated to the GPU's count."""
     import numpy as np
     from cufoam_amd import mesh
     from cufoam_amd.gis import Gis
@@ -248,7 +268,7 @@ def dropin(cpu_sample: int):
         g = Gis(0)
         g.set_pattern(s.n_cells, s.lower_addr, s.upper_addr)
         g.set_values(s.diag, s.lower, s.upper)
-        sched = g.autotune(fp32=False)[0]                # once per mesh, cached
+        sched = g.get_schedule()                         # the pattern's own schedule
         psi = np.zeros(s.n_cells)
         g.foam_solve(name, psi, s.b, 1e-8, max_iter)     # warm-up (code load, workspaces)
         psi = np.zeros(s.n_cells)
@@ -265,7 +285,17 @@ def dropin(cpu_sample: int):
                "converged": bool(perf.final_residual <= 1e-8 * perf.norm_factor),
                "solve_s": round(t2 - t1, 5), "with_set_values_s": round(t2 - t0, 5),
                "us_per_iteration": round((t2 - t1) / max(perf.n_iterations, 1) * 1e6, 2)}
-        if O is not None and cpu_sample > 0:
+        if O is not None and cpu_sample > 0 and name in ("CG", "BiCG"):
+            A = O.ldu2csr(s)
+            f = O.pcg_openfoam_omp if name == "CG" else O.pbicg_openfoam_omp
+            _, it_of, res_of, t_of = f(A, s.b, 1e-8, max_iter)
+            rec["cpu_openfoam_all_threads"] = {
+                "solver": "PCG" if name == "CG" else "PBiCG", "preconditioner": "none",
+                "threads": O.baseline_threads(), "s": round(t_of, 4), "iterations": it_of,
+                "scaled_l1_residual": res_of,
+                "note": "OpenFOAM's loop restated, full run to tolerance 1e-8 on its own "
+                        "residual (sum|r| / normFactor)"}
+        elif O is not None and cpu_sample > 0:
             A = O.ldu2csr(s)
             t = []
             for k in (cpu_sample, 2 * cpu_sample):   # the difference drops set-up work
@@ -507,42 +537,68 @@ def main():
                       "GFLOP/s": round(2 * nnz / (us * 1e-6) / 1e9, 1)}
         log(f"[bench] spmv {name}: {us:.1f} us {gbs:.0f} GB/s")
 
-    # ---- roofline of the dominant kernel (fused FP32 iteration kernel) -----
+    # ---- roofline of the dominant kernel (the FP32 SpMV-carrying kernel) -----
     mx1_avg_s = mx1_ms / max(mx1_n, 1) / 1e3
     split32 = sched["fp32_direction"] == 1
     kname = "k_mx1s" if split32 else "k_mx1"
-    ach = mx1_bytes(N, nnz, split32) / mx1_avg_s / 1e9
+    lay_b = mx1_layout_bytes(g, N, sched["spmv32"], split32)
+    csr_b = mx1_bytes(N, nnz, split32)
+    ach = lay_b / mx1_avg_s / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_poisson256.json")
     if os.path.exists(tf):
         try:
-            traffic = json.load(open(tf)).get(f"{kname}_bytes_per_launch")
+            tj = json.load(open(tf))
+            # counter bytes of the variant that ran (kernel form and SpMV layout)
+            if tj.get(f"{kname}_spmv_schedule") == sched["spmv32"]:
+                traffic = tj.get(f"{kname}_bytes_per_launch")
         except Exception:
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic,
                 "kernel": ("k_mx1s (FP32 SpMV of the direction + (p, q))" if split32 else
                            "k_mx1 (FP32 SpMV + direction + deferred correction)"),
-                "bytes_per_launch": mx1_bytes(N, nnz, split32),
+                "spmv_schedule": sched["spmv32"],
+                "bytes_per_launch": lay_b,
+                "bytes_model": "stored matrix of the running layout (values, offset tables, "
+                               "row masks) + x read once + q written once"
+                               + ("" if split32 else " + r, p_old, xc read + xc written"),
+                "achieved_csr": round(csr_b / mx1_avg_s / 1e9, 1),
+                "frac_csr": round(csr_b / mx1_avg_s / 1e9 / PEAK_HBM_GBS, 4),
+                "bytes_per_launch_csr": csr_b,
                 "avg_launch_us": round(mx1_avg_s * 1e6, 2), "launches_timed": mx1_n,
                 "timed_one_launch_in": TIMING_PERIOD}
     if traffic:
-        # `achieved` counts the CSR algorithmic bytes (an int32 column per
-        # nonzero); the sliced layout with per-slice column-offset tables moves
-        # fewer, so the rate of the bytes actually moved is given beside it
-        roofline["moved_GBps"] = round(traffic / mx1_avg_s / 1e9, 1)
-        roofline["moved_frac"] = round(traffic / mx1_avg_s / 1e9 / PEAK_HBM_GBS, 4)
+        roofline["traffic_GBps"] = round(traffic / mx1_avg_s / 1e9, 1)
+        roofline["traffic_over_bytes"] = round(traffic / lay_b, 4)
+    # bytes of one FP64 CG iteration and one mixed inner iteration in the running
+    # layout (split form: direction pass 5 vector passes, SpMV 2, update 3), for a
+    # check of the solve times against the peak
+    cg_iter_b = g.spmv_bytes(8, 8, sched["spmv64"]) + 8 * N * 8
+    mx_iter_b = g.spmv_bytes(4, 4, sched["spmv32"]) + 8 * N * 4
 
     literal = None
     if args.literal:
-        x = torch.zeros_like(b)
+        # the configuration as stated: restarted refinement, inner relTol 1e-4, timed
+        # with the same warm-up and K-step protocol as `value`
+        for _ in range(W):
+            g.solve_mixed(b, torch.zeros_like(b), 1e-10, 1e-4, max_iter, 1)
         torch.cuda.synchronize()
+        xl = [torch.zeros_like(b) for _ in range(K)]
+        step_l, res_l = [], []
         t0 = time.perf_counter()
-        _, hl, itl, rell, stl = g.solve_mixed(b, x, 1e-10, 1e-4, max_iter, 1)
+        for i in range(K):
+            ts = time.perf_counter()
+            res_l.append(g.solve_mixed(b, xl[i], 1e-10, 1e-4, max_iter, 1))
+            step_l.append(time.perf_counter() - ts)
         torch.cuda.synchronize()
+        t_lit = (time.perf_counter() - t0) / K
+        _, hl, itl, rell, stl = res_l[-1]
         literal = {"mode": "restarted refinement", "inner_rel_tol": 1e-4,
-                   "s": round(time.perf_counter() - t0, 5), "iterations": itl,
-                   "fp64_updates": len(hl) - 1, "rel_residual": rell, "status": stl}
+                   "s": round(t_lit, 5), "median_step_s": round(float(np.median(step_l)), 5),
+                   "steps": K, "warmup": W, "iterations": itl,
+                   "fp64_updates": len(hl) - 1, "rel_residual": rell, "status": stl,
+                   "speedup_vs_fp64": round(t_fp64 / t_lit, 4)}
         log(f"[bench] literal restarted refinement: {literal}")
 
     batched = None
@@ -556,10 +612,15 @@ def main():
     plumb = plumbing_32() if args.dropin else None
 
     cpu = None
-    if not args.no_cpu and n == 256:
+    if not args.no_cpu:
         try:
-            cpu = cpu_baseline(s, it_d, args.cpu_sample)
-            log(f"[bench] cpu baseline: {cpu['value']:.1f}s extrapolated")
+            from oracle import oracle as O
+            A_host = O.ldu2csr(s)
+            cpu = cpu_baseline(s, A_host, xd[-1].cpu().numpy(), xs[-1].cpu().numpy(), it_d,
+                               args.cpu_sample)
+            del A_host
+            log(f"[bench] cpu baseline: {cpu['value']:.1f}s, full solve on {cpu['cores']} "
+                f"threads; GPU FP64 bit-exact: {cpu['gpu_fp64_bit_exact']}")
         except Exception as e:  # the oracle is optional at run time
             log(f"[bench] cpu baseline skipped: {e}")
 
@@ -592,6 +653,10 @@ def main():
         "rel_residual_mixed": rel_true_m,
         "rel_residual_fp64": rel_true_d,
         "rel_error_mixed_vs_fp64": err_m,
+        "rel_error_mixed_vs_oracle_fp64": (cpu or {}).get("rel_error_mixed_vs_oracle_fp64"),
+        "layout_bytes_per_iteration": {"fp64_cg": cg_iter_b, "mixed_inner": mx_iter_b,
+                                       "fp64_cg_TBps": round(cg_iter_b * it_d / t_fp64 / 1e12, 3),
+                                       "mixed_TBps": round(mx_iter_b * it_m / t_mixed / 1e12, 3)},
         "fp64_iteration_us": round(t_fp64 / max(it_d, 1) * 1e6, 2),
         "mixed_iteration_us": round(t_mixed / max(it_m, 1) * 1e6, 2),
         "kernel_avg_us": {"mixed_spmv_kernel": avg_us((mx1_ms, mx1_n)),

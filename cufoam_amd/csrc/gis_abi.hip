@@ -41,6 +41,10 @@ int set_values_common(gis_context* h) {
     return kOK;
 }
 
+bool detail_ok(const gis_params* p) {
+    return !p || p->detail_type == GIS_DETAIL_ERROR || p->detail_type == GIS_DETAIL_TIME;
+}
+
 SolveArgs to_args(int method, const gis_params* p, int hist_cap) {
     gis_params d;
     gis_default_params(&d);
@@ -53,6 +57,7 @@ SolveArgs to_args(int method, const gis_params* p, int hist_cap) {
     a.power_steps = p->power_steps;
     a.int_bound = p->int_bound;
     a.hist_cap = hist_cap;
+    a.detail = p->detail_type;
     return a;
 }
 
@@ -96,6 +101,7 @@ GIS_API void gis_default_params(gis_params* p) {
     p->omega = 0.0;
     p->power_steps = 30;
     p->int_bound = 1e-10;
+    p->detail_type = GIS_DETAIL_ERROR;
 }
 
 GIS_API const char* gis_status_string(int s) {
@@ -129,6 +135,10 @@ GIS_API int gis_create(gis_handle* out, int device) {
         return GIS_ERR_HIP;
     }
     h->own_stream = true;
+    int wall_khz = 0;
+    if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device) == hipSuccess &&
+        wall_khz > 0)
+        h->wall_tick = 1.0 / (1000.0 * wall_khz);
     if (ensure_state(h) != kOK || noalloc_setup(h) != kOK) {
         if (h->st_host) (void)hipHostFree(h->st_host);
         (void)hipStreamDestroy(h->stream);
@@ -205,6 +215,7 @@ static int set_pattern_impl(gis_handle h, int n_cells, int n_faces, const int* l
         GIS_TRY(upload(h, h->upper_addr, ua, b));
     }
     GIS_TRY(build_pattern(h, n_cells, n_faces, h->lower_addr.as<int>(), h->upper_addr.as<int>()));
+    pattern_schedule(h);
     GIS_TRY(ensure_reduction(h, n_cells));
     GIS_HIP(h, hipStreamSynchronize(h->stream));
     return GIS_OK;
@@ -293,6 +304,7 @@ GIS_API int gis_set_interfaces(gis_handle h, int n_couples, const int* row, cons
     }
     h->n_cpl = n_couples;
     GIS_TRY(build_pattern(h, N, h->M, h->lower_addr.as<int>(), h->upper_addr.as<int>()));
+    pattern_schedule(h);
     GIS_TRY(ensure_reduction(h, N));
     GIS_HIP(h, hipStreamSynchronize(h->stream));
     return GIS_OK;
@@ -327,6 +339,20 @@ GIS_API int gis_get_size(gis_handle h, int* n_rows, long long* nnz) {
     if (!h->has_pattern) return fail(h, GIS_ERR_STATE, "pattern not set");
     if (n_rows) *n_rows = h->N;
     if (nnz) *nnz = h->nnz;
+    return GIS_OK;
+}
+
+GIS_API int gis_get_layout(gis_handle h, gis_layout* out) {
+    if (!h || !out) return GIS_ERR_ARG;
+    if (!h->has_pattern) return fail(h, GIS_ERR_STATE, "pattern not set");
+    out->n_rows = h->N;
+    out->nnz = h->nnz;
+    out->slices = h->lay_slices;
+    out->sell_slots = h->n_slots;
+    out->table_slices = h->lay_table_slices;
+    out->table_entries = h->lay_table_entries;
+    out->byte_mask_slices = h->lay_byte_mask_slices;
+    out->untabled_slots = h->lay_untabled_slots;
     return GIS_OK;
 }
 
@@ -512,6 +538,7 @@ static int solve_impl(gis_handle h, int method, double* x, const double* b,
                       bool dev) {
     if (!h || !x || !b) return GIS_ERR_ARG;
     if (method < GIS_CG || method > GIS_RICHARDSON) return fail(h, GIS_ERR_ARG, "unknown method");
+    if (!detail_ok(params)) return fail(h, GIS_ERR_ARG, "detail_type must be ERROR or TIME");
     if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
     DeviceGuard g(h->device);
     SolveArgs a = to_args(method, params, std::max(1, hist_cap));
@@ -535,6 +562,7 @@ static int pcg_impl(gis_handle h, int precond, double* x, const double* b,
     if (!h || !x || !b) return GIS_ERR_ARG;
     if (precond < GIS_PRECOND_NONE || precond > GIS_PRECOND_GAMG32)
         return fail(h, GIS_ERR_ARG, "unknown preconditioner");
+    if (!detail_ok(params)) return fail(h, GIS_ERR_ARG, "detail_type must be ERROR or TIME");
     if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
     DeviceGuard g(h->device);
     SolveArgs a = to_args(8 + precond, params, std::max(1, hist_cap));

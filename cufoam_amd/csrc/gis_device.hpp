@@ -55,6 +55,9 @@ struct KState {
     int slot_fault;                   // failure-path tests only: > 0 makes reduction
                                       // collectors see no slot written and give up
                                       // after that many polls
+    int detail;                       // history: 0 residual norms, 1 elapsed seconds
+    long long t0;                     // device wall clock when the loop started
+    double tick;                      // seconds per device wall-clock tick
 };
 
 // OpenFOAM's loop rule (lduMatrix::solver): iterate while (k < maxIter and not
@@ -668,13 +671,15 @@ __device__ __forceinline__ void tile_sell_c(const int* __restrict__ soff,
 // sell_row, with no column index per slot.  Values: slot-major from 64 * ooff[s].
 //
 // The loop body has no per-slot branches: slot indices are clamped into the
-// slice (min(k, U-1)), and a lane whose bit is clear gathers its own row
-// (`safe`, a valid row) and leaves the sum untouched through a select.  Every
-// load of a batch is then issued before the first wait.
+// slice (min(k, U-1)), every lane gathers column row + offset clamped into
+// [0, last] whatever its mask bit, and a clear bit leaves the sum untouched
+// through a select.  The gather addresses therefore depend only on the
+// (scalar) offset table, not on the mask: values, masks and gathers of a batch
+// are all in flight together, one memory round trip per batch.
 template <typename T, class XOp>
 __device__ __forceinline__ T sell_row_d(const int* __restrict__ offs, unsigned mask,
                                         const T* __restrict__ dval, int a, int len, int lane,
-                                        int row, int safe, XOp&& xop) {
+                                        int row, int last, XOp&& xop) {
     T acc = T(0);
     for (int j = 0; j < len; j += 8) {
         T v[8], xv[8];
@@ -687,10 +692,7 @@ __device__ __forceinline__ T sell_row_d(const int* __restrict__ offs, unsigned m
 #pragma unroll
         for (int u = 0; u < 8; ++u) o[u] = offs[min(j + u, len - 1)];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const bool on = (mask >> (j + u)) & 1u;
-            xv[u] = xop(on ? row + o[u] : safe);
-        }
+        for (int u = 0; u < 8; ++u) xv[u] = xop(min(max(row + o[u], 0), last));
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const bool on = (mask >> (j + u)) & 1u;
@@ -704,9 +706,9 @@ __device__ __forceinline__ T sell_row_d(const int* __restrict__ offs, unsigned m
 // Two offset-table slices per lane at a time (their loads in flight together).
 template <typename T, class XOp>
 __device__ __forceinline__ void sell_row2_d(const int* __restrict__ offs0, unsigned m0, int a0,
-                                            int len0, int row0, int safe0,
+                                            int len0, int row0,
                                             const int* __restrict__ offs1, unsigned m1, int a1,
-                                            int len1, int row1, int safe1,
+                                            int len1, int row1, int last,
                                             const T* __restrict__ dval, int lane, XOp&& xop,
                                             T& y0, T& y1) {
     T acc0 = T(0), acc1 = T(0);
@@ -727,9 +729,8 @@ __device__ __forceinline__ void sell_row2_d(const int* __restrict__ offs0, unsig
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const bool on0 = (m0 >> (j + u)) & 1u, on1 = (m1 >> (j + u)) & 1u;
-            x0[u] = xop(on0 ? row0 + o0[u] : safe0);
-            x1[u] = xop(on1 ? row1 + o1[u] : safe1);
+            x0[u] = xop(min(max(row0 + o0[u], 0), last));
+            x1[u] = xop(min(max(row1 + o1[u], 0), last));
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
@@ -790,13 +791,12 @@ __device__ __forceinline__ void tile_sell_d(const int* __restrict__ soff,
         }
         T y0 = T(0), y1 = T(0);
         const int r0 = s * kSlice + lane, r1 = (s + 1) * kSlice + lane;
-        // rows past N have an empty mask; their (unused) gathers read row N-1
-        const int q0 = min(r0, N - 1), q1 = min(r1, N - 1);
+        // rows past N have an empty mask; their (unused) gathers stay inside [0, N)
         if (u0 > 0 && u1 > 0) {
             const unsigned m0 = row_mask(dmask, dmask8, u0, r0, N);
             const unsigned m1 = row_mask(dmask, dmask8, u1, r1, N);
-            sell_row2_d<T>(doffs + o0, m0, o0 * kSlice, u0, r0, q0, doffs + o1, m1, o1 * kSlice,
-                           u1, r1, q1, dval, lane, xop, y0, y1);
+            sell_row2_d<T>(doffs + o0, m0, o0 * kSlice, u0, r0, doffs + o1, m1, o1 * kSlice,
+                           u1, r1, N - 1, dval, lane, xop, y0, y1);
         } else {
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
@@ -806,8 +806,8 @@ __device__ __forceinline__ void tile_sell_d(const int* __restrict__ soff,
                 T y;
                 if (up > 0) {
                     const unsigned mp = row_mask(dmask, dmask8, up, rp, N);
-                    y = sell_row_d<T>(doffs + op, mp, dval, op * kSlice, up, lane, rp,
-                                      p ? q1 : q0, xop);
+                    y = sell_row_d<T>(doffs + op, mp, dval, op * kSlice, up, lane, rp, N - 1,
+                                      xop);
                 } else {
                     const int a = soff[sp];
                     y = sell_row<T>(scol, sval, a, (soff[sp + 1] - a) / kSlice, lane, xop);
@@ -915,6 +915,62 @@ __device__ __forceinline__ void store4nt(float* __restrict__ p, int i0, int n,
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             if (i0 + k < n) p[i0 + k] = o[k];
+    }
+}
+
+// A tile's kTile FP64 results from LDS (tile row order) to y with streaming
+// stores that write whole cache lines per instruction: thread t stores rows
+// 2t, 2t+1 and 512+2t, 513+2t of the tile as 16-byte vectors, consecutive
+// threads on consecutive rows.  (Per-thread rows 4t..4t+3 as two 16-byte
+// stores would leave every 128-byte line half-written by each instruction,
+// and streaming stores reach memory as those partial lines.)  The caller has
+// made sh_y complete with a workgroup barrier.
+__device__ __forceinline__ void store_tile_nt(double* __restrict__ y,
+                                              const double* __restrict__ sh_y, int row0, int n) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int lr = 2 * (int)threadIdx.x + h * (kTile / 2);
+        const int g = row0 + lr;
+        if (g + 1 < n) {
+            gis_d2 v;
+            v.x = sh_y[lr];
+            v.y = sh_y[lr + 1];
+            __builtin_nontemporal_store(v, reinterpret_cast<gis_d2*>(y + g));
+        } else if (g < n) {
+            y[g] = sh_y[lr];
+        }
+    }
+}
+
+// Coalesced rows of a tile for streaming kernels without a reduction: thread
+// t owns rows row0 + 2t, +1 (o[0], o[1]) and row0 + 512 + 2t, +1 (o[2], o[3]),
+// so each 16-byte access of a wave covers 1 KB of consecutive rows.
+__device__ __forceinline__ void load_pairs(const double* __restrict__ p, int row0, int n,
+                                           double (&o)[4]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int g = row0 + 2 * (int)threadIdx.x + h * (kTile / 2);
+        if (g + 1 < n) {
+            const double2 a = *reinterpret_cast<const double2*>(p + g);
+            o[2 * h] = a.x;
+            o[2 * h + 1] = a.y;
+        } else {
+            o[2 * h] = g < n ? p[g] : 0.0;
+            o[2 * h + 1] = 0.0;
+        }
+    }
+}
+
+__device__ __forceinline__ void store_pairs(double* __restrict__ p, int row0, int n,
+                                            const double (&o)[4]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int g = row0 + 2 * (int)threadIdx.x + h * (kTile / 2);
+        if (g + 1 < n) {
+            *reinterpret_cast<double2*>(p + g) = make_double2(o[2 * h], o[2 * h + 1]);
+        } else if (g < n) {
+            p[g] = o[2 * h];
+        }
     }
 }
 

@@ -142,6 +142,7 @@ struct AmgData {
 
 struct gis_context {
     int device = 0;
+    double wall_tick = 0.0;     // seconds per tick of the device wall clock (wall_clock64)
     hipStream_t stream = nullptr;
     bool own_stream = false;
 
@@ -175,6 +176,11 @@ struct gis_context {
     // and the slot values (64 slots per table entry)
     gis::DevMem ooff, doffs, dmask, dmask8, dperm;
     long long n_dslots = 0;
+    bool selld_full = false;                    // every slice has an offset table
+    // SellD layout counts (gis_get_layout): slices, slices with a table, their
+    // table entries, tabled slices with one-byte masks, sliced-row slots of the others
+    long long lay_slices = 0, lay_table_slices = 0, lay_table_entries = 0;
+    long long lay_byte_mask_slices = 0, lay_untabled_slots = 0;
     gis::DevMem dval, dvalT, dval32, dval32T;
     gis::DevMem dinv;                           // per-row helper (1/a_ii or sqrt(1/a_ii))
     gis::AmgData* amg = nullptr;                // GAMG hierarchy (built on first use)
@@ -290,6 +296,7 @@ struct SolveArgs {
     int hist_cap;
     double rel_tol = 0.0;     // > 0: stop also at rel_tol * initial residual
     long long min_steps = 0;  // no stop on convergence before this many iterations
+    int detail = 0;           // history: 0 residual norms, 1 elapsed seconds
 };
 
 // Preconditioned CG (gis_amg.hip): pc 0 none, 1 diagonal, 2 GAMG.  As
@@ -350,6 +357,10 @@ struct MixedArgs {
 // schedule autotuner (gis_tune.hip); what: 1 FP64 kernels, 2 FP32 kernels
 int device_autotune(gis_context* h, int what, int iterations, int* spmv, int* direction,
                     int* from_cache, double* best_ms);
+// the schedule a new pattern starts with (no timing): a cached autotuner choice
+// for (device, N, nnz), else SellD with the separate direction pass when every
+// slice has an offset table, else sliced rows
+void pattern_schedule(gis_context* h);
 void autotune_forget();
 
 int device_solve_mixed(gis_context* h, const MixedArgs& a, double* d_x, const double* d_b,

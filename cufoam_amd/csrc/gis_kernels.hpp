@@ -173,11 +173,15 @@ struct SuperTile {
     }
 };
 
+// History entry k: the residual norm, or (detail = 1, DetailType::Time of
+// support.h:7-9) the seconds since the loop started, read from the device's
+// constant-rate wall clock by the workgroup that finishes iteration k.
 __device__ __forceinline__ void record_norm(KState* st, double norm) {
     const int k = st->k + 1;
     st->k = k;
     st->norm = norm;
-    if (k < st->hist_cap) st->hist[k] = norm;
+    if (k < st->hist_cap)
+        st->hist[k] = st->detail ? (double)(wall_clock64() - st->t0) * st->tick : norm;
 }
 
 // ---------------------------------------------------------------------------
@@ -219,7 +223,8 @@ __global__ __launch_bounds__(kWG) void k_setup(Mat<double> m, int N,
         st->norm0 = norm;
         st->k = 0;
         st->status = kOK;
-        if (st->hist_cap > 0) st->hist[0] = norm;
+        if (st->hist_cap > 0) st->hist[0] = st->detail ? 0.0 : norm;
+        st->t0 = wall_clock64();
         if (st->rel_tol > 0.0) st->error = fmax(st->error, st->rel_tol * norm);
         st->done = (done_rule == 0) ? (!(norm > st->error) && st->min_steps <= 0) : 0;
     });
@@ -241,7 +246,6 @@ __global__ __launch_bounds__(kWG) void k_cg1s(Mat<double> m, int N,
     const SuperTile<C> sup(N, false);
     const double* p = ((st->k + 1) & 1) ? pb1 : pb0;
     double leaf[C][1];
-    double qv[C][4];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         leaf[c][0] = 0.0;
@@ -251,17 +255,14 @@ __global__ __launch_bounds__(kWG) void k_cg1s(Mat<double> m, int N,
         P::run(m, N, t.row0, t.nrows, [&](int j) { return p[j]; }, sh, sh_y);
         double pv[4];
         load4(p, t.i0, N, pv);
+        __syncthreads();
+        store_tile_nt(q, sh_y, t.row0, N);
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-            qv[c][kk] = sh_y[threadIdx.x * kLeaf + kk];
-            if (t.i0 + kk < N) leaf[c][0] = fma(pv[kk], qv[c][kk], leaf[c][0]);
-        }
+        for (int kk = 0; kk < 4; ++kk)
+            if (t.i0 + kk < N) leaf[c][0] = fma(pv[kk], sh_y[threadIdx.x * kLeaf + kk], leaf[c][0]);
     }
     SlotReduce<1, C> tr;
     tr.begin(red, sup.tile0, sup.nt, leaf, shr);
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-        if (c < sup.nt) store4nt(q, TileRows(N, sup.tile0 + c).i0, N, qv[c]);
     tr.end(red, st, shr, [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
 }
 

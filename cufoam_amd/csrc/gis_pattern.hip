@@ -168,9 +168,10 @@ __global__ __launch_bounds__(kBlk) void k_sort_lower(int N, const int* __restric
 // pads the slice by at most a quarter.  ucount[s] = table size (0: none).
 __global__ __launch_bounds__(kBlk) void k_selld_union(int N, int S,
                                                       const int* __restrict__ rowptr,
-                                                      const int* __restrict__ col, int* ucount,
+                                                      const int* __restrict__ col,
+                                                      const int* __restrict__ soff, int* ucount,
                                                       int* tmp_offs, unsigned* dmask,
-                                                      unsigned char* dmask8) {
+                                                      unsigned char* dmask8, int* counts) {
     const int lane = threadIdx.x & 63;
     const int s = blockIdx.x * (kBlk / 64) + (threadIdx.x >> 6);
     if (s >= S) return;   // whole waves
@@ -210,7 +211,17 @@ __global__ __launch_bounds__(kBlk) void k_selld_union(int N, int S,
         dmask[row] = fit ? mask : 0u;
         dmask8[row] = (unsigned char)(fit && U <= 8 ? mask : 0u);
     }
-    if (lane == 0) ucount[s] = fit ? U : 0;
+    // counts: [0] slices without a table, [1] tabled slices with one-byte masks,
+    // [2] sliced-row slots of the slices without a table
+    if (lane == 0) {
+        ucount[s] = fit ? U : 0;
+        if (!fit) {
+            atomicAdd(&counts[0], 1);
+            atomicAdd(&counts[2], soff[s + 1] - soff[s]);
+        } else if (U <= 8) {
+            atomicAdd(&counts[1], 1);
+        }
+    }
 }
 
 // offsets compacted to doffs[ooff[s] ..]; slot (table entry k, lane) -> value
@@ -518,23 +529,33 @@ int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua
     // column-offset tables (SellD)
     {
         DevMem ucount, tmp_offs;
-        GIS_HIP(h, ucount.ensure(sizeof(int) * ((size_t)S + 1)));
+        // ucount[0..S]: table sizes (ucount[S] = 0 for the scan); ucount[S+1..S+3]:
+        // the layout counts of k_selld_union
+        GIS_HIP(h, ucount.ensure(sizeof(int) * ((size_t)S + 4)));
         GIS_HIP(h, tmp_offs.ensure(sizeof(int) * 32 * (size_t)S));
         GIS_HIP(h, h->ooff.ensure(sizeof(int) * ((size_t)S + 1)));
         GIS_HIP(h, h->dmask.ensure(sizeof(unsigned) * (size_t)N));
         GIS_HIP(h, h->dmask8.ensure((size_t)N));
-        GIS_HIP(h, hipMemsetAsync(ucount.p, 0, sizeof(int) * ((size_t)S + 1), h->stream));
+        GIS_HIP(h, hipMemsetAsync(ucount.p, 0, sizeof(int) * ((size_t)S + 4), h->stream));
         const dim3 gs((unsigned)((S + kBlk / 64 - 1) / (kBlk / 64)));
         hipLaunchKernelGGL(k_selld_union, gs, dim3(kBlk), 0, h->stream, N, S,
-                           h->rowptr.as<int>(), h->col.as<int>(), ucount.as<int>(),
-                           tmp_offs.as<int>(), h->dmask.as<unsigned>(),
-                           h->dmask8.as<unsigned char>());
+                           h->rowptr.as<int>(), h->col.as<int>(), h->soff.as<int>(),
+                           ucount.as<int>(), tmp_offs.as<int>(), h->dmask.as<unsigned>(),
+                           h->dmask8.as<unsigned char>(), ucount.as<int>() + S + 1);
         GIS_HIP(h, hipGetLastError());
         GIS_TRY(scan_exclusive(h, ucount.as<int>(), h->ooff.as<int>(), S + 1, tmp, scan_tmp));
-        int n_tab = 0;
+        int n_tab = 0, counts[3] = {0, 0, 0};
         GIS_HIP(h, hipMemcpyAsync(&n_tab, h->ooff.as<int>() + S, sizeof(int),
                                   hipMemcpyDeviceToHost, h->stream));
+        GIS_HIP(h, hipMemcpyAsync(counts, ucount.as<int>() + S + 1, sizeof(counts),
+                                  hipMemcpyDeviceToHost, h->stream));
         GIS_HIP(h, hipStreamSynchronize(h->stream));
+        h->selld_full = (counts[0] == 0);
+        h->lay_slices = S;
+        h->lay_table_slices = S - counts[0];
+        h->lay_table_entries = n_tab;
+        h->lay_byte_mask_slices = counts[1];
+        h->lay_untabled_slots = counts[2];
         h->n_dslots = 64LL * n_tab;
         GIS_HIP(h, h->doffs.ensure(sizeof(int) * std::max(n_tab, 1)));
         GIS_HIP(h, h->dperm.ensure(sizeof(int) * (size_t)std::max<long long>(h->n_dslots, 1)));

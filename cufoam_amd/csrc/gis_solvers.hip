@@ -26,10 +26,15 @@ __global__ __launch_bounds__(kWG) void k_spmv(Mat<T> m, int N, const T* __restri
     __shared__ T sh_y[kTile];
     TileRows t(N);
     P::run(m, N, t.row0, t.nrows, [&](int c) { return x[c]; }, sh, sh_y);
-    T o[4];
+    if constexpr (sizeof(T) == 8) {
+        __syncthreads();
+        store_tile_nt(y, sh_y, t.row0, N);
+    } else {
+        T o[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) o[k] = sh_y[threadIdx.x * kLeaf + k];
-    store4nt(y, t.i0, N, o);
+        for (int k = 0; k < 4; ++k) o[k] = sh_y[threadIdx.x * kLeaf + k];
+        store4nt(y, t.i0, N, o);   // 16 bytes of consecutive rows per thread: whole lines
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -149,8 +154,9 @@ __global__ __launch_bounds__(kWG) void k_cg1(Mat<double> m, int N, const double*
             if (t.i0 + kk < N) leaf[c][0] = fma(pn[kk], qv[kk], leaf[c][0]);
         }
         store4(p_new, t.i0, N, pn);
-        store4nt(q, t.i0, N, qv);
         if (!first) store4(x, t.i0, N, xv);
+        __syncthreads();
+        store_tile_nt(q, sh_y, t.row0, N);
     }
     SlotReduce<1, C> tr;
     tr.begin(red, sup.tile0, sup.nt, leaf, shr);
@@ -174,22 +180,22 @@ __global__ __launch_bounds__(kWG) void k_cg_dir(int N, const double* __restrict_
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         if (c >= sup.nt) continue;
-        const TileRows t(N, sup.tile0 + c);
-        double rv[4], pv[4], xv[4];
-        load4(r, t.i0, N, rv);
+        const int row0 = (sup.tile0 + c) * kTile;
+        double rv[4], pv[4], xv[4];   // rows as load_pairs maps them (no reduction here)
+        load_pairs(r, row0, N, rv);
         if (k == 1) {
-            store4(p_new, t.i0, N, rv);
+            store_pairs(p_new, row0, N, rv);
             continue;
         }
-        load4(p_old, t.i0, N, pv);
-        load4(x, t.i0, N, xv);
+        load_pairs(p_old, row0, N, pv);
+        load_pairs(x, row0, N, xv);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             xv[kk] = fma(alpha_prev, pv[kk], xv[kk]);
             pv[kk] = fma(beta, pv[kk], rv[kk]);
         }
-        store4(p_new, t.i0, N, pv);
-        store4(x, t.i0, N, xv);
+        store_pairs(p_new, row0, N, pv);
+        store_pairs(x, row0, N, xv);
     }
 }
 
@@ -477,6 +483,8 @@ int prepare_solve(gis_context* h, const SolveArgs& a, int nvec) {
     s.rel_tol = a.rel_tol;
     s.min_steps = a.min_steps;
     s.slot_fault = h->slot_fault;
+    s.detail = a.detail;
+    s.tick = h->wall_tick;
     s.hist = h->hist.as<double>();
     s.hist_cap = cap;
     s.aux[4] = INFINITY;   // running min for FindDiagAbsMin

@@ -17,6 +17,7 @@ import numpy as np
 from . import build as _build
 
 OK, ERR_ARG, ERR_HIP, ERR_SHAPE, ERR_STATE, ERR_NOMEM = 0, 1, 2, 3, 4, 5
+DETAILS = {"error": 0, "time": 1}   # what a solve's history records (DetailType)
 BREAKDOWN, NOT_CONVERGED, NAN = 10, 11, 12
 ERR_SYNC = 13   # a device reduction did not complete: an error, raised
 
@@ -33,13 +34,19 @@ class GisError(RuntimeError):
 
 class _Params(C.Structure):
     _fields_ = [("error", C.c_double), ("max_steps", C.c_longlong), ("omega", C.c_double),
-                ("power_steps", C.c_int), ("int_bound", C.c_double)]
+                ("power_steps", C.c_int), ("int_bound", C.c_double), ("detail_type", C.c_int)]
 
 
 class _Schedule(C.Structure):
     _fields_ = [("spmv64", C.c_int), ("spmv32", C.c_int), ("block_iters", C.c_int),
                 ("fp64_direction", C.c_int), ("fp32_direction", C.c_int),
                 ("coarsen64", C.c_int), ("coarsen32", C.c_int)]
+
+
+class _Layout(C.Structure):
+    _fields_ = [(f, C.c_longlong) for f in ("n_rows", "nnz", "slices", "sell_slots",
+                                            "table_slices", "table_entries",
+                                            "byte_mask_slices", "untabled_slots")]
 
 
 class _Perf(C.Structure):
@@ -69,6 +76,7 @@ SIGNATURES = {
     "gis_set_coupled_values": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "gis_set_coupled_values_device": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "gis_get_size": (_i, [_vp, _ip, _llp]),
+    "gis_get_layout": (_i, [_vp, C.POINTER(_Layout)]),
     "gis_get_csr": (_i, [_vp, _vp, _vp, _vp]),
     "gis_get_csr_transpose": (_i, [_vp, _vp, _vp, _vp]),
     "gis_spmv_device": (_i, [_vp, _vp, _vp]),
@@ -385,6 +393,30 @@ class Gis:
         else:
             self.set_values(sys.diag, sys.lower, sys.upper)
 
+    def layout(self) -> dict:
+        """The sliced layouts of the pattern (gis_layout): slice, slot and table counts."""
+        lay = _Layout()
+        self._check(self._L.gis_get_layout(self._h, C.byref(lay)), "get_layout")
+        return {f: getattr(lay, f) for f, _ in _Layout._fields_}
+
+    def spmv_bytes(self, value_size: int, vector_size: int, schedule: int = 6) -> int:
+        """Bytes one SpMV of this pattern must move in the given layout: the stored
+        matrix (values, column indices or offset tables and row masks, slice
+        offsets), x read once and y written once.  Schedule 6 (column-offset
+        tables) and 0 (sliced rows) are modelled; others count the CSR bytes."""
+        lay = self.layout()
+        n, sv, sx = lay["n_rows"], value_size, vector_size
+        vec = 2 * n * sx
+        if schedule == 6:
+            tabled_rows = min(64 * lay["table_slices"], n)
+            byte_rows = min(64 * lay["byte_mask_slices"], tabled_rows)
+            return (64 * lay["table_entries"] * sv + byte_rows + 4 * (tabled_rows - byte_rows)
+                    + 4 * (lay["table_entries"] + lay["slices"] + 1)
+                    + lay["untabled_slots"] * (sv + 4) + vec)
+        if schedule in (0, 4):
+            return lay["sell_slots"] * (sv + 4) + 4 * (lay["slices"] + 1) + vec
+        return lay["nnz"] * (sv + 4) + 4 * (n + 1) + vec
+
     def get_csr(self, transpose: bool = False):
         rp = np.zeros(self.n + 1, np.int32)
         col = np.zeros(max(self.nnz, 1), np.int32)
@@ -505,9 +537,10 @@ class Gis:
 
     # -- solvers ---------------------------------------------------------------
     def solve_pcg(self, precond: str, b, x0=None, error: float = 1e-4, max_steps: int = 0,
-                  hist_cap: int | None = None):
-        """Preconditioned CG (precond none / diagonal / GAMG) -> (x, hist, status)."""
-        p = _Params(error, max_steps, 0.0, 30, 1e-10)
+                  hist_cap: int | None = None, detail: str = "error"):
+        """Preconditioned CG (precond none / diagonal / GAMG) -> (x, hist, status).
+        detail "time": hist holds the seconds since the loop started instead of norms."""
+        p = _Params(error, max_steps, 0.0, 30, 1e-10, DETAILS[detail])
         if hist_cap is None:
             hist_cap = (max_steps if max_steps > 0 else 10000) + 1
         hist = np.zeros(max(hist_cap, 1))
@@ -555,13 +588,16 @@ class Gis:
 
     def solve(self, method: str, b, x0=None, error: float = 1e-4, max_steps: int = 0,
               omega: float = 0.0, power_steps: int = 30, int_bound: float = 1e-10,
-              hist_cap: int | None = None):
-        """SingleGPU::<method>(x, A, b, error, max_steps, ...) -> (x, hist, status).
+              hist_cap: int | None = None, detail: str = "error"):
+        """SingleGPU::<method>(x, A, b, error, max_steps, ..., detail, detail_type)
+        -> (x, hist, status).
 
         With numpy inputs x is a new numpy array; with device tensors x0 is
-        updated in place and returned.
+        updated in place and returned.  detail "error" (DetailType::Error): hist
+        holds ||r_k||_2; "time" (DetailType::Time): the seconds since the loop
+        started, 0.0 first.
         """
-        p = _Params(error, max_steps, omega, power_steps, int_bound)
+        p = _Params(error, max_steps, omega, power_steps, int_bound, DETAILS[detail])
         if hist_cap is None:
             hist_cap = (max_steps if max_steps > 0 else 10000) + 1
         hist = np.zeros(max(hist_cap, 1))

@@ -56,12 +56,21 @@ extern "C" {
 typedef struct gis_context *gis_handle;
 
 /* Solve controls.  Defaults (gis_default_params) reproduce gis/gis.h:13-87. */
+/* What the history records (DetailType, support.h:7-9): the residual norm
+ * ||r_k||_2 per iteration, or the elapsed time in seconds since the loop started
+ * (entry 0 is 0.0), stamped on the device's constant-rate wall clock when
+ * iteration k completes (SolverSingleGPU.h:397-444 and every other method). */
+#define GIS_DETAIL_ERROR 0
+#define GIS_DETAIL_TIME 1
+
 typedef struct {
     double error;        /* absolute ||r||_2 threshold (CG :412 uses norm > error) */
     long long max_steps; /* <= 0: unbounded (SolverSingleGPU.h:394-395)            */
     double omega;        /* Richardson factor; |omega| < 1e-15 -> OptimizePower     */
     int power_steps;     /* Richardson power iterations (30)                        */
     double int_bound;    /* BiCG breakdown bound (1e-10)                            */
+    int detail_type;     /* GIS_DETAIL_ERROR (default) or GIS_DETAIL_TIME: what
+                            gis_solve / gis_solve_pcg write into hist            */
 } gis_params;
 
 /* solverPerformance fields the wrappers fill (cuFoamCG.C:80-83) */
@@ -100,6 +109,22 @@ GIS_API int gis_set_values(gis_handle h, const double *diag, const double *lower
 GIS_API int gis_set_values_device(gis_handle h, const double *d_diag, const double *d_lower,
                                   const double *d_upper);
 GIS_API int gis_get_size(gis_handle h, int *n_rows, long long *nnz);
+/* The sliced layouts of the pattern (64-row slices): what one SpMV reads.
+ * Schedule 0 reads sell_slots values and int32 columns; schedule 6 reads, per
+ * slice with a column-offset table, 64 values per table entry, one mask per
+ * row (1 byte in byte_mask_slices, else 4) and the table itself, and the
+ * sliced-row slots (value + int32 column) of the slices without a table. */
+typedef struct {
+    long long n_rows;
+    long long nnz;
+    long long slices;            /* 64-row slices */
+    long long sell_slots;        /* padded slots of the sliced-row layout */
+    long long table_slices;      /* slices with a column-offset table */
+    long long table_entries;     /* sum of their table lengths */
+    long long byte_mask_slices;  /* tabled slices with one-byte row masks (<= 8 entries) */
+    long long untabled_slots;    /* sliced-row slots of the slices without a table */
+} gis_layout;
+GIS_API int gis_get_layout(gis_handle h, gis_layout *out);
 /* Copy the device CSR (row_offsets[N+1], column_indices[nnz], values[nnz]) to host. */
 GIS_API int gis_get_csr(gis_handle h, int *row_offsets, int *column_indices, double *values);
 /* CSR of A^T (Transpose, gis/gis.cu:4-7): same pattern for LDU, lower <-> upper. */
@@ -139,7 +164,11 @@ typedef struct gis_schedule {
                             iteration kernels: 1, 2 or 4 */
     int coarsen32;       /* the same for the FP32 inner loop */
 } gis_schedule;
-/* Negative fields leave the current value unchanged. */
+/* Negative fields leave the current value unchanged.  gis_set_pattern (and
+ * gis_set_interfaces) start every new pattern on a schedule chosen without
+ * timing: the autotuner's cached choice for (device, N, nnz) if there is one,
+ * else schedule 6 with the separate direction pass (form 1) when every 64-row
+ * slice has a column-offset table (structured meshes), else schedule 0. */
 GIS_API int gis_set_schedule(gis_handle h, const gis_schedule *s);
 GIS_API int gis_get_schedule(gis_handle h, gis_schedule *s);
 
@@ -148,8 +177,10 @@ GIS_API int gis_get_schedule(gis_handle h, gis_schedule *s);
  * handle's matrix with `iterations` solver iterations per run (<= 0: 24) and
  * applies the fastest.  A candidate is admitted only if its SpMV equals the
  * sliced-row schedule bit for bit on a fixed vector and its run ends in the
- * bit-identical solver state.  The choice is cached per (device, N, nnz):
- * later calls (any handle) apply it without timing and set *from_cache = 1.
+ * bit-identical solver state.  The schedule in force is kept unless a
+ * candidate is more than 3% faster.  The choice is cached per (device, N, nnz):
+ * later calls (any handle) apply it without timing and set *from_cache = 1,
+ * and so does gis_set_pattern for a pattern of that size.
  * targets: GIS_TUNE_FP64 | GIS_TUNE_FP32.  best_ms[2] (may be NULL): time of
  * the chosen candidate's run per target (0 when cached).  Requires values. ---- */
 #define GIS_TUNE_FP64 1

@@ -183,3 +183,153 @@ BL_API int bl_pcg_openfoam(int n, int m, const int* la, const int* ua, const dou
     free(sum_a);
     return it;
 }
+
+/*
+ * OpenFOAM's PCG and PBiCG with preconditioner "none" on all host threads.
+ * The arithmetic is OpenFOAM's published loops; the matrix is applied row by
+ * row from CSR (A and, for PBiCG, A^T) instead of by the serial face loop, as a
+ * threaded (or domain-decomposed) run distributes it.  Residuals are
+ * sum|r| / normFactor, normFactor = sum(|A psi - xRef A 1| + |b - xRef A 1|)
+ * + 1e-20 with xRef = mean(psi); the solve stops when the scaled residual is
+ * below `tolerance` (relTol 0) or after max_iter iterations.  Reductions are
+ * OpenMP's (a timing routine, not a parity one).  Return the iteration count;
+ * *final_res receives the last scaled residual.
+ */
+static void csr_mul(int n, const int* rowptr, const int* col, const double* val,
+                    const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) s += val[j] * x[col[j]];
+        y[i] = s;
+    }
+}
+
+static double omp_norm_factor(int n, const int* rowptr, const double* val, const double* b,
+                              const double* x, const double* ax) {
+    double xref = 0.0;
+#pragma omp parallel for reduction(+ : xref) schedule(static)
+    for (int i = 0; i < n; ++i) xref += x[i];
+    xref /= n;
+    double nf = 0.0;
+#pragma omp parallel for reduction(+ : nf) schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double sum_a = 0.0;   /* (A 1)_i */
+        for (int j = rowptr[i]; j < rowptr[i + 1]; ++j) sum_a += val[j];
+        const double xa = xref * sum_a;
+        nf += fabs(ax[i] - xa) + fabs(b[i] - xa);
+    }
+    return nf + 1e-20;
+}
+
+BL_API int bl_pcg_openfoam_omp(int n, const int* rowptr, const int* col, const double* val,
+                               const double* b, double* x, double tolerance, int max_iter,
+                               double* final_res) {
+    double* r = malloc(sizeof(double) * (size_t)n);
+    double* p = malloc(sizeof(double) * (size_t)n);
+    double* w = malloc(sizeof(double) * (size_t)n);
+    csr_mul(n, rowptr, col, val, x, w);
+    const double nf = omp_norm_factor(n, rowptr, val, b, x, w);
+    double res = 0.0;
+#pragma omp parallel for reduction(+ : res) schedule(static)
+    for (int i = 0; i < n; ++i) {
+        r[i] = b[i] - w[i];
+        res += fabs(r[i]);
+    }
+    res /= nf;
+    int it = 0;
+    double rho_old = 1.0;
+    while (it < max_iter && !(res < tolerance)) {
+        double rho = 0.0;
+#pragma omp parallel for reduction(+ : rho) schedule(static)
+        for (int i = 0; i < n; ++i) rho += r[i] * r[i];
+        const double beta = it == 0 ? 0.0 : rho / rho_old;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) p[i] = it == 0 ? r[i] : r[i] + beta * p[i];
+        rho_old = rho;
+        csr_mul(n, rowptr, col, val, p, w);
+        double pw = 0.0;
+#pragma omp parallel for reduction(+ : pw) schedule(static)
+        for (int i = 0; i < n; ++i) pw += w[i] * p[i];
+        if (fabs(pw) / nf < 1e-300) break;
+        const double alpha = rho / pw;
+        res = 0.0;
+#pragma omp parallel for reduction(+ : res) schedule(static)
+        for (int i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * w[i];
+            res += fabs(r[i]);
+        }
+        res /= nf;
+        ++it;
+    }
+    *final_res = res;
+    free(r);
+    free(p);
+    free(w);
+    return it;
+}
+
+/* PBiCG, preconditioner none: the shadow system uses A^T (At in CSR) and starts
+ * from the same residual, r~ = b - A^T psi as OpenFOAM's Tmul gives it. */
+BL_API int bl_pbicg_openfoam_omp(int n, const int* rowptr, const int* col, const double* val,
+                                 const int* trowptr, const int* tcol, const double* tval,
+                                 const double* b, double* x, double tolerance, int max_iter,
+                                 double* final_res) {
+    double* rA = malloc(sizeof(double) * (size_t)n);
+    double* rT = malloc(sizeof(double) * (size_t)n);
+    double* pA = malloc(sizeof(double) * (size_t)n);
+    double* pT = malloc(sizeof(double) * (size_t)n);
+    double* wA = malloc(sizeof(double) * (size_t)n);
+    double* wT = malloc(sizeof(double) * (size_t)n);
+    csr_mul(n, rowptr, col, val, x, wA);
+    csr_mul(n, trowptr, tcol, tval, x, wT);
+    const double nf = omp_norm_factor(n, rowptr, val, b, x, wA);
+    double res = 0.0;
+#pragma omp parallel for reduction(+ : res) schedule(static)
+    for (int i = 0; i < n; ++i) {
+        rA[i] = b[i] - wA[i];
+        rT[i] = b[i] - wT[i];
+        res += fabs(rA[i]);
+    }
+    res /= nf;
+    int it = 0;
+    double wArT_old = 0.0;
+    while (it < max_iter && !(res < tolerance)) {
+        double wArT = 0.0;   /* (r, r~) with the identity preconditioner */
+#pragma omp parallel for reduction(+ : wArT) schedule(static)
+        for (int i = 0; i < n; ++i) wArT += rA[i] * rT[i];
+        const double beta = it == 0 ? 0.0 : wArT / wArT_old;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            pA[i] = it == 0 ? rA[i] : rA[i] + beta * pA[i];
+            pT[i] = it == 0 ? rT[i] : rT[i] + beta * pT[i];
+        }
+        wArT_old = wArT;
+        csr_mul(n, rowptr, col, val, pA, wA);
+        csr_mul(n, trowptr, tcol, tval, pT, wT);
+        double wApT = 0.0;
+#pragma omp parallel for reduction(+ : wApT) schedule(static)
+        for (int i = 0; i < n; ++i) wApT += wA[i] * pT[i];
+        if (fabs(wApT) / nf < 1e-300) break;
+        const double alpha = wArT / wApT;
+        res = 0.0;
+#pragma omp parallel for reduction(+ : res) schedule(static)
+        for (int i = 0; i < n; ++i) {
+            x[i] += alpha * pA[i];
+            rA[i] -= alpha * wA[i];
+            rT[i] -= alpha * wT[i];
+            res += fabs(rA[i]);
+        }
+        res /= nf;
+        ++it;
+    }
+    *final_res = res;
+    free(rA);
+    free(rT);
+    free(pA);
+    free(pT);
+    free(wA);
+    free(wT);
+    return it;
+}

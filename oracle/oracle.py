@@ -700,6 +700,13 @@ def _baseline_lib():
         L.bl_cg_ldu.restype = C.c_double
         L.bl_cg_ldu.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
                                 _f64p, C.c_int]
+        L.bl_pcg_openfoam_omp.restype = C.c_int
+        L.bl_pcg_openfoam_omp.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double,
+                                          C.c_int, C.POINTER(C.c_double)]
+        L.bl_pbicg_openfoam_omp.restype = C.c_int
+        L.bl_pbicg_openfoam_omp.argtypes = [C.c_int, _i32p, _i32p, _f64p, _i32p, _i32p, _f64p,
+                                            _f64p, _f64p, C.c_double, C.c_int,
+                                            C.POINTER(C.c_double)]
         _bl = L
     return _bl
 
@@ -743,4 +750,33 @@ def pcg_openfoam(sys, tolerance: float, max_iter: int):
                                          _c(sys.lower, np.float64), _c(sys.upper, np.float64),
                                          _c(sys.b, np.float64), x, tolerance, max_iter,
                                          C.byref(res))
+    return x, int(it), res.value, time.perf_counter() - t0
+
+
+def pcg_openfoam_omp(A: Csr, b, tolerance: float, max_iter: int):
+    """OpenFOAM's PCG (preconditioner none) on all OpenMP threads, matrix applied row
+    by row: (x, iterations, final scaled residual, seconds)."""
+    import time
+    x = np.zeros(A.n)
+    res = C.c_double(0.0)
+    t0 = time.perf_counter()
+    it = _baseline_lib().bl_pcg_openfoam_omp(A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32),
+                                             _c(A.val, np.float64), _c(b, np.float64), x,
+                                             tolerance, max_iter, C.byref(res))
+    return x, int(it), res.value, time.perf_counter() - t0
+
+
+def pbicg_openfoam_omp(A: Csr, b, tolerance: float, max_iter: int, At: Csr | None = None):
+    """OpenFOAM's PBiCG (preconditioner none) on all OpenMP threads: (x, iterations,
+    final scaled residual, seconds)."""
+    import time
+    if At is None:
+        At = transpose(A)
+    x = np.zeros(A.n)
+    res = C.c_double(0.0)
+    t0 = time.perf_counter()
+    it = _baseline_lib().bl_pbicg_openfoam_omp(
+        A.n, _c(A.rowptr, np.int32), _c(A.col, np.int32), _c(A.val, np.float64),
+        _c(At.rowptr, np.int32), _c(At.col, np.int32), _c(At.val, np.float64),
+        _c(b, np.float64), x, tolerance, max_iter, C.byref(res))
     return x, int(it), res.value, time.perf_counter() - t0

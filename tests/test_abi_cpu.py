@@ -47,7 +47,10 @@ def test_default_params_match_reference_defaults():
     lib = G.load()
     p = G._Params()
     lib.gis_default_params(ctypes.byref(p))
+    p.detail_type = 7
+    lib.gis_default_params(ctypes.byref(p))
     assert (p.error, p.max_steps, p.omega, p.power_steps, p.int_bound) == (1e-4, 0, 0.0, 30, 1e-10)
+    assert p.detail_type == 0   # DetailType::Error, the reference's default
 
 
 def test_create_without_gpu_reports_error():

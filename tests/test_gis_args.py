@@ -120,3 +120,14 @@ def test_system_of_the_right_size_passes_the_checks():
     assert not dev and x.shape == (27,) and np.all(x == 0) and b.flags.c_contiguous
     X, B, dev = g._xb(np.ones((2, 27)), None, "multi", 54, (2, 27))
     assert X.shape == (2, 27) and B.shape == (2, 27)
+
+
+def test_binding_has_every_public_method():
+    for name in ("set_pattern", "set_values", "set_interfaces", "set_coupled_values",
+                 "set_system", "get_csr", "layout", "spmv_bytes", "spmv", "spmv_async", "dot",
+                 "norm_factor", "norm_factor_noalloc", "set_schedule", "get_schedule",
+                 "autotune", "set_timing", "get_timing", "synchronize", "solve_pcg",
+                 "set_slot_fault", "set_gamg", "gamg_levels", "gamg_level", "solve",
+                 "solve_mixed", "solve_multi", "solve_multi_mixed", "adjoint",
+                 "batch_set_systems", "batch_values", "batch_solve", "foam_solve"):
+        assert callable(getattr(Gis, name, None)), name

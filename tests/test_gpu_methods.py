@@ -188,3 +188,42 @@ def test_foam_selection_rules():
         foam.New("cuFoamRichardson", "p", m, {})  # symmetric-only (cuFoamRichardson.C:11-12)
     with pytest.raises(foam.FatalError):
         foam.New("cuFoamGAMG", "p", m, {})
+
+
+@pytest.mark.parametrize("method", ["CG", "BiCG", "Jacobi", "MR", "MRJacobi", "JMR", "MRS",
+                                    "Richardson"])
+def test_time_history_mode(gpu, method):
+    """DetailType::Time (support.h:7-9, SolverSingleGPU.h:397-444): the history holds
+    0.0 and then the seconds since the loop started, one stamp per iteration, from the
+    device wall clock; iterates and iteration counts equal the norm-history run."""
+    import time
+    s = _case(24, 7, True, 0.3)
+    gpu.set_system(s)
+    err = 1e-8 * float(np.linalg.norm(s.b))
+    x_e, h_e, st_e = gpu.solve(method, s.b, s.x0, error=err, max_steps=400)
+    t0 = time.perf_counter()
+    x_t, h_t, st_t = gpu.solve(method, s.b, s.x0, error=err, max_steps=400, detail="time")
+    wall = time.perf_counter() - t0
+    assert st_t == st_e and len(h_t) == len(h_e) > 2
+    np.testing.assert_array_equal(x_t, x_e)
+    assert h_t[0] == 0.0
+    assert np.all(np.diff(h_t) > 0.0), h_t[:8]
+    assert h_t[-1] < wall                       # inside the call's own wall time
+    assert h_t[-1] > 1e-7 * (len(h_t) - 1)      # at least 0.1 us per iteration
+
+
+def test_time_history_mode_pcg_and_argument_check(gpu):
+    from cufoam_amd.gis import GisError, _Params
+    import ctypes as C
+    s = _case(20, 8, True, 0.3)
+    gpu.set_system(s)
+    x_e, h_e, _ = gpu.solve_pcg("GAMG", s.b, s.x0, error=1e-9, max_steps=200)
+    x_t, h_t, _ = gpu.solve_pcg("GAMG", s.b, s.x0, error=1e-9, max_steps=200, detail="time")
+    np.testing.assert_array_equal(x_t, x_e)
+    assert len(h_t) == len(h_e) and h_t[0] == 0.0 and np.all(np.diff(h_t) > 0.0)
+    p = _Params(1e-4, 10, 0.0, 30, 1e-10, 2)
+    x = np.zeros(s.n_cells)
+    hl = C.c_int(0)
+    st = gpu._L.gis_solve(gpu.handle, 0, x.ctypes.data_as(C.c_void_p),
+                          s.b.ctypes.data_as(C.c_void_p), C.byref(p), None, 0, C.byref(hl))
+    assert st == 1   # GIS_ERR_ARG: unknown detail type

@@ -47,3 +47,32 @@ def test_autotune_rejects_bad_targets(gpu):
     gpu.set_system(s)
     with pytest.raises(GisError):
         gpu.autotune(fp64=False, fp32=False)
+
+
+def test_pattern_picks_its_schedule_without_timing(gpu):
+    """gis_set_pattern starts a structured mesh (every 64-row slice tabled) on
+    SellD with the separate direction pass, an unstructured one on sliced rows;
+    gis_get_layout reports the counts the SellD byte model uses."""
+    from cufoam_amd import mesh
+    from cufoam_amd.gis import Gis
+    Gis.autotune_clear()
+    s = mesh.laplacian(32, seed=0, shift=1e-3)
+    gpu.set_system(s)
+    sch = gpu.get_schedule()
+    assert (sch["spmv64"], sch["spmv32"], sch["fp64_direction"], sch["fp32_direction"]) == (6, 6, 1, 1)
+    lay = gpu.layout()
+    assert lay["n_rows"] == 32768 and lay["nnz"] == s.nnz and lay["slices"] == 512
+    assert lay["table_slices"] == 512 and lay["untabled_slots"] == 0
+    assert lay["byte_mask_slices"] == 512
+    # a slice lies on one x-line: offsets -n^2, -n, -1, 0, +1, +n, +n^2 minus the
+    # missing neighbours of boundary lines
+    assert 5 * 512 <= lay["table_entries"] <= 7 * 512
+    csr = s.nnz * 12 + 4 * (s.n_cells + 1) + 16 * s.n_cells
+    assert gpu.spmv_bytes(8, 8, 6) < csr
+    r = mesh.random_ldu(5000, 15000, seed=1)
+    gpu.set_system(r)
+    assert gpu.get_schedule()["spmv64"] == 0
+    lay = gpu.layout()
+    assert lay["table_slices"] + (lay["untabled_slots"] > 0) >= 1
+    assert lay["sell_slots"] >= r.nnz
+
