@@ -312,6 +312,60 @@ def dropin(cpu_sample: int):
     return out
 
 
+def unstructured_spmv():
+    """SpMV where the offset tables do not apply: the 128^3 SPD system with its cells
+    renumbered at random (every row's neighbours scattered over the whole vector)
+    and then by reverse Cuthill-McKee (banded, as mesh tools order cells).  The
+    schedule is the one gis_set_pattern picks; the structured numbering is timed
+    beside them.  Bytes are those of the layout that runs (Gis.spmv_bytes)."""
+    import numpy as np
+    import torch
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    from cufoam_amd import mesh
+    from cufoam_amd.gis import Gis
+
+    s = mesh.config_system("poisson128")
+    N = s.n_cells
+    rng = np.random.default_rng(7)
+    rand = mesh.renumber(s, rng.permutation(N))
+    adj = csr_matrix((np.ones(rand.n_faces), (rand.lower_addr, rand.upper_addr)), shape=(N, N))
+    order = reverse_cuthill_mckee(adj + adj.T, symmetric_mode=True)
+    perm = np.empty(N, np.int64)
+    perm[order] = np.arange(N)
+    rcm = mesh.renumber(rand, perm)
+    out = {}
+    for name, sys_ in (("structured", s), ("random", rand), ("rcm", rcm)):
+        g = Gis(0)
+        g.set_system(sys_)
+        sched = g.get_schedule()
+        lay = g.layout()
+        rec = {"spmv_schedule": sched["spmv64"],
+               "tabled_slot_share": round(1.0 - lay["untabled_slots"] / max(lay["sell_slots"], 1)
+                                          if lay["table_slices"] else 0.0, 4),
+               "sell_slots_per_nnz": round(lay["sell_slots"] / lay["nnz"], 4)}
+        x64 = torch.randn(N, dtype=torch.float64, device="cuda")
+        for tag, xx, sv in (("fp64", x64, 8), ("fp32", x64.float(), 4)):
+            yy = torch.empty_like(xx)
+            for _ in range(3):
+                g.spmv_async(xx, yy)
+            g.synchronize()
+            g.set_timing("spmv")
+            for _ in range(20):
+                g.spmv_async(xx, yy)
+            ms, cnt = g.get_timing("spmv")
+            g.set_timing()
+            us = ms / cnt * 1e3
+            b = g.spmv_bytes(sv, sv, sched["spmv64"] if sv == 8 else sched["spmv32"])
+            rec[tag] = {"us": round(us, 2), "layout_GBps": round(b / (us * 1e-6) / 1e9, 1),
+                        "layout_frac": round(b / (us * 1e-6) / 1e9 / PEAK_HBM_GBS, 4),
+                        "csr_GBps": round(spmv_bytes(N, sys_.nnz, sv, sv) / (us * 1e-6) / 1e9, 1)}
+        g.close()
+        out[name] = rec
+        log(f"[bench] spmv 128^3 {name}: {rec}")
+    return out
+
+
 def plumbing_32():
     """The CPU-runnable case at 32^3 (SPD, shift 1e-3, x0 = 0, abs tol 1e-6, maxIter 2000):
     OpenFOAM's PCG (preconditioner none) restated on one host thread, the host ldu2csr the
@@ -610,6 +664,7 @@ def main():
 
     drop = dropin(0 if args.no_cpu else 10) if args.dropin else None
     plumb = plumbing_32() if args.dropin else None
+    unstructured = unstructured_spmv() if args.dropin else None
 
     cpu = None
     if not args.no_cpu:
@@ -681,6 +736,7 @@ def main():
     if drop is not None:
         out["dropin_128"] = drop
         out["plumbing_32"] = plumb
+        out["spmv_128_numberings"] = unstructured
     if batched is not None:
         out["batched_cg_64_systems"] = batched
         out["spmm_bicg_3_rhs"] = multi

@@ -236,6 +236,7 @@ int batch_set_pattern(gis_context* h, int S, const int* n_cells, const int* n_fa
     }
     GIS_TRY(build_pattern(h, (int)rows, (int)faces, h->lower_addr.as<int>(),
                           h->upper_addr.as<int>()));
+    pattern_schedule(h);   // SellD when every slice of the stack has an offset table
     // per-system reduction workspaces and states
     const int T = B.tile0[S];
     std::vector<int> tile_sys(T);
@@ -338,7 +339,10 @@ int batch_set_values(gis_context* h, const double* diag, const double* lower,
 static int batch_graph(gis_context* h, int iters, const Mat<double>& m, int N, double* r,
                        double* pb0, double* pb1, double* x, double* q, const BatchView& bv) {
     BatchInfo& B = h->batch;
-    if (B.graph_valid && B.graph_iters == iters && B.graph_key == (void*)x) return kOK;
+    const bool selld = h->sched.spmv64 == SpmvKind::SellD;
+    if (B.graph_valid && B.graph_iters == iters && B.graph_key == (void*)x &&
+        B.graph_selld == selld)
+        return kOK;
     if (B.graph_exec) (void)hipGraphExecDestroy(B.graph_exec);
     B.graph_exec = nullptr;
     if (!B.cap_stream) GIS_HIP(h, hipStreamCreateWithFlags(&B.cap_stream, hipStreamNonBlocking));
@@ -346,8 +350,12 @@ static int batch_graph(gis_context* h, int iters, const Mat<double>& m, int N, d
     GIS_HIP(h, hipStreamBeginCapture(B.cap_stream, hipStreamCaptureModeThreadLocal));
     const dim3 g = tiles_grid(N);
     for (int i = 0; i < iters; ++i) {
-        hipLaunchKernelGGL((k_bcg1<Sell>), g, dim3(kWG), 0, B.cap_stream, m, N, r, pb0, pb1, x,
-                           q, bv);
+        if (selld)
+            hipLaunchKernelGGL((k_bcg1<SellD>), g, dim3(kWG), 0, B.cap_stream, m, N, r, pb0, pb1,
+                               x, q, bv);
+        else
+            hipLaunchKernelGGL((k_bcg1<Sell>), g, dim3(kWG), 0, B.cap_stream, m, N, r, pb0, pb1,
+                               x, q, bv);
         hipLaunchKernelGGL(k_bcg2, g, dim3(kWG), 0, B.cap_stream, N, r, q, bv);
     }
     hipError_t e = hipStreamEndCapture(B.cap_stream, &graph);
@@ -358,6 +366,7 @@ static int batch_graph(gis_context* h, int iters, const Mat<double>& m, int N, d
     B.graph_valid = true;
     B.graph_iters = iters;
     B.graph_key = (void*)x;
+    B.graph_selld = selld;
     return kOK;
 }
 
@@ -431,8 +440,12 @@ int batch_solve(gis_context* h, double* x_packed, const double* b_packed, bool d
             }
         } else {
             for (int i = 0; i < block; ++i) {
-                hipLaunchKernelGGL((k_bcg1<Sell>), g, dim3(kWG), 0, h->stream, m, N, r, pb0, pb1,
-                                   x, q, bv);
+                if (h->sched.spmv64 == SpmvKind::SellD)
+                    hipLaunchKernelGGL((k_bcg1<SellD>), g, dim3(kWG), 0, h->stream, m, N, r, pb0,
+                                       pb1, x, q, bv);
+                else
+                    hipLaunchKernelGGL((k_bcg1<Sell>), g, dim3(kWG), 0, h->stream, m, N, r, pb0,
+                                       pb1, x, q, bv);
                 hipLaunchKernelGGL(k_bcg2, g, dim3(kWG), 0, h->stream, N, r, q, bv);
             }
         }

@@ -104,6 +104,7 @@ struct BatchInfo {
     bool graph_valid = false;
     int graph_iters = 0;
     void* graph_key = nullptr;
+    bool graph_selld = false;            // the SpMV layout the captured graph uses
 };
 
 // GAMG hierarchy (preconditioned CG).  Level 0 is the handle's own matrix;
@@ -176,7 +177,6 @@ struct gis_context {
     // and the slot values (64 slots per table entry)
     gis::DevMem ooff, doffs, dmask, dmask8, dperm;
     long long n_dslots = 0;
-    bool selld_full = false;                    // every slice has an offset table
     // SellD layout counts (gis_get_layout): slices, slices with a table, their
     // table entries, tabled slices with one-byte masks, sliced-row slots of the others
     long long lay_slices = 0, lay_table_slices = 0, lay_table_entries = 0;

@@ -29,45 +29,93 @@ constexpr int kMaxRhs = 8;
 __device__ __forceinline__ double fmat(double a, double b, double c) { return fma(a, b, c); }
 __device__ __forceinline__ float fmat(float a, float b, float c) { return fmaf(a, b, c); }
 
-// y[c][k] for the four 64-row slices of this wave, columns with bit k set in `active`
+// y[c][k] for the four 64-row slices of this wave, columns with bit k set in
+// `active`.  A slice with a column-offset table (m.ooff set: the handle's
+// schedule is SellD) reads 64 values per table entry and the row masks; the
+// others read the int32 slot columns.  Either way one value load feeds the
+// gathers of every column, the loop has no per-slot branch (a clear mask bit
+// or a padding slot gathers a valid row and leaves the sums untouched through
+// a select), and each row sums its entries in storage order with rounded
+// products: the bits of tile_sell / tile_sell_d, column by column.
 template <int NR, typename T, class XOp>
 __device__ __forceinline__ void sell_multi(const Mat<T>& m, int N, int row0, XOp&& xop,
                                            unsigned active, T (&y)[4][NR]) {
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nslices = (N + kSlice - 1) / kSlice;
     const int s0 = row0 / kSlice + w * 4;
+    const int last = N - 1;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
 #pragma unroll
         for (int k = 0; k < NR; ++k) y[c][k] = T(0);
         const int s = s0 + c;
         if (s >= nslices) continue;
-        const int a = m.soff[s];
-        const int len = (m.soff[s + 1] - a) / kSlice;
-        for (int j = 0; j < len; j += 4) {
-            int cc[4];
-            T v[4];
+        const int row = s * kSlice + lane;
+        const int o = m.ooff ? m.ooff[s] : 0;
+        const int U = m.ooff ? m.ooff[s + 1] - o : 0;
+        if (U > 0) {
+            const unsigned mask = row_mask(m.dmask, m.dmask8, U, row, N);
+            const T* dv = m.dval + (size_t)o * kSlice + lane;
+            const int* offs = m.doffs + o;
+            // a structured slice (at most 8 table entries) in one batch: every
+            // value load and gather of the slice in flight together
+            for (int j = 0; j < U; j += 8) {
+                T v[8];
+                int cc[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int idx = a + (j + u) * kSlice + lane;
-                if (j + u < len) {
-                    cc[u] = __builtin_nontemporal_load(m.scol + idx);
-                    v[u] = __builtin_nontemporal_load(m.sval + idx);
-                } else {
-                    cc[u] = -1;
-                    v[u] = T(0);
+                for (int u = 0; u < 8; ++u) {
+                    const int kk = min(j + u, U - 1);
+                    v[u] = __builtin_nontemporal_load(dv + (size_t)kk * kSlice);
+                    cc[u] = min(max(row + offs[kk], 0), last);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const bool on = (mask >> (j + u)) & 1u;
+#pragma unroll
+                    for (int k = 0; k < NR; ++k) {
+                        if (!(active & (1u << k))) continue;
+                        const T t = y[c][k] + v[u] * xop(k, cc[u]);
+                        y[c][k] = on ? t : y[c][k];
+                    }
                 }
             }
+        } else {
+            const int a = m.soff[s];
+            const int len = (m.soff[s + 1] - a) / kSlice;
+            const int safe = min(row, last);
+            for (int j = 0; j < len; j += 4) {
+                int cc[4];
+                T v[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (cc[u] < 0) continue;
+                for (int u = 0; u < 4; ++u) {
+                    const int idx = a + min(j + u, len - 1) * kSlice + lane;
+                    cc[u] = __builtin_nontemporal_load(m.scol + idx);
+                    v[u] = __builtin_nontemporal_load(m.sval + idx);
+                }
 #pragma unroll
-                for (int k = 0; k < NR; ++k)
-                    if (active & (1u << k)) y[c][k] = y[c][k] + v[u] * xop(k, cc[u]);
+                for (int u = 0; u < 4; ++u) {
+                    const bool on = (j + u < len) && cc[u] >= 0;
+                    const int col = on ? cc[u] : safe;
+#pragma unroll
+                    for (int k = 0; k < NR; ++k) {
+                        if (!(active & (1u << k))) continue;
+                        const T t = y[c][k] + v[u] * xop(k, col);
+                        y[c][k] = on ? t : y[c][k];
+                    }
+                }
             }
         }
     }
+}
+
+// The matrix view the multi-RHS kernels read: the offset tables only when the
+// handle's SpMV schedule for that precision is SellD (sell_multi then reads
+// the tabled slices through them), else the int32 slot columns throughout.
+template <typename T>
+Mat<T> multi_view(Mat<T> m, SpmvKind kind) {
+    if (kind != SpmvKind::SellD) m.ooff = nullptr;
+    return m;
 }
 
 // column k of the wave's slices -> this thread's kLeaf consecutive rows
@@ -652,8 +700,8 @@ int run_multi(gis_context* h, int method, const Mat<T>& mA, const Mat<T>& mAt,
 template <int NR>
 int solve_multi_nr(gis_context* h, int method, bool transpose, const SolveArgs& a, double* X,
                    const double* B, MultiBufs& mb) {
-    const Mat<double> mA = transpose ? mat64T(h) : mat64(h);
-    const Mat<double> mAt = transpose ? mat64(h) : mat64T(h);
+    const Mat<double> mA = multi_view(transpose ? mat64T(h) : mat64(h), h->sched.spmv64);
+    const Mat<double> mAt = multi_view(transpose ? mat64(h) : mat64T(h), h->sched.spmv64);
     return run_multi<NR, double>(h, method, mA, mAt, a.max_steps, X, B, h->vec.as<double>(), mb,
                                  0u);
 }
@@ -667,9 +715,9 @@ int solve_multi_mixed_nr(gis_context* h, int method, const MultiMixedArgs& a, do
     const int N = h->N;
     const size_t n = (size_t)N;
     const dim3 g = tiles_grid(N);
-    const Mat<double> m64 = mat64(h);
-    const Mat<float> m32 = mat32(h);
-    const Mat<float> m32T = mat32T(h);
+    const Mat<double> m64 = multi_view(mat64(h), h->sched.spmv64);
+    const Mat<float> m32 = multi_view(mat32(h), h->sched.spmv32);
+    const Mat<float> m32T = multi_view(mat32T(h), h->sched.spmv32);
     float* base = h->vec.as<float>();
     float* R32 = base + 6 * n * NR;
     float* D = base + 7 * n * NR;

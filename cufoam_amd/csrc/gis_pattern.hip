@@ -550,7 +550,6 @@ int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua
         GIS_HIP(h, hipMemcpyAsync(counts, ucount.as<int>() + S + 1, sizeof(counts),
                                   hipMemcpyDeviceToHost, h->stream));
         GIS_HIP(h, hipStreamSynchronize(h->stream));
-        h->selld_full = (counts[0] == 0);
         h->lay_slices = S;
         h->lay_table_slices = S - counts[0];
         h->lay_table_entries = n_tab;

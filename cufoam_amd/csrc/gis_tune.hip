@@ -282,9 +282,16 @@ int device_autotune(gis_context* h, int what, int iterations, int* spmv, int* di
 
 void pattern_schedule(gis_context* h) {
     Schedule& s = h->sched;
-    const SpmvKind k = h->selld_full ? SpmvKind::SellD : SpmvKind::Sell;
+    // SellD when the offset tables cover all but a sliver of the matrix (the
+    // slices without a table read int32 columns inside the same kernels; a
+    // batch's padding slices are such slices)
+    const bool d = h->lay_table_slices > 0 && h->lay_untabled_slots * 20 <= h->n_slots;
+    const SpmvKind k = d ? SpmvKind::SellD : SpmvKind::Sell;
     s.spmv64 = s.spmv32 = k;
-    s.fp64_direction = s.fp32_direction = h->selld_full ? 1 : 0;
+    // FP64: the separate direction pass; FP32: the direction formed at the
+    // gathers (the forms that measured fastest on structured meshes)
+    s.fp64_direction = d ? 1 : 0;
+    s.fp32_direction = 0;
     s.coarsen64 = s.coarsen32 = 1;
     std::lock_guard<std::mutex> lk(g_cache_mu);
     for (int what = 1; what <= 2; ++what) {

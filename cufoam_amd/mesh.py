@@ -223,6 +223,27 @@ def random_ldu(n_cells: int, n_faces: int, seed: int = 0, symmetric: bool = Fals
                      meta={"random": True, "seed": seed, "symmetric": symmetric})
 
 
+def renumber(sys: LduSystem, perm: np.ndarray) -> LduSystem:
+    """The same system with cell c renamed perm[c] (an unstructured numbering of a
+    structured mesh): faces re-oriented so owner < neighbour (upper and lower
+    swap with the orientation), then sorted by owner and neighbour as OpenFOAM
+    orders them; diag, b and x0 move with their cells."""
+    perm = np.asarray(perm, np.int64)
+    a = perm[sys.lower_addr]
+    c = perm[sys.upper_addr]
+    flip = a > c
+    own = np.where(flip, c, a)
+    nbr = np.where(flip, a, c)
+    up = np.where(flip, sys.lower, sys.upper)   # A(own, nbr)
+    lo = np.where(flip, sys.upper, sys.lower)   # A(nbr, own)
+    order = np.lexsort((nbr, own))
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.shape[0])
+    return LduSystem(sys.n_cells, own[order].astype(np.int32), nbr[order].astype(np.int32),
+                     sys.diag[inv], lo[order], up[order], sys.b[..., inv], sys.x0[..., inv],
+                     meta=dict(sys.meta, renumbered=True))
+
+
 def batch_edges(count: int = 64, lo: int = 16, hi: int = 64, seed: int = 0) -> np.ndarray:
     """Mesh edges of an uneven batch, n_s ~ U{lo..hi}."""
     rng = np.random.default_rng(seed)

@@ -167,8 +167,9 @@ typedef struct gis_schedule {
 /* Negative fields leave the current value unchanged.  gis_set_pattern (and
  * gis_set_interfaces) start every new pattern on a schedule chosen without
  * timing: the autotuner's cached choice for (device, N, nnz) if there is one,
- * else schedule 6 with the separate direction pass (form 1) when every 64-row
- * slice has a column-offset table (structured meshes), else schedule 0. */
+ * else schedule 6 (FP64 direction form 1, FP32 form 0) when the slices
+ * with a column-offset table hold at least 95% of the sliced-row slots
+ * (structured meshes, batches of them), else schedule 0. */
 GIS_API int gis_set_schedule(gis_handle h, const gis_schedule *s);
 GIS_API int gis_get_schedule(gis_handle h, gis_schedule *s);
 

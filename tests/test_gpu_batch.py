@@ -17,11 +17,14 @@ def _systems(edges, seed0=0, shift=1e-3):
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_bit_exact_vs_oracle(gpu, use_graph):
+@pytest.mark.parametrize("schedule", [None, 6, 0])
+def test_batch_bit_exact_vs_oracle(gpu, use_graph, schedule):
     edges = [2, 5, 3, 12, 1, 9, 7]
     systems = _systems(edges, 10)
-    print(f"[case] batch edges={edges} graph={use_graph}", flush=True)
+    print(f"[case] batch edges={edges} graph={use_graph} schedule={schedule}", flush=True)
     gpu.batch_set_systems(systems)
+    if schedule is not None:
+        gpu.set_schedule(spmv64=schedule)
     b = np.concatenate([s.b for s in systems])
     err = np.array([1e-9 * np.linalg.norm(s.b) for s in systems])
     for rep in range(3):

@@ -25,9 +25,13 @@ def _case(n, seed, symmetric, shift):
 @pytest.mark.parametrize("nrhs", [1, 2, 3, 5, 8])
 @pytest.mark.parametrize("method", ["CG", "BiCG"])
 @pytest.mark.parametrize("transpose", [False, True])
-def test_multi_columns_equal_single_solves(gpu, nrhs, method, transpose):
+@pytest.mark.parametrize("schedule", [6, 0])
+def test_multi_columns_equal_single_solves(gpu, nrhs, method, transpose, schedule):
+    """Both layouts of the SpMM: column-offset tables (the default on this mesh)
+    and int32 slot columns."""
     s = _case(12, nrhs, method == "CG", 0.05)
     gpu.set_system(s)
+    gpu.set_schedule(spmv64=schedule, spmv32=schedule)
     A = O.ldu2csr(s)
     At = O.transpose(A)
     rng = np.random.default_rng(nrhs)

@@ -51,7 +51,8 @@ def test_autotune_rejects_bad_targets(gpu):
 
 def test_pattern_picks_its_schedule_without_timing(gpu):
     """gis_set_pattern starts a structured mesh (every 64-row slice tabled) on
-    SellD with the separate direction pass, an unstructured one on sliced rows;
+    SellD (FP64 with the separate direction pass, FP32 with the direction formed
+    at the gathers), an unstructured one on sliced rows;
     gis_get_layout reports the counts the SellD byte model uses."""
     from cufoam_amd import mesh
     from cufoam_amd.gis import Gis
@@ -59,7 +60,7 @@ def test_pattern_picks_its_schedule_without_timing(gpu):
     s = mesh.laplacian(32, seed=0, shift=1e-3)
     gpu.set_system(s)
     sch = gpu.get_schedule()
-    assert (sch["spmv64"], sch["spmv32"], sch["fp64_direction"], sch["fp32_direction"]) == (6, 6, 1, 1)
+    assert (sch["spmv64"], sch["spmv32"], sch["fp64_direction"], sch["fp32_direction"]) == (6, 6, 1, 0)
     lay = gpu.layout()
     assert lay["n_rows"] == 32768 and lay["nnz"] == s.nnz and lay["slices"] == 512
     assert lay["table_slices"] == 512 and lay["untabled_slots"] == 0

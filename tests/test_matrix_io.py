@@ -150,3 +150,22 @@ def test_rb_matrix_through_ldu_is_the_same_csr(name):
     np.testing.assert_array_equal(C.rowptr, A.row_offsets)
     np.testing.assert_array_equal(C.col, A.column_indices)
     np.testing.assert_array_equal(C.val, A.values)
+
+
+def test_renumbered_mesh_is_the_permuted_matrix():
+    """mesh.renumber: A' = P A P^T with the faces back in OpenFOAM order."""
+    from cufoam_amd import mesh
+    from oracle import oracle as O
+    s = mesh.laplacian(5, seed=3, shift=0.2, symmetric=False)
+    perm = np.random.default_rng(4).permutation(s.n_cells)
+    r = mesh.renumber(s, perm)
+    assert np.all(r.lower_addr < r.upper_addr)
+    key = r.lower_addr.astype(np.int64) * s.n_cells + r.upper_addr
+    assert np.all(np.diff(key) > 0)
+    D = mesh.ldu_to_dense(s)
+    P = np.zeros((s.n_cells, s.n_cells))
+    P[perm, np.arange(s.n_cells)] = 1.0
+    np.testing.assert_array_equal(mesh.ldu_to_dense(r), P @ D @ P.T)
+    np.testing.assert_array_equal(r.b, P @ s.b)
+    A = O.ldu2csr(r)
+    np.testing.assert_allclose(A.scipy() @ (P @ s.b), P @ (D @ s.b), rtol=1e-13)

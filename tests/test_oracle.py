@@ -351,6 +351,44 @@ def test_minimal_residual_monotone(m):
     assert np.all(np.diff(res.hist) <= 1e-12 * res.hist[0])
 
 
+@pytest.mark.parametrize("m", ["MR", "MRJacobi", "MRS", "JMR"])
+def test_minimal_residual_step_is_the_least_squares_minimiser(m):
+    """Each step of the MR family moves x along its direction d by the omega that
+    minimises the (scaled) residual norm over the line: omega* = (r, W) / (W, W)
+    with W the residual's change per unit step (SolverSingleGPU.h:825-890,
+    :1000-1073, :1079-1155, :1236-1303), recomputed here in numpy from the true
+    residual.  A wrong numerator, denominator, sign or direction fails it."""
+    s = mesh.laplacian(7, seed=12, shift=0.2, symmetric=False)
+    A = O.ldu2csr(s)
+    As = A.scipy()
+    D = s.diag
+    S = 1.0 / np.sqrt(D)
+    x0 = s.x0.copy()
+    for k in range(1, 7):
+        x1 = O.solve(m, A, s.b, s.x0, 0.0, k).x
+        r = s.b - As @ x0
+        if m == "MR":
+            rr, d = r, r
+            W = As @ d
+        elif m in ("MRJacobi", "MRS"):
+            rr, d = r, r / D
+            W = As @ d
+        else:   # JMR: residual scaled by D^-1/2, direction D^-1/2 of it
+            rr = S * r
+            d = S * rr
+            W = S * (As @ d)
+        omega_star = (rr @ W) / (W @ W)
+        step = x1 - x0
+        omega = (step @ d) / (d @ d)
+        assert abs(omega - omega_star) <= 1e-9 * abs(omega_star), (k, omega, omega_star)
+        assert np.linalg.norm(step - omega * d) <= 1e-10 * np.linalg.norm(step), k
+        # and it is a minimum: nearby steps along d leave a larger residual
+        base = np.linalg.norm(rr - omega_star * W)
+        for e in (1e-3, -1e-3):
+            assert np.linalg.norm(rr - omega_star * (1 + e) * W) > base
+        x0 = x1
+
+
 def test_richardson_power_estimate():
     s = mesh.laplacian(6, seed=11, shift=0.1)
     A = O.ldu2csr(s)
