@@ -235,6 +235,26 @@ struct SlotReduce {
         }
     }
 
+    // begin() for a kernel that formed its tiles' subtree sums itself (thread 0
+    // holds them, e.g. pair_tile_sum): publish them as begin() would
+    __device__ void begin_sums(const RedWs& red, int tile0, int nt, const double (&v)[C][NV]) {
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) flat[c * NV + i] = v[c][i];
+        last_tile = nt > 0 ? tile0 + nt - 1 : -1;
+        if (red.count[0] != 1 && threadIdx.x == 0) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                if (c < nt) {
+#pragma unroll
+                    for (int i = 0; i < NV; ++i)
+                        st_agent(red.slot + (size_t)(tile0 + c) * NV + i, flat[c * NV + i]);
+                }
+            }
+        }
+    }
+
     // st[0 .. nst) are marked failed if a wait runs out
     template <class Fin>
     __device__ void end(const RedWs& red, KState* st, RedShared& sh, Fin&& fin, int nst = 1) {
@@ -972,6 +992,48 @@ __device__ __forceinline__ void store_pairs(double* __restrict__ p, int row0, in
             p[g] = o[2 * h];
         }
     }
+}
+
+// The tile's subtree sum of squares sum v_i^2 for values held in the pair
+// layout of load_pairs (thread t: rows 2t, 2t+1 and 512+2t, 513+2t), in the
+// canonical order: a leaf (rows 4j..4j+3, accumulated from +0.0, rows >= n
+// skipped) is started by the even lane of a pair and finished by the odd one;
+// each wave's 32 leaves of a half tile are folded pairwise across the odd
+// lanes; the halves' wave sums are combined ((w0 + w1) + (w2 + w3)) and the two
+// halves added.  These are the bits block_tree gives for per-thread leaves of
+// four rows.  The result is in thread 0; every thread of the workgroup calls it.
+__device__ __forceinline__ double pair_tile_sumsq(const double (&v)[4], int row0, int n,
+                                                  RedShared& sh) {
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    double half[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int g = row0 + 2 * (int)threadIdx.x + h * (kTile / 2);
+        double a = 0.0;
+        if (g < n) a = fma(v[2 * h], v[2 * h], a);
+        if (g + 1 < n) a = fma(v[2 * h + 1], v[2 * h + 1], a);
+        const double first = __shfl_xor(a, 1, 64);   // the even lane's two rows
+        double leaf = first;
+        if (g < n) leaf = fma(v[2 * h], v[2 * h], leaf);
+        if (g + 1 < n) leaf = fma(v[2 * h + 1], v[2 * h + 1], leaf);
+#pragma unroll
+        for (int m = 2; m < 64; m <<= 1) leaf = leaf + __shfl_xor(leaf, m, 64);
+        half[h] = leaf;   // in the odd lanes: the sum of this wave's 32 leaves
+    }
+    if (lane == 1) {
+        sh.w[w][0] = half[0];
+        sh.w[w][1] = half[1];
+    }
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        const double s0 = (sh.w[0][0] + sh.w[1][0]) + (sh.w[2][0] + sh.w[3][0]);
+        const double s1 = (sh.w[0][1] + sh.w[1][1]) + (sh.w[2][1] + sh.w[3][1]);
+        s = s0 + s1;
+    }
+    __syncthreads();   // sh.w is reused by the reduction that follows
+    return s;
 }
 
 __device__ __forceinline__ void load4(const float* __restrict__ p, int i0, int n, float (&o)[4]) {

@@ -207,27 +207,24 @@ __global__ __launch_bounds__(kWG) void k_cg2(int N, double* __restrict__ r,
     __shared__ RedShared shr;
     const SuperTile<C> sup(N, true);
     const double alpha = st->alpha;
-    double leaf[C][1];
-    double rv[C][4];
+    // rows in the pair layout (whole cache lines per access); the tile's sum of
+    // squares in the canonical order by pair_tile_sumsq
+    double tsum[C][1];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-        leaf[c][0] = 0.0;
+        tsum[c][0] = 0.0;
         if (c >= sup.nt) continue;
-        const TileRows t(N, sup.tile0 + c);
-        double qv[4];
-        load4(r, t.i0, N, rv[c]);
-        load4(q, t.i0, N, qv);
+        const int row0 = (sup.tile0 + c) * kTile;
+        double rv[4], qv[4];
+        load_pairs(r, row0, N, rv);
+        load_pairs(q, row0, N, qv);
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-            rv[c][kk] = fma(-alpha, qv[kk], rv[c][kk]);
-            if (t.i0 + kk < N) leaf[c][0] = fma(rv[c][kk], rv[c][kk], leaf[c][0]);
-        }
+        for (int kk = 0; kk < 4; ++kk) rv[kk] = fma(-alpha, qv[kk], rv[kk]);
+        store_pairs(r, row0, N, rv);
+        tsum[c][0] = pair_tile_sumsq(rv, row0, N, shr);
     }
     SlotReduce<1, C> tr;
-    tr.begin(red, sup.tile0, sup.nt, leaf, shr);
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-        if (c < sup.nt) store4(r, TileRows(N, sup.tile0 + c).i0, N, rv[c]);
+    tr.begin_sums(red, sup.tile0, sup.nt, tsum);
     tr.end(red, st, shr, [&](const double (&v)[1]) {
         st->rho1 = st->rho0;
         st->rho0 = v[0];
