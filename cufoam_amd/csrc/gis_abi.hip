@@ -560,7 +560,7 @@ static int pcg_impl(gis_handle h, int precond, double* x, const double* b,
                     const gis_params* params, double* hist, int hist_cap, int* hist_len,
                     bool dev) {
     if (!h || !x || !b) return GIS_ERR_ARG;
-    if (precond < GIS_PRECOND_NONE || precond > GIS_PRECOND_GAMG32)
+    if (precond < GIS_PRECOND_NONE || precond > GIS_PRECOND_DIC)
         return fail(h, GIS_ERR_ARG, "unknown preconditioner");
     if (!detail_ok(params)) return fail(h, GIS_ERR_ARG, "detail_type must be ERROR or TIME");
     if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
@@ -873,7 +873,7 @@ int foam_solve_impl(gis_context* h, int method, double* psi, const double* sourc
                     double tolerance, double rel_tol, long long min_iter, long long max_iter,
                     gis_performance* perf) {
     if (!h || !psi || !source) return GIS_ERR_ARG;
-    if (method < GIS_CG || method > 8 + GIS_PRECOND_GAMG32)
+    if (method < GIS_CG || method > 8 + GIS_PRECOND_DIC)
         return fail(h, GIS_ERR_ARG, "unknown method");
     if (!(rel_tol >= 0.0) || min_iter < 0)
         return fail(h, GIS_ERR_ARG, "rel_tol and min_iter must be non-negative");
@@ -915,7 +915,7 @@ GIS_API int gis_foam_solve(gis_handle h, int method, double* psi, const double* 
 GIS_API int gis_foam_solve_pcg(gis_handle h, int precond, double* psi, const double* source,
                                double tolerance, double rel_tol, long long min_iter,
                                long long max_iter, gis_performance* perf) {
-    if (h && (precond < GIS_PRECOND_NONE || precond > GIS_PRECOND_GAMG32))
+    if (h && (precond < GIS_PRECOND_NONE || precond > GIS_PRECOND_DIC))
         return fail(h, GIS_ERR_ARG, "unknown preconditioner");
     return foam_solve_impl(h, 8 + precond, psi, source, tolerance, rel_tol, min_iter, max_iter,
                            perf);

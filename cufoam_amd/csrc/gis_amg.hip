@@ -750,7 +750,8 @@ __global__ __launch_bounds__(kWG) void k_pcg_dir(int N, const double* __restrict
 }
 
 // r = fma(-alpha, q, r); ||r||, history, stop test.  PC 0: rho = (r, r);
-// PC 1: z = dinv r, rho = (r, z); PC 2, 3: rho is left to the V-cycle.
+// PC 1: z = dinv r, rho = (r, z); PC 2, 3: rho is left to the V-cycle;
+// PC 4: rho is left to the DIC sweeps (k_pcg_rz).
 // PC 2 also writes the V-cycle's first sweep from zero, z = sm .* r (dinv = sm);
 // PC 3 writes r32 = (float) r and z32 = sm32 .* r32.
 template <int C, int PC>
@@ -1210,11 +1211,203 @@ void AmgData::clear() {
     built = false;
     values_ok = false;
     values32_ok = false;
+    dic_levels = false;
+    dic_values = false;
+    dic_ptr_f.clear();
+    dic_ptr_b.clear();
+    dic_cells_f.release();
+    dic_cells_b.release();
+    dic_rd.release();
+}
+
+// ---------------------------------------------------------------------------
+// DIC, OpenFOAM's DICPreconditioner restated from its published algorithm (the
+// reference has no preconditioners, README.md:71-73):
+//   rD = diag; for each face f in face order: rD[u] -= upper[f] upper[f] / rD[l];
+//   rD = 1 / rD;
+//   z = rD .* r; for f in face order: z[u] -= rD[u] upper[f] z[l];
+//   for f in reverse face order: z[l] -= rD[l] upper[f] z[u].
+// The face loops are sequential; here cells run in dependency levels (a cell's
+// forward level is one more than its lower neighbours', its backward level one
+// more than its upper neighbours'), one launch per level, and each cell applies
+// its own faces in the order the face loop would: its lower faces in face
+// order (the CSR row's lower entries, perm codes f), its upper faces in reverse
+// face order.  Every value is then formed by the same operations in the same
+// order as the face loops form it.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlk) void k_dic_factor(int ncell, const int* __restrict__ cells,
+                                                     const int* __restrict__ rowptr,
+                                                     const int* __restrict__ nlow,
+                                                     const int* __restrict__ col,
+                                                     const int* __restrict__ perm,
+                                                     const double* __restrict__ diag,
+                                                     const double* __restrict__ upper,
+                                                     double* __restrict__ F) {
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    if (i >= ncell) return;
+    const int c = cells[i];
+    double f = diag[c];
+    const int a = rowptr[c];
+    const int e = a + nlow[c];
+    for (int p = a; p < e; ++p) {
+        const double u = upper[perm[p]];
+        f = f - u * u / F[col[p]];
+    }
+    F[c] = f;
+}
+
+__global__ __launch_bounds__(kBlk) void k_dic_fwd(int ncell, const int* __restrict__ cells,
+                                                  const int* __restrict__ rowptr,
+                                                  const int* __restrict__ nlow,
+                                                  const int* __restrict__ col,
+                                                  const int* __restrict__ perm,
+                                                  const double* __restrict__ rD,
+                                                  const double* __restrict__ upper,
+                                                  const double* __restrict__ r,
+                                                  double* __restrict__ z, const KState* st) {
+    if (st->done) return;
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    if (i >= ncell) return;
+    const int c = cells[i];
+    const double d = rD[c];
+    double w = d * r[c];
+    const int a = rowptr[c];
+    const int e = a + nlow[c];
+    for (int p = a; p < e; ++p) w = w - d * upper[perm[p]] * z[col[p]];
+    z[c] = w;
+}
+
+__global__ __launch_bounds__(kBlk) void k_dic_bwd(int ncell, const int* __restrict__ cells,
+                                                  const int* __restrict__ rowptr,
+                                                  const int* __restrict__ nlow,
+                                                  const int* __restrict__ col,
+                                                  const int* __restrict__ perm, int base,
+                                                  const double* __restrict__ rD,
+                                                  const double* __restrict__ upper,
+                                                  double* __restrict__ z, const KState* st) {
+    if (st->done) return;
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    if (i >= ncell) return;
+    const int c = cells[i];
+    const double d = rD[c];
+    double w = z[c];
+    const int b0 = rowptr[c] + nlow[c] + 1;   // after the diagonal: upper faces, face order
+    for (int p = rowptr[c + 1] - 1; p >= b0; --p) w = w - d * upper[perm[p] - base] * z[col[p]];
+    z[c] = w;
+}
+
+// rho = (r, z) after a DIC application; first: the set-up's rho, else also beta
+template <int FIRST>
+__global__ __launch_bounds__(kWG) void k_pcg_rz(int N, const double* __restrict__ r,
+                                                const double* __restrict__ z, KState* st,
+                                                RedWs red) {
+    if (st->done) return;
+    __shared__ RedShared shr;
+    StreamRows t(N);
+    double rv[4], zv[4];
+    load4(r, t.i0, N, rv);
+    load4(z, t.i0, N, zv);
+    double leaf[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (t.i0 + k < N) leaf[0] = fma(rv[k], zv[k], leaf[0]);
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) {
+        if (FIRST) {
+            st->rho0 = v[0];
+        } else {
+            st->rho1 = st->rho0;
+            st->rho0 = v[0];
+            st->beta = st->rho0 / st->rho1;
+        }
+    });
+}
+
+// dependency levels of both sweeps from the face addressing (one pass each:
+// faces are in owner order, so a face's lower cell is final when it is met
+// going forward, its upper cell going backward), cells bucketed by level
+int dic_levels(gis_context* h) {
+    AmgData& A = *h->amg;
+    const int N = h->N, M = h->M;
+    std::vector<int> la, ua;
+    GIS_TRY(download(h, la, h->lower_addr, (size_t)M));
+    GIS_TRY(download(h, ua, h->upper_addr, (size_t)M));
+    GIS_HIP(h, hipStreamSynchronize(h->stream));
+    std::vector<int> lf(N, 0), lb(N, 0);
+    for (int f = 0; f < M; ++f) lf[ua[f]] = std::max(lf[ua[f]], lf[la[f]] + 1);
+    for (int f = M - 1; f >= 0; --f) lb[la[f]] = std::max(lb[la[f]], lb[ua[f]] + 1);
+    auto group = [&](const std::vector<int>& lev, std::vector<int>& ptr, DevMem& cells) -> int {
+        const int nl = N > 0 ? 1 + *std::max_element(lev.begin(), lev.end()) : 0;
+        std::vector<int> all(N), list;
+        for (int i = 0; i < N; ++i) all[i] = i;
+        bucket(nl, lev, all, ptr, list);
+        return upload_vec(h, cells, list);
+    };
+    GIS_TRY(group(lf, A.dic_ptr_f, A.dic_cells_f));
+    GIS_TRY(group(lb, A.dic_ptr_b, A.dic_cells_b));
+    A.dic_levels = true;
+    return kOK;
+}
+
+int dic_prepare(gis_context* h) {
+    if (h->n_cpl > 0)
+        return fail(h, kErrState, "DIC does not take interface entries (gis_set_interfaces)");
+    if (h->batch.ready) return fail(h, kErrState, "DIC needs a single-system pattern");
+    if (!h->amg) {
+        h->amg = new (std::nothrow) AmgData();
+        if (!h->amg) return fail(h, kErrNoMem, "preconditioner data");
+    }
+    AmgData& A = *h->amg;
+    if (!A.dic_levels) {
+        GIS_TRY(dic_levels(h));
+        A.dic_values = false;
+    }
+    if (!A.dic_values) {
+        const int N = h->N;
+        GIS_HIP(h, A.dic_rd.ensure(sizeof(double) * (size_t)N));
+        GIS_HIP(h, h->dinv.ensure(sizeof(double) * (size_t)N));
+        double* F = h->dinv.as<double>();   // the factored diagonal before its reciprocal
+        const int nl = (int)A.dic_ptr_f.size() - 1;
+        for (int l = 0; l < nl; ++l) {
+            const int n = A.dic_ptr_f[l + 1] - A.dic_ptr_f[l];
+            hipLaunchKernelGGL(k_dic_factor, grid_of(n), dim3(kBlk), 0, h->stream, n,
+                               A.dic_cells_f.as<int>() + A.dic_ptr_f[l], h->rowptr.as<int>(),
+                               h->nbr_cnt.as<int>(), h->col.as<int>(), h->perm.as<int>(),
+                               h->diag.as<double>(), h->upper.as<double>(), F);
+        }
+        hipLaunchKernelGGL(k_recip, grid_of(N), dim3(kBlk), 0, h->stream, N, F,
+                           A.dic_rd.as<double>());
+        GIS_HIP(h, hipGetLastError());
+        A.dic_values = true;
+    }
+    return kOK;
+}
+
+// z = M^-1 r by the two sweeps, level by level
+int dic_apply(gis_context* h, const double* r, double* z) {
+    AmgData& A = *h->amg;
+    const int nf = (int)A.dic_ptr_f.size() - 1, nb = (int)A.dic_ptr_b.size() - 1;
+    for (int l = 0; l < nf; ++l) {
+        const int n = A.dic_ptr_f[l + 1] - A.dic_ptr_f[l];
+        hipLaunchKernelGGL(k_dic_fwd, grid_of(n), dim3(kBlk), 0, h->stream, n,
+                           A.dic_cells_f.as<int>() + A.dic_ptr_f[l], h->rowptr.as<int>(),
+                           h->nbr_cnt.as<int>(), h->col.as<int>(), h->perm.as<int>(),
+                           A.dic_rd.as<double>(), h->upper.as<double>(), r, z, h->st);
+    }
+    for (int l = 0; l < nb; ++l) {
+        const int n = A.dic_ptr_b[l + 1] - A.dic_ptr_b[l];
+        hipLaunchKernelGGL(k_dic_bwd, grid_of(n), dim3(kBlk), 0, h->stream, n,
+                           A.dic_cells_b.as<int>() + A.dic_ptr_b[l], h->rowptr.as<int>(),
+                           h->nbr_cnt.as<int>(), h->col.as<int>(), h->perm.as<int>(),
+                           h->M + h->N, A.dic_rd.as<double>(), h->upper.as<double>(), z, h->st);
+    }
+    GIS_HIP(h, hipGetLastError());
+    return kOK;
 }
 
 int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const double* b) {
-    if (pc < 0 || pc > 3) return fail(h, kErrArg, "unknown preconditioner");
-    if (pc >= 2) GIS_TRY(amg_prepare(h));
+    if (pc < 0 || pc > 4) return fail(h, kErrArg, "unknown preconditioner");
+    if (pc == 2 || pc == 3) GIS_TRY(amg_prepare(h));
+    if (pc == 4) GIS_TRY(dic_prepare(h));
     if (pc == 3 && !h->amg->values32_ok) GIS_TRY(amg_values32(h));
     GIS_TRY(prepare_solve(h, a, 5));
     const int N = h->N;
@@ -1258,6 +1451,10 @@ int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const doubl
                                h->st);
             (void)vcycle<P, float>(h, 0, r32, z, r);
         }
+        if (pc == 4) {
+            GIS_TRY(dic_apply(h, r, z));
+            hipLaunchKernelGGL(k_pcg_rz<1>, g, dim3(kWG), 0, h->stream, N, r, z, h->st, h->red);
+        }
         GIS_HIP(h, hipGetLastError());
         GIS_COARSEN_DISPATCH(h->sched.coarsen64, CC, {
             const dim3 gc = super_grid(N, CC);
@@ -1276,13 +1473,19 @@ int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const doubl
                     hipLaunchKernelGGL((k_pcg2<CC, 2>), gc, dim3(kWG), 0, h->stream, N, r, q,
                                        dinv, z0, sm32, r32, z32, h->st, h->red);
                     (void)vcycle<P, double>(h, 0, r, z, r);
-                } else {
+                } else if (pc == 3) {
                     hipLaunchKernelGGL((k_pcg2<CC, 3>), gc, dim3(kWG), 0, h->stream, N, r, q,
                                        dinv, z, sm32, r32, z32, h->st, h->red);
                     (void)vcycle<P, float>(h, 0, r32, z, r);
+                } else {
+                    hipLaunchKernelGGL((k_pcg2<CC, 4>), gc, dim3(kWG), 0, h->stream, N, r, q,
+                                       dinv, z, sm32, r32, z32, h->st, h->red);
+                    GIS_TRY(dic_apply(h, r, z));
+                    hipLaunchKernelGGL(k_pcg_rz<0>, g, dim3(kWG), 0, h->stream, N, r, z, h->st,
+                                       h->red);
                 }
                 return kOK;
-            }, pc == 2));
+            }, pc == 2 || pc == 4));
         });
     });
     hipLaunchKernelGGL(k_cg_flush, g, dim3(kWG), 0, h->stream, N, x, pb0, pb1, h->st);

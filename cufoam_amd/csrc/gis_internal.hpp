@@ -135,6 +135,12 @@ struct AmgData {
     DevMem tail, tail32;          // per-level pointers of the single-workgroup tail
     int tail_start = 0;           // first tail level (0: none)
     int tail_levels = 0;
+    // DIC: cells grouped by dependency level of the forward (lower faces) and
+    // backward (upper faces) sweeps, and the reciprocal factored diagonal rD
+    bool dic_levels = false;      // level lists of the current pattern
+    bool dic_values = false;      // rD of the current values
+    std::vector<int> dic_ptr_f, dic_ptr_b;   // level offsets (host: launch sizes)
+    DevMem dic_cells_f, dic_cells_b, dic_rd;
     void clear();
     ~AmgData();
 };

@@ -620,7 +620,8 @@ int device_solve(gis_context* h, const SolveArgs& a, double* d_x, const double* 
         case 8:
         case 9:
         case 10:
-        case 11: st = solve_pcg(h, a, a.method - 8, d_x, d_b); break;
+        case 11:
+        case 12: st = solve_pcg(h, a, a.method - 8, d_x, d_b); break;
         default: return fail(h, kErrArg, "unknown solver method");
     }
     if (st != kOK) return st;

@@ -603,7 +603,10 @@ static int gather_both(gis_context* h, bool transpose, DevMem& csr, DevMem& sell
 int gather_values(gis_context* h) {
     if (!h->has_pattern) return fail(h, kErrState, "pattern not set");
     GIS_TRY(gather_both<double>(h, false, h->val, h->sval, h->dval));
-    if (h->amg) h->amg->values_ok = false;
+    if (h->amg) {
+        h->amg->values_ok = false;
+        h->amg->dic_values = false;
+    }
     h->has_values = true;
     h->has_val32 = false;
     h->has_valT = false;

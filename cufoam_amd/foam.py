@@ -108,8 +108,8 @@ class cuFoamSolver:
             raise FatalError(f"convergence must be reference or openfoam, not "
                              f"{self.convergence}")
         self.preconditioner = str(controls.get("preconditioner", "none"))
-        if self.preconditioner not in ("none", "diagonal", "GAMG", "GAMG32"):
-            raise FatalError(f"preconditioner must be none, diagonal, GAMG or GAMG32, not "
+        if self.preconditioner not in ("none", "diagonal", "DIC", "GAMG", "GAMG32"):
+            raise FatalError(f"preconditioner must be none, diagonal, DIC, GAMG or GAMG32, not "
                              f"{self.preconditioner}")
         self.couple = str(controls.get("interfaces", "ignored"))
         if self.couple not in ("ignored", "coupled"):

@@ -23,7 +23,7 @@ ERR_SYNC = 13   # a device reduction did not complete: an error, raised
 
 METHODS = {"CG": 0, "BiCG": 1, "Jacobi": 2, "MR": 3, "MRJacobi": 4, "JMR": 5, "MRS": 6,
            "Richardson": 7}
-PRECONDITIONERS = {"none": 0, "diagonal": 1, "GAMG": 2, "GAMG32": 3}
+PRECONDITIONERS = {"none": 0, "diagonal": 1, "GAMG": 2, "GAMG32": 3, "DIC": 4}
 
 
 class GisError(RuntimeError):

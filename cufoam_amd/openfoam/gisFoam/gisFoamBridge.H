@@ -159,7 +159,7 @@ inline Outcome solve(int method, const Addressing& a, const Coefficients& c, dou
     return o;
 }
 
-// Preconditioned CG (GIS_PRECOND_NONE / DIAGONAL / GAMG / GAMG32) with the same
+// Preconditioned CG (GIS_PRECOND_NONE / DIAGONAL / DIC / GAMG / GAMG32) with the same
 // sequence: gis_foam_solve_pcg, relTol / minIter applied when `openfoam` is set.
 inline Outcome solvePcg(int precond, const Addressing& a, const Coefficients& c, double* psi,
                         const double* source, const Controls& ctl) {

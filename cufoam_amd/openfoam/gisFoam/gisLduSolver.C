@@ -20,8 +20,9 @@ int Foam::gisLduSolver::preconditionerId() const {
     if (name == "diagonal") return GIS_PRECOND_DIAGONAL;
     if (name == "GAMG") return GIS_PRECOND_GAMG;
     if (name == "GAMG32") return GIS_PRECOND_GAMG32;
+    if (name == "DIC") return GIS_PRECOND_DIC;
     FatalErrorInFunction << "unknown preconditioner " << name
-                         << " (valid: none diagonal GAMG GAMG32)" << abort(FatalError);
+                         << " (valid: none diagonal DIC GAMG GAMG32)" << abort(FatalError);
     return GIS_PRECOND_NONE;
 }
 

@@ -11,7 +11,7 @@
  *                maxIter only, as the reference wrappers; openfoam: relTol
  *                and minIter too, through gis_foam_solve_controls)
  *   device       GPU index (default 0)
- *   preconditioner  none | diagonal | GAMG | GAMG32  (cuFoamPCG only)
+ *   preconditioner  none | diagonal | DIC | GAMG | GAMG32  (cuFoamPCG only)
  */
 #ifndef GIS_LDU_SOLVER_H
 #define GIS_LDU_SOLVER_H

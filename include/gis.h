@@ -281,6 +281,13 @@ GIS_API int gis_set_coupled_values_device(gis_handle h, const double *d_diag,
  * rounded to float, the result widened); the Krylov loop, residuals and
  * reductions stay FP64, so the attainable residual is unchanged. */
 #define GIS_PRECOND_GAMG32 3
+/* OpenFOAM's DIC (diagonal incomplete Cholesky) from its published algorithm:
+ * rD = 1 / (diag - sum over lower faces of upper^2 rD_owner) in face order,
+ * z = rD r, a forward sweep over the faces and a backward sweep in reverse face
+ * order.  The sweeps run level by level (cells whose faces are independent),
+ * one launch per level: about 3n launches per application on an n^3 mesh.
+ * Symmetric matrices (it reads upper); no interface entries. */
+#define GIS_PRECOND_DIC 4
 GIS_API int gis_solve_pcg(gis_handle h, int precond, double *x, const double *b,
                           const gis_params *params, double *hist, int hist_cap, int *hist_len);
 GIS_API int gis_solve_pcg_device(gis_handle h, int precond, double *d_x, const double *d_b,

@@ -19,6 +19,11 @@
  *                  z = M r; rho' = (r, z); beta = rho' / rho.
  *   With M = I this is CG, bit for bit.
  * diagonal:  z_i = (1 / a_ii) r_i.
+ * DIC: OpenFOAM's DICPreconditioner, restated from its published algorithm:
+ *   rD = diag; for each face f in face order: rD[u] -= upper[f] * upper[f] / rD[l];
+ *   rD = 1 / rD;
+ *   z = rD .* r; for f in face order: z[u] -= rD[u] * upper[f] * z[l];
+ *   for f in reverse face order: z[l] -= rD[l] * upper[f] * z[u].
  * GAMG:
  *   agglomeration of a level: two pairwise passes (pair_pass) over the face
  *     graph with weights |upper_f| + |lower_f|; each pass visits cells in
@@ -67,7 +72,7 @@
 #define OR_AMG_DENSE 64
 
 enum { OR_OK = 0, OR_ERR_ARG = 1, OR_ERR_NOMEM = 5 };
-enum { PC_NONE = 0, PC_DIAGONAL = 1, PC_GAMG = 2, PC_GAMG32 = 3 };
+enum { PC_NONE = 0, PC_DIAGONAL = 1, PC_GAMG = 2, PC_GAMG32 = 3, PC_DIC = 4 };
 
 /* from gis_oracle.c */
 double or_dot(int64_t n, const double *a, const double *b);
@@ -576,9 +581,34 @@ OR_API int or_amg_vcycle32(void *p, const double *r, double *z) {
 /* ------------------------------------------------------------------------ */
 /* PCG                                                                        */
 /* ------------------------------------------------------------------------ */
-static void precond(int pc, const amg_t *amg, int n, const double *dinv, const double *r,
-                    double *z) {
-    if (pc == PC_NONE) {
+/* DIC: the reciprocal factored diagonal rD (face loop in face order) */
+OR_API void or_dic_rd(int n, int m, const int *la, const int *ua, const double *diag,
+                      const double *upper, double *rD) {
+    for (int i = 0; i < n; ++i) rD[i] = diag[i];
+    for (int f = 0; f < m; ++f) rD[ua[f]] -= upper[f] * upper[f] / rD[la[f]];
+    for (int i = 0; i < n; ++i) rD[i] = 1.0 / rD[i];
+}
+
+/* DIC: z = M^-1 r (forward sweep in face order, backward in reverse face order) */
+OR_API void or_dic_apply(int n, int m, const int *la, const int *ua, const double *rD,
+                         const double *upper, const double *r, double *z) {
+    for (int i = 0; i < n; ++i) z[i] = rD[i] * r[i];
+    for (int f = 0; f < m; ++f) z[ua[f]] -= rD[ua[f]] * upper[f] * z[la[f]];
+    for (int f = m - 1; f >= 0; --f) z[la[f]] -= rD[la[f]] * upper[f] * z[ua[f]];
+}
+
+typedef struct {
+    int m;
+    const int *la, *ua;
+    const double *upper;
+    double *rD;
+} dic_t;
+
+static void precond(int pc, const amg_t *amg, const dic_t *dic, int n, const double *dinv,
+                    const double *r, double *z) {
+    if (pc == PC_DIC) {
+        or_dic_apply(n, dic->m, dic->la, dic->ua, dic->rD, dic->upper, r, z);
+    } else if (pc == PC_NONE) {
         memcpy(z, r, sizeof(double) * (size_t)n);
     } else if (pc == PC_DIAGONAL) {
         for (int i = 0; i < n; ++i) z[i] = dinv[i] * r[i];
@@ -598,7 +628,7 @@ OR_API int or_pcg(int pc, int N, int M, const int *la, const int *ua, const doub
                   const double *lower, const double *upper, const double *b, double *x,
                   double error, int max_steps, double rel_tol, int64_t min_steps, double omega,
                   int nu, double cscale, double *hist, int hist_cap, int *hist_len) {
-    if (pc < PC_NONE || pc > PC_GAMG32 || N < 1) return OR_ERR_ARG;
+    if (pc < PC_NONE || pc > PC_DIC || N < 1) return OR_ERR_ARG;
     amg_t *amg = NULL;
     level_t A;
     memset(&A, 0, sizeof(A));
@@ -623,6 +653,11 @@ OR_API int or_pcg(int pc, int N, int M, const int *la, const int *ua, const doub
     double *r = v, *z = v + N, *p = v + 2 * (size_t)N, *q = v + 3 * (size_t)N;
     double *dinv = v + 4 * (size_t)N;
     for (int i = 0; i < N; ++i) dinv[i] = 1.0 / diag[i];
+    dic_t dic = {M, la, ua, upper, NULL};
+    if (pc == PC_DIC) {
+        dic.rD = xcalloc((size_t)N, sizeof(double));
+        or_dic_rd(N, M, la, ua, diag, upper, dic.rD);
+    }
     or_spmv(N, A.rowptr, A.col, A.val, x, q);
     for (int i = 0; i < N; ++i) r[i] = b[i] - q[i];
     double norm = sqrt(or_dot(N, r, r));
@@ -633,7 +668,7 @@ OR_API int or_pcg(int pc, int N, int M, const int *la, const int *ua, const doub
     const int64_t ms = max_steps <= 0 ? INT32_MAX : max_steps;
     double rho = 0.0, beta = 0.0;
     if ((norm > error) || min_steps > 0) {
-        precond(pc, amg, N, dinv, r, z);
+        precond(pc, amg, &dic, N, dinv, r, z);
         rho = or_dot(N, r, z);
     }
     for (int64_t k = 1; (k <= ms && norm > error) || k <= min_steps; ++k) {
@@ -649,13 +684,14 @@ OR_API int or_pcg(int pc, int N, int M, const int *la, const int *ua, const doub
         len++;
         if (!((k + 1 <= ms && norm > error) || k + 1 <= min_steps)) break;
         const double gamma = pc == PC_GAMG32 ? or_dot(N, r, z) : 0.0;
-        precond(pc, amg, N, dinv, r, z);
+        precond(pc, amg, &dic, N, dinv, r, z);
         const double rho_new = or_dot(N, r, z);
         beta = pc == PC_GAMG32 ? (rho_new - gamma) / rho : rho_new / rho;
         rho = rho_new;
     }
     if (hist_len) *hist_len = len;
     free(v);
+    free(dic.rD);
     free(A.rowptr); free(A.col); free(A.perm); free(A.val);
     or_amg_free(amg);
     return OR_OK;

@@ -366,7 +366,7 @@ def foam_solve(method: str, sys, psi, source, tolerance: float, max_iter: int,
 # ---------------------------------------------------------------------------
 # preconditioned CG (none / diagonal / GAMG), gis_oracle_amg.c
 # ---------------------------------------------------------------------------
-PRECONDITIONERS = {"none": 0, "diagonal": 1, "GAMG": 2, "GAMG32": 3}
+PRECONDITIONERS = {"none": 0, "diagonal": 1, "GAMG": 2, "GAMG32": 3, "DIC": 4}
 GAMG_OMEGA, GAMG_NU, GAMG_CSCALE = 0.8, 1, 1.8
 
 
@@ -406,6 +406,30 @@ def pair_pass(n: int, la, ua, w):
 
 def _z(a, dt):
     return _c(a if len(a) else np.zeros(1, dt), dt)
+
+
+def dic_rd(sys) -> np.ndarray:
+    """DIC's reciprocal factored diagonal (OpenFOAM's calcReciprocalD, face order)."""
+    L = lib()
+    L.or_dic_rd.restype = None
+    L.or_dic_rd.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+    rd = np.zeros(sys.n_cells)
+    L.or_dic_rd(sys.n_cells, sys.n_faces, _z(sys.lower_addr, np.int32),
+                _z(sys.upper_addr, np.int32), _c(sys.diag, np.float64),
+                _z(sys.upper, np.float64), rd)
+    return rd
+
+
+def dic_apply(sys, rd, r) -> np.ndarray:
+    """z = M^-1 r of DIC: z = rD r, forward face sweep, backward face sweep."""
+    L = lib()
+    L.or_dic_apply.restype = None
+    L.or_dic_apply.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p]
+    z = np.zeros(sys.n_cells)
+    L.or_dic_apply(sys.n_cells, sys.n_faces, _z(sys.lower_addr, np.int32),
+                   _z(sys.upper_addr, np.int32), _c(rd, np.float64), _z(sys.upper, np.float64),
+                   _c(r, np.float64), z)
+    return z
 
 
 def pcg(precond: str, sys, b, x0, error: float, max_steps: int, rel_tol: float = 0.0,

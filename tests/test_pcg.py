@@ -206,3 +206,60 @@ def test_fp32_cycle_tracks_the_fp64_cycle_and_converges():
     assert g32.hist[-1] <= err and g32.iterations <= g64.iterations + 3
     A = O.ldu2csr(s).scipy()
     assert np.linalg.norm(s.b - A @ g32.x) <= 2 * err
+
+
+# ---- DIC (OpenFOAM's DICPreconditioner, restated in gis_oracle_amg.c) ----------------
+def _lower_dense(s):
+    """Strict lower triangle of A in dense form (A(u, l) = lower, here = upper)."""
+    Lm = np.zeros((s.n_cells, s.n_cells))
+    Lm[s.upper_addr, s.lower_addr] = s.upper
+    return Lm
+
+
+@pytest.mark.parametrize("n,shift", [(5, 0.1), (7, 1e-3)])
+def test_dic_is_incomplete_cholesky_on_the_pattern(n, shift):
+    """M = (D* + L) D*^-1 (D* + L^T) with D* = 1 / rD: M equals A on A's pattern
+    (diagonal and every face), the defining property of IC(0); and z = M^-1 r
+    for the oracle's sweeps, against a dense solve."""
+    from cufoam_amd import mesh
+    s = mesh.laplacian(n, seed=3, shift=shift)
+    rd = O.dic_rd(s)
+    Ds = np.diag(1.0 / rd)
+    Lm = _lower_dense(s)
+    M = (Ds + Lm) @ np.diag(rd) @ (Ds + Lm.T)
+    A = mesh.ldu_to_dense(s)
+    np.testing.assert_allclose(np.diag(M), np.diag(A), rtol=1e-12)
+    np.testing.assert_allclose(M[s.upper_addr, s.lower_addr], A[s.upper_addr, s.lower_addr],
+                               rtol=1e-12)
+    r = np.random.default_rng(5).standard_normal(s.n_cells)
+    z = O.dic_apply(s, rd, r)
+    np.testing.assert_allclose(M @ z, r, rtol=1e-10, atol=1e-12 * np.abs(r).max())
+
+
+def test_dic_is_exact_cholesky_on_a_chain_and_diagonal_without_faces():
+    from cufoam_amd import mesh
+    n = 40   # a 1-D chain: no fill-in, so IC(0) is the exact factorisation
+    la = np.arange(n - 1, dtype=np.int32)
+    ua = la + 1
+    up = -np.random.default_rng(6).uniform(0.5, 1.5, n - 1)
+    d = np.full(n, 0.05)
+    np.add.at(d, la, -up)
+    np.add.at(d, ua, -up)
+    s = mesh.LduSystem(n, la, ua, d, up.copy(), up, np.ones(n), np.zeros(n))
+    r = np.random.default_rng(7).standard_normal(n)
+    z = O.dic_apply(s, O.dic_rd(s), r)
+    np.testing.assert_allclose(z, np.linalg.solve(mesh.ldu_to_dense(s), r), rtol=1e-9)
+    e = mesh.LduSystem(5, np.zeros(0, np.int32), np.zeros(0, np.int32), np.arange(1.0, 6.0),
+                       np.zeros(0), np.zeros(0), np.ones(5), np.zeros(5))
+    assert np.array_equal(O.dic_rd(e), 1.0 / np.arange(1.0, 6.0))
+
+
+def test_dic_pcg_converges_faster_than_cg():
+    from cufoam_amd import mesh
+    s = mesh.laplacian(16, seed=4, shift=1e-3)
+    err = 1e-8 * np.linalg.norm(s.b)
+    plain = O.pcg("none", s, s.b, s.x0, err, 5000)
+    dic = O.pcg("DIC", s, s.b, s.x0, err, 5000)
+    assert dic.iterations < 0.7 * plain.iterations
+    A = mesh.ldu_to_dense(s)
+    assert np.linalg.norm(s.b - A @ dic.x) <= 1.01 * err
