@@ -565,8 +565,46 @@ static int pcg_impl(gis_handle h, int precond, double* x, const double* b,
     if (!detail_ok(params)) return fail(h, GIS_ERR_ARG, "detail_type must be ERROR or TIME");
     if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
     DeviceGuard g(h->device);
-    SolveArgs a = to_args(8 + precond, params, std::max(1, hist_cap));
+    SolveArgs a = to_args(kMethodPcg0 + precond, params, std::max(1, hist_cap));
     return solve_staged(h, a, x, b, dev, hist, hist_cap, hist_len, nullptr, nullptr);
+}
+
+// GIS_PRECOND_NONE: BiCG itself; GIS_PRECOND_DIAGONAL: BiCG preconditioned by D
+static int pbicg_method(gis_handle h, int precond, int* method) {
+    if (precond == GIS_PRECOND_NONE) {
+        *method = GIS_BICG;
+        return GIS_OK;
+    }
+    if (precond == GIS_PRECOND_DIAGONAL) {
+        *method = kMethodPbicgDiag;
+        return GIS_OK;
+    }
+    return fail(h, GIS_ERR_ARG, "PBiCG takes the none or diagonal preconditioner");
+}
+
+static int pbicg_impl(gis_handle h, int precond, double* x, const double* b,
+                      const gis_params* params, double* hist, int hist_cap, int* hist_len,
+                      bool dev) {
+    if (!h || !x || !b) return GIS_ERR_ARG;
+    int method = 0;
+    GIS_TRY(pbicg_method(h, precond, &method));
+    if (!detail_ok(params)) return fail(h, GIS_ERR_ARG, "detail_type must be ERROR or TIME");
+    if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
+    DeviceGuard g(h->device);
+    SolveArgs a = to_args(method, params, std::max(1, hist_cap));
+    return solve_staged(h, a, x, b, dev, hist, hist_cap, hist_len, nullptr, nullptr);
+}
+
+GIS_API int gis_solve_pbicg(gis_handle h, int precond, double* x, const double* b,
+                            const gis_params* params, double* hist, int hist_cap,
+                            int* hist_len) {
+    return pbicg_impl(h, precond, x, b, params, hist, hist_cap, hist_len, false);
+}
+
+GIS_API int gis_solve_pbicg_device(gis_handle h, int precond, double* d_x, const double* d_b,
+                                   const gis_params* params, double* hist, int hist_cap,
+                                   int* hist_len) {
+    return pbicg_impl(h, precond, d_x, d_b, params, hist, hist_cap, hist_len, true);
 }
 
 GIS_API int gis_solve_pcg(gis_handle h, int precond, double* x, const double* b,
@@ -868,13 +906,12 @@ GIS_API int gis_batch_solve_device(gis_handle h, double* d_x, const double* d_b,
 
 namespace {
 
-// method: GIS_CG .. GIS_RICHARDSON, or 8 + preconditioner for PCG
+// method: GIS_CG .. GIS_RICHARDSON, kMethodPcg0 + preconditioner for PCG, or
+// kMethodPbicgDiag (the public entries check their own ranges)
 int foam_solve_impl(gis_context* h, int method, double* psi, const double* source,
                     double tolerance, double rel_tol, long long min_iter, long long max_iter,
                     gis_performance* perf) {
     if (!h || !psi || !source) return GIS_ERR_ARG;
-    if (method < GIS_CG || method > 8 + GIS_PRECOND_DIC)
-        return fail(h, GIS_ERR_ARG, "unknown method");
     if (!(rel_tol >= 0.0) || min_iter < 0)
         return fail(h, GIS_ERR_ARG, "rel_tol and min_iter must be non-negative");
     if (!h->has_values) return fail(h, GIS_ERR_STATE, "values not set");
@@ -909,6 +946,8 @@ int foam_solve_impl(gis_context* h, int method, double* psi, const double* sourc
 
 GIS_API int gis_foam_solve(gis_handle h, int method, double* psi, const double* source,
                            double tolerance, long long max_iter, gis_performance* perf) {
+    if (h && (method < GIS_CG || method > GIS_RICHARDSON))
+        return fail(h, GIS_ERR_ARG, "unknown method");
     return foam_solve_impl(h, method, psi, source, tolerance, 0.0, 0, max_iter, perf);
 }
 
@@ -917,14 +956,24 @@ GIS_API int gis_foam_solve_pcg(gis_handle h, int precond, double* psi, const dou
                                long long max_iter, gis_performance* perf) {
     if (h && (precond < GIS_PRECOND_NONE || precond > GIS_PRECOND_DIC))
         return fail(h, GIS_ERR_ARG, "unknown preconditioner");
-    return foam_solve_impl(h, 8 + precond, psi, source, tolerance, rel_tol, min_iter, max_iter,
-                           perf);
+    return foam_solve_impl(h, kMethodPcg0 + precond, psi, source, tolerance, rel_tol, min_iter,
+                           max_iter, perf);
+}
+
+GIS_API int gis_foam_solve_pbicg(gis_handle h, int precond, double* psi, const double* source,
+                                 double tolerance, double rel_tol, long long min_iter,
+                                 long long max_iter, gis_performance* perf) {
+    int method = 0;
+    if (h) GIS_TRY(pbicg_method(h, precond, &method));
+    return foam_solve_impl(h, method, psi, source, tolerance, rel_tol, min_iter, max_iter, perf);
 }
 
 GIS_API int gis_foam_solve_controls(gis_handle h, int method, double* psi,
                                     const double* source, double tolerance, double rel_tol,
                                     long long min_iter, long long max_iter,
                                     gis_performance* perf) {
+    if (h && (method < GIS_CG || method > GIS_RICHARDSON))
+        return fail(h, GIS_ERR_ARG, "unknown method");
     return foam_solve_impl(h, method, psi, source, tolerance, rel_tol, min_iter, max_iter, perf);
 }
 

@@ -305,8 +305,12 @@ struct SolveArgs {
     int detail = 0;           // history: 0 residual norms, 1 elapsed seconds
 };
 
-// Preconditioned CG (gis_amg.hip): pc 0 none, 1 diagonal, 2 GAMG.  As
-// device_solve methods 8, 9, 10.
+// device_solve method ids past GIS_RICHARDSON: 8 + pc for preconditioned CG
+// (pc 0 none, 1 diagonal, 2 GAMG, 3 GAMG32, 4 DIC), and BiCG preconditioned by
+// the diagonal.
+constexpr int kMethodPcg0 = 8;
+constexpr int kMethodPbicgDiag = 13;
+// Preconditioned CG (gis_amg.hip), as device_solve methods kMethodPcg0 + pc.
 int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const double* b);
 int amg_levels(gis_context* h, int* n_levels);
 int amg_level(gis_context* h, int l, int* n, int* m, int* agg, int* la, int* ua, double* diag,

@@ -403,13 +403,15 @@ __global__ __launch_bounds__(kWG) void k_bicg1s(Mat<double> m, int N,
     });
 }
 
-template <class P>
+// PRE: rho = (D^-1 r, rs) from dinv (preconditioned BiCG, M = D); else (r, rs).
+template <class P, bool PRE>
 __global__ __launch_bounds__(kWG) void k_bicg2(Mat<double> mt, int N, double* __restrict__ r,
                                                double* __restrict__ rs,
                                                const double* __restrict__ pb0,
                                                const double* __restrict__ pb1,
                                                const double* __restrict__ ps,
                                                const double* __restrict__ q,
+                                               const double* __restrict__ dinv,
                                                double* __restrict__ x, KState* st, RedWs red) {
     if (st->done) return;
     __shared__ typename P::template Shared<double> sh;
@@ -420,12 +422,13 @@ __global__ __launch_bounds__(kWG) void k_bicg2(Mat<double> mt, int N, double* __
     const double xi = st->alpha;
     const double* p = (k & 1) ? pb1 : pb0;
     P::run(mt, N, t.row0, t.nrows, [&](int c) { return ps[c]; }, sh, sh_y);
-    double xv[4], pv[4], rv[4], qv[4], rsv[4];
+    double xv[4], pv[4], rv[4], qv[4], rsv[4], dv[4];
     load4(x, t.i0, N, xv);
     load4(p, t.i0, N, pv);
     load4(r, t.i0, N, rv);
     load4(q, t.i0, N, qv);
     load4(rs, t.i0, N, rsv);
+    if (PRE) load4(dinv, t.i0, N, dv);
     double leaf[2] = {0.0, 0.0};
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) {
@@ -434,7 +437,7 @@ __global__ __launch_bounds__(kWG) void k_bicg2(Mat<double> mt, int N, double* __
         rsv[kk] = fma(-xi, sh_y[threadIdx.x * kLeaf + kk], rsv[kk]);
         if (t.i0 + kk < N) {
             leaf[0] = fma(rv[kk], rv[kk], leaf[0]);
-            leaf[1] = fma(rv[kk], rsv[kk], leaf[1]);
+            leaf[1] = fma(PRE ? dv[kk] * rv[kk] : rv[kk], rsv[kk], leaf[1]);
         }
     }
     store4(x, t.i0, N, xv);
@@ -451,6 +454,67 @@ __global__ __launch_bounds__(kWG) void k_bicg2(Mat<double> mt, int N, double* __
             st->beta = st->rho0 / st->rho1;
         }
     });
+}
+
+// ---------------------------------------------------------------------------
+// Preconditioned BiCG with M = D (split schedule): z = D^-1 r, z* = D^-1 r*,
+//   pdir: p_k = z (k = 1) or fma(theta, p_{k-1}, z), ps_k likewise from z*
+//   k_bicg1s: q = A p_k, (ps_k, q) -> xi, breakdown
+//   k_bicg2<P, true>: x, r, rs as BiCG; (r, r) -> norm; (D^-1 r, rs) -> rho
+// rho0 = (D^-1 r0, r0) after the set-up (rs0 = r0).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kWG) void k_pbicg_rho0(int N, const double* __restrict__ dinv,
+                                                    const double* __restrict__ r, KState* st,
+                                                    RedWs red) {
+    if (st->done) return;
+    __shared__ RedShared shr;
+    TileRows t(N);
+    double dv[4], rv[4];
+    load4(dinv, t.i0, N, dv);
+    load4(r, t.i0, N, rv);
+    double leaf[1] = {0.0};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+        if (t.i0 + kk < N) leaf[0] = fma(dv[kk] * rv[kk], rv[kk], leaf[0]);
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) { st->rho0 = v[0]; });
+}
+
+__global__ __launch_bounds__(kWG) void k_pbicg_dir(int N, const double* __restrict__ dinv,
+                                                   const double* __restrict__ r,
+                                                   const double* __restrict__ rs,
+                                                   double* __restrict__ pb0,
+                                                   double* __restrict__ pb1,
+                                                   double* __restrict__ ps, const KState* st) {
+    if (st->done) return;
+    StreamRows t(N);
+    const int k = st->k + 1;
+    const double* p_old = (k & 1) ? pb0 : pb1;
+    double* p_new = (k & 1) ? pb1 : pb0;
+    double dv[4], zv[4], zsv[4];
+    load4(dinv, t.i0, N, dv);
+    load4(r, t.i0, N, zv);
+    load4(rs, t.i0, N, zsv);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        zv[kk] = dv[kk] * zv[kk];
+        zsv[kk] = dv[kk] * zsv[kk];
+    }
+    if (k == 1) {
+        store4(p_new, t.i0, N, zv);
+        store4(ps, t.i0, N, zsv);
+        return;
+    }
+    const double theta = st->beta;
+    double pv[4], psv[4];
+    load4(p_old, t.i0, N, pv);
+    load4(ps, t.i0, N, psv);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        pv[kk] = fma(theta, pv[kk], zv[kk]);
+        psv[kk] = fma(theta, psv[kk], zsv[kk]);
+    }
+    store4(p_new, t.i0, N, pv);
+    store4(ps, t.i0, N, psv);
 }
 
 // ---------------------------------------------------------------------------
@@ -595,8 +659,43 @@ int solve_bicg(gis_context* h, const SolveArgs& a, double* x, const double* b) {
                 hipLaunchKernelGGL((k_bicg1<P>), g, dim3(kWG), 0, h->stream, m, N, r, rs, pb0,
                                    pb1, ps, q, h->st, h->red);
             }
-            hipLaunchKernelGGL((k_bicg2<P>), g, dim3(kWG), 0, h->stream, mt, N, r, rs, pb0,
-                               pb1, ps, q, x, h->st, h->red);
+            hipLaunchKernelGGL((k_bicg2<P, false>), g, dim3(kWG), 0, h->stream, mt, N, r, rs,
+                               pb0, pb1, ps, q, nullptr, x, h->st, h->red);
+            return kOK;
+        }));
+    });
+    return kOK;
+}
+
+// BiCG preconditioned by M = D (no counterpart in the reference): the BiCG
+// loop, breakdown test, stopping rule and history, with z = D^-1 r in the
+// directions and in rho.  The split schedule only (three launches per iteration).
+int solve_pbicg_diag(gis_context* h, const SolveArgs& a, double* x, const double* b) {
+    GIS_TRY(ensure_valT(h));
+    GIS_TRY(prepare_solve(h, a, 6));
+    GIS_TRY(diag_helper(h, 0));
+    const int N = h->N;
+    double* r = h->vec.as<double>();
+    double* rs = r + N;
+    double* pb0 = r + 2 * (size_t)N;
+    double* pb1 = r + 3 * (size_t)N;
+    double* ps = r + 4 * (size_t)N;
+    double* q = r + 5 * (size_t)N;
+    const double* dinv = h->dinv.as<double>();
+    const Mat<double> m = mat64(h);
+    const Mat<double> mt = mat64T(h);
+    GIS_TRY(launch_setup(h, m, b, x, nullptr, r, rs, 1));
+    const dim3 g = tiles_grid(N);
+    hipLaunchKernelGGL(k_pbicg_rho0, g, dim3(kWG), 0, h->stream, N, dinv, r, h->st, h->red);
+    GIS_HIP(h, hipGetLastError());
+    GIS_SPMV_DISPATCH(h->sched.spmv64, P, {
+        GIS_TRY(run_blocks(h, iteration_budget(h), [&]() -> int {
+            hipLaunchKernelGGL(k_pbicg_dir, g, dim3(kWG), 0, h->stream, N, dinv, r, rs, pb0,
+                               pb1, ps, h->st);
+            hipLaunchKernelGGL((k_bicg1s<P>), g, dim3(kWG), 0, h->stream, m, N, pb0, pb1, ps,
+                               q, h->st, h->red);
+            hipLaunchKernelGGL((k_bicg2<P, true>), g, dim3(kWG), 0, h->stream, mt, N, r, rs,
+                               pb0, pb1, ps, q, dinv, x, h->st, h->red);
             return kOK;
         }));
     });
@@ -617,11 +716,12 @@ int device_solve(gis_context* h, const SolveArgs& a, double* d_x, const double* 
         case 5:
         case 6: st = solve_mr_family(h, a, d_x, d_b); break;
         case 7: st = solve_richardson(h, a, d_x, d_b); break;
-        case 8:
-        case 9:
-        case 10:
-        case 11:
-        case 12: st = solve_pcg(h, a, a.method - 8, d_x, d_b); break;
+        case kMethodPcg0 + 0:
+        case kMethodPcg0 + 1:
+        case kMethodPcg0 + 2:
+        case kMethodPcg0 + 3:
+        case kMethodPcg0 + 4: st = solve_pcg(h, a, a.method - kMethodPcg0, d_x, d_b); break;
+        case kMethodPbicgDiag: st = solve_pbicg_diag(h, a, d_x, d_b); break;
         default: return fail(h, kErrArg, "unknown solver method");
     }
     if (st != kOK) return st;

@@ -81,6 +81,9 @@ _TABLE = {
     # no counterpart in the reference: CG with a preconditioner (controls key
     # "preconditioner": none / diagonal / GAMG), for symmetric matrices
     "cuFoamPCG": ("PCG", True, False),
+    # no counterpart in the reference: BiCG with a preconditioner (none /
+    # diagonal), for symmetric and asymmetric matrices
+    "cuFoamPBiCG": ("PBiCG", True, True),
 }
 
 
@@ -110,6 +113,9 @@ class cuFoamSolver:
         self.preconditioner = str(controls.get("preconditioner", "none"))
         if self.preconditioner not in ("none", "diagonal", "DIC", "GAMG", "GAMG32"):
             raise FatalError(f"preconditioner must be none, diagonal, DIC, GAMG or GAMG32, not "
+                             f"{self.preconditioner}")
+        if self.typeName == "cuFoamPBiCG" and self.preconditioner not in ("none", "diagonal"):
+            raise FatalError(f"cuFoamPBiCG takes the none or diagonal preconditioner, not "
                              f"{self.preconditioner}")
         self.couple = str(controls.get("interfaces", "ignored"))
         if self.couple not in ("ignored", "coupled"):
@@ -153,9 +159,10 @@ class cuFoamSolver:
             raise FatalError(f"psi and source must have {self.matrix.size()} cells")
         self._upload()
         src = np.ascontiguousarray(source, dtype=np.float64)
-        if self.typeName == "cuFoamPCG":
+        if self.typeName in ("cuFoamPCG", "cuFoamPBiCG"):
             of = self.convergence == "openfoam"
-            perf = self._gis.foam_solve("PCG", psi, src, self.tolerance, self.maxIter,
+            perf = self._gis.foam_solve(_TABLE[self.typeName][0], psi, src, self.tolerance,
+                                        self.maxIter,
                                         rel_tol=self.relTol if of else 0.0,
                                         min_iter=self.minIter if of else 0,
                                         preconditioner=self.preconditioner)
@@ -183,10 +190,11 @@ cuFoamMRJacobi = _make("cuFoamMRJacobi")
 cuFoamJMR = _make("cuFoamJMR")
 cuFoamMRS = _make("cuFoamMRS")
 cuFoamPCG = _make("cuFoamPCG")
+cuFoamPBiCG = _make("cuFoamPBiCG")
 
 _CLASSES = {c.typeName: c for c in (cuFoamCG, cuFoamBiCG, cuFoamJacobi, cuFoamRichardson,
                                       cuFoamMR, cuFoamMRJacobi, cuFoamJMR, cuFoamMRS,
-                                      cuFoamPCG)}
+                                      cuFoamPCG, cuFoamPBiCG)}
 
 
 def New(name: str, fieldName: str, matrix: lduMatrix, solverControls: dict,

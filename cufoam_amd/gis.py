@@ -100,6 +100,9 @@ SIGNATURES = {
     "gis_solve_pcg": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
     "gis_solve_pcg_device": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
     "gis_foam_solve_pcg": (_i, [_vp, _i, _vp, _vp, _d, _d, _ll, _ll, C.POINTER(_Perf)]),
+    "gis_solve_pbicg": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
+    "gis_solve_pbicg_device": (_i, [_vp, _i, _vp, _vp, C.POINTER(_Params), _vp, _i, _ip]),
+    "gis_foam_solve_pbicg": (_i, [_vp, _i, _vp, _vp, _d, _d, _ll, _ll, C.POINTER(_Perf)]),
     "gis_set_gamg": (_i, [_vp, _d, _i, _d]),
     "gis_set_slot_fault": (_i, [_vp, _i]),
     "gis_set_timing_sample": (_i, [_vp, _i]),
@@ -553,6 +556,24 @@ class Gis:
         self.last_iterations = max(hl.value - 1, 0)
         return x, hist[:min(hl.value, hist_cap)].copy(), st
 
+    def solve_pbicg(self, precond: str, b, x0=None, error: float = 1e-4, max_steps: int = 0,
+                    int_bound: float = 1e-10, hist_cap: int | None = None,
+                    detail: str = "error"):
+        """BiCG preconditioned by none / diagonal -> (x, hist, status).
+        Breakdown (status 11) is returned, not raised, as for solve("BiCG")."""
+        p = _Params(error, max_steps, 0.0, 30, int_bound, DETAILS[detail])
+        if hist_cap is None:
+            hist_cap = (max_steps if max_steps > 0 else 10000) + 1
+        hist = np.zeros(max(hist_cap, 1))
+        hl = C.c_int(0)
+        pc = PRECONDITIONERS[precond]
+        x, b, dev = self._xb(b, x0, "solve_pbicg")
+        f = self._L.gis_solve_pbicg_device if dev else self._L.gis_solve_pbicg
+        st = f(self._h, pc, _ptr(x), _ptr(b), C.byref(p), _ptr(hist), hist_cap, C.byref(hl))
+        self._check(st, f"solve_pbicg({precond})")
+        self.last_iterations = max(hl.value - 1, 0)
+        return x, hist[:min(hl.value, hist_cap)].copy(), st
+
     def set_slot_fault(self, polls: int = 0):
         """Failure-path tests only: polls > 0 makes every device reduction give up
         after that many polls (solves then raise GisError 13); 0 restores."""
@@ -758,7 +779,8 @@ class Gis:
 
         rel_tol / min_iter (OpenFOAM's relTol / minIter) select
         gis_foam_solve_controls; left at None / 0 the call is gis_foam_solve.
-        method "PCG" with a preconditioner calls gis_foam_solve_pcg."""
+        method "PCG" / "PBiCG" with a preconditioner calls gis_foam_solve_pcg /
+        gis_foam_solve_pbicg."""
         perf = _Perf()
         self._need_pattern("foam_solve")
         if not (isinstance(psi, np.ndarray) and psi.dtype == np.float64
@@ -766,11 +788,10 @@ class Gis:
             raise GisError(ERR_ARG, f"foam_solve: psi must be a contiguous float64 numpy array "
                                     f"of {self.n} entries (updated in place)")
         source = self._host_vec(source, self.n, "foam_solve source")
-        if method == "PCG":
-            st = self._L.gis_foam_solve_pcg(self._h, PRECONDITIONERS[preconditioner or "none"],
-                                            _ptr(psi), _ptr(source), tolerance,
-                                            float(rel_tol or 0.0), int(min_iter), max_iter,
-                                            C.byref(perf))
+        if method in ("PCG", "PBiCG"):
+            f = self._L.gis_foam_solve_pcg if method == "PCG" else self._L.gis_foam_solve_pbicg
+            st = f(self._h, PRECONDITIONERS[preconditioner or "none"], _ptr(psi), _ptr(source),
+                   tolerance, float(rel_tol or 0.0), int(min_iter), max_iter, C.byref(perf))
         elif rel_tol is None and min_iter == 0:
             st = self._L.gis_foam_solve(self._h, METHODS[method], _ptr(psi), _ptr(source),
                                         tolerance, max_iter, C.byref(perf))

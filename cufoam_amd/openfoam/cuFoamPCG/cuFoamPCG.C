@@ -19,5 +19,5 @@ Foam::cuFoamPCG::cuFoamPCG(const word& fieldName, const lduMatrix& matrix,
 
 Foam::solverPerformance Foam::cuFoamPCG::solve(scalarField& psi, const scalarField& source,
                                            const direction) const {
-    return solveWith(typeName, -1 - preconditionerId(), psi, source);
+    return solveWith(typeName, gisFoam::Family::PCG, preconditionerId(), psi, source);
 }

@@ -1,6 +1,8 @@
 /*
- * cuFoamPCG: CG with a none / diagonal / GAMG / GAMG32 preconditioner (symmetric matrices; no counterpart in the reference) on the GPU through the gis library, selected as
- * "cuFoamPCG" in fvSolution (the interface of the reference's cuFoamPCG/cuFoamPCG.H).
+ * cuFoamPCG: CG with a none / diagonal / DIC / GAMG / GAMG32 preconditioner
+ * (symmetric matrices; no counterpart in the reference) on the GPU through the
+ * gis library, selected as "cuFoamPCG" in fvSolution.  Its interface is that
+ * of the reference's wrappers (cuFoamCG/cuFoamCG.H:8-31).
  */
 #ifndef CUFOAMPCG_H
 #define CUFOAMPCG_H

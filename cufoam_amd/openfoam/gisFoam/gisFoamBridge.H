@@ -135,9 +135,18 @@ private:
     long long patterns_ = 0;
 };
 
-// cuFoam<Name>::solve: psi (host, nCells) is updated in place.
-inline Outcome solve(int method, const Addressing& a, const Coefficients& c, double* psi,
-                     const double* source, const Controls& ctl) {
+// Which gis entry a wrapper's solve ends in.
+enum class Family {
+    Plain,   // gis_foam_solve / gis_foam_solve_controls with a GIS_<method> id
+    PCG,     // gis_foam_solve_pcg with a GIS_PRECOND_<name> id
+    PBiCG,   // gis_foam_solve_pbicg with GIS_PRECOND_NONE or GIS_PRECOND_DIAGONAL
+};
+
+// cuFoam<Name>::solve: psi (host, nCells) is updated in place.  The handle of
+// the addressing, gis_set_values, then the family's entry; relTol / minIter
+// are applied when `openfoam` is set.
+inline Outcome solveAs(Family family, int id, const Addressing& a, const Coefficients& c,
+                       double* psi, const double* source, const Controls& ctl) {
     Outcome o;
     gis_handle h = nullptr;
     int st = Handles::global().get(a, ctl.device, &h, &o.message);
@@ -146,11 +155,25 @@ inline Outcome solve(int method, const Addressing& a, const Coefficients& c, dou
         if (st != GIS_OK) o.message = std::string("gis_set_values: ") + gis_last_error(h);
     }
     if (st == GIS_OK) {
-        st = ctl.openfoam
-                 ? gis_foam_solve_controls(h, method, psi, source, ctl.tolerance, ctl.relTol,
-                                           ctl.minIter, ctl.maxIter, &o.perf)
-                 : gis_foam_solve(h, method, psi, source, ctl.tolerance, ctl.maxIter, &o.perf);
-        if (isError(st)) o.message = std::string("gis_foam_solve: ") + gis_last_error(h);
+        const double relTol = ctl.openfoam ? ctl.relTol : 0.0;
+        const long long minIter = ctl.openfoam ? ctl.minIter : 0;
+        const char* entry = "gis_foam_solve";
+        if (family == Family::PCG) {
+            entry = "gis_foam_solve_pcg";
+            st = gis_foam_solve_pcg(h, id, psi, source, ctl.tolerance, relTol, minIter,
+                                    ctl.maxIter, &o.perf);
+        } else if (family == Family::PBiCG) {
+            entry = "gis_foam_solve_pbicg";
+            st = gis_foam_solve_pbicg(h, id, psi, source, ctl.tolerance, relTol, minIter,
+                                      ctl.maxIter, &o.perf);
+        } else if (ctl.openfoam) {
+            entry = "gis_foam_solve_controls";
+            st = gis_foam_solve_controls(h, id, psi, source, ctl.tolerance, relTol, minIter,
+                                         ctl.maxIter, &o.perf);
+        } else {
+            st = gis_foam_solve(h, id, psi, source, ctl.tolerance, ctl.maxIter, &o.perf);
+        }
+        if (isError(st)) o.message = std::string(entry) + ": " + gis_last_error(h);
     }
     o.status = st;
     o.fatal = isError(st);
@@ -159,28 +182,21 @@ inline Outcome solve(int method, const Addressing& a, const Coefficients& c, dou
     return o;
 }
 
-// Preconditioned CG (GIS_PRECOND_NONE / DIAGONAL / DIC / GAMG / GAMG32) with the same
-// sequence: gis_foam_solve_pcg, relTol / minIter applied when `openfoam` is set.
+inline Outcome solve(int method, const Addressing& a, const Coefficients& c, double* psi,
+                     const double* source, const Controls& ctl) {
+    return solveAs(Family::Plain, method, a, c, psi, source, ctl);
+}
+
+// Preconditioned CG (GIS_PRECOND_NONE / DIAGONAL / DIC / GAMG / GAMG32).
 inline Outcome solvePcg(int precond, const Addressing& a, const Coefficients& c, double* psi,
                         const double* source, const Controls& ctl) {
-    Outcome o;
-    gis_handle h = nullptr;
-    int st = Handles::global().get(a, ctl.device, &h, &o.message);
-    if (st == GIS_OK) {
-        st = gis_set_values(h, c.diag, c.lower, c.upper);
-        if (st != GIS_OK) o.message = std::string("gis_set_values: ") + gis_last_error(h);
-    }
-    if (st == GIS_OK) {
-        st = gis_foam_solve_pcg(h, precond, psi, source, ctl.tolerance,
-                                ctl.openfoam ? ctl.relTol : 0.0, ctl.openfoam ? ctl.minIter : 0,
-                                ctl.maxIter, &o.perf);
-        if (isError(st)) o.message = std::string("gis_foam_solve_pcg: ") + gis_last_error(h);
-    }
-    o.status = st;
-    o.fatal = isError(st);
-    o.warn = (st == GIS_NAN);
-    if (o.warn) o.message = gis_status_string(st);
-    return o;
+    return solveAs(Family::PCG, precond, a, c, psi, source, ctl);
+}
+
+// Preconditioned BiCG (GIS_PRECOND_NONE / DIAGONAL).
+inline Outcome solvePbicg(int precond, const Addressing& a, const Coefficients& c,
+                          double* psi, const double* source, const Controls& ctl) {
+    return solveAs(Family::PBiCG, precond, a, c, psi, source, ctl);
 }
 
 }  // namespace gisFoam

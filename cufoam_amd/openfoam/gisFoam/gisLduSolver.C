@@ -26,7 +26,8 @@ int Foam::gisLduSolver::preconditionerId() const {
     return GIS_PRECOND_NONE;
 }
 
-Foam::solverPerformance Foam::gisLduSolver::solveWith(const word& solverName, int method,
+Foam::solverPerformance Foam::gisLduSolver::solveWith(const word& solverName,
+                                                      gisFoam::Family family, int id,
                                                       scalarField& psi,
                                                       const scalarField& source) const {
     static_assert(sizeof(label) == sizeof(int), "gis takes 32-bit labels (WM_LABEL_SIZE=32)");
@@ -54,9 +55,7 @@ Foam::solverPerformance Foam::gisLduSolver::solveWith(const word& solverName, in
     ctl.openfoam = (convergence == "openfoam");
     ctl.device = controlDict_.lookupOrDefault<label>("device", 0);
 
-    const gisFoam::Outcome o =
-        method >= 0 ? gisFoam::solve(method, a, c, psi.data(), source.cdata(), ctl)
-                    : gisFoam::solvePcg(-1 - method, a, c, psi.data(), source.cdata(), ctl);
+    const gisFoam::Outcome o = gisFoam::solveAs(family, id, a, c, psi.data(), source.cdata(), ctl);
     if (o.fatal) {
         FatalErrorInFunction << solverName << " for " << fieldName() << ": " << o.message
                              << abort(FatalError);

@@ -11,7 +11,8 @@
  *                maxIter only, as the reference wrappers; openfoam: relTol
  *                and minIter too, through gis_foam_solve_controls)
  *   device       GPU index (default 0)
- *   preconditioner  none | diagonal | DIC | GAMG | GAMG32  (cuFoamPCG only)
+ *   preconditioner  none | diagonal | DIC | GAMG | GAMG32  (cuFoamPCG);
+ *                   none | diagonal  (cuFoamPBiCG)
  */
 #ifndef GIS_LDU_SOLVER_H
 #define GIS_LDU_SOLVER_H
@@ -32,11 +33,17 @@ protected:
 
     // psi is updated in place; the performance carries the reference's
     // unnormalised initial and final L2 residuals and the iteration count.
-    // method: a GIS_<method> id, or -1 - GIS_PRECOND_<name> for preconditioned CG.
-    solverPerformance solveWith(const word& solverName, int method, scalarField& psi,
-                                const scalarField& source) const;
+    // family Plain: id is a GIS_<method>; PCG / PBiCG: a GIS_PRECOND_<name>.
+    solverPerformance solveWith(const word& solverName, gisFoam::Family family, int id,
+                                scalarField& psi, const scalarField& source) const;
 
-    // the fvSolution preconditioner name as a GIS_PRECOND_ id (cuFoamPCG)
+    // the plain methods of the reference's wrappers
+    solverPerformance solveWith(const word& solverName, int method, scalarField& psi,
+                                const scalarField& source) const {
+        return solveWith(solverName, gisFoam::Family::Plain, method, psi, source);
+    }
+
+    // the fvSolution preconditioner name as a GIS_PRECOND_ id (cuFoamPCG, cuFoamPBiCG)
     int preconditionerId() const;
 };
 

@@ -297,6 +297,28 @@ GIS_API int gis_solve_pcg_device(gis_handle h, int precond, double *d_x, const d
 GIS_API int gis_foam_solve_pcg(gis_handle h, int precond, double *psi, const double *source,
                                double tolerance, double rel_tol, long long min_iter,
                                long long max_iter, gis_performance *perf);
+/* ---- preconditioned BiCG (no counterpart in the reference; its BiCG,
+ * SolverSingleGPU.h:568-675, and the cuFoamBiCG wrapper take no
+ * preconditioner).  The loop, breakdown test (|xi| < int_bound), stopping rule
+ * (||r||_2 < error) and history are those of GIS_BICG, with z = M^-1 r and
+ * z* = M^-T r* in the directions (p = z + theta p, p* = z* + theta p*) and in
+ * rho = (z, r*).  precond GIS_PRECOND_NONE gives GIS_BICG bit for bit;
+ * GIS_PRECOND_DIAGONAL uses M = D (OpenFOAM's diagonal preconditioner: z_i =
+ * r_i / a_ii); other preconditioners are GIS_ERR_ARG.  On a symmetric matrix with the diagonal preconditioner the
+ * iterates are those of gis_solve_pcg(GIS_PRECOND_DIAGONAL) bit for bit. ---- */
+GIS_API int gis_solve_pbicg(gis_handle h, int precond, double *x, const double *b,
+                            const gis_params *params, double *hist, int hist_cap,
+                            int *hist_len);
+GIS_API int gis_solve_pbicg_device(gis_handle h, int precond, double *d_x, const double *d_b,
+                                   const gis_params *params, double *hist, int hist_cap,
+                                   int *hist_len);
+/* The wrapper sequence of cuFoamBiCG (int_bound 1e-10, error = tolerance *
+ * NormFactor) with a preconditioner, and relTol / minIter as
+ * gis_foam_solve_controls (0 and 0: the reference's rule). */
+GIS_API int gis_foam_solve_pbicg(gis_handle h, int precond, double *psi, const double *source,
+                                 double tolerance, double rel_tol, long long min_iter,
+                                 long long max_iter, gis_performance *perf);
+
 /* GAMG settings (defaults omega 0.8, nu 1, cscale 1.8). */
 GIS_API int gis_set_gamg(gis_handle h, double omega, int nu, double cscale);
 

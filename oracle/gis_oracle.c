@@ -63,7 +63,7 @@
 #define OR_HOT __attribute__((target_clones("fma", "default")))
 
 enum { OR_CG = 0, OR_BICG = 1, OR_JACOBI = 2, OR_MR = 3, OR_MRJACOBI = 4,
-       OR_JMR = 5, OR_MRS = 6, OR_RICHARDSON = 7 };
+       OR_JMR = 5, OR_MRS = 6, OR_RICHARDSON = 7, OR_PBICG_DIAG = 13 };
 
 enum { OR_OK = 0, OR_ERR_ARG = 1, OR_ERR_NOMEM = 5, OR_BREAKDOWN = 10 };
 
@@ -411,6 +411,62 @@ static int bicg(const csr_t *A, const csr_t *At, const double *b, double *x, dou
 }
 
 /* ------------------------------------------------------------------------ */
+/* Preconditioned BiCG (no counterpart in the reference; the loop, breakdown  */
+/* test, stopping rule and history of bicg above, SolverSingleGPU.h:568-675): */
+/*   z = M^-1 r, z* = M^-T r*, rho = (z, r*), p = z (k = 1) or                 */
+/*   fma(theta, p, z), p* likewise from z*.  M = D: z_i = (1 / a_ii) r_i, and  */
+/*   M^-T = M^-1.  With M = I this is bicg bit for bit.                        */
+/* ------------------------------------------------------------------------ */
+static int pbicg_diag(const csr_t *A, const csr_t *At, const double *b, double *x, double error,
+                      int max_steps, double int_bound, const ctl_t *c, hist_t *h) {
+    const int N = A->n;
+    double *r = (double *)malloc(sizeof(double) * (size_t)N * 8);
+    if (!r) return OR_ERR_NOMEM;
+    double *rs = r + N, *p = r + 2 * (size_t)N, *ps = r + 3 * (size_t)N;
+    double *q = r + 4 * (size_t)N, *qs = r + 5 * (size_t)N, *dinv = r + 6 * (size_t)N;
+    double *z = r + 7 * (size_t)N;
+    find_diag(A, dinv);
+    for (int i = 0; i < N; ++i) dinv[i] = 1.0 / dinv[i];
+    residual(A, b, x, r);
+    memcpy(rs, r, sizeof(double) * N);
+    for (int i = 0; i < N; ++i) {
+        p[i] = dinv[i] * r[i];
+        ps[i] = dinv[i] * rs[i];
+    }
+    double rho0 = or_dot(N, p, rs);   /* (z, r*) with z = p at k = 1 */
+    double rho1 = 0;
+    double norm_r = norm2(N, r);
+    int64_t ms = unbounded(max_steps);
+    hpush(h, norm_r);
+    error = ctl_error(c, error, norm_r);
+    int status = OR_OK;
+    for (int64_t k = 1; k <= ms || k <= c->min_steps; ++k) {
+        spmv(A, p, q);
+        double xi = rho0 / or_dot(N, ps, q);
+        if (xi > -int_bound && xi < int_bound) {
+            status = OR_BREAKDOWN;
+            break;
+        }
+        axpy(N, xi, p, x, x);
+        axpy(N, -xi, q, r, r);
+        norm_r = norm2(N, r);
+        hpush(h, norm_r);
+        if (norm_r < error && k >= c->min_steps) break;
+        spmv(At, ps, qs);
+        axpy(N, -xi, qs, rs, rs);
+        for (int i = 0; i < N; ++i) z[i] = dinv[i] * r[i];
+        rho1 = rho0;
+        rho0 = or_dot(N, z, rs);
+        double theta = rho0 / rho1;
+        axpy(N, theta, p, z, p);
+        for (int i = 0; i < N; ++i) z[i] = dinv[i] * rs[i];
+        axpy(N, theta, ps, z, ps);
+    }
+    free(r);
+    return status;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Richardson + OptimizePower + FindDiagAbsMin  (SolverSingleGPU.h:686-819) */
 /* ------------------------------------------------------------------------ */
 OR_API double or_find_diag_abs_min(int N, const int *rowptr, const int *col, const double *val) {
@@ -635,8 +691,9 @@ static int mrs(const csr_t *A, const double *b, double *x, double error, int max
 /* dispatch                                                                   */
 /* ------------------------------------------------------------------------ */
 /*
- * Solve A x = b with one of the eight methods.  x holds x0 on entry.
- * At (CSR of A^T) is only read by BiCG.  hist receives up to hist_cap
+ * Solve A x = b with one of the eight methods, or BiCG preconditioned by the
+ * diagonal (OR_PBICG_DIAG).  x holds x0 on entry.
+ * At (CSR of A^T) is only read by the two BiCGs.  hist receives up to hist_cap
  * residual norms; *hist_len gets the full count (iterations = len - 1).
  * omega/power_steps: Richardson; int_bound: BiCG breakdown bound.
  * *omega_used receives Richardson's relaxation factor (may be NULL).
@@ -657,6 +714,10 @@ OR_API int or_solve_ctl(int method, int N, const int *rowptr, const int *col, co
         case OR_BICG:
             if (!t_rowptr || !t_col || !t_val) return OR_ERR_ARG;
             st = bicg(&A, &At, b, x, error, max_steps, int_bound, &c, &h);
+            break;
+        case OR_PBICG_DIAG:
+            if (!t_rowptr || !t_col || !t_val) return OR_ERR_ARG;
+            st = pbicg_diag(&A, &At, b, x, error, max_steps, int_bound, &c, &h);
             break;
         case OR_JACOBI: st = jacobi(&A, b, x, error, max_steps, &c, &h); break;
         case OR_MR: st = mr(&A, b, x, error, max_steps, &c, &h); break;

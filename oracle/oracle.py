@@ -21,6 +21,7 @@ REF_PATH = os.path.join(HERE, "_ref", "libref_sparsecpu.so")
 
 METHODS = {"CG": 0, "BiCG": 1, "Jacobi": 2, "MR": 3, "MRJacobi": 4, "JMR": 5, "MRS": 6,
            "Richardson": 7}
+_PBICG_DIAG = 13   # or_solve_ctl's id for BiCG preconditioned by the diagonal
 BREAKDOWN = 10
 NAN = 12
 
@@ -293,7 +294,7 @@ def solve(method: str, A: Csr, b, x0, error: float, max_steps: int, At: Csr | No
 
     rel_tol / min_steps add OpenFOAM's relTol and minIter (lduMatrix::solver);
     0 and 0 are the reference's loops."""
-    m = METHODS[method]
+    m = _PBICG_DIAG if method == "PBiCG-diagonal" else METHODS[method]
     x = _c(np.array(x0, dtype=np.float64, copy=True), np.float64)
     if hist_cap is None:
         hist_cap = max(max_steps if max_steps > 0 else 100000, min_steps) + 1
@@ -301,7 +302,7 @@ def solve(method: str, A: Csr, b, x0, error: float, max_steps: int, At: Csr | No
     hl = C.c_int(0)
     om = C.c_double(0.0)
     keep = []
-    if m == METHODS["BiCG"]:
+    if m in (METHODS["BiCG"], _PBICG_DIAG):
         if At is None:
             At = transpose(A)
         keep = [_c(At.rowptr, np.int32), _c(At.col, np.int32), _c(At.val, np.float64)]
@@ -322,6 +323,18 @@ def solve(method: str, A: Csr, b, x0, error: float, max_steps: int, At: Csr | No
     return SolveResult(x, hist[:n].copy(), hl.value, st, om.value)
 
 
+def pbicg(precond: str, A: Csr, b, x0, error: float, max_steps: int, At: Csr | None = None,
+          int_bound: float = 1e-10, hist_cap: int | None = None, rel_tol: float = 0.0,
+          min_steps: int = 0) -> SolveResult:
+    """BiCG preconditioned by none (BiCG itself) or the diagonal (no counterpart
+    in the reference; gis_oracle.c pbicg_diag)."""
+    if precond not in ("none", "diagonal"):
+        raise ValueError(f"PBiCG takes none or diagonal, not {precond}")
+    return solve("BiCG" if precond == "none" else "PBiCG-diagonal", A, b, x0, error, max_steps,
+                 At=At, int_bound=int_bound, hist_cap=hist_cap, rel_tol=rel_tol,
+                 min_steps=min_steps)
+
+
 @dataclass
 class Performance:
     """The three numbers cuFoam's wrappers put into solverPerformance."""
@@ -340,8 +353,8 @@ def foam_solve(method: str, sys, psi, source, tolerance: float, max_iter: int,
     error = tolerance * NormFactor(A, b, x); the solver runs with max_steps = maxIter;
     initialResidual = hist.front(), finalResidual = hist.back(), nIterations = size-1.
     rel_tol / min_iter: OpenFOAM's relTol and minIter, which the reference
-    wrappers do not pass on (0 and 0 reproduce them).  method "PCG": CG with
-    `preconditioner` (no counterpart in the reference).
+    wrappers do not pass on (0 and 0 reproduce them).  method "PCG" / "PBiCG":
+    CG / BiCG with `preconditioner` (no counterpart in the reference).
     """
     if A is None:
         A = ldu2csr(sys)
@@ -353,11 +366,15 @@ def foam_solve(method: str, sys, psi, source, tolerance: float, max_iter: int,
                            res.hist_len - 1, res.status)
         return res.x, perf
     At = None
-    if method == "BiCG":
+    if method in ("BiCG", "PBiCG"):
         coupled = getattr(sys, "cpl_row", None) is not None and len(sys.cpl_row)
         At = mirror_transpose(A) if coupled else transpose(A)
-    res = solve(method, A, source, psi, tolerance * nf, max_iter, At=At, rel_tol=rel_tol,
-                min_steps=min_iter)
+    if method == "PBiCG":
+        res = pbicg(preconditioner, A, source, psi, tolerance * nf, max_iter, At=At,
+                    rel_tol=rel_tol, min_steps=min_iter)
+    else:
+        res = solve(method, A, source, psi, tolerance * nf, max_iter, At=At, rel_tol=rel_tol,
+                    min_steps=min_iter)
     perf = Performance("cuFoam" + method, float(res.hist[0]), float(res.hist[-1]),
                        res.hist_len - 1, res.status)
     return res.x, perf

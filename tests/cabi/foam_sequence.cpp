@@ -8,6 +8,8 @@
 //   foam_sequence solve IN OUT      systems from IN, results to OUT (below)
 //
 // IN:  int32 K, then K times: int32 method, N, M, maxIter; float64 tolerance;
+//      (method < 100: a GIS_<method> id; 100 + p: PCG, 200 + p: PBiCG with
+//      the GIS_PRECOND_ id p)
 //      int32 lowerAddr[M], upperAddr[M]; float64 diag[N], lower[M], upper[M],
 //      source[N], psi[N].
 // OUT: int64 pattern builds, then per system: int32 status, nIterations,
@@ -81,9 +83,13 @@ int run(const char* in_path, const char* out_path) {
         ctl.maxIter = max_iter;
         Result r;
         std::vector<double> psi = psi0;
-        gisFoam::Outcome o1 = gisFoam::solve(method, a, c, psi.data(), source.data(), ctl);
+        const gisFoam::Family fam = method >= 200   ? gisFoam::Family::PBiCG
+                                    : method >= 100 ? gisFoam::Family::PCG
+                                                    : gisFoam::Family::Plain;
+        const int id = method % 100;
+        gisFoam::Outcome o1 = gisFoam::solveAs(fam, id, a, c, psi.data(), source.data(), ctl);
         std::vector<double> psi2 = psi0;
-        gisFoam::Outcome o2 = gisFoam::solve(method, a, c, psi2.data(), source.data(), ctl);
+        gisFoam::Outcome o2 = gisFoam::solveAs(fam, id, a, c, psi2.data(), source.data(), ctl);
         if (o1.fatal || o2.fatal) {
             std::fprintf(stderr, "system %d: %s%s\n", s, o1.message.c_str(), o2.message.c_str());
             return 3;

@@ -71,3 +71,14 @@ def test_foam_convergence_control_is_validated_before_any_gpu_call():
     m = foam.lduMatrix.from_system(s)
     with pytest.raises(foam.FatalError):
         foam.New("cuFoamCG", "p", m, {"convergence": "strict"})
+
+
+def test_foam_pbicg_registration_and_preconditioner_checks_before_any_gpu_call():
+    from cufoam_amd import foam, mesh
+    asym = foam.lduMatrix.from_system(mesh.laplacian(4, seed=0, shift=0.1, symmetric=False))
+    assert asym.asymmetric()
+    with pytest.raises(foam.FatalError):   # PCG is registered for symmetric matrices only
+        foam.New("cuFoamPCG", "U", asym, {"preconditioner": "diagonal"})
+    for pc in ("DIC", "GAMG", "GAMG32"):   # PBiCG takes none or diagonal
+        with pytest.raises(foam.FatalError):
+            foam.New("cuFoamPBiCG", "U", asym, {"preconditioner": pc})

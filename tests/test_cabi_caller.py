@@ -36,11 +36,19 @@ def test_header_layouts_match_the_ctypes_mirrors():
     assert got["gis_schedule"] == _offsets(G._Schedule, [f[0] for f in G._Schedule._fields_])
 
 
+def _code(method):
+    """foam_sequence's method code: a GIS_ id, 100 + p for PCG, 200 + p for PBiCG."""
+    if ":" in method:
+        family, pc = method.split(":")
+        return (100 if family == "PCG" else 200) + G.PRECONDITIONERS[pc]
+    return G.METHODS[method]
+
+
 def _write_systems(path, cases):
     with open(path, "wb") as f:
         np.array([len(cases)], np.int32).tofile(f)
         for method, s, psi, tol, max_iter in cases:
-            np.array([G.METHODS[method], s.n_cells, s.n_faces, max_iter], np.int32).tofile(f)
+            np.array([_code(method), s.n_cells, s.n_faces, max_iter], np.int32).tofile(f)
             np.array([tol], np.float64).tofile(f)
             for a, dt in ((s.lower_addr, np.int32), (s.upper_addr, np.int32),
                           (s.diag, np.float64), (s.lower, np.float64), (s.upper, np.float64),
@@ -58,12 +66,15 @@ def test_compiled_caller_runs_the_wrapper_sequence_bit_exact(tmp_path):
             ("BiCG", mesh.laplacian(16, seed=1, shift=0.1, symmetric=False), 1e-8, 2000),
             ("MRJacobi", mesh.random_ldu(3000, 9000, seed=2, symmetric=False), 1e-8, 3000),
             ("Richardson", mesh.laplacian(9, seed=3, shift=0.5), 1e-8, 500),
-            ("JMR", mesh.laplacian(11, seed=4, shift=0.3, symmetric=False), 1e-8, 2000)):
+            ("JMR", mesh.laplacian(11, seed=4, shift=0.3, symmetric=False), 1e-8, 2000),
+            ("PBiCG:diagonal", mesh.random_ldu(2000, 7000, seed=5, symmetric=False), 1e-8,
+             2000),
+            ("PCG:DIC", mesh.laplacian(12, seed=6, shift=1e-2), 1e-8, 2000)):
         cases.append((method, s, rng.uniform(-1, 1, s.n_cells), tol, max_iter))
     exe = build_caller.build()
     src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
     _write_systems(src, cases)
-    print("[case] compiled C++ caller, 5 systems", flush=True)
+    print(f"[case] compiled C++ caller, {len(cases)} systems", flush=True)
     r = subprocess.run([exe, "solve", str(src), str(dst)], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stderr
@@ -74,7 +85,9 @@ def test_compiled_caller_runs_the_wrapper_sequence_bit_exact(tmp_path):
             st, iters, same = np.fromfile(f, np.int32, 3)
             init, fin, nf = np.fromfile(f, np.float64, 3)
             got = np.fromfile(f, np.float64, s.n_cells)
-            ref_x, ref = O.foam_solve(method, s, psi, s.b, tol, max_iter)
+            family, _, pc = method.partition(":")
+            ref_x, ref = O.foam_solve(family, s, psi, s.b, tol, max_iter,
+                                      preconditioner=pc or "none")
             assert same == 1, method
             assert (iters, init, fin) == (ref.n_iterations, ref.initial_residual,
                                           ref.final_residual), method
