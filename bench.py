@@ -449,11 +449,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=30,
                     help="host CG iterations timed per one-thread variant (5x on all threads)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-tune", dest="tune", action="store_false",
-                    help="keep the default schedule instead of autotuning")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune (gis_autotune) instead of keeping the schedule that "
+                         "gis_set_pattern picks from the pattern (the drop-in's schedule)")
     ap.add_argument("--schedule", default=None,
-                    help="fixed schedule instead of autotuning: spmv64,spmv32,fp64_direction,"
-                         "fp32_direction,coarsen64,coarsen32 (profiling runs)")
+                    help="fixed schedule: spmv64,spmv32,fp64_direction,fp32_direction,"
+                         "coarsen64,coarsen32")
     ap.add_argument("--no-dropin", dest="dropin", action="store_false",
                     help="skip the 128^3 OpenFOAM drop-in wrapper timings")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
@@ -499,7 +500,8 @@ def main():
     else:
         sched, tune_ms = g.get_schedule(), (0.0, 0.0)
     t_tune = time.perf_counter() - t0
-    log(f"[bench] schedule {sched} (autotune {t_tune:.2f}s, best runs {tune_ms} ms)")
+    sched_source = "fixed" if args.schedule else ("autotune" if args.tune else "pattern")
+    log(f"[bench] schedule {sched} from {sched_source} ({t_tune:.2f}s, best runs {tune_ms} ms)")
     b = torch.from_numpy(s.b).cuda()
     bnorm = float(np.linalg.norm(s.b))
     max_iter = args.max_iter
@@ -760,6 +762,7 @@ def main():
                                        "update_k_cg2": avg_us(brk_d["iter_update"]),
                                        "direction_k_cg_dir": avg_us(brk_d["iter_direction"])}}},
         "schedule": sched,
+        "schedule_source": sched_source,
         "spmv": spmv,
         "setup_s": {"pattern": round(t_pattern, 4), "values": round(t_values, 4),
                     "autotune": round(t_tune, 4)},

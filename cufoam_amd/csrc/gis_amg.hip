@@ -25,6 +25,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "gis_agglom.hpp"
 #include "gis_kernels.hpp"
 
 namespace gis {
@@ -39,142 +40,6 @@ constexpr int kAmgLevels = 32;
 constexpr int kGammaSlot = 7;     // KState::aux slot of (r, z_prev) for GAMG32's beta
 
 inline dim3 grid_of(long long n) { return dim3((unsigned)((n + kBlk - 1) / kBlk)); }
-
-// ---------------------------------------------------------------------------
-// host agglomeration (same steps as the oracle's restatement)
-// ---------------------------------------------------------------------------
-struct HostLevel {
-    int n = 0, m = 0;
-    std::vector<int> la, ua;
-    std::vector<double> diag, lower, upper;
-};
-
-// rows of the face graph: faces where the cell is neighbour (face order), then
-// faces where it is owner (face order)
-void graph_rows(int n, int m, const int* la, const int* ua, std::vector<int>& rp,
-                std::vector<int>& face, std::vector<int>& nb) {
-    rp.assign((size_t)n + 1, 0);
-    face.resize(2 * (size_t)m);
-    nb.resize(2 * (size_t)m);
-    std::vector<int> lo_cnt(n, 0), fill(n, 0);
-    for (int f = 0; f < m; ++f) {
-        rp[ua[f] + 1]++;
-        rp[la[f] + 1]++;
-        lo_cnt[ua[f]]++;
-    }
-    for (int i = 0; i < n; ++i) rp[i + 1] += rp[i];
-    for (int f = 0; f < m; ++f) {
-        const int c = ua[f];
-        const int p = rp[c] + fill[c]++;
-        face[p] = f;
-        nb[p] = la[f];
-    }
-    std::fill(fill.begin(), fill.end(), 0);
-    for (int f = 0; f < m; ++f) {
-        const int c = la[f];
-        const int p = rp[c] + lo_cnt[c] + fill[c]++;
-        face[p] = f;
-        nb[p] = ua[f];
-    }
-}
-
-int pair_pass(int n, int m, const int* la, const int* ua, const double* w, std::vector<int>& agg) {
-    std::vector<int> rp, face, nb;
-    graph_rows(n, m, la, ua, rp, face, nb);
-    agg.assign(n, -1);
-    int nc = 0;
-    for (int i = 0; i < n; ++i) {
-        if (agg[i] >= 0) continue;
-        int best = -1;
-        double bw = -1.0;
-        for (int p = rp[i]; p < rp[i + 1]; ++p)
-            if (agg[nb[p]] < 0 && w[face[p]] > bw) {
-                bw = w[face[p]];
-                best = nb[p];
-            }
-        if (best >= 0) {
-            agg[i] = agg[best] = nc++;
-            continue;
-        }
-        for (int p = rp[i]; p < rp[i + 1]; ++p)
-            if (w[face[p]] > bw) {
-                bw = w[face[p]];
-                best = nb[p];
-            }
-        agg[i] = best >= 0 ? agg[best] : nc++;
-    }
-    return nc;
-}
-
-// Distinct aggregate pairs of the fine faces in upper-triangular order
-// (sorted by owner, then neighbour); fcode[f] = 2 F + flip, or -1 - C for a
-// face inside aggregate C.
-int coarse_faces(int m, const int* la, const int* ua, const int* agg, int nc,
-                 std::vector<int>& fcode, std::vector<int>& cla, std::vector<int>& cua) {
-    std::vector<int> start((size_t)nc + 1, 0);
-    for (int f = 0; f < m; ++f) {
-        const int a = agg[la[f]], b = agg[ua[f]];
-        if (a != b) start[std::min(a, b) + 1]++;
-    }
-    for (int c = 0; c < nc; ++c) start[c + 1] += start[c];
-    std::vector<int> nbr(start[nc]), fill(nc, 0);
-    for (int f = 0; f < m; ++f) {
-        const int a = agg[la[f]], b = agg[ua[f]];
-        if (a != b) {
-            const int o = std::min(a, b);
-            nbr[start[o] + fill[o]++] = std::max(a, b);
-        }
-    }
-    // per owner: sorted distinct neighbours; uniq[c] .. uniq[c+1] their face ids
-    std::vector<int> ustart((size_t)nc + 1, 0);
-    cla.clear();
-    cua.clear();
-    for (int c = 0; c < nc; ++c) {
-        auto b0 = nbr.begin() + start[c], b1 = nbr.begin() + start[c + 1];
-        std::sort(b0, b1);
-        auto e = std::unique(b0, b1);
-        ustart[c] = (int)cla.size();
-        for (auto it = b0; it != e; ++it) {
-            cla.push_back(c);
-            cua.push_back(*it);
-        }
-    }
-    ustart[nc] = (int)cla.size();
-    fcode.resize(m);
-    for (int f = 0; f < m; ++f) {
-        const int a = agg[la[f]], b = agg[ua[f]];
-        if (a == b) {
-            fcode[f] = -1 - a;
-            continue;
-        }
-        const int o = std::min(a, b), t = std::max(a, b);
-        const auto it = std::lower_bound(cua.begin() + ustart[o], cua.begin() + ustart[o + 1], t);
-        fcode[f] = 2 * (int)(it - cua.begin()) + (a > b ? 1 : 0);
-    }
-    return (int)cla.size();
-}
-
-// Galerkin values of the next level on the host (for the next level's weights)
-void restrict_host(const HostLevel& F, const std::vector<int>& agg,
-                   const std::vector<int>& fcode, HostLevel& C) {
-    C.diag.assign(C.n, 0.0);
-    C.upper.assign(C.m, 0.0);
-    C.lower.assign(C.m, 0.0);
-    for (int i = 0; i < F.n; ++i) C.diag[agg[i]] = C.diag[agg[i]] + F.diag[i];
-    for (int f = 0; f < F.m; ++f) {
-        const int code = fcode[f];
-        if (code < 0) {
-            const int c = -1 - code;
-            C.diag[c] = C.diag[c] + F.upper[f];
-            C.diag[c] = C.diag[c] + F.lower[f];
-        } else {
-            const int g = code >> 1;
-            const bool flip = code & 1;
-            C.upper[g] = C.upper[g] + (flip ? F.lower[f] : F.upper[f]);
-            C.lower[g] = C.lower[g] + (flip ? F.upper[f] : F.lower[f]);
-        }
-    }
-}
 
 // ---------------------------------------------------------------------------
 // device: coarse values
@@ -827,8 +692,10 @@ __global__ __launch_bounds__(kWG) void k_pcg2(int N, double* __restrict__ r,
 // ---------------------------------------------------------------------------
 // host: hierarchy on the device
 // ---------------------------------------------------------------------------
-template <typename T>
-int upload_vec(gis_context* h, DevMem& d, const std::vector<T>& v) {
+// V: a vector of trivially copyable elements (std::vector or agglom::hvec)
+template <typename V>
+int upload_vec(gis_context* h, DevMem& d, const V& v) {
+    using T = typename V::value_type;
     GIS_HIP(h, d.ensure(sizeof(T) * std::max<size_t>(v.size(), 1)));
     if (!v.empty())
         GIS_HIP(h, hipMemcpyAsync(d.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice,
@@ -836,21 +703,9 @@ int upload_vec(gis_context* h, DevMem& d, const std::vector<T>& v) {
     return kOK;
 }
 
-// counting-sort lists: for key k, the items i (ascending) with key[i] == k
-void bucket(int nkeys, const std::vector<int>& key, const std::vector<int>& item,
-            std::vector<int>& ptr, std::vector<int>& list) {
-    ptr.assign((size_t)nkeys + 1, 0);
-    for (int k : key)
-        if (k >= 0) ptr[k + 1]++;
-    for (int c = 0; c < nkeys; ++c) ptr[c + 1] += ptr[c];
-    list.resize(ptr[nkeys]);
-    std::vector<int> fill(nkeys, 0);
-    for (size_t i = 0; i < key.size(); ++i)
-        if (key[i] >= 0) list[ptr[key[i]] + fill[key[i]]++] = item[i];
-}
-
-template <typename T>
-int download(gis_context* h, std::vector<T>& out, const DevMem& d, size_t n) {
+template <typename V>
+int download(gis_context* h, V& out, const DevMem& d, size_t n) {
+    using T = typename V::value_type;
     out.resize(n);
     if (n)
         GIS_HIP(h, hipMemcpyAsync(out.data(), d.p, sizeof(T) * n, hipMemcpyDeviceToHost,
@@ -861,6 +716,7 @@ int download(gis_context* h, std::vector<T>& out, const DevMem& d, size_t n) {
 gis_context* level_ctx(gis_context* h, int l) { return l == 0 ? h : h->amg->lv[l].ctx; }
 
 int amg_build(gis_context* h) {
+    using agglom::HostLevel;
     AmgData& A = *h->amg;
     A.clear();
     HostLevel fine;
@@ -877,63 +733,34 @@ int amg_build(gis_context* h) {
     A.lv[0].m = fine.m;
     HostLevel cur = std::move(fine);
     while (cur.n > kDense && (int)A.lv.size() < kAmgLevels) {
-        std::vector<double> w(cur.m);
-        for (int f = 0; f < cur.m; ++f) w[f] = std::fabs(cur.upper[f]) + std::fabs(cur.lower[f]);
-        std::vector<int> agg1, fc1, la1, ua1, agg2, fcode, cla, cua;
-        const int n1 = pair_pass(cur.n, cur.m, cur.la.data(), cur.ua.data(), w.data(), agg1);
-        const int m1 = coarse_faces(cur.m, cur.la.data(), cur.ua.data(), agg1.data(), n1, fc1, la1,
-                                    ua1);
-        std::vector<double> w1(m1, 0.0);
-        for (int f = 0; f < cur.m; ++f)
-            if (fc1[f] >= 0) w1[fc1[f] >> 1] = w1[fc1[f] >> 1] + w[f];
-        const int n2 = pair_pass(n1, m1, la1.data(), ua1.data(), w1.data(), agg2);
-        if ((long long)n2 * 10 > (long long)cur.n * 9) break;   // stagnation
-        std::vector<int> agg(cur.n);
-        for (int i = 0; i < cur.n; ++i) agg[i] = agg2[agg1[i]];
-        const int mc = coarse_faces(cur.m, cur.la.data(), cur.ua.data(), agg.data(), n2, fcode,
-                                    cla, cua);
         HostLevel next;
-        next.n = n2;
-        next.m = mc;
-        restrict_host(cur, agg, fcode, next);
-        next.la = cla;
-        next.ua = cua;
+        agglom::Transfer t;
+        if (!agglom::coarsen(cur, next, t)) break;   // stagnation
         // device data of the transfer cur -> next
         AmgLevel& L = A.lv.back();
-        L.nc = n2;
-        std::vector<int> cells(cur.n), ptr, list;
-        for (int i = 0; i < cur.n; ++i) cells[i] = i;
-        GIS_TRY(upload_vec(h, L.agg, agg));
-        bucket(n2, agg, cells, ptr, list);
-        GIS_TRY(upload_vec(h, L.mptr, ptr));
-        GIS_TRY(upload_vec(h, L.mlist, list));
-        std::vector<int> inside(cur.m), faces(cur.m), coarse_face(cur.m), code(cur.m);
-        for (int f = 0; f < cur.m; ++f) {
-            inside[f] = fcode[f] < 0 ? -1 - fcode[f] : -1;
-            coarse_face[f] = fcode[f] >= 0 ? fcode[f] >> 1 : -1;
-            faces[f] = f;
-            code[f] = fcode[f] >= 0 ? 2 * f + (fcode[f] & 1) : 0;
-        }
-        bucket(n2, inside, faces, ptr, list);
-        GIS_TRY(upload_vec(h, L.iptr, ptr));
-        GIS_TRY(upload_vec(h, L.ilist, list));
-        bucket(mc, coarse_face, code, ptr, list);
-        GIS_TRY(upload_vec(h, L.fptr, ptr));
-        GIS_TRY(upload_vec(h, L.flist, list));
+        L.nc = next.n;
+        GIS_TRY(upload_vec(h, L.agg, t.agg));
+        GIS_TRY(upload_vec(h, L.mptr, t.mptr));
+        GIS_TRY(upload_vec(h, L.mlist, t.mlist));
+        GIS_TRY(upload_vec(h, L.iptr, t.iptr));
+        GIS_TRY(upload_vec(h, L.ilist, t.ilist));
+        GIS_TRY(upload_vec(h, L.fptr, t.fptr));
+        GIS_TRY(upload_vec(h, L.flist, t.flist));
         // the next level as a pattern of its own
         A.lv.emplace_back();
         AmgLevel& C = A.lv.back();
-        C.n = n2;
-        C.m = mc;
+        C.n = next.n;
+        C.m = next.m;
         C.ctx = new (std::nothrow) gis_context();
         if (!C.ctx) return fail(h, kErrNoMem, "GAMG level context");
         C.ctx->device = h->device;
         C.ctx->stream = h->stream;
         C.ctx->own_stream = false;
-        GIS_TRY(upload_vec(h, C.ctx->lower_addr, cla));
-        GIS_TRY(upload_vec(h, C.ctx->upper_addr, cua));
+        GIS_TRY(upload_vec(h, C.ctx->lower_addr, next.la));
+        GIS_TRY(upload_vec(h, C.ctx->upper_addr, next.ua));
+        // the uploads read t and next, which go out of scope below
         GIS_HIP(h, hipStreamSynchronize(h->stream));
-        if (build_pattern(C.ctx, n2, mc, C.ctx->lower_addr.as<int>(),
+        if (build_pattern(C.ctx, next.n, next.m, C.ctx->lower_addr.as<int>(),
                           C.ctx->upper_addr.as<int>()) != kOK)
             return fail(h, kErrHip, "GAMG level pattern: " + C.ctx->err);
         cur = std::move(next);
@@ -1337,9 +1164,11 @@ int dic_levels(gis_context* h) {
     for (int f = M - 1; f >= 0; --f) lb[la[f]] = std::max(lb[la[f]], lb[ua[f]] + 1);
     auto group = [&](const std::vector<int>& lev, std::vector<int>& ptr, DevMem& cells) -> int {
         const int nl = N > 0 ? 1 + *std::max_element(lev.begin(), lev.end()) : 0;
-        std::vector<int> all(N), list;
+        std::vector<int> all(N);
         for (int i = 0; i < N; ++i) all[i] = i;
-        bucket(nl, lev, all, ptr, list);
+        agglom::hvec<int> p, list;
+        agglom::bucket(nl, lev, all, p, list);
+        ptr.assign(p.begin(), p.end());
         return upload_vec(h, cells, list);
     };
     GIS_TRY(group(lf, A.dic_ptr_f, A.dic_cells_f));

@@ -63,6 +63,29 @@ def test_gamg_hierarchy_matches_the_oracle(gpu, n, symmetric):
             np.testing.assert_array_equal(a["agg"], b["agg"], err_msg=f"level {l} agg")
 
 
+@pytest.mark.parametrize("case", ["random", "renumbered"])
+def test_gamg_hierarchy_irregular_graphs(gpu, case):
+    """The parallel host set-up on graphs with irregular rows: a random
+    unstructured pattern, and a structured mesh under a random cell numbering."""
+    if case == "random":
+        s = mesh.random_ldu(30000, 110000, seed=8, symmetric=True, shift=0.05)
+    else:
+        base = mesh.laplacian(28, seed=9, shift=1e-3)
+        s = mesh.renumber(base, np.random.default_rng(9).permutation(base.n_cells))
+    print(f"[case] GAMG hierarchy on {case} n={s.n_cells} faces={s.n_faces}", flush=True)
+    gpu.set_system(s)
+    ref = O.Amg(s)
+    assert gpu.gamg_levels() == ref.n_levels > 2
+    for l in range(ref.n_levels):
+        a, b = gpu.gamg_level(l), ref.level(l)
+        assert (a["n"], a["m"]) == (b["n"], b["m"]), l
+        for k in ("la", "ua", "diag", "lower", "upper"):
+            np.testing.assert_array_equal(a[k], b[k], err_msg=f"level {l} {k}")
+        if b["agg"] is not None:
+            np.testing.assert_array_equal(a["agg"], b["agg"], err_msg=f"level {l} agg")
+    _check(gpu, "GAMG", s, s.x0, 0.0, 8)
+
+
 @pytest.mark.parametrize("pc", ["GAMG", "GAMG32"])
 @pytest.mark.parametrize("n,seed", [(4, 0), (8, 1), (16, 2), (24, 3), (40, 4)])
 def test_pcg_gamg_bit_exact(gpu, pc, n, seed):
