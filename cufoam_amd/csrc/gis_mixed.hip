@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kWG) void k_mx1(Mat<float> m, int N, const float* _
         float rv[4], pv[4], xv[4], pn[4], qv[4];
         load4(r, t.i0, N, rv);
         if (!first || !folded) load4(p_old, t.i0, N, pv);
-        if (!folded) load4(xc, t.i0, N, xv);
+        if (!folded) load4nt(xc, t.i0, N, xv);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             pn[kk] = first ? rv[kk] : fmaf(bf, pv[kk], rv[kk]);
@@ -137,9 +137,11 @@ __global__ __launch_bounds__(kWG) void k_mx1(Mat<float> m, int N, const float* _
             xv[kk] = folded ? 0.0f : fmaf(af, pv[kk], xv[kk]);
             if (t.i0 + kk < N) leaf[c][0] = fma((double)pn[kk], (double)qv[kk], leaf[c][0]);
         }
-        store4(p_new, t.i0, N, pn);
+        // non-temporal stores (and xc's load): measured at 256^3, 209.7 -> 205.2 us
+        // per inner iteration against cached ones
+        store4nt(p_new, t.i0, N, pn);
         store4nt(q, t.i0, N, qv);
-        store4(xc, t.i0, N, xv);
+        store4nt(xc, t.i0, N, xv);
     }
     SlotReduce<1, C> tr;
     tr.begin(red, sup.tile0, sup.nt, leaf, shr);
