@@ -288,9 +288,11 @@ void pattern_schedule(gis_context* h) {
     const bool d = h->lay_table_slices > 0 && h->lay_untabled_slots * 20 <= h->n_slots;
     const SpmvKind k = d ? SpmvKind::SellD : SpmvKind::Sell;
     s.spmv64 = s.spmv32 = k;
-    // FP64: the separate direction pass; FP32: the direction formed at the
-    // gathers (the forms that measured fastest on structured meshes)
-    s.fp64_direction = d ? 1 : 0;
+    // FP64: the separate direction pass from 2^19 rows up, the direction formed
+    // at the gathers below (one launch fewer per iteration; FP64 CG per
+    // iteration, fused / split: 64^3 18.3 / 20.6 us, 96^3 38.2 / 36.6 us,
+    // 256^3 457.6 / 417.1 us); FP32: formed at the gathers (equal at 256^3)
+    s.fp64_direction = (d && h->N >= (1 << 19)) ? 1 : 0;
     s.fp32_direction = 0;
     s.coarsen64 = s.coarsen32 = 1;
     std::lock_guard<std::mutex> lk(g_cache_mu);

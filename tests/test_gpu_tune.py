@@ -51,16 +51,16 @@ def test_autotune_rejects_bad_targets(gpu):
 
 def test_pattern_picks_its_schedule_without_timing(gpu):
     """gis_set_pattern starts a structured mesh (every 64-row slice tabled) on
-    SellD (FP64 with the separate direction pass, FP32 with the direction formed
-    at the gathers), an unstructured one on sliced rows;
-    gis_get_layout reports the counts the SellD byte model uses."""
+    SellD (FP64 with the separate direction pass from 2^19 rows up, the
+    direction formed at the gathers below and in FP32), an unstructured one on
+    sliced rows; gis_get_layout reports the counts the SellD byte model uses."""
     from cufoam_amd import mesh
     from cufoam_amd.gis import Gis
     Gis.autotune_clear()
     s = mesh.laplacian(32, seed=0, shift=1e-3)
     gpu.set_system(s)
     sch = gpu.get_schedule()
-    assert (sch["spmv64"], sch["spmv32"], sch["fp64_direction"], sch["fp32_direction"]) == (6, 6, 1, 0)
+    assert (sch["spmv64"], sch["spmv32"], sch["fp64_direction"], sch["fp32_direction"]) == (6, 6, 0, 0)
     lay = gpu.layout()
     assert lay["n_rows"] == 32768 and lay["nnz"] == s.nnz and lay["slices"] == 512
     assert lay["table_slices"] == 512 and lay["untabled_slots"] == 0
@@ -70,6 +70,9 @@ def test_pattern_picks_its_schedule_without_timing(gpu):
     assert 5 * 512 <= lay["table_entries"] <= 7 * 512
     csr = s.nnz * 12 + 4 * (s.n_cells + 1) + 16 * s.n_cells
     assert gpu.spmv_bytes(8, 8, 6) < csr
+    big = mesh.laplacian(81, seed=0, shift=1e-3)   # 531,441 rows
+    gpu.set_system(big)
+    assert gpu.get_schedule()["fp64_direction"] == 1
     r = mesh.random_ldu(5000, 15000, seed=1)
     gpu.set_system(r)
     assert gpu.get_schedule()["spmv64"] == 0
