@@ -346,6 +346,41 @@ def pcg_128():
     return out
 
 
+def pbicg_128():
+    """Preconditioners for BiCG on the 128^3 nonsymmetric convection-diffusion system
+    (x0 = 0, to ||r|| <= 1e-10 ||b||): none, diagonal and DILU (OpenFOAM's incomplete
+    LU, sweeps level by level).  Each is timed once after a one-iteration call that
+    builds its set-up."""
+    import numpy as np
+    import torch
+    from cufoam_amd import mesh
+    from cufoam_amd.gis import Gis
+
+    s = mesh.config_system("convdiff128")
+    g = Gis(0)
+    g.set_system(s)
+    b = torch.from_numpy(s.b).cuda()
+    err = 1e-10 * float(np.linalg.norm(s.b))
+    out = {}
+    for pc in ("none", "diagonal", "DILU"):
+        t0 = time.perf_counter()
+        g.solve_pbicg(pc, b, torch.zeros_like(b), error=0.0, max_steps=1)
+        t_setup = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, hist, st = g.solve_pbicg(pc, b, torch.zeros_like(b), error=err, max_steps=20000,
+                                    hist_cap=20001)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        it = len(hist) - 1
+        out[pc] = {"s": round(t, 4), "iterations": it,
+                   "ms_per_iteration": round(t / max(it, 1) * 1e3, 3),
+                   "first_call_s": round(t_setup, 3), "status": st}
+    g.close()
+    log(f"[bench] pbicg 128^3: {out}")
+    return out
+
+
 def unstructured_spmv():
     """SpMV where the offset tables do not apply: the 128^3 SPD system with its cells
     renumbered at random (every row's neighbours scattered over the whole vector)
@@ -702,6 +737,7 @@ def main():
     plumb = plumbing_32() if args.dropin else None
     unstructured = unstructured_spmv() if args.dropin else None
     pcg128 = pcg_128() if args.pcg else None
+    pbicg128 = pbicg_128() if args.pcg else None
 
     cpu = None
     if not args.no_cpu:
@@ -785,6 +821,8 @@ def main():
         out["pcg_gamg_to_1e-10"] = pcg
     if pcg128 is not None:
         out["pcg_128_preconditioners"] = pcg128
+    if pbicg128 is not None:
+        out["pbicg_128_preconditioners"] = pbicg128
     print(json.dumps(out), flush=True)
 
 

@@ -569,7 +569,8 @@ static int pcg_impl(gis_handle h, int precond, double* x, const double* b,
     return solve_staged(h, a, x, b, dev, hist, hist_cap, hist_len, nullptr, nullptr);
 }
 
-// GIS_PRECOND_NONE: BiCG itself; GIS_PRECOND_DIAGONAL: BiCG preconditioned by D
+// GIS_PRECOND_NONE: BiCG itself; GIS_PRECOND_DIAGONAL: BiCG preconditioned by D;
+// GIS_PRECOND_DILU: by OpenFOAM's DILU
 static int pbicg_method(gis_handle h, int precond, int* method) {
     if (precond == GIS_PRECOND_NONE) {
         *method = GIS_BICG;
@@ -579,7 +580,11 @@ static int pbicg_method(gis_handle h, int precond, int* method) {
         *method = kMethodPbicgDiag;
         return GIS_OK;
     }
-    return fail(h, GIS_ERR_ARG, "PBiCG takes the none or diagonal preconditioner");
+    if (precond == GIS_PRECOND_DILU) {
+        *method = kMethodPbicgDilu;
+        return GIS_OK;
+    }
+    return fail(h, GIS_ERR_ARG, "PBiCG takes the none, diagonal or DILU preconditioner");
 }
 
 static int pbicg_impl(gis_handle h, int precond, double* x, const double* b,

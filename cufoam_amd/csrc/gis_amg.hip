@@ -1040,6 +1040,7 @@ void AmgData::clear() {
     values32_ok = false;
     dic_levels = false;
     dic_values = false;
+    dic_kind = 0;
     dic_ptr_f.clear();
     dic_ptr_b.clear();
     dic_cells_f.release();
@@ -1061,6 +1062,11 @@ void AmgData::clear() {
 // order (the CSR row's lower entries, perm codes f), its upper faces in reverse
 // face order.  Every value is then formed by the same operations in the same
 // order as the face loops form it.
+//
+// DILU (OpenFOAM's DILUPreconditioner, for asymmetric matrices) runs the same
+// kernels with other coefficient arrays: rD[u] -= upper[f] lower[f] / rD[l];
+// the forward sweep reads lower, the backward one upper; M^-T exchanges them.
+// With lower == upper its bits are DIC's.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlk) void k_dic_factor(int ncell, const int* __restrict__ cells,
                                                      const int* __restrict__ rowptr,
@@ -1069,6 +1075,7 @@ __global__ __launch_bounds__(kBlk) void k_dic_factor(int ncell, const int* __res
                                                      const int* __restrict__ perm,
                                                      const double* __restrict__ diag,
                                                      const double* __restrict__ upper,
+                                                     const double* __restrict__ lower,
                                                      double* __restrict__ F) {
     const int i = blockIdx.x * kBlk + threadIdx.x;
     if (i >= ncell) return;
@@ -1076,10 +1083,7 @@ __global__ __launch_bounds__(kBlk) void k_dic_factor(int ncell, const int* __res
     double f = diag[c];
     const int a = rowptr[c];
     const int e = a + nlow[c];
-    for (int p = a; p < e; ++p) {
-        const double u = upper[perm[p]];
-        f = f - u * u / F[col[p]];
-    }
+    for (int p = a; p < e; ++p) f = f - upper[perm[p]] * lower[perm[p]] / F[col[p]];
     F[c] = f;
 }
 
@@ -1089,7 +1093,7 @@ __global__ __launch_bounds__(kBlk) void k_dic_fwd(int ncell, const int* __restri
                                                   const int* __restrict__ col,
                                                   const int* __restrict__ perm,
                                                   const double* __restrict__ rD,
-                                                  const double* __restrict__ upper,
+                                                  const double* __restrict__ cf,
                                                   const double* __restrict__ r,
                                                   double* __restrict__ z, const KState* st) {
     if (st->done) return;
@@ -1100,7 +1104,7 @@ __global__ __launch_bounds__(kBlk) void k_dic_fwd(int ncell, const int* __restri
     double w = d * r[c];
     const int a = rowptr[c];
     const int e = a + nlow[c];
-    for (int p = a; p < e; ++p) w = w - d * upper[perm[p]] * z[col[p]];
+    for (int p = a; p < e; ++p) w = w - d * cf[perm[p]] * z[col[p]];
     z[c] = w;
 }
 
@@ -1110,7 +1114,7 @@ __global__ __launch_bounds__(kBlk) void k_dic_bwd(int ncell, const int* __restri
                                                   const int* __restrict__ col,
                                                   const int* __restrict__ perm, int base,
                                                   const double* __restrict__ rD,
-                                                  const double* __restrict__ upper,
+                                                  const double* __restrict__ cb,
                                                   double* __restrict__ z, const KState* st) {
     if (st->done) return;
     const int i = blockIdx.x * kBlk + threadIdx.x;
@@ -1119,7 +1123,7 @@ __global__ __launch_bounds__(kBlk) void k_dic_bwd(int ncell, const int* __restri
     const double d = rD[c];
     double w = z[c];
     const int b0 = rowptr[c] + nlow[c] + 1;   // after the diagonal: upper faces, face order
-    for (int p = rowptr[c + 1] - 1; p >= b0; --p) w = w - d * upper[perm[p] - base] * z[col[p]];
+    for (int p = rowptr[c + 1] - 1; p >= b0; --p) w = w - d * cb[perm[p] - base] * z[col[p]];
     z[c] = w;
 }
 
@@ -1177,10 +1181,11 @@ int dic_levels(gis_context* h) {
     return kOK;
 }
 
-int dic_prepare(gis_context* h) {
+// kind 4: DIC (the factor from upper alone), 5: DILU (upper and lower)
+int ilu_prepare(gis_context* h, int kind) {
     if (h->n_cpl > 0)
-        return fail(h, kErrState, "DIC does not take interface entries (gis_set_interfaces)");
-    if (h->batch.ready) return fail(h, kErrState, "DIC needs a single-system pattern");
+        return fail(h, kErrState, "DIC/DILU do not take interface entries (gis_set_interfaces)");
+    if (h->batch.ready) return fail(h, kErrState, "DIC/DILU need a single-system pattern");
     if (!h->amg) {
         h->amg = new (std::nothrow) AmgData();
         if (!h->amg) return fail(h, kErrNoMem, "preconditioner data");
@@ -1190,7 +1195,7 @@ int dic_prepare(gis_context* h) {
         GIS_TRY(dic_levels(h));
         A.dic_values = false;
     }
-    if (!A.dic_values) {
+    if (!A.dic_values || A.dic_kind != kind) {
         const int N = h->N;
         GIS_HIP(h, A.dic_rd.ensure(sizeof(double) * (size_t)N));
         GIS_HIP(h, h->dinv.ensure(sizeof(double) * (size_t)N));
@@ -1201,18 +1206,23 @@ int dic_prepare(gis_context* h) {
             hipLaunchKernelGGL(k_dic_factor, grid_of(n), dim3(kBlk), 0, h->stream, n,
                                A.dic_cells_f.as<int>() + A.dic_ptr_f[l], h->rowptr.as<int>(),
                                h->nbr_cnt.as<int>(), h->col.as<int>(), h->perm.as<int>(),
-                               h->diag.as<double>(), h->upper.as<double>(), F);
+                               h->diag.as<double>(), h->upper.as<double>(),
+                               kind == 5 ? h->lower.as<double>() : h->upper.as<double>(), F);
         }
         hipLaunchKernelGGL(k_recip, grid_of(N), dim3(kBlk), 0, h->stream, N, F,
                            A.dic_rd.as<double>());
         GIS_HIP(h, hipGetLastError());
         A.dic_values = true;
+        A.dic_kind = kind;
     }
     return kOK;
 }
 
-// z = M^-1 r by the two sweeps, level by level
-int dic_apply(gis_context* h, const double* r, double* z) {
+int dic_prepare(gis_context* h) { return ilu_prepare(h, 4); }
+
+// z = M^-1 r by the two sweeps, level by level: cf the forward sweep's
+// coefficients (below the diagonal), cb the backward sweep's (above it)
+int ilu_apply(gis_context* h, const double* r, double* z, const double* cf, const double* cb) {
     AmgData& A = *h->amg;
     const int nf = (int)A.dic_ptr_f.size() - 1, nb = (int)A.dic_ptr_b.size() - 1;
     for (int l = 0; l < nf; ++l) {
@@ -1220,17 +1230,21 @@ int dic_apply(gis_context* h, const double* r, double* z) {
         hipLaunchKernelGGL(k_dic_fwd, grid_of(n), dim3(kBlk), 0, h->stream, n,
                            A.dic_cells_f.as<int>() + A.dic_ptr_f[l], h->rowptr.as<int>(),
                            h->nbr_cnt.as<int>(), h->col.as<int>(), h->perm.as<int>(),
-                           A.dic_rd.as<double>(), h->upper.as<double>(), r, z, h->st);
+                           A.dic_rd.as<double>(), cf, r, z, h->st);
     }
     for (int l = 0; l < nb; ++l) {
         const int n = A.dic_ptr_b[l + 1] - A.dic_ptr_b[l];
         hipLaunchKernelGGL(k_dic_bwd, grid_of(n), dim3(kBlk), 0, h->stream, n,
                            A.dic_cells_b.as<int>() + A.dic_ptr_b[l], h->rowptr.as<int>(),
                            h->nbr_cnt.as<int>(), h->col.as<int>(), h->perm.as<int>(),
-                           h->M + h->N, A.dic_rd.as<double>(), h->upper.as<double>(), z, h->st);
+                           h->M + h->N, A.dic_rd.as<double>(), cb, z, h->st);
     }
     GIS_HIP(h, hipGetLastError());
     return kOK;
+}
+
+int dic_apply(gis_context* h, const double* r, double* z) {
+    return ilu_apply(h, r, z, h->upper.as<double>(), h->upper.as<double>());
 }
 
 int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const double* b) {

@@ -141,6 +141,7 @@ struct AmgData {
     bool dic_values = false;      // rD of the current values
     std::vector<int> dic_ptr_f, dic_ptr_b;   // level offsets (host: launch sizes)
     DevMem dic_cells_f, dic_cells_b, dic_rd;
+    int dic_kind = 0;             // the factor in dic_rd: 4 DIC, 5 DILU
     void clear();
     ~AmgData();
 };
@@ -310,6 +311,12 @@ struct SolveArgs {
 // the diagonal.
 constexpr int kMethodPcg0 = 8;
 constexpr int kMethodPbicgDiag = 13;
+constexpr int kMethodPbicgDilu = 14;
+// DIC / DILU (gis_amg.hip): the reciprocal factored diagonal for kind 4 (DIC)
+// or 5 (DILU), and z = M^-1 r by the level-scheduled sweeps with the forward
+// sweep's coefficients cf and the backward sweep's cb
+int ilu_prepare(gis_context* h, int kind);
+int ilu_apply(gis_context* h, const double* r, double* z, const double* cf, const double* cb);
 // Preconditioned CG (gis_amg.hip), as device_solve methods kMethodPcg0 + pc.
 int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const double* b);
 int amg_levels(gis_context* h, int* n_levels);

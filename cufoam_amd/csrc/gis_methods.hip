@@ -403,8 +403,9 @@ __global__ __launch_bounds__(kWG) void k_bicg1s(Mat<double> m, int N,
     });
 }
 
-// PRE: rho = (D^-1 r, rs) from dinv (preconditioned BiCG, M = D); else (r, rs).
-template <class P, bool PRE>
+// RHO 0: rho = (r, rs); 1: rho = (D^-1 r, rs) from dinv (BiCG preconditioned by
+// M = D); 2: rho is left to a later kernel (k_pbicg_rz, after M^-1 r and M^-T rs).
+template <class P, int RHO>
 __global__ __launch_bounds__(kWG) void k_bicg2(Mat<double> mt, int N, double* __restrict__ r,
                                                double* __restrict__ rs,
                                                const double* __restrict__ pb0,
@@ -428,7 +429,7 @@ __global__ __launch_bounds__(kWG) void k_bicg2(Mat<double> mt, int N, double* __
     load4(r, t.i0, N, rv);
     load4(q, t.i0, N, qv);
     load4(rs, t.i0, N, rsv);
-    if (PRE) load4(dinv, t.i0, N, dv);
+    if (RHO == 1) load4(dinv, t.i0, N, dv);
     double leaf[2] = {0.0, 0.0};
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) {
@@ -437,7 +438,7 @@ __global__ __launch_bounds__(kWG) void k_bicg2(Mat<double> mt, int N, double* __
         rsv[kk] = fma(-xi, sh_y[threadIdx.x * kLeaf + kk], rsv[kk]);
         if (t.i0 + kk < N) {
             leaf[0] = fma(rv[kk], rv[kk], leaf[0]);
-            leaf[1] = fma(PRE ? dv[kk] * rv[kk] : rv[kk], rsv[kk], leaf[1]);
+            if (RHO < 2) leaf[1] = fma(RHO == 1 ? dv[kk] * rv[kk] : rv[kk], rsv[kk], leaf[1]);
         }
     }
     store4(x, t.i0, N, xv);
@@ -448,7 +449,7 @@ __global__ __launch_bounds__(kWG) void k_bicg2(Mat<double> mt, int N, double* __
         record_norm(st, norm);
         if (stop_after(st, norm < st->error)) {
             st->done = 1;
-        } else {
+        } else if (RHO < 2) {
             st->rho1 = st->rho0;
             st->rho0 = v[1];
             st->beta = st->rho0 / st->rho1;
@@ -515,6 +516,61 @@ __global__ __launch_bounds__(kWG) void k_pbicg_dir(int N, const double* __restri
     }
     store4(p_new, t.i0, N, pv);
     store4(ps, t.i0, N, psv);
+}
+
+// Preconditioned BiCG with a general M (DILU): the directions from z = M^-1 r
+// and zs = M^-T rs, formed by the sweeps; rho = (z, rs) by k_pbicg_rz.
+__global__ __launch_bounds__(kWG) void k_pbicg_dirz(int N, const double* __restrict__ z,
+                                                    const double* __restrict__ zs,
+                                                    double* __restrict__ pb0,
+                                                    double* __restrict__ pb1,
+                                                    double* __restrict__ ps, const KState* st) {
+    if (st->done) return;
+    StreamRows t(N);
+    const int k = st->k + 1;
+    const double* p_old = (k & 1) ? pb0 : pb1;
+    double* p_new = (k & 1) ? pb1 : pb0;
+    double zv[4], zsv[4];
+    load4(z, t.i0, N, zv);
+    load4(zs, t.i0, N, zsv);
+    if (k == 1) {
+        store4(p_new, t.i0, N, zv);
+        store4(ps, t.i0, N, zsv);
+        return;
+    }
+    const double theta = st->beta;
+    double pv[4], psv[4];
+    load4(p_old, t.i0, N, pv);
+    load4(ps, t.i0, N, psv);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        pv[kk] = fma(theta, pv[kk], zv[kk]);
+        psv[kk] = fma(theta, psv[kk], zsv[kk]);
+    }
+    store4(p_new, t.i0, N, pv);
+    store4(ps, t.i0, N, psv);
+}
+
+// rho = (z, rs); FIRST: the set-up's rho, else also theta = rho / rho_prev
+template <int FIRST>
+__global__ __launch_bounds__(kWG) void k_pbicg_rz(int N, const double* __restrict__ z,
+                                                  const double* __restrict__ rs, KState* st,
+                                                  RedWs red) {
+    if (st->done) return;
+    __shared__ RedShared shr;
+    TileRows t(N);
+    double zv[4], rsv[4];
+    load4(z, t.i0, N, zv);
+    load4(rs, t.i0, N, rsv);
+    double leaf[1] = {0.0};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+        if (t.i0 + kk < N) leaf[0] = fma(zv[kk], rsv[kk], leaf[0]);
+    finish_tile_slots<1>(red, st, t.tile, leaf, shr, [&](const double (&v)[1]) {
+        if (!FIRST) st->rho1 = st->rho0;
+        st->rho0 = v[0];
+        if (!FIRST) st->beta = st->rho0 / st->rho1;
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -659,7 +715,7 @@ int solve_bicg(gis_context* h, const SolveArgs& a, double* x, const double* b) {
                 hipLaunchKernelGGL((k_bicg1<P>), g, dim3(kWG), 0, h->stream, m, N, r, rs, pb0,
                                    pb1, ps, q, h->st, h->red);
             }
-            hipLaunchKernelGGL((k_bicg2<P, false>), g, dim3(kWG), 0, h->stream, mt, N, r, rs,
+            hipLaunchKernelGGL((k_bicg2<P, 0>), g, dim3(kWG), 0, h->stream, mt, N, r, rs,
                                pb0, pb1, ps, q, nullptr, x, h->st, h->red);
             return kOK;
         }));
@@ -694,10 +750,54 @@ int solve_pbicg_diag(gis_context* h, const SolveArgs& a, double* x, const double
                                pb1, ps, h->st);
             hipLaunchKernelGGL((k_bicg1s<P>), g, dim3(kWG), 0, h->stream, m, N, pb0, pb1, ps,
                                q, h->st, h->red);
-            hipLaunchKernelGGL((k_bicg2<P, true>), g, dim3(kWG), 0, h->stream, mt, N, r, rs,
+            hipLaunchKernelGGL((k_bicg2<P, 1>), g, dim3(kWG), 0, h->stream, mt, N, r, rs,
                                pb0, pb1, ps, q, dinv, x, h->st, h->red);
             return kOK;
         }));
+    });
+    return kOK;
+}
+
+// BiCG preconditioned by OpenFOAM's DILU (no counterpart in the reference):
+// z = M^-1 r and zs = M^-T rs by the level-scheduled sweeps of gis_amg.hip (the
+// forward sweep reads lower, the backward one upper; M^-T exchanges them).
+int solve_pbicg_dilu(gis_context* h, const SolveArgs& a, double* x, const double* b) {
+    GIS_TRY(ensure_valT(h));
+    GIS_TRY(ilu_prepare(h, 5));
+    GIS_TRY(prepare_solve(h, a, 8));
+    const int N = h->N;
+    double* r = h->vec.as<double>();
+    double* rs = r + N;
+    double* pb0 = r + 2 * (size_t)N;
+    double* pb1 = r + 3 * (size_t)N;
+    double* ps = r + 4 * (size_t)N;
+    double* q = r + 5 * (size_t)N;
+    double* z = r + 6 * (size_t)N;
+    double* zs = r + 7 * (size_t)N;
+    const double* lo = h->lower.as<double>();
+    const double* up = h->upper.as<double>();
+    const Mat<double> m = mat64(h);
+    const Mat<double> mt = mat64T(h);
+    GIS_TRY(launch_setup(h, m, b, x, nullptr, r, rs, 1));
+    const dim3 g = tiles_grid(N);
+    GIS_TRY(ilu_apply(h, r, z, lo, up));
+    GIS_TRY(ilu_apply(h, rs, zs, up, lo));
+    hipLaunchKernelGGL(k_pbicg_rz<1>, g, dim3(kWG), 0, h->stream, N, z, rs, h->st, h->red);
+    GIS_HIP(h, hipGetLastError());
+    GIS_SPMV_DISPATCH(h->sched.spmv64, P, {
+        GIS_TRY(run_blocks(h, iteration_budget(h), [&]() -> int {
+            hipLaunchKernelGGL(k_pbicg_dirz, g, dim3(kWG), 0, h->stream, N, z, zs, pb0, pb1, ps,
+                               h->st);
+            hipLaunchKernelGGL((k_bicg1s<P>), g, dim3(kWG), 0, h->stream, m, N, pb0, pb1, ps,
+                               q, h->st, h->red);
+            hipLaunchKernelGGL((k_bicg2<P, 2>), g, dim3(kWG), 0, h->stream, mt, N, r, rs,
+                               pb0, pb1, ps, q, nullptr, x, h->st, h->red);
+            GIS_TRY(ilu_apply(h, r, z, lo, up));
+            GIS_TRY(ilu_apply(h, rs, zs, up, lo));
+            hipLaunchKernelGGL(k_pbicg_rz<0>, g, dim3(kWG), 0, h->stream, N, z, rs, h->st,
+                               h->red);
+            return kOK;
+        }, true));
     });
     return kOK;
 }
@@ -722,6 +822,7 @@ int device_solve(gis_context* h, const SolveArgs& a, double* d_x, const double* 
         case kMethodPcg0 + 3:
         case kMethodPcg0 + 4: st = solve_pcg(h, a, a.method - kMethodPcg0, d_x, d_b); break;
         case kMethodPbicgDiag: st = solve_pbicg_diag(h, a, d_x, d_b); break;
+        case kMethodPbicgDilu: st = solve_pbicg_dilu(h, a, d_x, d_b); break;
         default: return fail(h, kErrArg, "unknown solver method");
     }
     if (st != kOK) return st;

@@ -82,7 +82,7 @@ _TABLE = {
     # "preconditioner": none / diagonal / GAMG), for symmetric matrices
     "cuFoamPCG": ("PCG", True, False),
     # no counterpart in the reference: BiCG with a preconditioner (none /
-    # diagonal), for symmetric and asymmetric matrices
+    # diagonal / DILU), for symmetric and asymmetric matrices
     "cuFoamPBiCG": ("PBiCG", True, True),
 }
 
@@ -111,12 +111,16 @@ class cuFoamSolver:
             raise FatalError(f"convergence must be reference or openfoam, not "
                              f"{self.convergence}")
         self.preconditioner = str(controls.get("preconditioner", "none"))
-        if self.preconditioner not in ("none", "diagonal", "DIC", "GAMG", "GAMG32"):
-            raise FatalError(f"preconditioner must be none, diagonal, DIC, GAMG or GAMG32, not "
+        if self.preconditioner not in ("none", "diagonal", "DIC", "DILU", "GAMG", "GAMG32"):
+            raise FatalError(f"preconditioner must be none, diagonal, DIC, DILU, GAMG or GAMG32, "
+                             f"not {self.preconditioner}")
+        if self.typeName == "cuFoamPBiCG" and self.preconditioner not in ("none", "diagonal",
+                                                                          "DILU"):
+            raise FatalError(f"cuFoamPBiCG takes the none, diagonal or DILU preconditioner, not "
                              f"{self.preconditioner}")
-        if self.typeName == "cuFoamPBiCG" and self.preconditioner not in ("none", "diagonal"):
-            raise FatalError(f"cuFoamPBiCG takes the none or diagonal preconditioner, not "
-                             f"{self.preconditioner}")
+        if self.typeName == "cuFoamPCG" and self.preconditioner == "DILU":
+            raise FatalError("cuFoamPCG takes none, diagonal, DIC, GAMG or GAMG32; DILU is for "
+                             "asymmetric matrices (cuFoamPBiCG)")
         self.couple = str(controls.get("interfaces", "ignored"))
         if self.couple not in ("ignored", "coupled"):
             raise FatalError(f"interfaces must be ignored or coupled, not {self.couple}")

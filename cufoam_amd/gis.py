@@ -23,7 +23,7 @@ ERR_SYNC = 13   # a device reduction did not complete: an error, raised
 
 METHODS = {"CG": 0, "BiCG": 1, "Jacobi": 2, "MR": 3, "MRJacobi": 4, "JMR": 5, "MRS": 6,
            "Richardson": 7}
-PRECONDITIONERS = {"none": 0, "diagonal": 1, "GAMG": 2, "GAMG32": 3, "DIC": 4}
+PRECONDITIONERS = {"none": 0, "diagonal": 1, "GAMG": 2, "GAMG32": 3, "DIC": 4, "DILU": 5}
 
 
 class GisError(RuntimeError):
@@ -559,7 +559,7 @@ class Gis:
     def solve_pbicg(self, precond: str, b, x0=None, error: float = 1e-4, max_steps: int = 0,
                     int_bound: float = 1e-10, hist_cap: int | None = None,
                     detail: str = "error"):
-        """BiCG preconditioned by none / diagonal -> (x, hist, status).
+        """BiCG preconditioned by none / diagonal / DILU -> (x, hist, status).
         Breakdown (status 11) is returned, not raised, as for solve("BiCG")."""
         p = _Params(error, max_steps, 0.0, 30, int_bound, DETAILS[detail])
         if hist_cap is None:

@@ -1,5 +1,5 @@
 /*
- * cuFoamPBiCG: BiCG with a none / diagonal preconditioner (symmetric and
+ * cuFoamPBiCG: BiCG with a none / diagonal / DILU preconditioner (symmetric and
  * asymmetric matrices; no counterpart in the reference) on the GPU through the
  * gis library, selected as "cuFoamPBiCG" in fvSolution.  Its interface is that
  * of the reference's wrappers (cuFoamBiCG/cuFoamBiCG.H:8-31).

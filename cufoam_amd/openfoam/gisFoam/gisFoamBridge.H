@@ -139,7 +139,7 @@ private:
 enum class Family {
     Plain,   // gis_foam_solve / gis_foam_solve_controls with a GIS_<method> id
     PCG,     // gis_foam_solve_pcg with a GIS_PRECOND_<name> id
-    PBiCG,   // gis_foam_solve_pbicg with GIS_PRECOND_NONE or GIS_PRECOND_DIAGONAL
+    PBiCG,   // gis_foam_solve_pbicg with GIS_PRECOND_NONE, _DIAGONAL or _DILU
 };
 
 // cuFoam<Name>::solve: psi (host, nCells) is updated in place.  The handle of
@@ -193,7 +193,7 @@ inline Outcome solvePcg(int precond, const Addressing& a, const Coefficients& c,
     return solveAs(Family::PCG, precond, a, c, psi, source, ctl);
 }
 
-// Preconditioned BiCG (GIS_PRECOND_NONE / DIAGONAL).
+// Preconditioned BiCG (GIS_PRECOND_NONE / DIAGONAL / DILU).
 inline Outcome solvePbicg(int precond, const Addressing& a, const Coefficients& c,
                           double* psi, const double* source, const Controls& ctl) {
     return solveAs(Family::PBiCG, precond, a, c, psi, source, ctl);

@@ -21,8 +21,9 @@ int Foam::gisLduSolver::preconditionerId() const {
     if (name == "GAMG") return GIS_PRECOND_GAMG;
     if (name == "GAMG32") return GIS_PRECOND_GAMG32;
     if (name == "DIC") return GIS_PRECOND_DIC;
+    if (name == "DILU") return GIS_PRECOND_DILU;
     FatalErrorInFunction << "unknown preconditioner " << name
-                         << " (valid: none diagonal DIC GAMG GAMG32)" << abort(FatalError);
+                         << " (valid: none diagonal DIC DILU GAMG GAMG32)" << abort(FatalError);
     return GIS_PRECOND_NONE;
 }
 

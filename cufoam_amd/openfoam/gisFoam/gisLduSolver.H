@@ -12,7 +12,7 @@
  *                and minIter too, through gis_foam_solve_controls)
  *   device       GPU index (default 0)
  *   preconditioner  none | diagonal | DIC | GAMG | GAMG32  (cuFoamPCG);
- *                   none | diagonal  (cuFoamPBiCG)
+ *                   none | diagonal | DILU  (cuFoamPBiCG)
  */
 #ifndef GIS_LDU_SOLVER_H
 #define GIS_LDU_SOLVER_H

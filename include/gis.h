@@ -288,6 +288,13 @@ GIS_API int gis_set_coupled_values_device(gis_handle h, const double *d_diag,
  * one launch per level: about 3n launches per application on an n^3 mesh.
  * Symmetric matrices (it reads upper); no interface entries. */
 #define GIS_PRECOND_DIC 4
+/* OpenFOAM's DILU (diagonal incomplete LU) for asymmetric matrices, from its
+ * published algorithm: rD = 1 / (diag - sum over lower faces of
+ * upper lower rD_owner) in face order; z = rD r, a forward sweep over the faces
+ * with lower, a backward sweep in reverse face order with upper; M^-T exchanges
+ * lower and upper.  The same level-scheduled sweeps as DIC.  PBiCG only
+ * (gis_solve_pbicg); gis_solve_pcg rejects it.  No interface entries. */
+#define GIS_PRECOND_DILU 5
 GIS_API int gis_solve_pcg(gis_handle h, int precond, double *x, const double *b,
                           const gis_params *params, double *hist, int hist_cap, int *hist_len);
 GIS_API int gis_solve_pcg_device(gis_handle h, int precond, double *d_x, const double *d_b,
@@ -304,8 +311,9 @@ GIS_API int gis_foam_solve_pcg(gis_handle h, int precond, double *psi, const dou
  * z* = M^-T r* in the directions (p = z + theta p, p* = z* + theta p*) and in
  * rho = (z, r*).  precond GIS_PRECOND_NONE gives GIS_BICG bit for bit;
  * GIS_PRECOND_DIAGONAL uses M = D (OpenFOAM's diagonal preconditioner: z_i =
- * r_i / a_ii); other preconditioners are GIS_ERR_ARG.  On a symmetric matrix with the diagonal preconditioner the
- * iterates are those of gis_solve_pcg(GIS_PRECOND_DIAGONAL) bit for bit. ---- */
+ * r_i / a_ii); GIS_PRECOND_DILU uses OpenFOAM's DILU; others are GIS_ERR_ARG.
+ * On a symmetric matrix the iterates with the diagonal (DILU) preconditioner
+ * are those of gis_solve_pcg with the diagonal (DIC) one, bit for bit. ---- */
 GIS_API int gis_solve_pbicg(gis_handle h, int precond, double *x, const double *b,
                             const gis_params *params, double *hist, int hist_cap,
                             int *hist_len);

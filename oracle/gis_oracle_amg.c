@@ -72,7 +72,8 @@
 #define OR_AMG_DENSE 64
 
 enum { OR_OK = 0, OR_ERR_ARG = 1, OR_ERR_NOMEM = 5 };
-enum { PC_NONE = 0, PC_DIAGONAL = 1, PC_GAMG = 2, PC_GAMG32 = 3, PC_DIC = 4 };
+enum { PC_NONE = 0, PC_DIAGONAL = 1, PC_GAMG = 2, PC_GAMG32 = 3, PC_DIC = 4, PC_DILU = 5 };
+enum { OR_BREAKDOWN_STATUS = 10 };
 
 /* from gis_oracle.c */
 double or_dot(int64_t n, const double *a, const double *b);
@@ -597,6 +598,29 @@ OR_API void or_dic_apply(int n, int m, const int *la, const int *ua, const doubl
     for (int f = m - 1; f >= 0; --f) z[la[f]] -= rD[la[f]] * upper[f] * z[ua[f]];
 }
 
+/* DILU (OpenFOAM's DILUPreconditioner, published algorithm; for asymmetric
+ * matrices): M = (D* + L) D*^-1 (D* + U) with D* chosen so that diag(M) = diag(A):
+ *   rD = diag; for f in face order: rD[u] -= upper[f] lower[f] / rD[l]; rD = 1 / rD;
+ *   z = rD r; for f in face order: z[u] -= rD[u] lower[f] z[l];
+ *   for f in reverse face order: z[l] -= rD[l] upper[f] z[u].
+ * transpose: z = M^-T r, the same sweeps with lower and upper exchanged.  With
+ * lower == upper every operation is DIC's, so the bits are DIC's. */
+OR_API void or_dilu_rd(int n, int m, const int *la, const int *ua, const double *diag,
+                       const double *lower, const double *upper, double *rD) {
+    for (int i = 0; i < n; ++i) rD[i] = diag[i];
+    for (int f = 0; f < m; ++f) rD[ua[f]] -= upper[f] * lower[f] / rD[la[f]];
+    for (int i = 0; i < n; ++i) rD[i] = 1.0 / rD[i];
+}
+
+OR_API void or_dilu_apply(int n, int m, const int *la, const int *ua, const double *rD,
+                          const double *lower, const double *upper, int transpose,
+                          const double *r, double *z) {
+    const double *cf = transpose ? upper : lower, *cb = transpose ? lower : upper;
+    for (int i = 0; i < n; ++i) z[i] = rD[i] * r[i];
+    for (int f = 0; f < m; ++f) z[ua[f]] -= rD[ua[f]] * cf[f] * z[la[f]];
+    for (int f = m - 1; f >= 0; --f) z[la[f]] -= rD[la[f]] * cb[f] * z[ua[f]];
+}
+
 typedef struct {
     int m;
     const int *la, *ua;
@@ -695,4 +719,97 @@ OR_API int or_pcg(int pc, int N, int M, const int *la, const int *ua, const doub
     free(A.rowptr); free(A.col); free(A.perm); free(A.val);
     or_amg_free(amg);
     return OR_OK;
+}
+
+/*
+ * BiCG preconditioned by M on an LDU system (no counterpart in the reference;
+ * the loop, breakdown test, stopping rule and history of gis_oracle.c's bicg,
+ * SolverSingleGPU.h:568-675): z = M^-1 r, z* = M^-T r*, rho = (z, r*),
+ * p = z or fma(theta, p, z), p* likewise from z*.  pc: PC_DIAGONAL (M = D) or
+ * PC_DILU.  A^T is A's pattern with lower and upper exchanged.  error /
+ * max_steps (<= 0 unbounded) / int_bound / rel_tol / min_steps as
+ * or_solve_ctl; returns 0, 10 (breakdown) or an error code.
+ */
+OR_API int or_pbicg(int pc, int N, int M, const int *la, const int *ua, const double *diag,
+                    const double *lower, const double *upper, const double *b, double *x,
+                    double error, int max_steps, double int_bound, double rel_tol,
+                    int64_t min_steps, double *hist, int hist_cap, int *hist_len) {
+    if ((pc != PC_DIAGONAL && pc != PC_DILU) || N < 1) return OR_ERR_ARG;
+    level_t A;
+    memset(&A, 0, sizeof(A));
+    A.n = N;
+    A.m = M;
+    A.la = (int *)la;
+    A.ua = (int *)ua;
+    if (level_csr(&A) != OR_OK) return OR_ERR_ARG;
+    const int64_t nnz = (int64_t)N + 2 * (int64_t)M;
+    double *valT = xcalloc((size_t)nnz, sizeof(double));
+    double *v = xcalloc((size_t)N * 9, sizeof(double));
+    if (!valT || !v) {
+        free(valT); free(v); free(A.rowptr); free(A.col); free(A.perm); free(A.val);
+        return OR_ERR_NOMEM;
+    }
+    or_ldu2csr_values(N, M, nnz, A.perm, diag, lower, upper, A.val);
+    or_ldu2csr_values(N, M, nnz, A.perm, diag, upper, lower, valT);
+    double *r = v, *rs = v + N, *p = v + 2 * (size_t)N, *ps = v + 3 * (size_t)N;
+    double *q = v + 4 * (size_t)N, *qs = v + 5 * (size_t)N, *z = v + 6 * (size_t)N;
+    double *zs = v + 7 * (size_t)N, *rD = v + 8 * (size_t)N;
+    if (pc == PC_DILU) {
+        or_dilu_rd(N, M, la, ua, diag, lower, upper, rD);
+    } else {
+        for (int i = 0; i < N; ++i) rD[i] = 1.0 / diag[i];
+    }
+#define PRECOND(src, dst, T)                                                      \
+    do {                                                                          \
+        if (pc == PC_DILU)                                                        \
+            or_dilu_apply(N, M, la, ua, rD, lower, upper, (T), (src), (dst));     \
+        else                                                                      \
+            for (int i_ = 0; i_ < N; ++i_) (dst)[i_] = rD[i_] * (src)[i_];        \
+    } while (0)
+    or_spmv(N, A.rowptr, A.col, A.val, x, q);
+    for (int i = 0; i < N; ++i) r[i] = b[i] - q[i];
+    memcpy(rs, r, sizeof(double) * (size_t)N);
+    PRECOND(r, z, 0);
+    PRECOND(rs, zs, 1);
+    memcpy(p, z, sizeof(double) * (size_t)N);
+    memcpy(ps, zs, sizeof(double) * (size_t)N);
+    double rho0 = or_dot(N, z, rs), rho1 = 0.0;
+    double norm = sqrt(or_dot(N, r, r));
+    int len = 0;
+    if (hist && len < hist_cap) hist[len] = norm;
+    len++;
+    if (rel_tol > 0.0) error = fmax(error, rel_tol * norm);
+    const int64_t ms = max_steps <= 0 ? INT32_MAX : max_steps;
+    int status = OR_OK;
+    for (int64_t k = 1; k <= ms || k <= min_steps; ++k) {
+        or_spmv(N, A.rowptr, A.col, A.val, p, q);
+        const double xi = rho0 / or_dot(N, ps, q);
+        if (xi > -int_bound && xi < int_bound) {
+            status = OR_BREAKDOWN_STATUS;
+            break;
+        }
+        for (int i = 0; i < N; ++i) {
+            x[i] = fma(xi, p[i], x[i]);
+            r[i] = fma(-xi, q[i], r[i]);
+        }
+        norm = sqrt(or_dot(N, r, r));
+        if (hist && len < hist_cap) hist[len] = norm;
+        len++;
+        if (norm < error && k >= min_steps) break;
+        or_spmv(N, A.rowptr, A.col, valT, ps, qs);
+        for (int i = 0; i < N; ++i) rs[i] = fma(-xi, qs[i], rs[i]);
+        PRECOND(r, z, 0);
+        rho1 = rho0;
+        rho0 = or_dot(N, z, rs);
+        const double theta = rho0 / rho1;
+        for (int i = 0; i < N; ++i) p[i] = fma(theta, p[i], z[i]);
+        PRECOND(rs, zs, 1);
+        for (int i = 0; i < N; ++i) ps[i] = fma(theta, ps[i], zs[i]);
+    }
+#undef PRECOND
+    if (hist_len) *hist_len = len;
+    free(v);
+    free(valT);
+    free(A.rowptr); free(A.col); free(A.perm); free(A.val);
+    return status;
 }

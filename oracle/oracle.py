@@ -369,7 +369,10 @@ def foam_solve(method: str, sys, psi, source, tolerance: float, max_iter: int,
     if method in ("BiCG", "PBiCG"):
         coupled = getattr(sys, "cpl_row", None) is not None and len(sys.cpl_row)
         At = mirror_transpose(A) if coupled else transpose(A)
-    if method == "PBiCG":
+    if method == "PBiCG" and preconditioner == "DILU":
+        res = pbicg_ldu("DILU", sys, source, psi, tolerance * nf, max_iter, rel_tol=rel_tol,
+                        min_steps=min_iter)
+    elif method == "PBiCG":
         res = pbicg(preconditioner, A, source, psi, tolerance * nf, max_iter, At=At,
                     rel_tol=rel_tol, min_steps=min_iter)
     else:
@@ -383,7 +386,7 @@ def foam_solve(method: str, sys, psi, source, tolerance: float, max_iter: int,
 # ---------------------------------------------------------------------------
 # preconditioned CG (none / diagonal / GAMG), gis_oracle_amg.c
 # ---------------------------------------------------------------------------
-PRECONDITIONERS = {"none": 0, "diagonal": 1, "GAMG": 2, "GAMG32": 3, "DIC": 4}
+PRECONDITIONERS = {"none": 0, "diagonal": 1, "GAMG": 2, "GAMG32": 3, "DIC": 4, "DILU": 5}
 GAMG_OMEGA, GAMG_NU, GAMG_CSCALE = 0.8, 1, 1.8
 
 
@@ -447,6 +450,60 @@ def dic_apply(sys, rd, r) -> np.ndarray:
                    _z(sys.upper_addr, np.int32), _c(rd, np.float64), _z(sys.upper, np.float64),
                    _c(r, np.float64), z)
     return z
+
+
+def dilu_rd(sys) -> np.ndarray:
+    """DILU's reciprocal factored diagonal (rD[u] -= upper lower / rD[l], face order)."""
+    L = lib()
+    L.or_dilu_rd.restype = None
+    L.or_dilu_rd.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p]
+    rd = np.zeros(sys.n_cells)
+    L.or_dilu_rd(sys.n_cells, sys.n_faces, _z(sys.lower_addr, np.int32),
+                 _z(sys.upper_addr, np.int32), _c(sys.diag, np.float64),
+                 _z(sys.lower, np.float64), _z(sys.upper, np.float64), rd)
+    return rd
+
+
+def dilu_apply(sys, rd, r, transpose: bool = False) -> np.ndarray:
+    """z = M^-1 r (transpose: M^-T r) of DILU: z = rD r, forward and backward face sweeps."""
+    L = lib()
+    L.or_dilu_apply.restype = None
+    L.or_dilu_apply.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int,
+                                _f64p, _f64p]
+    z = np.zeros(sys.n_cells)
+    L.or_dilu_apply(sys.n_cells, sys.n_faces, _z(sys.lower_addr, np.int32),
+                    _z(sys.upper_addr, np.int32), _c(rd, np.float64),
+                    _z(sys.lower, np.float64), _z(sys.upper, np.float64), int(transpose),
+                    _c(r, np.float64), z)
+    return z
+
+
+def pbicg_ldu(precond: str, sys, b, x0, error: float, max_steps: int,
+              int_bound: float = 1e-10, rel_tol: float = 0.0, min_steps: int = 0,
+              hist_cap: int | None = None) -> SolveResult:
+    """BiCG preconditioned by the diagonal or DILU on an LduSystem
+    (gis_oracle_amg.c or_pbicg); x0 is copied."""
+    if precond not in ("diagonal", "DILU"):
+        raise ValueError(f"pbicg_ldu takes diagonal or DILU, not {precond}")
+    L = lib()
+    L.or_pbicg.restype = C.c_int
+    L.or_pbicg.argtypes = [C.c_int, C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
+                           _f64p, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int64,
+                           _f64p, C.c_int, C.POINTER(C.c_int)]
+    x = _c(np.array(x0, dtype=np.float64, copy=True), np.float64)
+    if hist_cap is None:
+        hist_cap = max(max_steps if max_steps > 0 else 100000, min_steps) + 1
+    hist = np.zeros(max(hist_cap, 1))
+    hl = C.c_int(0)
+    st = L.or_pbicg(PRECONDITIONERS[precond], sys.n_cells, sys.n_faces,
+                    _z(sys.lower_addr, np.int32), _z(sys.upper_addr, np.int32),
+                    _c(sys.diag, np.float64), _z(sys.lower, np.float64),
+                    _z(sys.upper, np.float64), _c(b, np.float64), x, error, max_steps,
+                    int_bound, rel_tol, min_steps, hist, hist_cap, C.byref(hl))
+    if st not in (0, BREAKDOWN):
+        raise RuntimeError(f"or_pbicg({precond}) failed with status {st}")
+    n = min(hl.value, hist_cap)
+    return SolveResult(x, hist[:n].copy(), hl.value, st)
 
 
 def pcg(precond: str, sys, b, x0, error: float, max_steps: int, rel_tol: float = 0.0,

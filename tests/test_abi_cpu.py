@@ -79,6 +79,9 @@ def test_foam_pbicg_registration_and_preconditioner_checks_before_any_gpu_call()
     assert asym.asymmetric()
     with pytest.raises(foam.FatalError):   # PCG is registered for symmetric matrices only
         foam.New("cuFoamPCG", "U", asym, {"preconditioner": "diagonal"})
-    for pc in ("DIC", "GAMG", "GAMG32"):   # PBiCG takes none or diagonal
+    for pc in ("DIC", "GAMG", "GAMG32"):   # PBiCG takes none, diagonal or DILU
         with pytest.raises(foam.FatalError):
             foam.New("cuFoamPBiCG", "U", asym, {"preconditioner": pc})
+    sym = foam.lduMatrix.from_system(mesh.laplacian(4, seed=0, shift=0.1))
+    with pytest.raises(foam.FatalError):   # DILU is for the asymmetric solver
+        foam.New("cuFoamPCG", "p", sym, {"preconditioner": "DILU"})

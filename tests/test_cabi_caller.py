@@ -69,7 +69,9 @@ def test_compiled_caller_runs_the_wrapper_sequence_bit_exact(tmp_path):
             ("JMR", mesh.laplacian(11, seed=4, shift=0.3, symmetric=False), 1e-8, 2000),
             ("PBiCG:diagonal", mesh.random_ldu(2000, 7000, seed=5, symmetric=False), 1e-8,
              2000),
-            ("PCG:DIC", mesh.laplacian(12, seed=6, shift=1e-2), 1e-8, 2000)):
+            ("PCG:DIC", mesh.laplacian(12, seed=6, shift=1e-2), 1e-8, 2000),
+            ("PBiCG:DILU", mesh.laplacian(10, seed=7, shift=1e-2, symmetric=False), 1e-8,
+             2000)):
         cases.append((method, s, rng.uniform(-1, 1, s.n_cells), tol, max_iter))
     exe = build_caller.build()
     src, dst = tmp_path / "in.bin", tmp_path / "out.bin"

@@ -1,8 +1,9 @@
-"""GPU parity of BiCG preconditioned by the diagonal (gis_solve_pbicg,
-gis_foam_solve_pbicg, the cuFoamPBiCG wrapper; no counterpart in the
-reference) against the oracle (oracle/gis_oracle.c pbicg_diag): histories,
-iteration counts and solutions bit for bit on nonsymmetric structured and
-unstructured systems under every SpMV schedule, GIS_PRECOND_NONE = BiCG,
+"""GPU parity of BiCG preconditioned by the diagonal and by DILU
+(gis_solve_pbicg, gis_foam_solve_pbicg, the cuFoamPBiCG wrapper; no counterpart
+in the reference) against the oracle (oracle/gis_oracle.c pbicg_diag,
+oracle/gis_oracle_amg.c or_pbicg): histories, iteration counts and solutions
+bit for bit on nonsymmetric structured and unstructured systems under every
+SpMV schedule, GIS_PRECOND_NONE = BiCG, DILU on symmetric matrices = PCG-DIC,
 breakdown, and the argument checks."""
 import numpy as np
 import pytest
@@ -92,7 +93,54 @@ def test_pbicg_breakdown_is_an_outcome(gpu):
     np.testing.assert_array_equal(x, ref.x)
 
 
-@pytest.mark.parametrize("pc", ["none", "diagonal"])
+def _check_dilu(gpu, s, x0, error, max_steps):
+    ref = O.pbicg_ldu("DILU", s, s.b, x0, error, max_steps)
+    x, hist, st = gpu.solve_pbicg("DILU", s.b, x0, error=error, max_steps=max_steps,
+                                  hist_cap=ref.hist_len + 4)
+    assert st == ref.status
+    assert len(hist) == ref.hist_len, (len(hist), ref.hist_len)
+    np.testing.assert_array_equal(hist, ref.hist)
+    np.testing.assert_array_equal(x, ref.x)
+    return ref
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (4, 1), (13, 2), (24, 3)])
+def test_pbicg_dilu_bit_exact(gpu, n, seed):
+    s = _nonsym(n, seed, shift=1e-2)
+    gpu.set_system(s)
+    x0 = np.random.default_rng(seed).uniform(-1, 1, s.n_cells)
+    _check_dilu(gpu, s, x0, 0.0, 12 if n > 1 else 1)
+    ref = _check_dilu(gpu, s, s.x0, 1e-10 * np.linalg.norm(s.b), 3000)
+    assert ref.status == 0
+
+
+def test_pbicg_dilu_unstructured_values_refresh_and_schedules(gpu):
+    s = mesh.random_ldu(4000, 15000, seed=3, symmetric=False, shift=0.2)
+    print(f"[case] DILU random LDU n={s.n_cells} faces={s.n_faces}", flush=True)
+    gpu.set_system(s)
+    _check_dilu(gpu, s, s.x0, 1e-10 * np.linalg.norm(s.b), 3000)
+    # new values: the factor is recomputed
+    t = mesh.LduSystem(s.n_cells, s.lower_addr, s.upper_addr, 1.5 * s.diag, s.lower,
+                       0.5 * s.upper, s.b, s.x0)
+    gpu.set_values(t.diag, t.lower, t.upper)
+    _check_dilu(gpu, t, t.x0, 0.0, 15)
+    for sched in (0, 6):
+        gpu.set_schedule(sched, -1, -1, -1, -1, 1)
+        _check_dilu(gpu, t, t.x0, 0.0, 10)
+    gpu.set_schedule(0, -1, -1, -1, -1, 1)
+
+
+def test_pbicg_dilu_symmetric_equals_pcg_dic(gpu):
+    s = mesh.laplacian(12, seed=4, shift=1e-3)
+    gpu.set_system(s)
+    x, hist, st = gpu.solve_pbicg("DILU", s.b, s.x0, error=0.0, max_steps=30)
+    # DIC after DILU on the same handle: the factor is rebuilt for DIC (same bits here)
+    x2, hist2, st2 = gpu.solve_pcg("DIC", s.b, s.x0, error=0.0, max_steps=30)
+    np.testing.assert_array_equal(hist, hist2)
+    np.testing.assert_array_equal(x, x2)
+
+
+@pytest.mark.parametrize("pc", ["none", "diagonal", "DILU"])
 def test_foam_pbicg_wrapper(gpu, pc):
     from cufoam_amd import foam
     s = _nonsym(16, 10, shift=1e-3)
@@ -121,6 +169,9 @@ def test_pbicg_argument_checks(gpu):
         with pytest.raises(GisError) as e:
             gpu.solve_pbicg(pc, s.b, s.x0, error=1e-8, max_steps=10)
         assert e.value.status == 1
+    with pytest.raises(GisError) as e:   # DILU is for BiCG
+        gpu.solve_pcg("DILU", s.b, s.x0, error=1e-8, max_steps=10)
+    assert e.value.status == 1
     with pytest.raises(foam.FatalError):
         foam.New("cuFoamPBiCG", "U", foam.lduMatrix.from_system(s),
                  {"preconditioner": "DIC"}, gis=gpu)
