@@ -288,6 +288,13 @@ inline int pair_pass(int n, int m, const int* la, const int* ua, const double* w
                      hvec<int>& agg) {
     hvec<int> rp, face, nb;
     graph_rows(n, m, la, ua, rp, face, nb);
+    // the weights in row order, gathered in parallel, so that the sequential
+    // pass below reads only streams and the aggregate map
+    const int T = threads_for(2 * (long long)m);
+    hvec<double> wt(2 * (size_t)m);
+    par_for(2 * (long long)m, T, [&](long long p0, long long p1) {
+        for (long long p = p0; p < p1; ++p) wt[p] = w[face[p]];
+    });
     agg = filled(n, -1, threads_for(n));
     int nc = 0;
     for (int i = 0; i < n; ++i) {
@@ -295,8 +302,8 @@ inline int pair_pass(int n, int m, const int* la, const int* ua, const double* w
         int best = -1;
         double bw = -1.0;
         for (int p = rp[i]; p < rp[i + 1]; ++p)
-            if (agg[nb[p]] < 0 && w[face[p]] > bw) {
-                bw = w[face[p]];
+            if (agg[nb[p]] < 0 && wt[p] > bw) {
+                bw = wt[p];
                 best = nb[p];
             }
         if (best >= 0) {
@@ -304,8 +311,8 @@ inline int pair_pass(int n, int m, const int* la, const int* ua, const double* w
             continue;
         }
         for (int p = rp[i]; p < rp[i + 1]; ++p)
-            if (w[face[p]] > bw) {
-                bw = w[face[p]];
+            if (wt[p] > bw) {
+                bw = wt[p];
                 best = nb[p];
             }
         agg[i] = best >= 0 ? agg[best] : nc++;
