@@ -22,6 +22,9 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <exception>
+#include <new>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -900,7 +903,22 @@ int amg_prepare(gis_context* h) {
         h->amg = new (std::nothrow) AmgData();
         if (!h->amg) return fail(h, kErrNoMem, "GAMG data");
     }
-    if (!h->amg->built) GIS_TRY(amg_build(h));
+    if (!h->amg->built) {
+        // the host set-up allocates and starts threads: no C++ exception may
+        // cross the C ABI
+        int st = kOK;
+        try {
+            st = amg_build(h);
+        } catch (const std::bad_alloc&) {
+            st = fail(h, kErrNoMem, "GAMG set-up: host memory exhausted");
+        } catch (const std::exception& e) {
+            st = fail(h, kErrNoMem, std::string("GAMG set-up: ") + e.what());
+        }
+        if (st != kOK) {
+            h->amg->clear();
+            return st;
+        }
+    }
     if (!h->amg->values_ok) GIS_TRY(amg_values(h));
     return kOK;
 }
@@ -1192,7 +1210,15 @@ int ilu_prepare(gis_context* h, int kind) {
     }
     AmgData& A = *h->amg;
     if (!A.dic_levels) {
-        GIS_TRY(dic_levels(h));
+        int st = kOK;
+        try {
+            st = dic_levels(h);
+        } catch (const std::bad_alloc&) {
+            st = fail(h, kErrNoMem, "DIC/DILU set-up: host memory exhausted");
+        } catch (const std::exception& e) {
+            st = fail(h, kErrNoMem, std::string("DIC/DILU set-up: ") + e.what());
+        }
+        if (st != kOK) return st;
         A.dic_values = false;
     }
     if (!A.dic_values || A.dic_kind != kind) {
