@@ -117,6 +117,67 @@ __global__ __launch_bounds__(kWG) void k_bcg1(Mat<double> m, int N, const double
                    [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
 }
 
+// Split direction form (the schedule's fp64_direction = 1): bdir streams
+// p_k = r (k = 1) or fma(beta, p_{k-1}, r) and the deferred x += alpha_{k-1} p_{k-1};
+// bcg1s is then a pure SpMV of p_k with (p_k, q).  The same fma per row and the
+// same leaves as k_bcg1, so the same bits.
+__global__ __launch_bounds__(kWG) void k_bdir(int N, const double* __restrict__ r,
+                                              double* __restrict__ pb0, double* __restrict__ pb1,
+                                              double* __restrict__ x, BatchView bv) {
+    TileRows t(N);
+    const SysTile s(bv, t.tile);
+    const KState* st = &bv.st[s.sys];
+    if (st->done) return;
+    const int k = st->k + 1;
+    const double* p_old = (k & 1) ? pb0 : pb1;
+    double* p_new = (k & 1) ? pb1 : pb0;
+    double rv[4];
+    load4(r, t.i0, N, rv);
+    if (k == 1) {
+        store4(p_new, t.i0, N, rv);
+        return;
+    }
+    const double beta = st->beta;
+    const double alpha_prev = st->alpha;
+    double pv[4], xv[4];
+    load4(p_old, t.i0, N, pv);
+    load4(x, t.i0, N, xv);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        xv[kk] = fma(alpha_prev, pv[kk], xv[kk]);
+        pv[kk] = fma(beta, pv[kk], rv[kk]);
+    }
+    store4(p_new, t.i0, N, pv);
+    store4(x, t.i0, N, xv);
+}
+
+template <class P>
+__global__ __launch_bounds__(kWG) void k_bcg1s(Mat<double> m, int N,
+                                               const double* __restrict__ pb0,
+                                               const double* __restrict__ pb1,
+                                               double* __restrict__ q, BatchView bv) {
+    TileRows t(N);
+    const SysTile s(bv, t.tile);
+    KState* st = &bv.st[s.sys];
+    if (st->done) return;
+    __shared__ typename P::template Shared<double> sh;
+    __shared__ double sh_y[kTile];
+    __shared__ RedShared shr;
+    const double* p = ((st->k + 1) & 1) ? pb1 : pb0;
+    P::run(m, N, t.row0, t.nrows, [&](int c) { return p[c]; }, sh, sh_y);
+    double pv[4], qv[4];
+    load4(p, t.i0, N, pv);
+    double leaf[1] = {0.0};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+        qv[kk] = sh_y[threadIdx.x * kLeaf + kk];
+        if (t.i0 + kk < N) leaf[0] = fma(pv[kk], qv[kk], leaf[0]);
+    }
+    store4(q, t.i0, N, qv);
+    finish_tile_slots<1>(bv.red[s.sys], st, s.local, leaf, shr,
+                         [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
+}
+
 __global__ __launch_bounds__(kWG) void k_bcg2(int N, double* __restrict__ r,
                                               const double* __restrict__ q, BatchView bv) {
     TileRows t(N);
@@ -336,28 +397,40 @@ int batch_set_values(gis_context* h, const double* diag, const double* lower,
     return kOK;
 }
 
+// one CG iteration of every system: the fused direction form, or the split one
+static void launch_iteration(hipStream_t s, bool selld, bool split, const Mat<double>& m, int N,
+                             double* r, double* pb0, double* pb1, double* x, double* q,
+                             const BatchView& bv) {
+    const dim3 g = tiles_grid(N);
+    if (split) {
+        hipLaunchKernelGGL(k_bdir, g, dim3(kWG), 0, s, N, r, pb0, pb1, x, bv);
+        if (selld)
+            hipLaunchKernelGGL((k_bcg1s<SellD>), g, dim3(kWG), 0, s, m, N, pb0, pb1, q, bv);
+        else
+            hipLaunchKernelGGL((k_bcg1s<Sell>), g, dim3(kWG), 0, s, m, N, pb0, pb1, q, bv);
+    } else if (selld) {
+        hipLaunchKernelGGL((k_bcg1<SellD>), g, dim3(kWG), 0, s, m, N, r, pb0, pb1, x, q, bv);
+    } else {
+        hipLaunchKernelGGL((k_bcg1<Sell>), g, dim3(kWG), 0, s, m, N, r, pb0, pb1, x, q, bv);
+    }
+    hipLaunchKernelGGL(k_bcg2, g, dim3(kWG), 0, s, N, r, q, bv);
+}
+
 static int batch_graph(gis_context* h, int iters, const Mat<double>& m, int N, double* r,
                        double* pb0, double* pb1, double* x, double* q, const BatchView& bv) {
     BatchInfo& B = h->batch;
     const bool selld = h->sched.spmv64 == SpmvKind::SellD;
+    const bool split = h->sched.fp64_direction != 0;
     if (B.graph_valid && B.graph_iters == iters && B.graph_key == (void*)x &&
-        B.graph_selld == selld)
+        B.graph_selld == selld && B.graph_split == split)
         return kOK;
     if (B.graph_exec) (void)hipGraphExecDestroy(B.graph_exec);
     B.graph_exec = nullptr;
     if (!B.cap_stream) GIS_HIP(h, hipStreamCreateWithFlags(&B.cap_stream, hipStreamNonBlocking));
     hipGraph_t graph = nullptr;
     GIS_HIP(h, hipStreamBeginCapture(B.cap_stream, hipStreamCaptureModeThreadLocal));
-    const dim3 g = tiles_grid(N);
-    for (int i = 0; i < iters; ++i) {
-        if (selld)
-            hipLaunchKernelGGL((k_bcg1<SellD>), g, dim3(kWG), 0, B.cap_stream, m, N, r, pb0, pb1,
-                               x, q, bv);
-        else
-            hipLaunchKernelGGL((k_bcg1<Sell>), g, dim3(kWG), 0, B.cap_stream, m, N, r, pb0, pb1,
-                               x, q, bv);
-        hipLaunchKernelGGL(k_bcg2, g, dim3(kWG), 0, B.cap_stream, N, r, q, bv);
-    }
+    for (int i = 0; i < iters; ++i) launch_iteration(B.cap_stream, selld, split, m, N, r, pb0,
+                                                     pb1, x, q, bv);
     hipError_t e = hipStreamEndCapture(B.cap_stream, &graph);
     if (e != hipSuccess) return hip_fail(h, e, "hipStreamEndCapture");
     e = hipGraphInstantiate(&B.graph_exec, graph, nullptr, nullptr, 0);
@@ -367,6 +440,7 @@ static int batch_graph(gis_context* h, int iters, const Mat<double>& m, int N, d
     B.graph_iters = iters;
     B.graph_key = (void*)x;
     B.graph_selld = selld;
+    B.graph_split = split;
     return kOK;
 }
 
@@ -439,15 +513,9 @@ int batch_solve(gis_context* h, double* x_packed, const double* b_packed, bool d
                 break;
             }
         } else {
-            for (int i = 0; i < block; ++i) {
-                if (h->sched.spmv64 == SpmvKind::SellD)
-                    hipLaunchKernelGGL((k_bcg1<SellD>), g, dim3(kWG), 0, h->stream, m, N, r, pb0,
-                                       pb1, x, q, bv);
-                else
-                    hipLaunchKernelGGL((k_bcg1<Sell>), g, dim3(kWG), 0, h->stream, m, N, r, pb0,
-                                       pb1, x, q, bv);
-                hipLaunchKernelGGL(k_bcg2, g, dim3(kWG), 0, h->stream, N, r, q, bv);
-            }
+            for (int i = 0; i < block; ++i)
+                launch_iteration(h->stream, h->sched.spmv64 == SpmvKind::SellD,
+                                 h->sched.fp64_direction != 0, m, N, r, pb0, pb1, x, q, bv);
         }
         launched += block;
         if (hipMemcpyAsync(B.st_host, B.st.p, sizeof(KState) * S, hipMemcpyDeviceToHost,

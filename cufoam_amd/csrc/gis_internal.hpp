@@ -105,6 +105,7 @@ struct BatchInfo {
     int graph_iters = 0;
     void* graph_key = nullptr;
     bool graph_selld = false;            // the SpMV layout the captured graph uses
+    bool graph_split = false;            // the direction form the captured graph uses
 };
 
 // GAMG hierarchy (preconditioned CG).  Level 0 is the handle's own matrix;

@@ -18,13 +18,16 @@ def _systems(edges, seed0=0, shift=1e-3):
 
 @pytest.mark.parametrize("use_graph", [False, True])
 @pytest.mark.parametrize("schedule", [None, 6, 0])
-def test_batch_bit_exact_vs_oracle(gpu, use_graph, schedule):
+@pytest.mark.parametrize("direction", [0, 1])
+def test_batch_bit_exact_vs_oracle(gpu, use_graph, schedule, direction):
+    """direction 0: the direction formed at the gathers (k_bcg1); 1: the split
+    form (k_bdir + k_bcg1s), which the pattern picks for large stacks."""
     edges = [2, 5, 3, 12, 1, 9, 7]
     systems = _systems(edges, 10)
-    print(f"[case] batch edges={edges} graph={use_graph} schedule={schedule}", flush=True)
+    print(f"[case] batch edges={edges} graph={use_graph} schedule={schedule} "
+          f"direction={direction}", flush=True)
     gpu.batch_set_systems(systems)
-    if schedule is not None:
-        gpu.set_schedule(spmv64=schedule)
+    gpu.set_schedule(spmv64=-1 if schedule is None else schedule, fp64_direction=direction)
     b = np.concatenate([s.b for s in systems])
     err = np.array([1e-9 * np.linalg.norm(s.b) for s in systems])
     for rep in range(3):
