@@ -404,13 +404,20 @@ int device_solve_mixed(gis_context* h, const MixedArgs& a, double* x, const doub
     });
     GIS_HIP(h, hipGetLastError());
     const long long max_iter = a.max_iter <= 0 ? (long long)INT_MAX : a.max_iter;
-    // at most one FP64 update launch follows each inner iteration
+    // The FP64 update (k_mxu) follows every kGroup inner iterations instead of
+    // every one: its launch is a no-op unless an inner iteration has raised
+    // F_UPD, and the inner kernels launched after that raise return at once,
+    // so the state sequence, and every result, is the same; the cap on inner
+    // iterations is the device's (k >= max_steps), the host budget counts groups.
+    constexpr int kGroup = 4;
+    const long long max_groups = max_iter / kGroup + 1;
     GIS_SPMV_DISPATCH(h->sched.spmv32, P32, {
         GIS_SPMV_DISPATCH(h->sched.spmv64, P64, {
             const bool split = h->sched.fp32_direction != 0;
             GIS_COARSEN_DISPATCH(h->sched.coarsen32, CC, {
                 const dim3 gc = super_grid(N, CC);
-                GIS_TRY(run_blocks(h, max_iter, [&]() -> int {
+                GIS_TRY(run_blocks(h, max_groups, [&]() -> int {
+                  for (int gi = 0; gi < kGroup; ++gi) {
                     if (split) {
                         GIS_TRY(timer_begin(h, 4));
                         // no reduction to amortise: one tile per workgroup (measured best)
@@ -430,6 +437,7 @@ int device_solve_mixed(gis_context* h, const MixedArgs& a, double* x, const doub
                     hipLaunchKernelGGL((k_mx2<CC>), gc, dim3(kWG), 0, h->stream, N, r32, q,
                                        h->st, h->red);
                     GIS_TRY(timer_end(h));
+                  }
                     GIS_TRY(timer_begin(h, 2));
                     hipLaunchKernelGGL((k_mxu<P64>), gu, dim3(kWG), 0, h->stream, m64, N, b, x,
                                        xb1, pb0, pb1, xc, r32, h->st, h->red);
