@@ -1357,7 +1357,7 @@ int solve_pcg(gis_context* h, const SolveArgs& a, int pc, double* x, const doubl
             }, pc == 2 || pc == 4));
         });
     });
-    hipLaunchKernelGGL(k_cg_flush, g, dim3(kWG), 0, h->stream, N, x, pb0, pb1, h->st);
+    hipLaunchKernelGGL(k_cg_flush, g, dim3(kWG), 0, h->stream, N, x, pb0, pb1, h->st, 0);
     GIS_HIP(h, hipGetLastError());
     return kOK;
 }

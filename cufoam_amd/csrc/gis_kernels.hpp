@@ -234,6 +234,10 @@ __global__ __launch_bounds__(kWG) void k_setup(Mat<double> m, int N,
 // CG pieces shared by CG (gis_solvers.hip) and PCG (gis_amg.hip)
 // ---------------------------------------------------------------------------
 // Split schedule, pass 2: q = A p_k with one gathered vector, (p_k, q) -> alpha_k.
+// KState::aux slot of alpha_{k-1} (written by k_cg1s, read by the paired x
+// updates of k_cg_dir and k_cg_flush)
+constexpr int kAlphaPrevSlot = 5;
+
 template <class P, int C>
 __global__ __launch_bounds__(kWG) void k_cg1s(Mat<double> m, int N,
                                               const double* __restrict__ pb0,
@@ -263,26 +267,39 @@ __global__ __launch_bounds__(kWG) void k_cg1s(Mat<double> m, int N,
     }
     SlotReduce<1, C> tr;
     tr.begin(red, sup.tile0, sup.nt, leaf, shr);
-    tr.end(red, st, shr, [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
+    tr.end(red, st, shr, [&](const double (&v)[1]) {
+        st->aux[kAlphaPrevSlot] = st->alpha;   // alpha_{k-1}, for the paired x updates
+        st->alpha = st->rho0 / v[0];
+    });
 }
 
 namespace {
 
-// x += alpha_k p_k for the last iteration (the update cg1 defers)
+// x += alpha_k p_k for the last iteration (the update the direction pass
+// defers).  paired (k_cg_dir): after an even iteration count K the update of
+// iteration K - 1 is pending too, applied first: x = fma(alpha_K, p_K,
+// fma(alpha_{K-1}, p_{K-1}, x)), the order of the per-iteration updates.
 __global__ __launch_bounds__(kWG) void k_cg_flush(int N, double* __restrict__ x,
                                                   const double* __restrict__ pb0,
                                                   const double* __restrict__ pb1,
-                                                  const KState* st) {
+                                                  const KState* st, int paired) {
     const int k = st->k;
     if (k < 1) return;
     const double alpha = st->alpha;
     const double* p = (k & 1) ? pb1 : pb0;
+    const bool two = paired && (k & 1) == 0;
+    const double alpha_prev = st->aux[kAlphaPrevSlot];
+    const double* pp = (k & 1) ? pb0 : pb1;
     StreamRows t(N);
-    double xv[4], pv[4];
+    double xv[4], pv[4], ppv[4];
     load4(x, t.i0, N, xv);
     load4(p, t.i0, N, pv);
+    if (two) load4(pp, t.i0, N, ppv);
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) xv[kk] = fma(alpha, pv[kk], xv[kk]);
+    for (int kk = 0; kk < 4; ++kk) {
+        if (two) xv[kk] = fma(alpha_prev, ppv[kk], xv[kk]);
+        xv[kk] = fma(alpha, pv[kk], xv[kk]);
+    }
     store4(x, t.i0, N, xv);
 }
 

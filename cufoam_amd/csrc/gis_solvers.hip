@@ -163,8 +163,12 @@ __global__ __launch_bounds__(kWG) void k_cg1(Mat<double> m, int N, const double*
     tr.end(red, st, shr, [&](const double (&v)[1]) { st->alpha = st->rho0 / v[0]; });
 }
 
-// Split schedule, pass 1: p_k = r + beta p_{k-1} and the deferred
-// x += alpha_{k-1} p_{k-1}, the same two fma per row as k_cg1, streamed.
+// Split schedule, pass 1: p_k = r + beta p_{k-1} and the deferred x updates,
+// the same fma per row as k_cg1, streamed.  The x updates go in pairs: an even
+// iteration k leaves alpha_{k-1} p_{k-1} pending, and the odd iteration k + 1
+// applies alpha_{k-1} p_{k-1}, then alpha_k p_k (p_{k-1} is still in the buffer
+// p_{k+1} overwrites, read first row by row).  x takes the same fma in the same
+// order, and a streamed read and write of x are saved every other iteration.
 template <int C>
 __global__ __launch_bounds__(kWG) void k_cg_dir(int N, const double* __restrict__ r,
                                                 double* __restrict__ pb0,
@@ -176,7 +180,8 @@ __global__ __launch_bounds__(kWG) void k_cg_dir(int N, const double* __restrict_
     const double* p_old = (k & 1) ? pb0 : pb1;
     double* p_new = (k & 1) ? pb1 : pb0;
     const double beta = st->beta;
-    const double alpha_prev = st->alpha;
+    const double alpha_prev = st->alpha;                        // alpha_{k-1}
+    const double alpha_prev2 = st->aux[kAlphaPrevSlot];         // alpha_{k-2}
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         if (c >= sup.nt) continue;
@@ -188,9 +193,18 @@ __global__ __launch_bounds__(kWG) void k_cg_dir(int N, const double* __restrict_
             continue;
         }
         load_pairs(p_old, row0, N, pv);
+        if ((k & 1) == 0) {   // even: alpha_{k-1} p_{k-1} stays pending
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) pv[kk] = fma(beta, pv[kk], rv[kk]);
+            store_pairs(p_new, row0, N, pv);
+            continue;
+        }
+        double ppv[4];   // p_{k-2}, in the buffer p_k is about to take
+        load_pairs(p_new, row0, N, ppv);
         load_pairs(x, row0, N, xv);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
+            xv[kk] = fma(alpha_prev2, ppv[kk], xv[kk]);
             xv[kk] = fma(alpha_prev, pv[kk], xv[kk]);
             pv[kk] = fma(beta, pv[kk], rv[kk]);
         }
@@ -531,7 +545,8 @@ int solve_cg(gis_context* h, const SolveArgs& a, double* x, const double* b) {
             }));
         });
     });
-    hipLaunchKernelGGL(k_cg_flush, g, dim3(kWG), 0, h->stream, N, x, pb0, pb1, h->st);
+    hipLaunchKernelGGL(k_cg_flush, g, dim3(kWG), 0, h->stream, N, x, pb0, pb1, h->st,
+                       h->sched.fp64_direction != 0 ? 1 : 0);
     GIS_HIP(h, hipGetLastError());
     return kOK;
 }
