@@ -44,21 +44,24 @@ def spmv_bytes(N, nnz, sv, sx):
 
 
 def mx1_bytes(N, nnz, direction):
-    """CSR algorithmic bytes of the FP32 SpMV-carrying iteration kernel.  Fused form
-    (k_mx1): FP32 matrix (values + columns + row offsets) and six FP32 vector passes
-    (read r, p_{k-1}, xc; write p_k, q, xc).  Split form (k_mx1s): the matrix, p_k read
-    once and q written, i.e. the bytes of one FP32 SpMV."""
+    """CSR algorithmic bytes of the FP32 SpMV-carrying iteration kernel, per launch on
+    average.  Fused form (k_mx1): FP32 matrix (values + columns + row offsets) and
+    FP32 vector passes: r and p_{k-1} read, p_k and q written, and every other launch
+    (the deferred correction goes into xc in pairs) xc and p_{k-2} read and xc
+    written, 5.5 passes on average.  Split form (k_mx1s): the matrix, p_k read once
+    and q written, i.e. the bytes of one FP32 SpMV."""
     if direction:
         return spmv_bytes(N, nnz, 4, 4)
-    return nnz * (4 + 4) + 4 * (N + 1) + 6 * N * 4
+    return nnz * (4 + 4) + 4 * (N + 1) + 22 * N
 
 
 def mx1_layout_bytes(g, N, schedule, direction):
     """The same kernel's bytes in the layout it runs on: one FP32 SpMV of the stored
-    matrix (Gis.spmv_bytes) plus, for the fused form, the four extra FP32 vector
-    passes (r, p_{k-1} and xc read, xc written)."""
+    matrix (Gis.spmv_bytes) plus, for the fused form, the extra FP32 vector passes:
+    r and p_{k-1} read, and every other launch xc and p_{k-2} read and xc written
+    (3.5 passes on average)."""
     b = g.spmv_bytes(4, 4, schedule)
-    return b if direction else b + 4 * N * 4
+    return b if direction else b + 14 * N
 
 
 def cpu_baseline(sys_, A, x_gpu_fp64, x_gpu_mixed, iters_fp64: int, sample_iters: int):
@@ -687,7 +690,8 @@ def main():
                 "bytes_per_launch": lay_b,
                 "bytes_model": "stored matrix of the running layout (values, offset tables, "
                                "row masks) + x read once + q written once"
-                               + ("" if split32 else " + r, p_old, xc read + xc written"),
+                               + ("" if split32 else " + r, p_old read + (every other "
+                                                     "launch) xc, p_{k-2} read and xc written"),
                 "achieved_csr": round(csr_b / mx1_avg_s / 1e9, 1),
                 "frac_csr": round(csr_b / mx1_avg_s / 1e9 / PEAK_HBM_GBS, 4),
                 "bytes_per_launch_csr": csr_b,
@@ -697,10 +701,12 @@ def main():
         roofline["traffic_GBps"] = round(traffic / mx1_avg_s / 1e9, 1)
         roofline["traffic_over_bytes"] = round(traffic / lay_b, 4)
     # bytes of one FP64 CG iteration and one mixed inner iteration in the running
-    # layout (split form: direction pass 5 vector passes, SpMV 2, update 3), for a
-    # check of the solve times against the peak
-    cg_iter_b = g.spmv_bytes(8, 8, sched["spmv64"]) + 8 * N * 8
-    mx_iter_b = g.spmv_bytes(4, 4, sched["spmv32"]) + 8 * N * 4
+    # layout, for a check of the solve times against the peak.  Split form: the
+    # direction pass 4.5 vector passes on average (r, p_{k-1} read, p_k written;
+    # every other iteration x or xc and p_{k-2} read and x or xc written), SpMV 2,
+    # update 3; the fused mixed form moves one pass less, so this bounds both
+    cg_iter_b = g.spmv_bytes(8, 8, sched["spmv64"]) + 60 * N
+    mx_iter_b = g.spmv_bytes(4, 4, sched["spmv32"]) + 30 * N
 
     literal = None
     if args.literal:

@@ -17,6 +17,11 @@
 //   mx2: r -= alpha_k q, rho = (r, r), trigger test
 //   mxu: (only after a trigger) x' = x + (double)(xc + alpha_k p_k) on the fly,
 //        r64 = b - A x' in FP64, r = (float) r64, ||r64||, rho
+// The deferred correction goes into xc in pairs: after a fold the first
+// iteration leaves alpha_{k-1} p_{k-1} pending as well (F_PEND), the next one
+// applies both, xc = fmaf(alpha_{k-1}, p_{k-1}, fmaf(alpha_{k-2}, p_{k-2}, xc)),
+// the same float operations in the same order, so xc (and everything after it)
+// keeps its bits while its read and write are saved every other iteration.
 // The arithmetic is the one of oracle/gis_oracle_ext.c:or_mixed_cg, bit for bit.
 #include <algorithm>
 #include <climits>
@@ -34,11 +39,32 @@ enum : int {
     F_RESTART = 4,   // next mx1 uses p = r
     F_XALT = 8,      // current x lives in the alternate buffer
     F_CONV = 16,     // converged
+    F_PEND = 32,     // alpha_{k-1} p_{k-1} is pending too (aux[kAlphaPrevSlot])
 };
 
 // KState use: rho0 = rho, rho1 = rho_old, alpha, beta, norm (last true residual),
 // norm0, error = target, aux[0] = inner_rel_tol, aux[1] = r0 (true norm at the
-// last update), aux[3] = ||b||, max_steps = max_iter, aux[6] = mode.
+// last update), aux[3] = ||b||, max_steps = max_iter, aux[6] = mode,
+// aux[kAlphaPrevSlot] = alpha_{k-1}.
+
+// xc's pending terms with x_c = xc[c]: alpha_k p_k (pa = p_k), and with F_PEND
+// alpha_{k-1} p_{k-1} (pb = p_{k-1}) applied first
+__device__ __forceinline__ float pending_xc(bool pend, float af, float afp, float pa, float pb,
+                                            float x_c) {
+    return fmaf(af, pa, pend ? fmaf(afp, pb, x_c) : x_c);
+}
+
+// alpha_k from (p_k, q), and the flags for the next iteration: restart and
+// fold are consumed; a pair of pending terms is opened after a plain
+// iteration and closed by the next one
+__device__ __forceinline__ void finish_alpha(KState* st, double pq) {
+    const int f0 = st->flag;
+    int f = f0 & ~(F_RESTART | F_FOLDED | F_PEND);
+    if (!(f0 & (F_FOLDED | F_PEND))) f |= F_PEND;
+    st->flag = f;
+    st->aux[kAlphaPrevSlot] = st->alpha;
+    st->alpha = st->rho0 / pq;
+}
 
 __device__ __forceinline__ void push_hist(KState* st, double v) {
     if (st->nhist < st->hist_cap) st->hist[st->nhist] = v;
@@ -109,8 +135,10 @@ __global__ __launch_bounds__(kWG) void k_mx1(Mat<float> m, int N, const float* _
     const int flag = st->flag;
     const bool first = (flag & F_RESTART) != 0;
     const bool folded = (flag & F_FOLDED) != 0;
+    const bool pair = !folded && (flag & F_PEND) != 0;   // apply both pending terms
     const float bf = (float)st->beta;
     const float af = (float)st->alpha;
+    const float afp = (float)st->aux[kAlphaPrevSlot];
     const float* p_old = (k & 1) ? pb0 : pb1;
     float* p_new = (k & 1) ? pb1 : pb0;
     double leaf[C][1];
@@ -126,29 +154,30 @@ __global__ __launch_bounds__(kWG) void k_mx1(Mat<float> m, int N, const float* _
             P::run(m, N, t.row0, t.nrows, [&](int j) { return fmaf(bf, p_old[j], r[j]); }, sh,
                    sh_y);
         }
-        float rv[4], pv[4], xv[4], pn[4], qv[4];
+        float rv[4], pv[4], xv[4], pp[4], pn[4], qv[4];
         load4(r, t.i0, N, rv);
         if (!first || !folded) load4(p_old, t.i0, N, pv);
-        if (!folded) load4nt(xc, t.i0, N, xv);
+        if (pair) {
+            load4nt(xc, t.i0, N, xv);
+            load4(p_new, t.i0, N, pp);   // p_{k-2}, own rows, before p_k replaces it
+        }
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             pn[kk] = first ? rv[kk] : fmaf(bf, pv[kk], rv[kk]);
             qv[kk] = sh_y[threadIdx.x * kLeaf + kk];
-            xv[kk] = folded ? 0.0f : fmaf(af, pv[kk], xv[kk]);
+            xv[kk] = folded ? 0.0f : pair ? pending_xc(true, af, afp, pv[kk], pp[kk], xv[kk])
+                                          : 0.0f;
             if (t.i0 + kk < N) leaf[c][0] = fma((double)pn[kk], (double)qv[kk], leaf[c][0]);
         }
         // non-temporal stores (and xc's load): measured at 256^3, 209.7 -> 205.2 us
         // per inner iteration against cached ones
         store4nt(p_new, t.i0, N, pn);
         store4nt(q, t.i0, N, qv);
-        store4nt(xc, t.i0, N, xv);
+        if (folded || pair) store4nt(xc, t.i0, N, xv);
     }
     SlotReduce<1, C> tr;
     tr.begin(red, sup.tile0, sup.nt, leaf, shr);
-    tr.end(red, st, shr, [&](const double (&v)[1]) {
-        st->alpha = st->rho0 / v[0];
-        st->flag = st->flag & ~(F_RESTART | F_FOLDED);
-    });
+    tr.end(red, st, shr, [&](const double (&v)[1]) { finish_alpha(st, v[0]); });
 }
 
 // Split schedule, pass 1: the direction and the deferred correction of k_mx1,
@@ -163,8 +192,10 @@ __global__ __launch_bounds__(kWG) void k_mx_dir(int N, const float* __restrict__
     const int flag = st->flag;
     const bool first = (flag & F_RESTART) != 0;
     const bool folded = (flag & F_FOLDED) != 0;
+    const bool pair = !folded && (flag & F_PEND) != 0;
     const float bf = (float)st->beta;
     const float af = (float)st->alpha;
+    const float afp = (float)st->aux[kAlphaPrevSlot];
     const float* p_old = (k & 1) ? pb0 : pb1;
     float* p_new = (k & 1) ? pb1 : pb0;
 #pragma unroll
@@ -172,16 +203,20 @@ __global__ __launch_bounds__(kWG) void k_mx_dir(int N, const float* __restrict__
         if (c >= sup.nt) continue;
         const TileRows t(N, sup.tile0 + c);
         float rv[4], pv[4] = {0.0f, 0.0f, 0.0f, 0.0f}, xv[4] = {0.0f, 0.0f, 0.0f, 0.0f}, pn[4];
+        float pp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         load4(r, t.i0, N, rv);
         if (!first || !folded) load4(p_old, t.i0, N, pv);
-        if (!folded) load4(xc, t.i0, N, xv);
+        if (pair) {
+            load4(xc, t.i0, N, xv);
+            load4(p_new, t.i0, N, pp);   // p_{k-2}, own rows, before p_k replaces it
+        }
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             pn[kk] = first ? rv[kk] : fmaf(bf, pv[kk], rv[kk]);
-            xv[kk] = folded ? 0.0f : fmaf(af, pv[kk], xv[kk]);
+            xv[kk] = pair ? pending_xc(true, af, afp, pv[kk], pp[kk], xv[kk]) : 0.0f;
         }
         store4(p_new, t.i0, N, pn);
-        store4(xc, t.i0, N, xv);
+        if (folded || pair) store4(xc, t.i0, N, xv);
     }
 }
 
@@ -218,10 +253,7 @@ __global__ __launch_bounds__(kWG) void k_mx1s(Mat<float> m, int N, const float* 
 #pragma unroll
     for (int c = 0; c < C; ++c)
         if (c < sup.nt) store4nt(q, TileRows(N, sup.tile0 + c).i0, N, qv[c]);
-    tr.end(red, st, shr, [&](const double (&v)[1]) {
-        st->alpha = st->rho0 / v[0];
-        st->flag = st->flag & ~(F_RESTART | F_FOLDED);
-    });
+    tr.end(red, st, shr, [&](const double (&v)[1]) { finish_alpha(st, v[0]); });
 }
 
 template <int C>
@@ -290,11 +322,16 @@ __global__ __launch_bounds__(kWG) void k_mxu(Mat<double> m, int N, const double*
     __shared__ RedShared shr;
     const int k = st->k;
     const int flag = st->flag;
+    const bool pend = (flag & F_PEND) != 0;
     const float af = (float)st->alpha;
+    const float afp = (float)st->aux[kAlphaPrevSlot];
     const float* p = (k & 1) ? pb1 : pb0;
+    const float* pq = (k & 1) ? pb0 : pb1;   // p_{k-1}, read only with F_PEND
     const double* x_old = (flag & F_XALT) ? xb1 : xb0;
     double* x_new = (flag & F_XALT) ? xb0 : xb1;
-    auto xn = [&](int c) { return x_old[c] + (double)fmaf(af, p[c], xc[c]); };
+    auto xn = [&](int c) {
+        return x_old[c] + (double)pending_xc(pend, af, afp, p[c], pend ? pq[c] : 0.0f, xc[c]);
+    };
     const int ntiles = tiles_of(N);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         TileRows t(N, tile);
@@ -320,7 +357,7 @@ __global__ __launch_bounds__(kWG) void k_mxu(Mat<double> m, int N, const double*
             const double rnorm = sqrt(v[0]);
             st->norm = rnorm;
             push_hist(st, rnorm);
-            int f = (st->flag ^ F_XALT) & ~F_UPD;
+            int f = (st->flag ^ F_XALT) & ~(F_UPD | F_PEND);
             f |= F_FOLDED;
             if (!(rnorm > st->error)) {
                 f |= F_CONV;
@@ -352,8 +389,11 @@ __global__ __launch_bounds__(kWG) void k_mx_final(int N, double* __restrict__ xb
     const int k = st->k;
     const bool conv = (flag & F_CONV) != 0;
     const bool folded = (flag & F_FOLDED) != 0 || k < 1;
+    const bool pend = (flag & F_PEND) != 0;
     const float af = (float)st->alpha;
+    const float afp = (float)st->aux[kAlphaPrevSlot];
     const float* p = (k & 1) ? pb1 : pb0;
+    const float* pq = (k & 1) ? pb0 : pb1;
     const double* xcur = (flag & F_XALT) ? xb1 : xb0;
     StreamRows t(N);
 #pragma unroll
@@ -361,7 +401,9 @@ __global__ __launch_bounds__(kWG) void k_mx_final(int N, double* __restrict__ xb
         const int i = t.i0 + kk;
         if (i >= N) continue;
         double v = xcur[i];
-        if (!conv) v = v + (double)(folded ? 0.0f : fmaf(af, p[i], xc[i]));
+        if (!conv)
+            v = v + (double)(folded ? 0.0f
+                                    : pending_xc(pend, af, afp, p[i], pend ? pq[i] : 0.0f, xc[i]));
         xb0[i] = v;
     }
 }
