@@ -424,6 +424,23 @@ GIS_API int gis_set_timing(gis_handle h, int slot_mask) {
     return GIS_OK;
 }
 
+GIS_API int gis_set_symmetric_storage(gis_handle h, int enable) {
+    if (!h || (enable != 0 && enable != 1)) return GIS_ERR_ARG;
+    h->sym_enable = enable != 0;
+    if (h->amg)   // the GAMG levels follow the handle
+        for (auto& L : h->amg->lv)
+            if (L.ctx) L.ctx->sym_enable = h->sym_enable;
+    return GIS_OK;
+}
+
+GIS_API int gis_get_symmetric_storage(gis_handle h, int* active, long long* stored_slots) {
+    if (!h) return GIS_ERR_ARG;
+    if (!h->has_pattern) return fail(h, GIS_ERR_STATE, "pattern not set");
+    if (active) *active = use_sym(h) ? 1 : 0;
+    if (stored_slots) *stored_slots = h->sym_layout ? h->n_sym_slots : 0;
+    return GIS_OK;
+}
+
 GIS_API int gis_set_timing_sample(gis_handle h, int period) {
     if (!h) return GIS_ERR_ARG;
     if (period < 1) return fail(h, GIS_ERR_ARG, "period must be >= 1");

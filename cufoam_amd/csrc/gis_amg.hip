@@ -759,6 +759,7 @@ int amg_build(gis_context* h) {
         C.ctx->device = h->device;
         C.ctx->stream = h->stream;
         C.ctx->own_stream = false;
+        C.ctx->sym_enable = h->sym_enable;
         GIS_TRY(upload_vec(h, C.ctx->lower_addr, next.la));
         GIS_TRY(upload_vec(h, C.ctx->upper_addr, next.ua));
         // the uploads read t and next, which go out of scope below

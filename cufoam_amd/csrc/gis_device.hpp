@@ -841,6 +841,126 @@ __device__ __forceinline__ void tile_sell_d(const int* __restrict__ soff,
     wave_sync_lds();
 }
 
+// Symmetric storage of a SellD layout (a bitwise symmetric matrix, every slice
+// tabled): only the slots of table entries with offset >= 0 are stored (dsv).
+// Each table entry has one 16-byte record e = {offset o, base a, base b, 0}
+// (one scalar load per slot: three separate scalar loads serialised the loop).
+// Slot k of a row reads dsv[base + (r & 63)], r = row + min(o, 0): its own slot
+// for o >= 0 (r = row, base = a), the +d slot of row r = row - d for o = -d,
+// whose slice is the one of row 64 s - d (base a) or the next (base b).  Record
+// and slice test are uniform per slot, so the loop keeps the branch-free form
+// of sell_row2_d: same values, order and sum.  The loads are cached: each
+// stored upper slot is read a second time as a lower slot, by the same wave
+// (d < 64), a nearby one, or the workgroup 64 tiles on.
+template <typename T>
+__device__ __forceinline__ T sym_load(const T* __restrict__ dsv, const int4& e, int srow, int row) {
+    const int so = min(e.x, 0);
+    const int r = row + so;
+    const int s1 = (srow + so) >> 6;   // floor
+    const int base = (r >> 6) == s1 ? e.y : e.z;
+    return dsv[base + (r & 63)];
+}
+
+template <typename T, class XOp>
+__device__ __forceinline__ void sell_row2_ds(const int4* __restrict__ tab0, unsigned m0, int len0,
+                                             int srow0, const int4* __restrict__ tab1,
+                                             unsigned m1, int len1, int srow1, int lane, int last,
+                                             const T* __restrict__ dsv, XOp&& xop, T& y0, T& y1) {
+    T acc0 = T(0), acc1 = T(0);
+    const int row0 = srow0 + lane, row1 = srow1 + lane;
+    const int len = len0 > len1 ? len0 : len1;
+    for (int j = 0; j < len; j += 8) {
+        T v0[8], v1[8], x0[8], x1[8];
+        int4 e0[8], e1[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            e0[u] = tab0[min(j + u, len0 - 1)];
+            e1[u] = tab1[min(j + u, len1 - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            v0[u] = sym_load(dsv, e0[u], srow0, row0);
+            v1[u] = sym_load(dsv, e1[u], srow1, row1);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            x0[u] = xop(min(max(row0 + e0[u].x, 0), last));
+            x1[u] = xop(min(max(row1 + e1[u].x, 0), last));
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const bool on0 = (m0 >> (j + u)) & 1u, on1 = (m1 >> (j + u)) & 1u;
+            const T t0 = acc0 + v0[u] * x0[u];
+            const T t1 = acc1 + v1[u] * x1[u];
+            acc0 = on0 ? t0 : acc0;
+            acc1 = on1 ? t1 : acc1;
+        }
+    }
+    y0 = acc0;
+    y1 = acc1;
+}
+
+template <typename T, class XOp>
+__device__ __forceinline__ T sell_row_ds(const int4* __restrict__ tab, unsigned mask, int len,
+                                         int srow, int lane, int last,
+                                         const T* __restrict__ dsv, XOp&& xop) {
+    T acc = T(0);
+    const int row = srow + lane;
+    for (int j = 0; j < len; j += 8) {
+        T v[8], xv[8];
+        int4 e[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) e[u] = tab[min(j + u, len - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = sym_load(dsv, e[u], srow, row);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = xop(min(max(row + e[u].x, 0), last));
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const bool on = (mask >> (j + u)) & 1u;
+            const T t = acc + v[u] * xv[u];
+            acc = on ? t : acc;
+        }
+    }
+    return acc;
+}
+
+// tile over the slices two at a time in the symmetric storage (every slice
+// has a table; slices past the end are skipped)
+template <typename T, class XOp>
+__device__ __forceinline__ void tile_sell_ds(const int* __restrict__ ooff,
+                                             const unsigned* __restrict__ dmask,
+                                             const unsigned char* __restrict__ dmask8,
+                                             const int4* __restrict__ symx,
+                                             const T* __restrict__ dsv, int N, int row0,
+                                             XOp&& xop, T* __restrict__ sh_y) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nslices = (N + kSlice - 1) / kSlice;
+    const int s0 = row0 / kSlice + w * 4;
+#pragma unroll
+    for (int c = 0; c < 4; c += 2) {
+        const int s = s0 + c;
+        T y0 = T(0), y1 = T(0);
+        if (s + 1 < nslices) {
+            const int o0 = ooff[s], o1 = ooff[s + 1];
+            const int u0 = o1 - o0, u1 = ooff[s + 2] - o1;
+            const unsigned m0 = row_mask(dmask, dmask8, u0, s * kSlice + lane, N);
+            const unsigned m1 = row_mask(dmask, dmask8, u1, (s + 1) * kSlice + lane, N);
+            sell_row2_ds<T>(symx + o0, m0, u0, s * kSlice, symx + o1, m1, u1, (s + 1) * kSlice,
+                            lane, N - 1, dsv, xop, y0, y1);
+        } else if (s < nslices) {
+            const int o0 = ooff[s];
+            const int u0 = ooff[s + 1] - o0;
+            const unsigned m0 = row_mask(dmask, dmask8, u0, s * kSlice + lane, N);
+            y0 = sell_row_ds<T>(symx + o0, m0, u0, s * kSlice, lane, N - 1, dsv, xop);
+        }
+        sh_y[(w * 4 + c) * kSlice + lane] = y0;
+        sh_y[(w * 4 + c + 1) * kSlice + lane] = y1;
+    }
+    wave_sync_lds();
+}
+
 template <typename T, class XOp>
 __device__ __forceinline__ void tile_sell(const int* __restrict__ soff,
                                           const int* __restrict__ scol,

@@ -190,6 +190,17 @@ struct gis_context {
     long long lay_slices = 0, lay_table_slices = 0, lay_table_entries = 0;
     long long lay_byte_mask_slices = 0, lay_untabled_slots = 0;
     gis::DevMem dval, dvalT, dval32, dval32T;
+    // symmetric storage of the SellD layout (every slice tabled): the FP64 slot
+    // values of the table entries with column offset >= 0 only (dsv), and per
+    // table entry the value bases its slots read (syma, symb); built per pattern
+    // (sym_layout), used while the values are bitwise symmetric (sym_values) and
+    // the caller has not switched it off (sym_enable)
+    gis::DevMem syma, symb, dsv;
+    gis::DevMem symx;   // per table entry {offset, syma, symb, 0}: one 16-byte load a slot
+    long long n_sym_slots = 0;
+    bool sym_layout = false;
+    bool sym_values = false;
+    bool sym_enable = true;
     gis::DevMem dinv;                           // per-row helper (1/a_ii or sqrt(1/a_ii))
     gis::AmgData* amg = nullptr;                // GAMG hierarchy (built on first use)
 
@@ -277,6 +288,11 @@ int read_state(gis_context* h);
 
 // pattern + values (gis_pattern.hip)
 int build_pattern(gis_context* h, int N, int M, const int* d_lower_addr, const int* d_upper_addr);
+// symmetric storage in use: a layout built for it, values checked symmetric,
+// not switched off
+inline bool use_sym(const gis_context* h) {
+    return h->sym_layout && h->sym_values && h->sym_enable;
+}
 int gather_values(gis_context* h);          // val (and valT / val32 when present) from diag/lower/upper
 int ensure_val32(gis_context* h);
 int ensure_valT(gis_context* h);

@@ -390,6 +390,116 @@ __global__ __launch_bounds__(kSlice) void k_sell_compress(int S, const int* __re
     if (lane == 0) sflag[s] = (unsigned char)ok;
 }
 
+// ---------------------------------------------------------------------------
+// Symmetric storage of a SellD layout whose slices all have a table.  A
+// bitwise symmetric matrix keeps the slots of its table entries with column
+// offset >= 0 only: dsv, slot-major per slice from vb[s] = 64 x (entries with
+// offset >= 0 of the earlier slices).  The entry of offset -d of row i is the
+// entry of offset +d of row i - d, in the slice (i - d) / 64 at lane
+// (i - d) % 64.  Per table entry e of slice s (offset o):
+//   o >= 0: syma[e] = symb[e] = vb[s] + 64 x (e's rank among the entries >= 0);
+//   o = -d: syma[e] = base of the +d entry in slice S1 = floor((64 s - d) / 64),
+//           symb[e] = the same in slice S1 + 1 (the rows i - d of the slice lie
+//           in these two); 0 where that slice does not exist or has no +d entry
+//           (no row there holds the entry, so no set mask bit reads it).
+// The values a row sums, their order and so the result are those of the full
+// layout; k_sym_check confirms the symmetry on the values before use.
+// ---------------------------------------------------------------------------
+
+// slots of slice s that the symmetric storage keeps (64 per entry with offset >= 0)
+__global__ __launch_bounds__(kBlk) void k_sym_count(int S, const int* __restrict__ ooff,
+                                                    const int* __restrict__ doffs, int* cnt) {
+    const int s = blockIdx.x * kBlk + threadIdx.x;
+    if (s > S) return;
+    int c = 0;
+    if (s < S)
+        for (int e = ooff[s]; e < ooff[s + 1]; ++e) c += doffs[e] >= 0 ? kSlice : 0;
+    cnt[s] = c;
+}
+
+__device__ __forceinline__ int sym_base_of(int S, const int* __restrict__ ooff,
+                                           const int* __restrict__ doffs,
+                                           const int* __restrict__ vb, int sp, int d) {
+    if (sp < 0 || sp >= S) return 0;
+    int rank = 0;
+    for (int e = ooff[sp]; e < ooff[sp + 1]; ++e) {
+        const int o = doffs[e];
+        if (o == d) return vb[sp] + rank * kSlice;
+        if (o >= 0) ++rank;
+    }
+    return 0;
+}
+
+__global__ __launch_bounds__(kBlk) void k_sym_fill(int S, const int* __restrict__ ooff,
+                                                   const int* __restrict__ doffs,
+                                                   const int* __restrict__ vb, int* syma,
+                                                   int* symb) {
+    const int s = blockIdx.x * kBlk + threadIdx.x;
+    if (s >= S) return;
+    int rank = 0;
+    for (int e = ooff[s]; e < ooff[s + 1]; ++e) {
+        const int o = doffs[e];
+        if (o >= 0) {
+            syma[e] = symb[e] = vb[s] + rank * kSlice;
+            ++rank;
+        } else {
+            const int s1 = (s * kSlice + o) >> 6;   // floor: o < 0 may take it below 0
+            syma[e] = sym_base_of(S, ooff, doffs, vb, s1, -o);
+            symb[e] = sym_base_of(S, ooff, doffs, vb, s1 + 1, -o);
+        }
+    }
+}
+
+// per table entry the record the SpMV reads: {offset, syma, symb, 0}
+__global__ __launch_bounds__(kBlk) void k_sym_pack(int n_tab, const int* __restrict__ doffs,
+                                                   const int* __restrict__ syma,
+                                                   const int* __restrict__ symb, int4* symx) {
+    const int e = blockIdx.x * kBlk + threadIdx.x;
+    if (e < n_tab) symx[e] = make_int4(doffs[e], syma[e], symb[e], 0);
+}
+
+// dsv: the kept slots of the full slot values (one thread per full slot)
+template <typename T>
+__global__ __launch_bounds__(kBlk) void k_sym_compact(long long n_dslots,
+                                                      const int* __restrict__ doffs,
+                                                      const int* __restrict__ syma,
+                                                      const T* __restrict__ dval, T* dsv) {
+    const long long i = (long long)blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n_dslots) return;
+    const int e = (int)(i / kSlice);
+    if (doffs[e] >= 0) dsv[syma[e] + (int)(i % kSlice)] = dval[i];
+}
+
+// bad = 1 unless every set slot of an entry with offset < 0 holds the same bits
+// as the slot the symmetric storage reads for it.  One wave per slice.
+__global__ __launch_bounds__(kBlk) void k_sym_check(int N, int S, const int* __restrict__ ooff,
+                                                    const int* __restrict__ doffs,
+                                                    const unsigned* __restrict__ dmask,
+                                                    const int* __restrict__ syma,
+                                                    const int* __restrict__ symb,
+                                                    const double* __restrict__ dval,
+                                                    const double* __restrict__ dsv, int* bad) {
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * (kBlk / 64) + (threadIdx.x >> 6);
+    if (s >= S) return;   // whole waves
+    const int row = s * kSlice + lane;
+    const unsigned m = row < N ? dmask[row] : 0u;
+    int mismatch = 0;
+    for (int e = ooff[s], k = 0; e < ooff[s + 1]; ++e, ++k) {
+        const int o = doffs[e];
+        if (o >= 0) break;   // offsets ascend
+        if (!((m >> k) & 1u)) continue;
+        const int r = row + o;
+        const int s1 = (s * kSlice + o) >> 6;
+        const int base = (r >> 6) == s1 ? syma[e] : symb[e];
+        if (__double_as_longlong(dval[(long long)e * kSlice + lane]) !=
+            __double_as_longlong(dsv[base + (r & 63)]))
+            mismatch = 1;
+    }
+    const unsigned long long any = __ballot(mismatch);
+    if (lane == 0 && any) atomicOr(bad, 1);
+}
+
 int scan_exclusive(gis_context* h, const int* in, int* out, int n, int* tmp, size_t tmp_ints) {
     const int nb = (n + kBlk * 4 - 1) / (kBlk * 4);
     if (nb <= 1) {
@@ -564,6 +674,41 @@ int build_pattern(gis_context* h, int N, int M, const int* d_la, const int* d_ua
         GIS_HIP(h, hipGetLastError());
         GIS_HIP(h, hipStreamSynchronize(h->stream));   // the temporaries are freed next
     }
+    // symmetric storage: only where every slice has a table
+    h->sym_layout = false;
+    h->sym_values = false;
+    h->n_sym_slots = 0;
+    if (S > 0 && h->lay_table_slices == S && h->n_dslots < (1LL << 31)) {
+        DevMem cnt;
+        GIS_HIP(h, cnt.ensure(sizeof(int) * ((size_t)S + 1)));
+        GIS_HIP(h, h->syma.ensure(sizeof(int) * (size_t)h->lay_table_entries));
+        GIS_HIP(h, h->symb.ensure(sizeof(int) * (size_t)h->lay_table_entries));
+        hipLaunchKernelGGL(k_sym_count, grid_for((long long)S + 1), dim3(kBlk), 0, h->stream, S,
+                           h->ooff.as<int>(), h->doffs.as<int>(), cnt.as<int>());
+        GIS_HIP(h, hipGetLastError());
+        DevMem vb;
+        GIS_HIP(h, vb.ensure(sizeof(int) * ((size_t)S + 1)));
+        GIS_TRY(scan_exclusive(h, cnt.as<int>(), vb.as<int>(), S + 1, tmp, scan_tmp));
+        hipLaunchKernelGGL(k_sym_fill, grid_for(S), dim3(kBlk), 0, h->stream, S,
+                           h->ooff.as<int>(), h->doffs.as<int>(), vb.as<int>(), h->syma.as<int>(),
+                           h->symb.as<int>());
+        GIS_HIP(h, hipGetLastError());
+        const int n_tab = (int)h->lay_table_entries;
+        GIS_HIP(h, h->symx.ensure(sizeof(int4) * (size_t)n_tab));
+        hipLaunchKernelGGL(k_sym_pack, grid_for(n_tab), dim3(kBlk), 0, h->stream, n_tab,
+                           h->doffs.as<int>(), h->syma.as<int>(), h->symb.as<int>(),
+                           h->symx.as<int4>());
+        GIS_HIP(h, hipGetLastError());
+        int n_sym = 0;
+        GIS_HIP(h, hipMemcpyAsync(&n_sym, vb.as<int>() + S, sizeof(int), hipMemcpyDeviceToHost,
+                                  h->stream));
+        GIS_HIP(h, hipStreamSynchronize(h->stream));   // cnt and vb are freed next
+        // every base + 63 must lie in the kept slots, the fallback base 0 included
+        if (n_sym >= kSlice) {
+            h->n_sym_slots = n_sym;
+            h->sym_layout = true;
+        }
+    }
     h->n_slots = n_slots;
     h->N = N;
     h->M = M;
@@ -600,9 +745,38 @@ static int gather_both(gis_context* h, bool transpose, DevMem& csr, DevMem& sell
     return kOK;
 }
 
+// Symmetric storage of T values: dsv from the full slot values
+template <typename T>
+static int sym_compact(gis_context* h, const DevMem& dval, DevMem& dsv) {
+    GIS_HIP(h, dsv.ensure(sizeof(T) * (size_t)h->n_sym_slots));
+    hipLaunchKernelGGL(k_sym_compact<T>, grid_for(h->n_dslots), dim3(kBlk), 0, h->stream,
+                       h->n_dslots, h->doffs.as<int>(), h->syma.as<int>(), dval.as<T>(),
+                       dsv.as<T>());
+    GIS_HIP(h, hipGetLastError());
+    return kOK;
+}
+
 int gather_values(gis_context* h) {
     if (!h->has_pattern) return fail(h, kErrState, "pattern not set");
     GIS_TRY(gather_both<double>(h, false, h->val, h->sval, h->dval));
+    h->sym_values = false;
+    if (h->sym_layout) {
+        GIS_TRY(sym_compact<double>(h, h->dval, h->dsv));
+        DevMem flag;
+        GIS_HIP(h, flag.ensure(sizeof(int)));
+        GIS_HIP(h, hipMemsetAsync(flag.p, 0, sizeof(int), h->stream));
+        const int S = (int)h->lay_slices;
+        hipLaunchKernelGGL(k_sym_check, dim3((unsigned)((S + kBlk / 64 - 1) / (kBlk / 64))),
+                           dim3(kBlk), 0, h->stream, h->N, S, h->ooff.as<int>(),
+                           h->doffs.as<int>(), h->dmask.as<unsigned>(), h->syma.as<int>(),
+                           h->symb.as<int>(), h->dval.as<double>(), h->dsv.as<double>(),
+                           flag.as<int>());
+        GIS_HIP(h, hipGetLastError());
+        int bad = 1;
+        GIS_HIP(h, hipMemcpyAsync(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        GIS_HIP(h, hipStreamSynchronize(h->stream));
+        h->sym_values = bad == 0;
+    }
     if (h->amg) {
         h->amg->values_ok = false;
         h->amg->dic_values = false;

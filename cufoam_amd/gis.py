@@ -106,6 +106,8 @@ SIGNATURES = {
     "gis_set_gamg": (_i, [_vp, _d, _i, _d]),
     "gis_set_slot_fault": (_i, [_vp, _i]),
     "gis_set_timing_sample": (_i, [_vp, _i]),
+    "gis_set_symmetric_storage": (_i, [_vp, _i]),
+    "gis_get_symmetric_storage": (_i, [_vp, _ip, C.POINTER(C.c_longlong)]),
     "gis_gamg_levels": (_i, [_vp, _ip]),
     "gis_gamg_level": (_i, [_vp, _i, _ip, _ip, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gis_solve_multi": (_i, [_vp, _i, _i, _vp, _vp, C.POINTER(_Params), _i, _vp, _vp, _vp]),
@@ -402,18 +404,37 @@ class Gis:
         self._check(self._L.gis_get_layout(self._h, C.byref(lay)), "get_layout")
         return {f: getattr(lay, f) for f, _ in _Layout._fields_}
 
+    def set_symmetric_storage(self, enable: bool = True):
+        """Switch the symmetric storage of the offset-table schedule on or off
+        (gis_set_symmetric_storage; on by default, used where it applies)."""
+        self._check(self._L.gis_set_symmetric_storage(self._h, 1 if enable else 0),
+                    "set_symmetric_storage")
+
+    def symmetric_storage(self) -> tuple[bool, int]:
+        """(in use, value slots it keeps) for the current pattern and values."""
+        a = C.c_int(0)
+        n = C.c_longlong(0)
+        self._check(self._L.gis_get_symmetric_storage(self._h, C.byref(a), C.byref(n)),
+                    "get_symmetric_storage")
+        return bool(a.value), int(n.value)
+
     def spmv_bytes(self, value_size: int, vector_size: int, schedule: int = 6) -> int:
         """Bytes one SpMV of this pattern must move in the given layout: the stored
         matrix (values, column indices or offset tables and row masks, slice
         offsets), x read once and y written once.  Schedule 6 (column-offset
-        tables) and 0 (sliced rows) are modelled; others count the CSR bytes."""
+        tables) and 0 (sliced rows) are modelled; others count the CSR bytes.
+        Under the symmetric storage the FP64 schedule 6 stores only the kept slots,
+        plus a 16-byte record per table entry."""
         lay = self.layout()
         n, sv, sx = lay["n_rows"], value_size, vector_size
         vec = 2 * n * sx
         if schedule == 6:
             tabled_rows = min(64 * lay["table_slices"], n)
             byte_rows = min(64 * lay["byte_mask_slices"], tabled_rows)
-            return (64 * lay["table_entries"] * sv + byte_rows + 4 * (tabled_rows - byte_rows)
+            sym, kept = self.symmetric_storage()
+            vals = kept * sv + 16 * lay["table_entries"] if sym and sv == 8 else \
+                64 * lay["table_entries"] * sv
+            return (vals + byte_rows + 4 * (tabled_rows - byte_rows)
                     + 4 * (lay["table_entries"] + lay["slices"] + 1)
                     + lay["untabled_slots"] * (sv + 4) + vec)
         if schedule in (0, 4):

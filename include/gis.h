@@ -125,6 +125,17 @@ typedef struct {
     long long untabled_slots;    /* sliced-row slots of the slices without a table */
 } gis_layout;
 GIS_API int gis_get_layout(gis_handle h, gis_layout *out);
+/* Symmetric storage (no counterpart in the reference; a storage form of the
+ * SpMV that getSparseCSRMat, gis/gis.cu:51-104, feeds): when every 64-row slice
+ * of the pattern has a column-offset table and the values are bitwise symmetric
+ * (checked on the device at each gis_set_values), the FP64 offset-table
+ * schedule stores only the entries on and above the diagonal and reads each
+ * entry below it from its mirror; results are bit-identical.  enable 0 / 1 (default 1;
+ * the GAMG levels' own matrices follow the handle).
+ * gis_get_symmetric_storage: active = 1 while in use; stored_slots = value
+ * slots it keeps (0 where the pattern does not allow it). */
+GIS_API int gis_set_symmetric_storage(gis_handle h, int enable);
+GIS_API int gis_get_symmetric_storage(gis_handle h, int *active, long long *stored_slots);
 /* Copy the device CSR (row_offsets[N+1], column_indices[nnz], values[nnz]) to host. */
 GIS_API int gis_get_csr(gis_handle h, int *row_offsets, int *column_indices, double *values);
 /* CSR of A^T (Transpose, gis/gis.cu:4-7): same pattern for LDU, lower <-> upper. */
