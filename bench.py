@@ -26,8 +26,10 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 # the dominant kernel of a timed solve is timed live with HIP events on one
 # launch in TIMING_PERIOD: an event pair adds a few microseconds of launch gap,
-# which would otherwise land in the measured solve time
-TIMING_PERIOD = 16
+# which would otherwise land in the measured solve time.  Odd, so that the
+# samples alternate between the two kinds of k_mx1 launch (with and without the
+# paired correction update, about 168 and 139 us at 256^3)
+TIMING_PERIOD = 15
 sys.path.insert(0, ROOT)
 
 METRIC = "CG time-to-1e-10 rel resid, 256^3 Poisson: FP64 vs mixed-prec; SpMV HBM GB/s"
@@ -540,6 +542,10 @@ def main():
     t_tune = time.perf_counter() - t0
     sched_source = "fixed" if args.schedule else ("autotune" if args.tune else "pattern")
     log(f"[bench] schedule {sched} from {sched_source} ({t_tune:.2f}s, best runs {tune_ms} ms)")
+    sym_on, sym_kept = g.symmetric_storage()
+    sym_state = {"in_use": sym_on, "kept_slots": sym_kept,
+                 "full_slots": 64 * g.layout()["table_entries"]}
+    log(f"[bench] FP64 symmetric storage {sym_state}")
     b = torch.from_numpy(s.b).cuda()
     bnorm = float(np.linalg.norm(s.b))
     max_iter = args.max_iter
@@ -660,8 +666,11 @@ def main():
         g.set_timing()
         us = ms / cnt * 1e3
         gbs = spmv_bytes(N, nnz, sv, sv) / (us * 1e-6) / 1e9
+        lb = g.spmv_bytes(sv, sv, sched["spmv64"] if sv == 8 else sched["spmv32"])
         spmv[name] = {"us": round(us, 2), "GB/s": round(gbs, 1),
                       "frac": round(gbs / PEAK_HBM_GBS, 4),
+                      "layout_bytes": lb, "layout_GBps": round(lb / (us * 1e-6) / 1e9, 1),
+                      "layout_frac": round(lb / (us * 1e-6) / 1e9 / PEAK_HBM_GBS, 4),
                       "GFLOP/s": round(2 * nnz / (us * 1e-6) / 1e9, 1)}
         log(f"[bench] spmv {name}: {us:.1f} us {gbs:.0f} GB/s")
 
@@ -805,6 +814,7 @@ def main():
                                        "direction_k_cg_dir": avg_us(brk_d["iter_direction"])}}},
         "schedule": sched,
         "schedule_source": sched_source,
+        "symmetric_storage_fp64": sym_state,
         "spmv": spmv,
         "setup_s": {"pattern": round(t_pattern, 4), "values": round(t_values, 4),
                     "autotune": round(t_tune, 4)},
