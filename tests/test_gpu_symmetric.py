@@ -35,13 +35,16 @@ def _spmv_all(gpu, s, A, seed):
     return y64, y32.cpu().numpy()
 
 
-@pytest.mark.parametrize("n,seed", [(8, 1), (16, 2), (40, 3)])
+# 35^3 = 669 x 64 + 59 rows: a last slice with a table and 5 rows past N
+@pytest.mark.parametrize("n,seed", [(8, 1), (16, 2), (35, 4), (40, 3)])
 def test_spmv_symmetric_storage_bit_exact(gpu, n, seed):
     s = _lap(n, seed)
     gpu.set_system(s)
     gpu.set_schedule(6, 6, -1)
     try:
         A = O.ldu2csr(s)
+        lay = gpu.layout()
+        assert lay["table_slices"] == lay["slices"]
         active, kept = gpu.symmetric_storage()
         full = 64 * gpu.layout()["table_entries"]
         # a 7-point stencil keeps the diagonal and the upper offsets: 4 of 7 inside
