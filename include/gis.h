@@ -126,7 +126,7 @@ typedef struct {
 } gis_layout;
 GIS_API int gis_get_layout(gis_handle h, gis_layout *out);
 /* Symmetric storage (no counterpart in the reference; a storage form of the
- * SpMV that getSparseCSRMat, gis/gis.cu:51-104, feeds): when every 64-row slice
+ * SpMV that getSparseCSRMat, gis/ldu2csr.h:7-42, feeds): when every 64-row slice
  * of the pattern has a column-offset table and the values are bitwise symmetric
  * (checked on the device at each gis_set_values), the FP64 offset-table
  * schedule stores only the entries on and above the diagonal and reads each
