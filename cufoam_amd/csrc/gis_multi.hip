@@ -62,21 +62,24 @@ __device__ __forceinline__ void sell_multi(const Mat<T>& m, int N, int row0, XOp
             // value load and gather of the slice in flight together
             for (int j = 0; j < U; j += 8) {
                 T v[8];
-                int cc[8];
+                int o8[8], cc[8];
+                // the offsets as one block of scalar loads before any use: loaded
+                // one by one between the gathers, each waited for its own load
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int kk = min(j + u, U - 1);
-                    v[u] = __builtin_nontemporal_load(dv + (size_t)kk * kSlice);
-                    cc[u] = min(max(row + offs[kk], 0), last);
-                }
+                for (int u = 0; u < 8; ++u) o8[u] = offs[min(j + u, U - 1)];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    v[u] = __builtin_nontemporal_load(dv + (size_t)min(j + u, U - 1) * kSlice);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) cc[u] = min(max(row + o8[u], 0), last);
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const bool on = (mask >> (j + u)) & 1u;
 #pragma unroll
                     for (int k = 0; k < NR; ++k) {
-                        if (!(active & (1u << k))) continue;
+                        const bool use = on && ((active >> k) & 1u);
                         const T t = y[c][k] + v[u] * xop(k, cc[u]);
-                        y[c][k] = on ? t : y[c][k];
+                        y[c][k] = use ? t : y[c][k];
                     }
                 }
             }
@@ -99,9 +102,9 @@ __device__ __forceinline__ void sell_multi(const Mat<T>& m, int N, int row0, XOp
                     const int col = on ? cc[u] : safe;
 #pragma unroll
                     for (int k = 0; k < NR; ++k) {
-                        if (!(active & (1u << k))) continue;
+                        const bool use = on && ((active >> k) & 1u);
                         const T t = y[c][k] + v[u] * xop(k, col);
-                        y[c][k] = on ? t : y[c][k];
+                        y[c][k] = use ? t : y[c][k];
                     }
                 }
             }
