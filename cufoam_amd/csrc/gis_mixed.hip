@@ -273,7 +273,9 @@ __global__ __launch_bounds__(kWG) void k_mx2(int N, float* __restrict__ r,
         const TileRows t(N, sup.tile0 + c);
         float qv[4];
         load4(r, t.i0, N, rv[c]);
-        load4(q, t.i0, N, qv);
+        // q is dead after this pass: a non-temporal load leaves the caches to
+        // r, which the next k_mx1 gathers (35.9 against 37.3 us at 256^3)
+        load4nt(q, t.i0, N, qv);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             rv[c][kk] = fmaf(-af, qv[kk], rv[c][kk]);
