@@ -234,7 +234,7 @@ __global__ __launch_bounds__(kWG) void k_cg2(int N, double* __restrict__ r,
         load_pairs(q, row0, N, qv);
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) rv[kk] = fma(-alpha, qv[kk], rv[kk]);
-        store_pairs(r, row0, N, rv);
+        store_pairs_cached(r, row0, N, rv);   // read by k_cg_dir next
         tsum[c][0] = pair_tile_sumsq(rv, row0, N, shr);
     }
     SlotReduce<1, C> tr;
