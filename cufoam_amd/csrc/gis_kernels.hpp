@@ -273,7 +273,7 @@ __global__ __launch_bounds__(kWG) void k_cg1s(Mat<double> m, int N,
         double pv[4];
         load4(p, t.i0, N, pv);
         __syncthreads();
-        store_tile_nt(q, sh_y, t.row0, N);
+        store_tile_cached(q, sh_y, t.row0, N);   // k_cg2 reads q next
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk)
             if (t.i0 + kk < N) leaf[c][0] = fma(pv[kk], sh_y[threadIdx.x * kLeaf + kk], leaf[c][0]);
